@@ -1,0 +1,591 @@
+// C ABI of the per-step cloth hot path: context, topology set-up, Forces::fill,
+// velocity solve (solver "none") and the step driver.  Host C++; every device
+// computation is a hand-written gfx950 kernel (k_*.hip).
+//
+// Reference interfaces replaced (see include/eolc.h):
+//   Forces::fill                      Forces.cpp:912-930
+//   Cloth::solve                      Cloth.cpp:343-357
+//   GeneralizedSolver::velocitySolve  GeneralizedSolver.cpp:107-172
+//   Cloth::step                       Cloth.cpp:359-413
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "eolc_internal.h"
+
+using eolc::Topology;
+
+namespace {
+
+#define EOLC_HIP(call)                                                                            \
+    do {                                                                                          \
+        hipError_t e__ = (call);                                                                  \
+        if (e__ != hipSuccess) {                                                                  \
+            c->err = std::string("HIP error ") + hipGetErrorString(e__) + " at " #call;           \
+            return EOLC_ERR_HIP;                                                                  \
+        }                                                                                         \
+    } while (0)
+
+int fail(eolc_ctx *c, int code, const std::string &msg)
+{
+    c->err = msg;
+    return code;
+}
+
+template <class T> hipError_t dalloc(eolc_ctx *c, T **p, size_t n)
+{
+    if (*p) {
+        hipError_t e = hipFree(*p);
+        if (e != hipSuccess) return e;
+        *p = nullptr;
+    }
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e == hipSuccess) c->alloc_bytes += n * sizeof(T);
+    return e;
+}
+
+template <class T> void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_topology_buffers(eolc_ctx *c)
+{
+    dfree(c->X); dfree(c->tri); dfree(c->hinge);
+    dfree(c->rowptr); dfree(c->col); dfree(c->diagpos);
+    dfree(c->cptr); dfree(c->clist); dfree(c->fptr); dfree(c->flist);
+    dfree(c->mptr); dfree(c->mlist);
+    dfree(c->stageK); dfree(c->stageF); dfree(c->stageM);
+    dfree(c->x); dfree(c->v); dfree(c->f); dfree(c->rhs);
+    dfree(c->A64); dfree(c->A32); dfree(c->Mval); dfree(c->D64); dfree(c->D32);
+    dfree(c->fixmask); dfree(c->vfix);
+    dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
+    dfree(c->p64[0]); dfree(c->p64[1]);
+    dfree(c->r32); dfree(c->z32); dfree(c->q32); dfree(c->xl32);
+    dfree(c->p32[0]); dfree(c->p32[1]);
+    dfree(c->cedges);
+    c->alloc_bytes = 0;
+}
+
+// Local blocks of a face (fillxMI / fillxxMI order) and of a hinge (fillxB / fillxxB).
+const int kFacePairs[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+const int kHingePairs[10][2] = {{0, 0}, {1, 1}, {2, 2}, {3, 3}, {0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+
+int find_block(const Topology &t, int i, int j)
+{
+    const int *b = t.col.data() + t.rowptr[i];
+    const int *e = t.col.data() + t.rowptr[i + 1];
+    const int *p = std::lower_bound(b, e, j);
+    return (p != e && *p == j) ? (int)(p - t.col.data()) : -1;
+}
+
+// Builds the BSR pattern and the gather maps.  Returns an error message or "".
+std::string build_topology(Topology &t, int N, int F, const int *tri, int H, const int *hinge)
+{
+    t.N = N; t.F = F; t.H = H;
+    for (int i = 0; i < 3 * F; ++i)
+        if (tri[i] < 0 || tri[i] >= N) return "triangle vertex index out of range";
+    for (int i = 0; i < 4 * H; ++i)
+        if (hinge[i] < 0 || hinge[i] >= N) return "hinge vertex index out of range";
+    // neighbour sets
+    std::vector<int> deg(N + 1, 1);
+    for (int f = 0; f < F; ++f)
+        for (int v = 0; v < 3; ++v) deg[tri[3 * f + v]] += 2;
+    for (int h = 0; h < H; ++h)
+        for (int v = 0; v < 4; ++v) deg[hinge[4 * h + v]] += 3;
+    std::vector<int64_t> off(N + 1, 0);
+    for (int i = 0; i < N; ++i) off[i + 1] = off[i] + deg[i];
+    std::vector<int> nbr((size_t)off[N]);
+    std::vector<int64_t> cur(off.begin(), off.end() - 1);
+    for (int i = 0; i < N; ++i) nbr[cur[i]++] = i;
+    for (int f = 0; f < F; ++f)
+        for (int v = 0; v < 3; ++v)
+            for (int w = 0; w < 3; ++w)
+                if (v != w) nbr[cur[tri[3 * f + v]]++] = tri[3 * f + w];
+    for (int h = 0; h < H; ++h)
+        for (int v = 0; v < 4; ++v)
+            for (int w = 0; w < 4; ++w)
+                if (v != w) nbr[cur[hinge[4 * h + v]]++] = hinge[4 * h + w];
+    t.rowptr.assign(N + 1, 0);
+    t.col.clear();
+    t.col.reserve((size_t)off[N]);
+    for (int i = 0; i < N; ++i) {
+        auto b = nbr.begin() + off[i], e = nbr.begin() + cur[i];
+        std::sort(b, e);
+        e = std::unique(b, e);
+        t.col.insert(t.col.end(), b, e);
+        if (t.col.size() > (size_t)INT32_MAX) return "too many blocks for 32-bit indices";
+        t.rowptr[i + 1] = (int)t.col.size();
+    }
+    t.nb = (int64_t)t.col.size();
+    t.nbp = (t.nb + 63) & ~(int64_t)63;
+    t.diagpos.resize(N);
+    for (int i = 0; i < N; ++i) t.diagpos[i] = find_block(t, i, i);
+    // block contributions in reference insertion order (slot order)
+    const int64_t nslots = 6 * (int64_t)F + 10 * (int64_t)H;
+    if (2 * nslots + 1 > (int64_t)INT32_MAX) return "too many element slots for 32-bit indices";
+    std::vector<int> cnt(t.nb + 1, 0);
+    std::vector<int> slotblk((size_t)nslots * 2);
+    auto visit = [&](int64_t slot, int i, int j) {
+        int b1 = find_block(t, i, j);
+        slotblk[2 * slot] = b1;
+        cnt[b1 + 1]++;
+        if (i != j) {
+            int b2 = find_block(t, j, i);
+            slotblk[2 * slot + 1] = b2;
+            cnt[b2 + 1]++;
+        } else {
+            slotblk[2 * slot + 1] = -1;
+        }
+    };
+    for (int f = 0; f < F; ++f)
+        for (int p = 0; p < 6; ++p)
+            visit(6 * (int64_t)f + p, tri[3 * f + kFacePairs[p][0]], tri[3 * f + kFacePairs[p][1]]);
+    for (int h = 0; h < H; ++h)
+        for (int p = 0; p < 10; ++p)
+            visit(6 * (int64_t)F + 10 * (int64_t)h + p, hinge[4 * h + kHingePairs[p][0]],
+                  hinge[4 * h + kHingePairs[p][1]]);
+    t.cptr.assign(t.nb + 1, 0);
+    for (int64_t b = 0; b < t.nb; ++b) t.cptr[b + 1] = t.cptr[b] + cnt[b + 1];
+    t.clist.assign(t.cptr[t.nb], 0);
+    std::vector<int> fill(t.cptr.begin(), t.cptr.end() - 1);
+    for (int64_t slot = 0; slot < nslots; ++slot) {
+        int b1 = slotblk[2 * slot], b2 = slotblk[2 * slot + 1];
+        t.clist[fill[b1]++] = (int)(slot << 1);
+        if (b2 >= 0) t.clist[fill[b2]++] = (int)((slot << 1) | 1);
+    }
+    // vertex force slots: faces (3f+v), then hinges (3F + 4h + v)
+    t.fptr.assign(N + 1, 0);
+    for (int f = 0; f < F; ++f)
+        for (int v = 0; v < 3; ++v) t.fptr[tri[3 * f + v] + 1]++;
+    for (int h = 0; h < H; ++h)
+        for (int v = 0; v < 4; ++v) t.fptr[hinge[4 * h + v] + 1]++;
+    for (int i = 0; i < N; ++i) t.fptr[i + 1] += t.fptr[i];
+    t.flist.assign(t.fptr[N], 0);
+    std::vector<int> ff(t.fptr.begin(), t.fptr.end() - 1);
+    for (int f = 0; f < F; ++f)
+        for (int v = 0; v < 3; ++v) t.flist[ff[tri[3 * f + v]]++] = 3 * f + v;
+    for (int h = 0; h < H; ++h)
+        for (int v = 0; v < 4; ++v) t.flist[ff[hinge[4 * h + v]]++] = 3 * F + 4 * h + v;
+    return "";
+}
+
+template <class T> hipError_t upload(eolc_ctx *c, T **dst, const std::vector<T> &src)
+{
+    hipError_t e = dalloc(c, dst, src.size());
+    if (e != hipSuccess) return e;
+    if (src.empty()) return hipSuccess;
+    return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return (double)ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int eolc_create(int device, eolc_ctx **out)
+{
+    if (!out) return EOLC_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return EOLC_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return EOLC_ERR_NO_DEVICE;
+    eolc_ctx *c = new (std::nothrow) eolc_ctx();
+    if (!c) return EOLC_ERR_INVALID;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return EOLC_ERR_HIP;
+    }
+    if (hipMalloc((void **)&c->scal, sizeof(eolc::SolverScalars)) != hipSuccess ||
+        hipMalloc((void **)&c->partials, sizeof(double) * 4 * eolc::kMaxPartials) != hipSuccess ||
+        hipMalloc((void **)&c->counter, sizeof(unsigned) * 4) != hipSuccess ||
+        hipHostMalloc((void **)&c->hscal, sizeof(eolc::SolverScalars), hipHostMallocDefault) != hipSuccess ||
+        hipMemset(c->counter, 0, sizeof(unsigned) * 4) != hipSuccess ||
+        hipMemset(c->scal, 0, sizeof(eolc::SolverScalars)) != hipSuccess) {
+        eolc_destroy(c);
+        return EOLC_ERR_HIP;
+    }
+    *out = c;
+    return EOLC_OK;
+}
+
+void eolc_destroy(eolc_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_topology_buffers(c);
+    dfree(c->scal); dfree(c->partials); dfree(c->counter);
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *eolc_last_error(const eolc_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int eolc_set_topology(eolc_ctx *c, int N, const double *X, int F, const int *tri, int H, const int *hinge)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (N <= 0 || F < 0 || H < 0 || !X || (F && !tri) || (H && !hinge))
+        return fail(c, EOLC_ERR_INVALID, "set_topology: bad sizes or null arrays");
+    EOLC_HIP(hipSetDevice(c->device));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    for (int f = 0; f < F; ++f) {
+        const int a = tri[3 * f], b = tri[3 * f + 1], cc = tri[3 * f + 2];
+        if (a < 0 || a >= N || b < 0 || b >= N || cc < 0 || cc >= N)
+            return fail(c, EOLC_ERR_INVALID, "set_topology: triangle index out of range");
+        double A2 = (X[2 * b] - X[2 * a]) * (X[2 * cc + 1] - X[2 * a + 1]) -
+                    (X[2 * b + 1] - X[2 * a + 1]) * (X[2 * cc] - X[2 * a]);
+        if (!(A2 > 0.0))
+            return fail(c, EOLC_ERR_DEGENERATE,
+                        "set_topology: face " + std::to_string(f) + " has non-positive material area (must be CCW)");
+    }
+    Topology t;
+    std::string msg = build_topology(t, N, F, tri, H, hinge);
+    if (!msg.empty()) return fail(c, EOLC_ERR_INVALID, "set_topology: " + msg);
+    free_topology_buffers(c);
+    c->have_topo = c->have_state = c->have_forces = false;
+    c->topo = std::move(t);
+    const Topology &T = c->topo;
+    std::vector<double> Xv(X, X + 2 * (size_t)N);
+    std::vector<int> triv(tri, tri + 3 * (size_t)F), hv(hinge, hinge + 4 * (size_t)H);
+    EOLC_HIP(upload(c, &c->X, Xv));
+    EOLC_HIP(upload(c, &c->tri, triv));
+    EOLC_HIP(upload(c, &c->hinge, hv));
+    EOLC_HIP(upload(c, &c->rowptr, T.rowptr));
+    EOLC_HIP(upload(c, &c->col, T.col));
+    EOLC_HIP(upload(c, &c->diagpos, T.diagpos));
+    EOLC_HIP(upload(c, &c->cptr, T.cptr));
+    EOLC_HIP(upload(c, &c->clist, T.clist));
+    EOLC_HIP(upload(c, &c->fptr, T.fptr));
+    EOLC_HIP(upload(c, &c->flist, T.flist));
+    const size_t nslots = 6 * (size_t)F + 10 * (size_t)H;
+    const size_t n3 = 3 * (size_t)N;
+    EOLC_HIP(dalloc(c, &c->stageK, 9 * nslots));
+    EOLC_HIP(dalloc(c, &c->stageF, 3 * (3 * (size_t)F + 4 * (size_t)H)));
+    EOLC_HIP(dalloc(c, &c->stageM, 6 * (size_t)F));
+    EOLC_HIP(dalloc(c, &c->x, n3));
+    EOLC_HIP(dalloc(c, &c->v, n3));
+    EOLC_HIP(dalloc(c, &c->f, n3));
+    EOLC_HIP(dalloc(c, &c->rhs, n3));
+    EOLC_HIP(dalloc(c, &c->A64, 9 * (size_t)T.nbp));
+    EOLC_HIP(dalloc(c, &c->A32, 9 * (size_t)T.nbp));
+    EOLC_HIP(dalloc(c, &c->Mval, (size_t)T.nb));
+    EOLC_HIP(dalloc(c, &c->D64, 9 * (size_t)N));
+    EOLC_HIP(dalloc(c, &c->D32, 9 * (size_t)N));
+    EOLC_HIP(dalloc(c, &c->fixmask, ((size_t)N + 3) & ~(size_t)3));
+    EOLC_HIP(dalloc(c, &c->vfix, n3));
+    EOLC_HIP(hipMemset(c->fixmask, 0, ((size_t)N + 3) & ~(size_t)3));
+    EOLC_HIP(hipMemset(c->vfix, 0, n3 * sizeof(double)));
+    for (double **p : {&c->b64, &c->xs64, &c->r64, &c->z64, &c->q64, &c->p64[0], &c->p64[1]})
+        EOLC_HIP(dalloc(c, p, n3));
+    for (float **p : {&c->r32, &c->z32, &c->q32, &c->xl32, &c->p32[0], &c->p32[1]}) EOLC_HIP(dalloc(c, p, n3));
+    c->have_topo = true;
+    return EOLC_OK;
+}
+
+int eolc_set_material(eolc_ctx *c, double density, double e, double nu, double beta, double dampingA,
+                      double dampingB)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!(std::isfinite(density) && std::isfinite(e) && std::isfinite(nu) && std::isfinite(beta) &&
+          std::isfinite(dampingB)) || nu >= 0.5 || nu <= -1.0)
+        return fail(c, EOLC_ERR_INVALID, "set_material: non-finite value or Poisson ratio outside (-1, 0.5)");
+    c->rho = density; c->E = e; c->nu = nu; c->beta = beta; c->dampA = dampingA; c->dampB = dampingB;
+    c->have_mat = true;
+    c->have_forces = false;
+    return EOLC_OK;
+}
+
+int eolc_set_options(eolc_ctx *c, int flags)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (flags & ~(EOLC_OPT_INCLUDE_BENDING_FORCE | EOLC_OPT_SPD_PROJECT_BENDING))
+        return fail(c, EOLC_ERR_INVALID, "set_options: unknown flag");
+    c->flags = flags;
+    c->have_forces = false;
+    return EOLC_OK;
+}
+
+int eolc_upload_state(eolc_ctx *c, const double *x, const double *v)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_topo) return fail(c, EOLC_ERR_INVALID, "upload_state: no topology");
+    if (!x || !v) return fail(c, EOLC_ERR_INVALID, "upload_state: null array");
+    EOLC_HIP(hipSetDevice(c->device));
+    const size_t n = 3 * (size_t)c->topo.N;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(v[i]))
+            return fail(c, EOLC_ERR_NONFINITE, "upload_state: non-finite entry at DOF " + std::to_string(i));
+    EOLC_HIP(hipMemcpyAsync(c->x, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    EOLC_HIP(hipMemcpyAsync(c->v, v, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    c->have_forces = false;
+    return EOLC_OK;
+}
+
+int eolc_download_state(eolc_ctx *c, double *x, double *v)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_state) return fail(c, EOLC_ERR_INVALID, "download_state: no state");
+    EOLC_HIP(hipSetDevice(c->device));
+    const size_t n = 3 * (size_t)c->topo.N;
+    if (x) EOLC_HIP(hipMemcpyAsync(x, c->x, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (v) EOLC_HIP(hipMemcpyAsync(v, c->v, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    return EOLC_OK;
+}
+
+static int forces_fill_async(eolc_ctx *c, const double grav[3], double h, bool want32)
+{
+    EOLC_HIP(hipMemsetAsync(c->counter + 1, 0, 2 * sizeof(unsigned), c->stream));
+    EOLC_HIP(eolc::launch_face_eval(c, h, grav));
+    EOLC_HIP(eolc::launch_hinge_eval(c, h));
+    EOLC_HIP(eolc::launch_assemble(c, want32));
+    c->h_last = h;
+    c->have_forces = true;
+    return EOLC_OK;
+}
+
+int eolc_forces_fill(eolc_ctx *c, const double grav[3], double h)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_topo || !c->have_mat || !c->have_state)
+        return fail(c, EOLC_ERR_INVALID, "forces_fill: topology, material and state must be set first");
+    if (!grav || !std::isfinite(h) || h <= 0.0) return fail(c, EOLC_ERR_INVALID, "forces_fill: bad gravity or h");
+    EOLC_HIP(hipSetDevice(c->device));
+    int rc = forces_fill_async(c, grav, h, true);
+    if (rc) return rc;
+    EOLC_HIP(eolc::launch_rhs(c, h));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    return EOLC_OK;
+}
+
+int eolc_download_forces(eolc_ctx *c, double *f)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_forces || !f) return fail(c, EOLC_ERR_INVALID, "download_forces: no forces");
+    EOLC_HIP(hipSetDevice(c->device));
+    EOLC_HIP(hipMemcpyAsync(f, c->f, 3 * (size_t)c->topo.N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    return EOLC_OK;
+}
+
+int eolc_download_csr(eolc_ctx *c, int which, int64_t *rowptr, int *col, double *val, int64_t cap, int64_t *nnz)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_forces) return fail(c, EOLC_ERR_INVALID, "download_csr: call forces_fill first");
+    if (which != EOLC_MAT_M && which != EOLC_MAT_MDK) return fail(c, EOLC_ERR_INVALID, "download_csr: bad matrix id");
+    if (!nnz) return fail(c, EOLC_ERR_INVALID, "download_csr: null nnz");
+    EOLC_HIP(hipSetDevice(c->device));
+    const Topology &T = c->topo;
+    const int N = T.N;
+    // blocks present: MDK = all; M = blocks with at least one triangle slot
+    std::vector<char> keep(T.nb, 1);
+    if (which == EOLC_MAT_M) {
+        const int ntri = 6 * T.F;
+        for (int64_t b = 0; b < T.nb; ++b) {
+            char k = 0;
+            for (int e = T.cptr[b]; e < T.cptr[b + 1]; ++e)
+                if ((T.clist[e] >> 1) < ntri) { k = 1; break; }
+            keep[b] = k;
+        }
+    }
+    int64_t need = 0;
+    for (int64_t b = 0; b < T.nb; ++b) need += keep[b] ? 9 : 0;
+    *nnz = need;
+    if (!rowptr || !col || !val || cap < need) return fail(c, EOLC_ERR_OVERFLOW, "download_csr: capacity too small");
+    std::vector<double> A((size_t)9 * T.nbp), m((size_t)T.nb);
+    EOLC_HIP(hipMemcpyAsync(A.data(), c->A64, A.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipMemcpyAsync(m.data(), c->Mval, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    int64_t p = 0;
+    for (int i = 0; i < N; ++i)
+        for (int a = 0; a < 3; ++a) {
+            rowptr[3 * i + a] = p;
+            for (int k = T.rowptr[i]; k < T.rowptr[i + 1]; ++k) {
+                if (!keep[k]) continue;
+                const int j = T.col[k];
+                for (int cc = 0; cc < 3; ++cc) {
+                    col[p] = 3 * j + cc;
+                    val[p] = (which == EOLC_MAT_MDK) ? A[(size_t)(3 * a + cc) * T.nbp + k] : (a == cc ? m[k] : 0.0);
+                    ++p;
+                }
+            }
+        }
+    rowptr[3 * N] = p;
+    return EOLC_OK;
+}
+
+// ---------------------------------------------------------------------------
+static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *fixed_val, double tol,
+                      int maxit, int mode, eolc_solve_stats *st)
+{
+    const int N = c->topo.N;
+    const bool mixed = (mode == EOLC_SOLVE_MIXED_BJ || mode == EOLC_SOLVE_MIXED_LEARNED);
+    // fixed DOFs -> device mask and values
+    for (int i = 0; i < n_fixed; ++i)
+        if (fixed_dof[i] < 0 || fixed_dof[i] >= 3 * N || !std::isfinite(fixed_val[i]))
+            return fail(c, EOLC_ERR_INVALID, "velocity_solve: fixed DOF index out of range or value non-finite");
+    int *d_dofs = nullptr;
+    double *d_vals = nullptr;
+    if (n_fixed > 0) {
+        EOLC_HIP(hipMalloc((void **)&d_dofs, sizeof(int) * n_fixed));
+        EOLC_HIP(hipMalloc((void **)&d_vals, sizeof(double) * n_fixed));
+        EOLC_HIP(hipMemcpyAsync(d_dofs, fixed_dof, sizeof(int) * n_fixed, hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(hipMemcpyAsync(d_vals, fixed_val, sizeof(double) * n_fixed, hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(eolc::set_fixed(c, n_fixed, d_dofs, d_vals, 1));
+    }
+    const int precond = (mode == EOLC_SOLVE_FP64_JACOBI) ? 1 : 0;
+    EOLC_HIP(eolc::pcg_setup(c, mixed, precond));
+    const double delta = mixed ? 0.1 : 0.0;
+    EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
+    EOLC_HIP(eolc::pcg_init(c, mixed, tol));
+    int parity = 0, done_iters = 0, refinements = 0, status = EOLC_ERR_MAXIT;
+    int chunk = 8;
+    double relres = NAN;
+    for (;;) {
+        int todo = std::min(chunk, maxit - done_iters);
+        if (todo <= 0) { status = EOLC_ERR_MAXIT; break; }
+        EOLC_HIP(eolc::pcg_chunk(c, mixed, todo, &parity));
+        EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(hipStreamSynchronize(c->stream));
+        const eolc::SolverScalars s = *c->hscal;
+        done_iters = s.iters;
+        parity = s.iters & 1;
+        relres = s.bnorm2 > 0 ? std::sqrt(s.rr / s.bnorm2) : 0.0;
+        if (chunk < 256) chunk *= 2;
+        if (!s.done) continue;
+        if (s.reason == 2) { status = EOLC_ERR_NOT_SPD; break; }
+        if (s.reason == 4) { status = EOLC_ERR_NONFINITE; break; }
+        if (!mixed) { status = EOLC_OK; break; }
+        // mixed: reliable update with the true FP64 residual
+        EOLC_HIP(eolc::pcg_reliable_update(c, tol, parity));
+        EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(hipStreamSynchronize(c->stream));
+        ++refinements;
+        const eolc::SolverScalars s2 = *c->hscal;
+        relres = s2.bnorm2 > 0 ? std::sqrt(s2.rmax2 / s2.bnorm2) : 0.0;
+        if (!std::isfinite(s2.rmax2)) { status = EOLC_ERR_NONFINITE; break; }
+        if (s2.rmax2 <= tol * tol * s2.bnorm2) { status = EOLC_OK; break; }
+        EOLC_HIP(eolc::pcg_restart_inner(c));
+        EOLC_HIP(eolc::scal_set(c, 0, 0, 0));
+    }
+    if (mixed && status != EOLC_OK && status != EOLC_ERR_NOT_SPD) {
+        // fold the last FP32 correction in before reporting
+        EOLC_HIP(eolc::pcg_reliable_update(c, tol, parity));
+    }
+    EOLC_HIP(eolc::pcg_finish(c, mixed));
+    EOLC_HIP(eolc::spmv_true_residual(c, nullptr));
+    EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
+    if (n_fixed > 0) EOLC_HIP(eolc::set_fixed(c, n_fixed, d_dofs, d_vals, 0));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    if (d_dofs) EOLC_HIP(hipFree(d_dofs));
+    if (d_vals) EOLC_HIP(hipFree(d_vals));
+    if (st) {
+        st->iters = c->hscal->iters;
+        st->refinements = refinements;
+        st->status = status;
+        st->used_fallback = 0;
+        st->relres = relres;
+        st->true_relres = c->hscal->bnorm2 > 0 ? std::sqrt(c->hscal->rr / c->hscal->bnorm2) : 0.0;
+    }
+    if (status == EOLC_ERR_NOT_SPD) c->err = "velocity_solve: p^T A p <= 0 (MDK is not positive definite)";
+    else if (status == EOLC_ERR_MAXIT) c->err = "velocity_solve: iteration cap reached";
+    else if (status == EOLC_ERR_NONFINITE) c->err = "velocity_solve: non-finite value in the solve";
+    return status;
+}
+
+int eolc_velocity_solve(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *fixed_val, double tol,
+                        int maxit, int mode, eolc_solve_stats *st)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_forces) return fail(c, EOLC_ERR_INVALID, "velocity_solve: call forces_fill first");
+    if (n_fixed < 0 || (n_fixed && (!fixed_dof || !fixed_val)) || !(tol > 0.0) || maxit <= 0)
+        return fail(c, EOLC_ERR_INVALID, "velocity_solve: bad arguments");
+    if (mode < EOLC_SOLVE_FP64_BJ || mode > EOLC_SOLVE_FP64_JACOBI)
+        return fail(c, EOLC_ERR_INVALID, "velocity_solve: unknown mode");
+    EOLC_HIP(hipSetDevice(c->device));
+    if (st) std::memset(st, 0, sizeof(*st));
+    return solve_impl(c, n_fixed, fixed_dof, fixed_val, tol, maxit, mode, st);
+}
+
+int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, const int *fixed_nodes,
+              const double *scripted_vel, double tol, int maxit, int mode, eolc_solve_stats *st)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_topo || !c->have_mat || !c->have_state)
+        return fail(c, EOLC_ERR_INVALID, "step: topology, material and state must be set first");
+    if (!grav || !(h > 0.0) || n_fixed_nodes < 0 || (n_fixed_nodes && !fixed_nodes) || !(tol > 0.0) || maxit <= 0)
+        return fail(c, EOLC_ERR_INVALID, "step: bad arguments");
+    if (mode < EOLC_SOLVE_FP64_BJ || mode > EOLC_SOLVE_FP64_JACOBI) return fail(c, EOLC_ERR_INVALID, "step: unknown mode");
+    EOLC_HIP(hipSetDevice(c->device));
+    const int N = c->topo.N;
+    for (int i = 0; i < n_fixed_nodes; ++i)
+        if (fixed_nodes[i] < 0 || fixed_nodes[i] >= N) return fail(c, EOLC_ERR_INVALID, "step: fixed node out of range");
+    hipEvent_t ev[5];
+    for (auto &e : ev) EOLC_HIP(hipEventCreate(&e));
+    EOLC_HIP(hipEventRecord(ev[0], c->stream));
+    const bool mixed = (mode == EOLC_SOLVE_MIXED_BJ || mode == EOLC_SOLVE_MIXED_LEARNED);
+    int rc = forces_fill_async(c, grav, h, mixed);
+    if (rc) return rc;
+    EOLC_HIP(hipEventRecord(ev[1], c->stream));
+    EOLC_HIP(eolc::launch_rhs(c, h));
+    EOLC_HIP(hipEventRecord(ev[2], c->stream));
+    // v_fixed = (1 - 0.01) v_old + scripted (Constraints.cpp:114-119)
+    std::vector<int> dofs(3 * (size_t)n_fixed_nodes);
+    std::vector<double> vals(3 * (size_t)n_fixed_nodes);
+    if (n_fixed_nodes > 0) {
+        std::vector<double> vold(3 * (size_t)n_fixed_nodes);
+        for (int i = 0; i < n_fixed_nodes; ++i)
+            EOLC_HIP(hipMemcpyAsync(&vold[3 * i], c->v + 3 * (size_t)fixed_nodes[i], 3 * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < n_fixed_nodes; ++i)
+            for (int k = 0; k < 3; ++k) {
+                dofs[3 * i + k] = 3 * fixed_nodes[i] + k;
+                vals[3 * i + k] = (1 - 0.01) * vold[3 * i + k] + (scripted_vel ? scripted_vel[3 * i + k] : 0.0);
+            }
+    }
+    eolc_solve_stats local;
+    std::memset(&local, 0, sizeof(local));
+    rc = solve_impl(c, 3 * n_fixed_nodes, dofs.data(), vals.data(), tol, maxit, mode, &local);
+    EOLC_HIP(hipEventRecord(ev[3], c->stream));
+    if (rc == EOLC_OK) EOLC_HIP(eolc::launch_update_x(c, h));
+    EOLC_HIP(hipEventRecord(ev[4], c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    local.ms_assembly = elapsed_ms(ev[0], ev[1]);
+    local.ms_rhs = elapsed_ms(ev[1], ev[2]);
+    local.ms_solve = elapsed_ms(ev[2], ev[3]);
+    local.ms_update = elapsed_ms(ev[3], ev[4]);
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (st) *st = local;
+    return rc;
+}
+
+int eolc_time_spmv(eolc_ctx *c, int precision, int reps, double *avg_ms)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_forces || (precision != 32 && precision != 64) || reps <= 0 || !avg_ms)
+        return fail(c, EOLC_ERR_INVALID, "time_spmv: bad arguments or no assembled matrix");
+    EOLC_HIP(hipSetDevice(c->device));
+    float ms = 0.f;
+    EOLC_HIP(eolc::time_spmv(c, precision == 32, reps, &ms));
+    *avg_ms = ms / reps;
+    return EOLC_OK;
+}
+
+}  // extern "C"

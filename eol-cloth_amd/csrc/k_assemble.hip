@@ -1,0 +1,108 @@
+// Deterministic, atomic-free assembly (gfx950).
+//
+// Replaces the triplet push + setFromTriplets of Forces::fill (Forces.cpp:103-125,
+// 522-539, 928-929).  The host builds, once per topology, the list of element
+// slots feeding every 3x3 block in the order the reference inserts its triplets
+// (faces in face order, then hinges in edge order), so each block is summed in
+// the same order Eigen's collapseDuplicates folds duplicates -- without atomics.
+//
+//   k_assemble : one thread per block; MDK block = sum of its staged slots (a slot
+//                flagged "transposed" is the (w,v) image of an element's (v,w)
+//                block); the mass scalar sums the triangle slots.  Writes the FP64
+//                SoA values and, for the mixed solver, an FP32 copy.
+//   k_rhs      : one thread per vertex; f_i = sum of its element force slots
+//                (faces, then hinges when the bending force is included) and
+//                rhs_i = sum_j m_ij v_j + h f_i, the right-hand side M v + h f of
+//                Cloth::solve (Cloth.cpp:345).
+#include "eolc_internal.h"
+
+namespace {
+
+__global__ void __launch_bounds__(256)
+k_assemble(int64_t nb, int64_t nbp, int ntrislots, const int *__restrict__ cptr, const int *__restrict__ clist,
+           const double *__restrict__ stageK, const double *__restrict__ stageM, double *__restrict__ A64,
+           float *__restrict__ A32, double *__restrict__ Mval)
+{
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double acc[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) acc[c] = 0.0;
+    double m = 0.0;
+    const int e0 = cptr[b], e1 = cptr[b + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int code = clist[e];
+        const int slot = code >> 1;
+        const double *sk = stageK + (size_t)9 * slot;
+        if (code & 1) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[3 * j + k] += sk[3 * k + j];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) acc[c] += sk[c];
+        }
+        if (slot < ntrislots) m += stageM[slot];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) A64[c * nbp + b] = acc[c];
+    if (A32) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) A32[c * nbp + b] = (float)acc[c];
+    }
+    Mval[b] = m;
+}
+
+__global__ void __launch_bounds__(256)
+k_rhs(int N, int ntriforce, int include_bending, double h, const int *__restrict__ fptr,
+      const int *__restrict__ flist, const double *__restrict__ stageF, const int *__restrict__ rowptr,
+      const int *__restrict__ col, const double *__restrict__ Mval, const double *__restrict__ v,
+      double *__restrict__ f, double *__restrict__ rhs)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    for (int e = fptr[i]; e < fptr[i + 1]; ++e) {
+        const int slot = flist[e];
+        if (slot >= ntriforce && !include_bending) continue;
+        const double *sf = stageF + (size_t)3 * slot;
+        f0 += sf[0]; f1 += sf[1]; f2 += sf[2];
+    }
+    f[3 * i] = f0; f[3 * i + 1] = f1; f[3 * i + 2] = f2;
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        const int j = col[k];
+        const double m = Mval[k];
+        y0 += m * v[3 * j]; y1 += m * v[3 * j + 1]; y2 += m * v[3 * j + 2];
+    }
+    rhs[3 * i] = y0 + h * f0;
+    rhs[3 * i + 1] = y1 + h * f1;
+    rhs[3 * i + 2] = y2 + h * f2;
+}
+
+inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+
+}  // namespace
+
+namespace eolc {
+
+hipError_t launch_assemble(eolc_ctx *c, bool want32)
+{
+    const int64_t nb = c->topo.nb;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, c->topo.nbp, 6 * c->topo.F,
+                       c->cptr, c->clist, c->stageK, c->stageM, c->A64, want32 ? c->A32 : nullptr, c->Mval);
+    return hipGetLastError();
+}
+
+hipError_t launch_rhs(eolc_ctx *c, double h)
+{
+    const int N = c->topo.N;
+    hipLaunchKernelGGL(k_rhs, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, 3 * c->topo.F,
+                       (c->flags & EOLC_OPT_INCLUDE_BENDING_FORCE) ? 1 : 0, h, c->fptr, c->flist, c->stageF,
+                       c->rowptr, c->col, c->Mval, c->v, c->f, c->rhs);
+    return hipGetLastError();
+}
+
+}  // namespace eolc

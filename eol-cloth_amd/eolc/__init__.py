@@ -1,0 +1,232 @@
+"""Python binding of the eolc C ABI (include/eolc.h) over ctypes.
+
+The library (``libeolc.so``, built in-tree by ``make -C eol-cloth_amd``) holds the
+gfx950 kernels; this module only moves host arrays across the ABI.  There is no
+CPU fallback: every compute call goes to the GPU, and loading fails loudly when
+the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libeolc.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_NOT_SPD, ERR_MAXIT, ERR_OVERFLOW, ERR_NONFINITE, ERR_DEGENERATE, ERR_NO_DEVICE = range(9)
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOT_SPD", 4: "MAXIT", 5: "OVERFLOW",
+                6: "NONFINITE", 7: "DEGENERATE", 8: "NO_DEVICE"}
+
+OPT_INCLUDE_BENDING_FORCE = 0x1
+OPT_SPD_PROJECT_BENDING = 0x2
+
+SOLVE_FP64_BJ, SOLVE_MIXED_BJ, SOLVE_MIXED_LEARNED, SOLVE_FP64_JACOBI = 0, 1, 2, 3
+MAT_M, MAT_MDK = 0, 1
+
+EXPORTED_SYMBOLS = (
+    "eolc_create", "eolc_destroy", "eolc_last_error", "eolc_set_topology", "eolc_set_material",
+    "eolc_set_options", "eolc_upload_state", "eolc_download_state", "eolc_forces_fill",
+    "eolc_download_forces", "eolc_download_csr", "eolc_velocity_solve", "eolc_step", "eolc_collide",
+    "eolc_time_spmv",
+)
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("refinements", C.c_int), ("status", C.c_int),
+                ("used_fallback", C.c_int), ("relres", C.c_double), ("true_relres", C.c_double),
+                ("ms_assembly", C.c_double), ("ms_rhs", C.c_double), ("ms_solve", C.c_double),
+                ("ms_update", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CollisionRec(C.Structure):
+    _fields_ = [("dist", C.c_double), ("nor1", C.c_double * 3), ("nor2", C.c_double * 3),
+                ("pos1", C.c_double * 3), ("pos2", C.c_double * 3), ("pos1_", C.c_double * 3),
+                ("count1", C.c_int), ("count2", C.c_int), ("verts1", C.c_int * 3),
+                ("verts2", C.c_int * 3), ("weights1", C.c_double * 3), ("weights2", C.c_double * 3),
+                ("tri1", C.c_int), ("tri2", C.c_int), ("nedge1", C.c_int), ("edge1", C.c_int * 3),
+                ("edge2", C.c_int), ("edgeDir", C.c_double * 3)]
+
+
+class EolcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"eolc error {STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load libeolc.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.eolc_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.eolc_destroy.argtypes = [vp]
+        L.eolc_destroy.restype = None
+        L.eolc_last_error.argtypes = [vp]
+        L.eolc_last_error.restype = C.c_char_p
+        L.eolc_set_topology.argtypes = [vp, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip]
+        L.eolc_set_material.argtypes = [vp] + [C.c_double] * 6
+        L.eolc_set_options.argtypes = [vp, C.c_int]
+        L.eolc_upload_state.argtypes = [vp, _dp, _dp]
+        L.eolc_download_state.argtypes = [vp, _dp, _dp]
+        L.eolc_forces_fill.argtypes = [vp, _dp, C.c_double]
+        L.eolc_download_forces.argtypes = [vp, _dp]
+        L.eolc_download_csr.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                        C.POINTER(C.c_int64)]
+        L.eolc_velocity_solve.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                          C.c_int, C.POINTER(SolveStats)]
+        L.eolc_step.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                C.c_int, C.c_int, C.POINTER(SolveStats)]
+        L.eolc_collide.argtypes = [vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(CollisionRec), C.c_int, C.POINTER(C.c_int)]
+        L.eolc_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One eolc_ctx: device buffers for one cloth on one GPU."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().eolc_create(int(device), C.byref(self._h))
+        if rc != OK:
+            raise EolcError(rc, "eolc_create failed (no usable GPU?)")
+        self.N = 0
+
+    def close(self):
+        if self._h:
+            lib().eolc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, allow=()):
+        if rc != OK and rc not in allow:
+            raise EolcError(rc, lib().eolc_last_error(self._h).decode())
+        return rc
+
+    def last_error(self):
+        return lib().eolc_last_error(self._h).decode()
+
+    def set_topology(self, X, tri, hinge):
+        X, tri, hinge = _d(X), _i(tri), _i(hinge)
+        self.N = X.size // 2
+        return self._check(lib().eolc_set_topology(self._h, self.N, X, tri.size // 3, tri,
+                                                   hinge.size // 4, hinge))
+
+    def set_material(self, density=0.1, e=500.0, nu=0.3, beta=1e-5, dampingA=0.0, dampingB=1.0):
+        return self._check(lib().eolc_set_material(self._h, density, e, nu, beta, dampingA, dampingB))
+
+    def set_options(self, flags):
+        return self._check(lib().eolc_set_options(self._h, int(flags)))
+
+    def upload_state(self, x, v):
+        return self._check(lib().eolc_upload_state(self._h, _d(x), _d(v)))
+
+    def download_state(self):
+        x, v = np.zeros(3 * self.N), np.zeros(3 * self.N)
+        self._check(lib().eolc_download_state(self._h, x, v))
+        return x, v
+
+    def forces_fill(self, grav, h):
+        return self._check(lib().eolc_forces_fill(self._h, _d(grav), float(h)))
+
+    def download_forces(self):
+        f = np.zeros(3 * self.N)
+        self._check(lib().eolc_download_forces(self._h, f))
+        return f
+
+    def download_csr(self, which):
+        """Scalar CSR (scipy.sparse.csr_matrix) of M or MDK."""
+        import scipy.sparse as sp
+        nnz = C.c_int64(0)
+        rc = lib().eolc_download_csr(self._h, int(which), None, None, None, 0, C.byref(nnz))
+        self._check(rc, allow=(ERR_OVERFLOW,))
+        n = 3 * self.N
+        rp = np.zeros(n + 1, dtype=np.int64)
+        col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        val = np.zeros(max(nnz.value, 1))
+        self._check(lib().eolc_download_csr(self._h, int(which), _ptr(rp), _ptr(col), _ptr(val),
+                                            nnz.value, C.byref(nnz)))
+        return sp.csr_matrix((val[: nnz.value], col[: nnz.value], rp), shape=(n, n))
+
+    def velocity_solve(self, fixed_dof=(), fixed_val=(), tol=1e-8, maxit=100000,
+                       mode=SOLVE_FP64_BJ, allow=()):
+        fd, fv = _i(fixed_dof), _d(fixed_val)
+        st = SolveStats()
+        rc = lib().eolc_velocity_solve(self._h, fd.size, _ptr(fd) if fd.size else None,
+                                       _ptr(fv) if fv.size else None, float(tol), int(maxit),
+                                       int(mode), C.byref(st))
+        self._check(rc, allow=allow)
+        return rc, st
+
+    def step(self, grav, h, fixed_nodes=(), scripted=None, tol=1e-8, maxit=100000,
+             mode=SOLVE_FP64_BJ, allow=()):
+        fn = _i(fixed_nodes)
+        sc = None if scripted is None else _d(scripted)
+        st = SolveStats()
+        rc = lib().eolc_step(self._h, _d(grav), float(h), fn.size, _ptr(fn) if fn.size else None,
+                             _ptr(sc), float(tol), int(maxit), int(mode), C.byref(st))
+        self._check(rc, allow=allow)
+        return rc, st
+
+    def time_spmv(self, precision, reps=20):
+        ms = C.c_double(0)
+        self._check(lib().eolc_time_spmv(self._h, int(precision), int(reps), C.byref(ms)))
+        return ms.value
+
+    def collide(self, threshold, pts=None, nrm=None, whd=None, E=None, eol_points=False,
+                box_npts=8, box_nedges=12, cap=1 << 16):
+        """CD2 (eol_points=False) or CD (True).  E: list of 4x4 row-major frames."""
+        pts = _d(pts if pts is not None else np.zeros((0, 3)))
+        nrm = _d(nrm if nrm is not None else np.zeros((0, 3)))
+        whd = _d(whd if whd is not None else np.zeros((0, 3)))
+        Es = np.zeros((0,)) if E is None or len(E) == 0 else \
+            np.concatenate([np.asarray(e, dtype=np.float64).T.reshape(-1) for e in E])
+        Es = _d(Es)
+        P, B = pts.size // 3, whd.size // 3
+        while True:
+            buf = (CollisionRec * cap)()
+            cnt = C.c_int(0)
+            rc = lib().eolc_collide(self._h, float(threshold), P, _ptr(pts), _ptr(nrm), B, _ptr(whd),
+                                    _ptr(Es), int(bool(eol_points)), int(box_npts), int(box_nedges),
+                                    buf, cap, C.byref(cnt))
+            if rc == ERR_OVERFLOW and cnt.value > cap:
+                cap = cnt.value
+                continue
+            self._check(rc)
+            return [buf[i] for i in range(cnt.value)]

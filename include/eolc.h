@@ -1,0 +1,153 @@
+/*
+ * eolc.h -- C ABI of the MI355X (gfx950) per-step cloth hot path.
+ *
+ * Replaces, for solver "none", the per-step work EOL-Cloth does in
+ *   Forces::fill            Forces.h:26-45, Forces.cpp:912-930
+ *   Cloth::solve            Cloth.cpp:343-357 (b = -(M v + h f))
+ *   GeneralizedSolver::velocitySolve   GeneralizedSolver.h:26-30, .cpp:107-172
+ *   Cloth::step update loop Cloth.cpp:394-410 (v = solution, x += h v)
+ *   CD / CD2                Collisions.h:9-11, Collisions.cpp:11-77
+ *   btc::pointTriCollision  boxTriCollision.h:151-158
+ *   btc::boxTriCollision    boxTriCollision.h:123-147
+ * Plain pointers and sizes only; every call returns an eolc_status and never
+ * aborts.  All arrays are caller-owned host memory unless a function says
+ * otherwise; the context owns its device buffers.  One context per host
+ * thread; calls are synchronous.
+ */
+#ifndef EOLC_H
+#define EOLC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct eolc_ctx eolc_ctx;
+
+typedef enum {
+    EOLC_OK = 0,
+    EOLC_ERR_INVALID = 1,     /* bad argument, shape or call order */
+    EOLC_ERR_HIP = 2,         /* HIP runtime error */
+    EOLC_ERR_NOT_SPD = 3,     /* PCG met p^T A p <= 0 (indefinite M + h^2 K) */
+    EOLC_ERR_MAXIT = 4,       /* iteration cap reached before the tolerance */
+    EOLC_ERR_OVERFLOW = 5,    /* caller buffer too small; required size returned */
+    EOLC_ERR_NONFINITE = 6,   /* NaN/Inf met in the state or the solve */
+    EOLC_ERR_DEGENERATE = 7,  /* non-positive material-space area (CW face) */
+    EOLC_ERR_NO_DEVICE = 8
+} eolc_status;
+
+/* option flags (eolc_set_options) */
+#define EOLC_OPT_INCLUDE_BENDING_FORCE 0x1 /* add f_bend to f; the reference's non-EOL
+                                              branch drops it (Forces.cpp:885-908) */
+#define EOLC_OPT_SPD_PROJECT_BENDING   0x2 /* clamp each hinge Hessian's negative
+                                              eigenvalues to 0 before assembly */
+
+/* solve modes (eolc_velocity_solve / eolc_step) */
+#define EOLC_SOLVE_FP64_BJ       0  /* FP64 3x3 block-Jacobi PCG */
+#define EOLC_SOLVE_MIXED_BJ      1  /* FP32 matrix/vectors, FP64 dots and true residual,
+                                       reliable updates; block-Jacobi */
+#define EOLC_SOLVE_MIXED_LEARNED 2  /* as MIXED_BJ with the learned block correction;
+                                       falls back to MIXED_BJ at the iteration cap */
+#define EOLC_SOLVE_FP64_JACOBI   3  /* FP64 scalar-Jacobi PCG (Eigen's default) */
+
+/* matrices (eolc_download_csr) */
+#define EOLC_MAT_M   0
+#define EOLC_MAT_MDK 1
+
+typedef struct {
+    int iters;              /* inner PCG iterations */
+    int refinements;        /* FP64 true-residual replacements (mixed modes) */
+    int status;             /* eolc_status of the solve */
+    int used_fallback;      /* 1 when the learned mode fell back to block-Jacobi */
+    double relres;          /* recursive residual estimate at exit */
+    double true_relres;     /* ||b - A v||_2 / ||b||_2, FP64, reduced system */
+    double ms_assembly;     /* element evaluation + assembly (device time) */
+    double ms_rhs;
+    double ms_solve;
+    double ms_update;
+} eolc_solve_stats;
+
+/* Collision record: btc::Collision (boxTriCollision.h:49-110) flattened. */
+typedef struct {
+    double dist;
+    double nor1[3], nor2[3], pos1[3], pos2[3], pos1_[3];
+    int count1, count2;
+    int verts1[3], verts2[3];
+    double weights1[3], weights2[3];
+    int tri1, tri2;
+    int nedge1;             /* valid entries of edge1 */
+    int edge1[3];
+    int edge2;
+    double edgeDir[3];
+} eolc_collision;
+
+/* ---- lifetime ------------------------------------------------------------ */
+int  eolc_create(int device, eolc_ctx **ctx);
+void eolc_destroy(eolc_ctx *ctx);
+const char *eolc_last_error(const eolc_ctx *ctx);
+
+/* ---- set-up (once per topology; again after every remesh) ------------------
+ * X: material coordinates Vert::u (2N); tri: Face::v[3] node indices (3F,
+ * counter-clockwise in X); hinge: interior edges as (n[0], n[1], apex of
+ * adjf[0], apex of adjf[1]) (4H), Forces.cpp:692-697.  Builds the 3x3 BSR
+ * pattern and the element-to-block gather maps on the host. */
+int eolc_set_topology(eolc_ctx *ctx, int N, const double *X, int F, const int *tri,
+                      int H, const int *hinge);
+/* Material (Cloth.h:28-35): density, Young's modulus, Poisson ratio, bending
+ * stiffness, dampingA (unused, Forces.cpp:397), dampingB (scales h^2 K). */
+int eolc_set_material(eolc_ctx *ctx, double density, double e, double nu, double beta,
+                      double dampingA, double dampingB);
+int eolc_set_options(eolc_ctx *ctx, int flags);
+
+/* ---- state ------------------------------------------------------------------ */
+int eolc_upload_state(eolc_ctx *ctx, const double *x, const double *v);      /* 3N each */
+int eolc_download_state(eolc_ctx *ctx, double *x, double *v);
+
+/* ---- Forces::fill ------------------------------------------------------------
+ * Evaluates every element at the device-resident x and assembles f (3N),
+ * M and MDK = M + dampingB h^2 K on the device. */
+int eolc_forces_fill(eolc_ctx *ctx, const double grav[3], double h);
+int eolc_download_forces(eolc_ctx *ctx, double *f);
+/* Scalar CSR (3N x 3N, sorted columns) of M or MDK.  On EOLC_ERR_OVERFLOW,
+ * *nnz holds the required capacity. */
+int eolc_download_csr(eolc_ctx *ctx, int which, int64_t *rowptr, int *col, double *val,
+                      int64_t cap, int64_t *nnz);
+
+/* ---- GeneralizedSolver::velocitySolve for solver "none" ----------------------
+ * Solves MDK v = M v_old + h f on the DOFs not listed in fixed_dof, with
+ * v[fixed_dof[i]] = fixed_val[i] (the QP's single-DOF equality rows,
+ * Constraints.cpp:114-119).  v_old is the device-resident velocity; the
+ * solution replaces it.  Warm start from v_old. */
+int eolc_velocity_solve(eolc_ctx *ctx, int n_fixed, const int *fixed_dof,
+                        const double *fixed_val, double tol, int maxit, int mode,
+                        eolc_solve_stats *stats);
+
+/* ---- Cloth::step (Cloth.cpp:359-413, no remesh, no EOL) ----------------------
+ * forces_fill, rhs, velocity solve with fixed nodes
+ * (v_fixed = 0.99 v_old + scripted velocity), then x += h v. */
+int eolc_step(eolc_ctx *ctx, const double grav[3], double h, int n_fixed_nodes,
+              const int *fixed_nodes, const double *scripted_vel /* 3 per node or NULL */,
+              double tol, int maxit, int mode, eolc_solve_stats *stats);
+
+/* ---- CD2 / CD ------------------------------------------------------------------
+ * Proximity queries of the device-resident cloth positions against P points
+ * (pts, nrm: 3P) and B boxes (whd: 3B; E: 16B, each a column-major 4x4 frame as
+ * Eigen stores Matrix4d).  eol_points = 0 gives CD2 (Collisions.cpp:55-77),
+ * 1 gives CD (Collisions.cpp:11-53, with its box feature-index remap; box_npts
+ * and box_nedges are Box::num_points / num_edges).  Records are written in the
+ * reference's order.  On EOLC_ERR_OVERFLOW *count is the required capacity. */
+int eolc_collide(eolc_ctx *ctx, double threshold, int P, const double *pts, const double *nrm,
+                 int B, const double *whd, const double *E, int eol_points, int box_npts,
+                 int box_nedges, eolc_collision *out, int cap, int *count);
+
+/* ---- measurement -------------------------------------------------------------
+ * Average device time (HIP events on the context's stream) of one launch of the
+ * PCG SpMV kernel (precision 64 or 32) over `reps` back-to-back launches on the
+ * current matrix.  For roofline reporting; leaves the solver state undefined. */
+int eolc_time_spmv(eolc_ctx *ctx, int precision, int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,93 @@
+"""Velocity solve and step parity against the CPU oracle.
+
+Solved velocities: both sides solved to 1e-13 relative residual (or dense
+elimination), compared to 1e-10 relative.  Trajectory: 200 steps of the 20x20
+configuration, positions within 1e-9 of the oracle's (relative to max |x|)."""
+import numpy as np
+import pytest
+
+import eolc
+import oracle as O
+from conftest import GRAV, H, rel_err
+from eolc import scene
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(ctx, sc, flags=0):
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(flags)
+    ctx.upload_state(sc.x0, sc.v0)
+
+
+def fixed_dofs(sc, v):
+    d = np.concatenate([3 * sc.fixed[:, None] + np.arange(3)[None, :]]).reshape(-1).astype(np.int32)
+    return d, 0.99 * v[d]
+
+
+@pytest.mark.parametrize("mode", [eolc.SOLVE_FP64_BJ, eolc.SOLVE_FP64_JACOBI])
+def test_velocity_solve_vs_dense(ctx, mode):
+    sc = scene.Scene(20, sigma_x=1e-3, sigma_v=0.05)
+    print(f"solve vs dense mode={mode}", flush=True)
+    setup(ctx, sc)
+    ctx.forces_fill(GRAV, H)
+    d, val = fixed_dofs(sc, sc.v0)
+    rc, st = ctx.velocity_solve(d, val, tol=1e-13, maxit=50000, mode=mode)
+    _, vg = ctx.download_state()
+    _, fo, Mo, Ko = O.forces_fill(sc.x0, sc.X, sc.tri, sc.hinge, O.material(), GRAV, H)
+    rhs = Mo @ sc.v0 + H * fo
+    rcd, vd = O.dense_velocity_solve(Ko, rhs, d, val)
+    assert rcd == 0
+    assert st.true_relres < 1e-12
+    assert rel_err(vg, vd) < 1e-10
+    assert np.array_equal(vg[d], val)
+
+
+def test_mixed_solve_reaches_tolerance(ctx):
+    sc = scene.Scene(48, sigma_x=1e-3, sigma_v=0.05)
+    setup(ctx, sc)
+    ctx.forces_fill(GRAV, H)
+    d, val = fixed_dofs(sc, sc.v0)
+    rc, st = ctx.velocity_solve(d, val, tol=1e-8, maxit=50000, mode=eolc.SOLVE_MIXED_BJ)
+    _, vm = ctx.download_state()
+    print("mixed", st.as_dict(), flush=True)
+    assert st.true_relres <= 1e-8 and st.refinements >= 1
+    ctx.upload_state(sc.x0, sc.v0)
+    ctx.forces_fill(GRAV, H)
+    rc, st64 = ctx.velocity_solve(d, val, tol=1e-13, maxit=50000, mode=eolc.SOLVE_FP64_BJ)
+    _, v64 = ctx.download_state()
+    assert rel_err(vm, v64) < 1e-5
+
+
+def test_trajectory_200_steps(ctx):
+    sc = scene.Scene(20)
+    setup(ctx, sc)
+    x, v = sc.x0.copy(), sc.v0.copy()
+    mat = O.material()
+    for k in range(200):
+        rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-12, maxit=20000, mode=eolc.SOLVE_FP64_BJ)
+        rco, sto, _ = O.step(x, v, sc.X, sc.tri, sc.hinge, mat, GRAV, H, sc.fixed, tol=1e-12, maxit=20000)
+        assert rc == 0 and rco == 0
+    xg, vg = ctx.download_state()
+    print("trajectory err x", rel_err(xg, x), "v", rel_err(vg, v), flush=True)
+    assert rel_err(xg, x) < 1e-9
+    assert rel_err(vg, v) < 1e-7
+
+
+def test_solve_deterministic(ctx):
+    sc = scene.Scene(24, sigma_x=2e-3, sigma_v=0.05)
+    out = []
+    for _ in range(2):
+        setup(ctx, sc)
+        rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-9, maxit=20000, mode=eolc.SOLVE_FP64_BJ)
+        out.append(ctx.download_state())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_iteration_cap_reports_maxit(ctx):
+    sc = scene.Scene(20, sigma_x=1e-3, sigma_v=0.05)
+    setup(ctx, sc)
+    rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-14, maxit=5, mode=eolc.SOLVE_FP64_BJ,
+                      allow=(eolc.ERR_MAXIT,))
+    assert rc == eolc.ERR_MAXIT and st.iters == 5
