@@ -209,11 +209,12 @@ int eolc_create(int device, eolc_ctx **out)
         return EOLC_ERR_HIP;
     }
     if (hipMalloc((void **)&c->scal, sizeof(eolc::SolverScalars)) != hipSuccess ||
-        hipMalloc((void **)&c->partials, sizeof(double) * 4 * eolc::kMaxPartials) != hipSuccess ||
+        hipMalloc((void **)&c->partials, sizeof(double) * 6 * eolc::kMaxPartials) != hipSuccess ||
         hipMalloc((void **)&c->counter, sizeof(unsigned) * 4) != hipSuccess ||
         hipHostMalloc((void **)&c->hscal, sizeof(eolc::SolverScalars), hipHostMallocDefault) != hipSuccess ||
         hipMemset(c->counter, 0, sizeof(unsigned) * 4) != hipSuccess ||
-        hipMemset(c->scal, 0, sizeof(eolc::SolverScalars)) != hipSuccess) {
+        hipMemset(c->scal, 0, sizeof(eolc::SolverScalars)) != hipSuccess ||
+        hipMemset(c->partials, 0, sizeof(double) * 6 * eolc::kMaxPartials) != hipSuccess) {
         eolc_destroy(c);
         return EOLC_ERR_HIP;
     }
@@ -453,19 +454,18 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     EOLC_HIP(eolc::pcg_setup(c, mixed, precond));
     const double delta = mixed ? 0.1 : 0.0;
     EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
-    EOLC_HIP(eolc::pcg_init(c, mixed, tol));
-    int parity = 0, done_iters = 0, refinements = 0, status = EOLC_ERR_MAXIT;
+    EOLC_HIP(eolc::pcg_init(c, mixed));
+    int k = 0, refinements = 0, status = EOLC_ERR_MAXIT;
     int chunk = 8;
     double relres = NAN;
     for (;;) {
-        int todo = std::min(chunk, maxit - done_iters);
+        const int todo = std::min(chunk, maxit - k);
         if (todo <= 0) { status = EOLC_ERR_MAXIT; break; }
-        EOLC_HIP(eolc::pcg_chunk(c, mixed, todo, &parity));
+        EOLC_HIP(eolc::pcg_chunk(c, mixed, k, todo));
         EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
         EOLC_HIP(hipStreamSynchronize(c->stream));
         const eolc::SolverScalars s = *c->hscal;
-        done_iters = s.iters;
-        parity = s.iters & 1;
+        k = s.iters;
         relres = s.bnorm2 > 0 ? std::sqrt(s.rr / s.bnorm2) : 0.0;
         if (chunk < 256) chunk *= 2;
         if (!s.done) continue;
@@ -473,7 +473,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         if (s.reason == 4) { status = EOLC_ERR_NONFINITE; break; }
         if (!mixed) { status = EOLC_OK; break; }
         // mixed: reliable update with the true FP64 residual
-        EOLC_HIP(eolc::pcg_reliable_update(c, tol, parity));
+        EOLC_HIP(eolc::pcg_reliable_update(c));
         EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
         EOLC_HIP(hipStreamSynchronize(c->stream));
         ++refinements;
@@ -481,15 +481,16 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         relres = s2.bnorm2 > 0 ? std::sqrt(s2.rmax2 / s2.bnorm2) : 0.0;
         if (!std::isfinite(s2.rmax2)) { status = EOLC_ERR_NONFINITE; break; }
         if (s2.rmax2 <= tol * tol * s2.bnorm2) { status = EOLC_OK; break; }
-        EOLC_HIP(eolc::pcg_restart_inner(c));
+        if (k >= maxit) { status = EOLC_ERR_MAXIT; break; }
+        EOLC_HIP(eolc::pcg_restart_inner(c, k));
         EOLC_HIP(eolc::scal_set(c, 0, 0, 0));
     }
     if (mixed && status != EOLC_OK && status != EOLC_ERR_NOT_SPD) {
         // fold the last FP32 correction in before reporting
-        EOLC_HIP(eolc::pcg_reliable_update(c, tol, parity));
+        EOLC_HIP(eolc::pcg_reliable_update(c));
     }
     EOLC_HIP(eolc::pcg_finish(c, mixed));
-    EOLC_HIP(eolc::spmv_true_residual(c, nullptr));
+    EOLC_HIP(eolc::spmv_true_residual(c));
     EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
     if (n_fixed > 0) EOLC_HIP(eolc::set_fixed(c, n_fixed, d_dofs, d_vals, 0));
     EOLC_HIP(hipStreamSynchronize(c->stream));
@@ -501,7 +502,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         st->status = status;
         st->used_fallback = 0;
         st->relres = relres;
-        st->true_relres = c->hscal->bnorm2 > 0 ? std::sqrt(c->hscal->rr / c->hscal->bnorm2) : 0.0;
+        st->true_relres = c->hscal->bnorm2 > 0 ? std::sqrt(c->hscal->true_rr / c->hscal->bnorm2) : 0.0;
     }
     if (status == EOLC_ERR_NOT_SPD) c->err = "velocity_solve: p^T A p <= 0 (MDK is not positive definite)";
     else if (status == EOLC_ERR_MAXIT) c->err = "velocity_solve: iteration cap reached";

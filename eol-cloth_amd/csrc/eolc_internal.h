@@ -11,10 +11,9 @@ namespace eolc {
 
 // Rows of the 3x3 BSR matrix handled by one workgroup pass of the PCG kernels:
 // 16 lanes per block row, 256 threads per workgroup.
-constexpr int kLanesPerRow = 16;
-constexpr int kWG = 256;
-constexpr int kRowsPerWG = kWG / kLanesPerRow;
-constexpr int kMaxPartials = 2048;   // workgroups of a reduction kernel
+constexpr int kPcgWG = 1024;        // threads per workgroup of the PCG row kernels
+constexpr int kPcgGrid = 512;       // maximum workgroups of the PCG row kernels
+constexpr int kMaxPartials = 2048;  // stride of one partial-sum array
 
 // Device-side solver scalars.  Written only by the last workgroup of a
 // reduction kernel (or by a one-thread set-up kernel); read by every
@@ -29,10 +28,11 @@ struct SolverScalars {
     double tol2;      // squared relative tolerance
     double rmax2;     // squared true residual at the last reliable update
     double delta2;    // reliable-update trigger (squared ratio; 0 in FP64 mode)
-    int done;         // 1: stop iterating (converged, trigger or breakdown)
-    int reason;       // 1 converged, 2 breakdown, 3 reliable-update trigger, 4 non-finite
-    int iters;
-    int pad;
+    double true_rr;   // ||rhs - A v||^2 of the returned solution
+    int done;         // 1: stop iterating
+    int reason;       // 1 residual below the running threshold, 2 breakdown, 4 non-finite
+    int iters;        // completed iterations
+    int first_iter;   // iteration that starts with beta = 0
 };
 
 struct Topology {
@@ -59,6 +59,7 @@ template <class T> struct DArr {
 struct eolc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    int lpr = 16;               // lanes per block row in the PCG SpMV (8 or 16)
     std::string err;
     int flags = 0;
     // material
@@ -88,7 +89,7 @@ struct eolc_ctx {
     double *p64[2] = {nullptr, nullptr};
     float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *xl32 = nullptr;
     float *p32[2] = {nullptr, nullptr};
-    double *partials = nullptr;   // kMaxPartials x 4
+    double *partials = nullptr;   // 6 x kMaxPartials
     unsigned int *counter = nullptr;
     eolc::SolverScalars *scal = nullptr;
     // host pinned copy of the scalars
@@ -108,16 +109,14 @@ hipError_t launch_assemble(eolc_ctx *c, bool want32);
 hipError_t launch_rhs(eolc_ctx *c, double h);
 hipError_t launch_update_x(eolc_ctx *c, double h);
 
-struct PcgLaunch {
-    int grid;      // workgroups of the row kernels
-};
+int pcg_grid(int N);
 hipError_t pcg_setup(eolc_ctx *c, bool mixed, int precond_kind);
-hipError_t pcg_init(eolc_ctx *c, bool mixed, double tol);
-hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int iters, int *parity);
-hipError_t pcg_reliable_update(eolc_ctx *c, double tol, int parity);
-hipError_t pcg_restart_inner(eolc_ctx *c);
+hipError_t pcg_init(eolc_ctx *c, bool mixed);
+hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters);
+hipError_t pcg_reliable_update(eolc_ctx *c);
+hipError_t pcg_restart_inner(eolc_ctx *c, int k);
 hipError_t pcg_finish(eolc_ctx *c, bool mixed);
-hipError_t spmv_true_residual(eolc_ctx *c, double *out_rr);
+hipError_t spmv_true_residual(eolc_ctx *c);
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int reps, float *ms);

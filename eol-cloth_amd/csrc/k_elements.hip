@@ -134,7 +134,8 @@ k_face_eval(int F, const int *__restrict__ tri, const double *__restrict__ x, co
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                double kk = A * (2.0 * mu * dd * RtR[j][k] + lam * comp(Rd[vv], j) * comp(Rd[ww], k));
+                // (Rd_v,j * Rd_w,k) is formed first so that diagonal blocks come out exactly symmetric
+                double kk = A * (2.0 * mu * dd * RtR[j][k] + lam * (comp(Rd[vv], j) * comp(Rd[ww], k)));
                 double mm = (j == k) ? mass : 0.0;
                 sk[9 * p + 3 * j + k] = mm + s * kk;
             }

@@ -4,28 +4,35 @@
 // as the Dirichlet-reduced solve of MDK v = M v_old + h f: fixed DOFs are masked
 // out of every matrix-vector product and carry their prescribed values.
 //
-// One iteration = two kernels:
-//   k_pcg_spmv   : p_new = z + beta p_old (rows and, on the fly, neighbours),
-//                  q = A p_new (16 lanes per block row), partial p_new.q;
-//                  the last workgroup turns the partials into alpha.
-//   k_pcg_update : x += alpha p_new, r -= alpha q, z = D^-1 r, partial r.z, r.r;
-//                  the last workgroup forms beta and the stop decision.
-// Scalars live in device memory (SolverScalars); a set `done` flag turns every
-// later launch into a no-op, so the host enqueues chunks of iterations and only
-// synchronises between chunks.  Reductions are deterministic: fixed in-workgroup
-// trees and a fixed-order sum over the per-workgroup partials.
+// Iteration k = two kernels:
+//   k_pcg_spmv<k>   : rz_k, rr_k <- sum of PB[k&1]; stop test; beta = rz_k / rz_{k-1};
+//                     p_new = z + beta p_old (own rows and, on the fly, neighbours);
+//                     q = A p_new; partial p_new.q -> PA.
+//   k_pcg_update<k> : pq <- sum of PA; alpha = rz_k / pq; x += alpha p_new,
+//                     r -= alpha q, z = D^-1 r; partials r.z, r.r -> PB[(k+1)&1].
+// There is no grid-wide fence inside a kernel: every workgroup re-reduces the
+// previous kernel's per-workgroup partials itself, in one fixed order, so all
+// workgroups reach bitwise-identical scalars and identical stop decisions.  The
+// launch boundary is the only synchronisation.  A `done` flag (set by workgroup 0
+// of the kernel that decides to stop) turns later launches into no-ops, so the host
+// enqueues chunks of iterations and synchronises only between chunks.
 //
 // Workgroup -> rows: launch w is mapped to logical chunk (w % 8) * (G / 8) + w / 8,
-// so the 8 XCDs (which receive workgroups round-robin) each stream one
-// contiguous eighth of the rows and keep their neighbour gathers in their own L2.
+// so each of the 8 XCDs (which receive workgroups round-robin) streams one
+// contiguous eighth of the rows and keeps its neighbour gathers in its own L2.
 //
 // Matrix values are stored SoA: A[c * nbp + k] is entry c (row-major 3x3) of
-// block k, so the 16 lanes of a row read consecutive blocks fully coalesced.
+// block k; the LPR lanes of a row read consecutive blocks, fully coalesced.
+// Products and row sums are accumulated in FP64 also when A, p, q are FP32.
+#include <type_traits>
+
 #include "eolc_internal.h"
 
 using eolc::SolverScalars;
 
 namespace {
+
+constexpr int WG = eolc::kPcgWG;
 
 __device__ __forceinline__ int logical_wg()
 {
@@ -34,257 +41,265 @@ __device__ __forceinline__ int logical_wg()
     return (w & 7) * (G >> 3) + (w >> 3);
 }
 
-// rows [begin, end) owned by this workgroup (G is a multiple of 8)
 __device__ __forceinline__ void wg_rows(int N, int *begin, int *end)
 {
     int G = gridDim.x;
     int per = (N + G - 1) / G;
     int lw = logical_wg();
-    int b = lw * per;
-    int e = b + per;
+    int b = lw * per, e = b + per;
     *begin = b < N ? b : N;
     *end = e < N ? e : N;
 }
 
-template <class T> __device__ __forceinline__ T shfl_xor16(T v, int m) { return __shfl_xor(v, m, 16); }
-
 __device__ __forceinline__ double wave_sum(double v)
 {
+#pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
 
-// Block sum of up to 4 per-thread values; result valid in thread 0.
+// Sum over the workgroup of K values per thread; result broadcast to all threads.
 template <int K>
-__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* 4*K */)
+__device__ __forceinline__ void block_allsum(double (&v)[K], double *lds /* K * WG/64 + K */)
 {
-    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    constexpr int NW = WG / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double s = wave_sum(v[k]);
-        if (lane == 0) lds[wid * K + k] = s;
+        if (lane == 0) lds[k * NW + wid] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wid == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = ((lds[k] + lds[K + k]) + lds[2 * K + k]) + lds[3 * K + k];
-    }
-}
-
-// Publishes this workgroup's partials and returns true in the last workgroup
-// to arrive; that workgroup then sees every partial.
-template <int K>
-__device__ bool publish_partials(const double (&v)[K], double *partials, unsigned *counter, int *lds_flag)
-{
-    if (threadIdx.x == 0) {
-        int lw = logical_wg();
+        for (int k = 0; k < K; ++k) {
+            double s = lane < NW ? lds[k * NW + lane] : 0.0;
 #pragma unroll
-        for (int k = 0; k < K; ++k) partials[k * eolc::kMaxPartials + lw] = v[k];
-        __threadfence();
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *lds_flag = (prev == gridDim.x - 1);
+            for (int o = NW / 2; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+            if (lane == 0) lds[K * NW + k] = s;
+        }
     }
     __syncthreads();
-    bool last = *lds_flag != 0;
-    if (last) __threadfence();
-    return last;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = lds[K * NW + k];
 }
 
-// In the last workgroup: fixed-order sum of partials[k][0..G) into out[k] (thread 0).
+// Fixed-order sum of K partial arrays (stride kMaxPartials) of length G, done by
+// every workgroup; the result is identical everywhere.
 template <int K>
-__device__ void reduce_partials(const double *partials, double (&out)[K], double *lds)
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ part, double (&out)[K], double *lds)
 {
-    int G = gridDim.x;
-    double acc[K];
+    const int G = gridDim.x;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double s = 0.0;
-        for (int i = threadIdx.x; i < G; i += blockDim.x) s += partials[k * eolc::kMaxPartials + i];
-        acc[k] = s;
+        for (int i = threadIdx.x; i < G; i += WG) s += part[k * eolc::kMaxPartials + i];
+        out[k] = s;
     }
-    block_sum<K>(acc, lds);
+    block_allsum<K>(out, lds);
+}
+
+template <int K>
+__device__ __forceinline__ void write_partials(double (&v)[K], double *__restrict__ part, double *lds)
+{
+    block_allsum<K>(v, lds);
+    if (threadIdx.x == 0) {
+        const int lw = logical_wg();
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[k] = acc[k];
+        for (int k = 0; k < K; ++k) part[k * eolc::kMaxPartials + lw] = v[k];
+    }
 }
 
 template <class T>
-__device__ __forceinline__ void load3(const T *__restrict__ a, int i, T &x, T &y, T &z)
+__device__ __forceinline__ void load3(const T *__restrict__ a, int i, double &x, double &y, double &z)
 {
-    x = a[3 * i]; y = a[3 * i + 1]; z = a[3 * i + 2];
+    x = (double)a[3 * i]; y = (double)a[3 * i + 1]; z = (double)a[3 * i + 2];
 }
+
+// sum over the LPR lanes of a row group
+template <int LPR>
+__device__ __forceinline__ double row_sum(double v)
+{
+#pragma unroll
+    for (int m = LPR / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, LPR);
+    return v;
+}
+
+// Partials layout (doubles, stride kMaxPartials): PA at slot 0, PB[parity] (r.z, r.r)
+// at slots 1 + 2 parity and 2 + 2 parity, residual-norm partials at slot 5.
+__device__ __forceinline__ double *PA(double *p) { return p; }
+__device__ __forceinline__ double *PB(double *p, int parity) { return p + (1 + 2 * parity) * eolc::kMaxPartials; }
 
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(256)
-k_pcg_spmv(int N, const int *__restrict__ rowptr, const int *__restrict__ col,
-           const T *__restrict__ A, int64_t nbp, const unsigned char *__restrict__ fixmask,
-           const T *__restrict__ z, const T *__restrict__ pold, T *__restrict__ pnew,
-           T *__restrict__ q, SolverScalars *scal, double *partials, unsigned *counter)
+template <class T, int LPR>
+__global__ void __launch_bounds__(WG)
+k_pcg_spmv(int k, int N, const int *__restrict__ rowptr, const int *__restrict__ col, const T *__restrict__ A,
+           int64_t nbp, const unsigned char *__restrict__ fixmask, const T *__restrict__ z,
+           const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, SolverScalars *scal,
+           double *partials)
 {
-    __shared__ double lds[8];
-    __shared__ int flag;
+    __shared__ double lds[2 * (WG / 64) + 2];
     if (scal->done) return;
-    const T beta = (T)scal->beta;
-    const int lane = threadIdx.x & 15;
-    const int sub = threadIdx.x >> 4;
+    double cur[2];
+    reduce_partials<2>(PB(partials, k & 1), cur, lds);
+    const double rzk = cur[0], rrk = cur[1];
+    const bool restart_beta0 = (k == scal->first_iter);
+    double prev[1] = {1.0};
+    if (!restart_beta0) reduce_partials<1>(PB(partials, (k + 1) & 1), prev, lds);
+    const double thr = fmax(scal->tol2 * scal->bnorm2, scal->delta2 * scal->rmax2);
+    if (!isfinite(rrk) || !isfinite(rzk) || rrk <= thr) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal->done = 1;
+            scal->reason = (isfinite(rrk) && isfinite(rzk)) ? 1 : 4;
+            scal->rr = rrk;
+            scal->rz = rzk;
+        }
+        return;
+    }
+    const double beta = restart_beta0 ? 0.0 : rzk / prev[0];
+    const int lane = threadIdx.x % LPR;
+    const int sub = threadIdx.x / LPR;
+    constexpr int RPP = WG / LPR;   // rows per pass
     int rb, re;
     wg_rows(N, &rb, &re);
     double dot = 0.0;
-    for (int base = rb; base < re; base += eolc::kRowsPerWG) {
-        int i = base + sub;
-        T y0 = 0, y1 = 0, y2 = 0;
+    for (int base = rb; base < re; base += RPP) {
+        const int i = base + sub;
+        double y0 = 0, y1 = 0, y2 = 0;
         if (i < re) {
-            int kb = rowptr[i], ke = rowptr[i + 1];
-            for (int k = kb + lane; k < ke; k += 16) {
-                int j = col[k];
-                T zj0, zj1, zj2, pj0, pj1, pj2;
+            const int kb = rowptr[i], ke = rowptr[i + 1];
+            for (int kk = kb + lane; kk < ke; kk += LPR) {
+                const int j = col[kk];
+                double zj0, zj1, zj2, pj0, pj1, pj2;
                 load3(z, j, zj0, zj1, zj2);
                 load3(pold, j, pj0, pj1, pj2);
-                T u0 = fma(beta, pj0, zj0), u1 = fma(beta, pj1, zj1), u2 = fma(beta, pj2, zj2);
-                const T *a = A + k;
-                y0 += a[0 * nbp] * u0 + a[1 * nbp] * u1 + a[2 * nbp] * u2;
-                y1 += a[3 * nbp] * u0 + a[4 * nbp] * u1 + a[5 * nbp] * u2;
-                y2 += a[6 * nbp] * u0 + a[7 * nbp] * u1 + a[8 * nbp] * u2;
+                // p_new_j formed exactly as row j forms (and stores) it below
+                const double u0 = (double)(T)fma(beta, pj0, zj0);
+                const double u1 = (double)(T)fma(beta, pj1, zj1);
+                const double u2 = (double)(T)fma(beta, pj2, zj2);
+                const T *a = A + kk;
+                y0 += (double)a[0 * nbp] * u0 + (double)a[1 * nbp] * u1 + (double)a[2 * nbp] * u2;
+                y1 += (double)a[3 * nbp] * u0 + (double)a[4 * nbp] * u1 + (double)a[5 * nbp] * u2;
+                y2 += (double)a[6 * nbp] * u0 + (double)a[7 * nbp] * u1 + (double)a[8 * nbp] * u2;
             }
         }
-#pragma unroll
-        for (int m = 8; m > 0; m >>= 1) {
-            y0 += shfl_xor16(y0, m);
-            y1 += shfl_xor16(y1, m);
-            y2 += shfl_xor16(y2, m);
-        }
+        y0 = row_sum<LPR>(y0);
+        y1 = row_sum<LPR>(y1);
+        y2 = row_sum<LPR>(y2);
         if (i < re && lane == 0) {
-            unsigned char fm = fixmask[i];
+            const unsigned char fm = fixmask[i];
             if (fm & 1) y0 = 0;
             if (fm & 2) y1 = 0;
             if (fm & 4) y2 = 0;
-            T zi0, zi1, zi2, pi0, pi1, pi2;
+            double zi0, zi1, zi2, pi0, pi1, pi2;
             load3(z, i, zi0, zi1, zi2);
             load3(pold, i, pi0, pi1, pi2);
-            T u0 = fma(beta, pi0, zi0), u1 = fma(beta, pi1, zi1), u2 = fma(beta, pi2, zi2);
+            const T u0 = (T)fma(beta, pi0, zi0), u1 = (T)fma(beta, pi1, zi1), u2 = (T)fma(beta, pi2, zi2);
             pnew[3 * i] = u0; pnew[3 * i + 1] = u1; pnew[3 * i + 2] = u2;
-            q[3 * i] = y0; q[3 * i + 1] = y1; q[3 * i + 2] = y2;
-            dot += ((double)u0 * (double)y0 + (double)u1 * (double)y1) + (double)u2 * (double)y2;
+            const T q0 = (T)y0, q1 = (T)y1, q2 = (T)y2;
+            q[3 * i] = q0; q[3 * i + 1] = q1; q[3 * i + 2] = q2;
+            dot += ((double)u0 * (double)q0 + (double)u1 * (double)q1) + (double)u2 * (double)q2;
         }
     }
-    double v[1] = { dot };
-    block_sum<1>(v, lds);
-    if (publish_partials<1>(v, partials, counter, &flag)) {
-        double tot[1];
-        reduce_partials<1>(partials, tot, lds);
-        if (threadIdx.x == 0) {
-            double pq = tot[0];
-            scal->pq = pq;
-            if (!(pq > 0.0)) {
-                scal->done = 1;
-                scal->reason = isfinite(pq) ? 2 : 4;
-            } else {
-                scal->alpha = scal->rz / pq;
-            }
-            *counter = 0u;
-        }
+    double v[1] = {dot};
+    write_partials<1>(v, PA(partials), lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal->rr = rrk;
+        scal->rz = rzk;
     }
 }
 
 // ---------------------------------------------------------------------------
-// X is the solution accumulator (double in FP64 mode, float in mixed mode).
+// TX is the solution accumulator (double in FP64 mode, float in mixed mode).
 template <class T, class TX>
-__global__ void __launch_bounds__(256)
-k_pcg_update(int N, const T *__restrict__ D, const T *__restrict__ pnew, const T *__restrict__ q,
-             TX *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
-             SolverScalars *scal, double *partials, unsigned *counter)
+__global__ void __launch_bounds__(WG)
+k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, const T *__restrict__ q,
+             TX *__restrict__ x, T *__restrict__ r, T *__restrict__ z, SolverScalars *scal, double *partials)
 {
-    __shared__ double lds[8];
-    __shared__ int flag;
+    __shared__ double lds[2 * (WG / 64) + 2];
     if (scal->done) return;
-    const T alpha = (T)scal->alpha;
+    double pq[1], rz[1];
+    reduce_partials<1>(PA(partials), pq, lds);
+    reduce_partials<1>(PB(partials, k & 1), rz, lds);
+    if (!(pq[0] > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal->done = 1;
+            scal->reason = isfinite(pq[0]) ? 2 : 4;
+            scal->pq = pq[0];
+        }
+        return;
+    }
+    const double alpha = rz[0] / pq[0];
     int rb, re;
     wg_rows(N, &rb, &re);
-    double rz = 0.0, rr = 0.0;
-    for (int i = rb + threadIdx.x; i < re; i += blockDim.x) {
-        T p0, p1, p2, q0, q1, q2, r0, r1, r2;
+    double srz = 0.0, srr = 0.0;
+    for (int i = rb + threadIdx.x; i < re; i += WG) {
+        double p0, p1, p2, q0, q1, q2, r0, r1, r2;
         load3(pnew, i, p0, p1, p2);
         load3(q, i, q0, q1, q2);
         load3(r, i, r0, r1, r2);
         x[3 * i] += (TX)(alpha * p0);
         x[3 * i + 1] += (TX)(alpha * p1);
         x[3 * i + 2] += (TX)(alpha * p2);
-        r0 = fma(-alpha, q0, r0);
-        r1 = fma(-alpha, q1, r1);
-        r2 = fma(-alpha, q2, r2);
-        r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
+        const T rn0 = (T)fma(-alpha, q0, r0), rn1 = (T)fma(-alpha, q1, r1), rn2 = (T)fma(-alpha, q2, r2);
+        r[3 * i] = rn0; r[3 * i + 1] = rn1; r[3 * i + 2] = rn2;
         const T *d = D + i;
-        T z0 = d[0 * N] * r0 + d[1 * N] * r1 + d[2 * N] * r2;
-        T z1 = d[3 * N] * r0 + d[4 * N] * r1 + d[5 * N] * r2;
-        T z2 = d[6 * N] * r0 + d[7 * N] * r1 + d[8 * N] * r2;
+        const double s0 = rn0, s1 = rn1, s2 = rn2;
+        const T z0 = (T)((double)d[0 * N] * s0 + (double)d[1 * N] * s1 + (double)d[2 * N] * s2);
+        const T z1 = (T)((double)d[3 * N] * s0 + (double)d[4 * N] * s1 + (double)d[5 * N] * s2);
+        const T z2 = (T)((double)d[6 * N] * s0 + (double)d[7 * N] * s1 + (double)d[8 * N] * s2);
         z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
-        rz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
-        rr += ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2;
+        srz += (s0 * (double)z0 + s1 * (double)z1) + s2 * (double)z2;
+        srr += (s0 * s0 + s1 * s1) + s2 * s2;
     }
-    double v[2] = { rz, rr };
-    block_sum<2>(v, lds);
-    if (publish_partials<2>(v, partials, counter, &flag)) {
-        double tot[2];
-        reduce_partials<2>(partials, tot, lds);
-        if (threadIdx.x == 0) {
-            double rzn = tot[0], rrn = tot[1];
-            scal->beta = rzn / scal->rz;
-            scal->rz_prev = scal->rz;
-            scal->rz = rzn;
-            scal->rr = rrn;
-            scal->iters += 1;
-            double thr = fmax(scal->tol2 * scal->bnorm2, scal->delta2 * scal->rmax2);
-            if (!isfinite(rrn) || !isfinite(rzn)) {
-                scal->done = 1; scal->reason = 4;
-            } else if (rrn <= thr) {
-                scal->done = 1; scal->reason = 1;
-            }
-            *counter = 0u;
-        }
+    double v[2] = {srz, srr};
+    write_partials<2>(v, PB(partials, (k + 1) & 1), lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal->iters = k + 1;
+        scal->pq = pq[0];
+        scal->alpha = alpha;
     }
 }
 
 // ---------------------------------------------------------------------------
-// out = b - A x (FP64), masked at fixed DOFs; b may be null (0).  Last
-// workgroup stores ||out||^2 into *rr_out.
-__global__ void __launch_bounds__(256)
-k_resid64(int N, const int *__restrict__ rowptr, const int *__restrict__ col,
-          const double *__restrict__ A, int64_t nbp, const unsigned char *__restrict__ fixmask,
-          const double *__restrict__ xin, const double *__restrict__ b, double *__restrict__ out,
-          double *rr_out, double *partials, unsigned *counter)
+// out = b - A x (FP64), masked at fixed DOFs; b may be null (0).  Partial
+// ||out||^2 -> partials (one array of G).
+template <int LPR>
+__global__ void __launch_bounds__(WG)
+k_resid64(int N, const int *__restrict__ rowptr, const int *__restrict__ col, const double *__restrict__ A,
+          int64_t nbp, const unsigned char *__restrict__ fixmask, const double *__restrict__ xin,
+          const double *__restrict__ b, double *__restrict__ out, double *partials)
 {
-    __shared__ double lds[8];
-    __shared__ int flag;
-    const int lane = threadIdx.x & 15;
-    const int sub = threadIdx.x >> 4;
+    __shared__ double lds[2 * (WG / 64) + 2];
+    const int lane = threadIdx.x % LPR;
+    const int sub = threadIdx.x / LPR;
+    constexpr int RPP = WG / LPR;
     int rb, re;
     wg_rows(N, &rb, &re);
     double acc = 0.0;
-    for (int base = rb; base < re; base += eolc::kRowsPerWG) {
-        int i = base + sub;
+    for (int base = rb; base < re; base += RPP) {
+        const int i = base + sub;
         double y0 = 0, y1 = 0, y2 = 0;
         if (i < re) {
-            int kb = rowptr[i], ke = rowptr[i + 1];
-            for (int k = kb + lane; k < ke; k += 16) {
-                int j = col[k];
-                double u0 = xin[3 * j], u1 = xin[3 * j + 1], u2 = xin[3 * j + 2];
-                const double *a = A + k;
+            const int kb = rowptr[i], ke = rowptr[i + 1];
+            for (int kk = kb + lane; kk < ke; kk += LPR) {
+                const int j = col[kk];
+                const double u0 = xin[3 * j], u1 = xin[3 * j + 1], u2 = xin[3 * j + 2];
+                const double *a = A + kk;
                 y0 += a[0 * nbp] * u0 + a[1 * nbp] * u1 + a[2 * nbp] * u2;
                 y1 += a[3 * nbp] * u0 + a[4 * nbp] * u1 + a[5 * nbp] * u2;
                 y2 += a[6 * nbp] * u0 + a[7 * nbp] * u1 + a[8 * nbp] * u2;
             }
         }
-#pragma unroll
-        for (int m = 8; m > 0; m >>= 1) {
-            y0 += shfl_xor16(y0, m);
-            y1 += shfl_xor16(y1, m);
-            y2 += shfl_xor16(y2, m);
-        }
+        y0 = row_sum<LPR>(y0);
+        y1 = row_sum<LPR>(y1);
+        y2 = row_sum<LPR>(y2);
         if (i < re && lane == 0) {
             double o0 = (b ? b[3 * i] : 0.0) - y0;
             double o1 = (b ? b[3 * i + 1] : 0.0) - y1;
             double o2 = (b ? b[3 * i + 2] : 0.0) - y2;
-            unsigned char fm = fixmask[i];
+            const unsigned char fm = fixmask[i];
             if (fm & 1) o0 = 0;
             if (fm & 2) o1 = 0;
             if (fm & 4) o2 = 0;
@@ -292,58 +307,44 @@ k_resid64(int N, const int *__restrict__ rowptr, const int *__restrict__ col,
             acc += (o0 * o0 + o1 * o1) + o2 * o2;
         }
     }
-    double v[1] = { acc };
-    block_sum<1>(v, lds);
-    if (publish_partials<1>(v, partials, counter, &flag)) {
-        double tot[1];
-        reduce_partials<1>(partials, tot, lds);
-        if (threadIdx.x == 0) {
-            *rr_out = tot[0];
-            *counter = 0u;
-        }
-    }
+    double v[1] = {acc};
+    write_partials<1>(v, partials, lds);
 }
 
-// z = D r (r converted from FP64 to T when T is float), partial r.z and r.r;
-// last workgroup sets rz (and beta = rz_new / rz_old when keep_p).
-template <class T>
-__global__ void __launch_bounds__(256)
-k_precond_init(int N, const T *__restrict__ D, const double *r64, T *r,
-               T *__restrict__ z, int keep_p, SolverScalars *scal, double *partials, unsigned *counter)
+// One workgroup: *dst = fixed-order sum of partials[0..G).
+__global__ void __launch_bounds__(WG) k_reduce_to(int G, const double *__restrict__ partials, double *dst)
 {
-    __shared__ double lds[8];
-    __shared__ int flag;
+    __shared__ double lds[2 * (WG / 64) + 2];
+    double s[1] = {0.0};
+    for (int i = threadIdx.x; i < G; i += WG) s[0] += partials[i];
+    block_allsum<1>(s, lds);
+    if (threadIdx.x == 0) *dst = s[0];
+}
+
+// z = D r (r converted from FP64 to T), partials r.z, r.r -> PB[parity].
+template <class T>
+__global__ void __launch_bounds__(WG)
+k_precond_init(int parity, int N, const T *__restrict__ D, const double *r64, T *r, T *__restrict__ z,
+               double *partials)
+{
+    __shared__ double lds[2 * (WG / 64) + 2];
     int rb, re;
     wg_rows(N, &rb, &re);
     double rz = 0.0, rr = 0.0;
-    for (int i = rb + threadIdx.x; i < re; i += blockDim.x) {
-        T r0 = (T)r64[3 * i], r1 = (T)r64[3 * i + 1], r2 = (T)r64[3 * i + 2];
+    for (int i = rb + threadIdx.x; i < re; i += WG) {
+        const T r0 = (T)r64[3 * i], r1 = (T)r64[3 * i + 1], r2 = (T)r64[3 * i + 2];
         r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
         const T *d = D + i;
-        T z0 = d[0 * N] * r0 + d[1 * N] * r1 + d[2 * N] * r2;
-        T z1 = d[3 * N] * r0 + d[4 * N] * r1 + d[5 * N] * r2;
-        T z2 = d[6 * N] * r0 + d[7 * N] * r1 + d[8 * N] * r2;
+        const double s0 = r0, s1 = r1, s2 = r2;
+        const T z0 = (T)((double)d[0 * N] * s0 + (double)d[1 * N] * s1 + (double)d[2 * N] * s2);
+        const T z1 = (T)((double)d[3 * N] * s0 + (double)d[4 * N] * s1 + (double)d[5 * N] * s2);
+        const T z2 = (T)((double)d[6 * N] * s0 + (double)d[7 * N] * s1 + (double)d[8 * N] * s2);
         z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
-        rz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
-        rr += ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2;
+        rz += (s0 * (double)z0 + s1 * (double)z1) + s2 * (double)z2;
+        rr += (s0 * s0 + s1 * s1) + s2 * s2;
     }
-    double v[2] = { rz, rr };
-    block_sum<2>(v, lds);
-    if (publish_partials<2>(v, partials, counter, &flag)) {
-        double tot[2];
-        reduce_partials<2>(partials, tot, lds);
-        if (threadIdx.x == 0) {
-            if (keep_p) {
-                scal->beta = tot[0] / scal->rz;
-            } else {
-                scal->beta = 0.0;
-            }
-            scal->rz_prev = scal->rz;
-            scal->rz = tot[0];
-            scal->rr = tot[1];
-            *counter = 0u;
-        }
-    }
+    double v[2] = {rz, rr};
+    write_partials<2>(v, PB(partials, parity), lds);
 }
 
 // Block-Jacobi inverse of the diagonal blocks, with fixed DOFs replaced by
@@ -378,11 +379,10 @@ k_block_jacobi(int N, const int *__restrict__ diagpos, const double *__restrict_
 #pragma unroll
         for (int c = 0; c < 9; ++c) o[c] = (c % 4 == 0) ? 1.0 : 0.0;
     }
-    // fixed DOFs: z must stay 0 there
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
         int r = c / 3, s = c % 3;
-        if (((fm >> r) & 1) || ((fm >> s) & 1)) o[c] = 0.0;
+        if (((fm >> r) & 1) || ((fm >> s) & 1)) o[c] = 0.0;   // z stays 0 on fixed DOFs
         D64[c * N + i] = o[c];
         D32[c * N + i] = (float)o[c];
     }
@@ -396,8 +396,8 @@ __global__ void k_set_fixed(int n, const int *__restrict__ dofs, const double *_
     int d = dofs[t];
     int vtx = d / 3, c = d % 3;
     if (set) {
-        __hip_atomic_fetch_or(reinterpret_cast<unsigned int *>(fixmask) + (vtx >> 2),
-                              1u << ((vtx & 3) * 8 + c), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_or(reinterpret_cast<unsigned int *>(fixmask) + (vtx >> 2), 1u << ((vtx & 3) * 8 + c),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         vfix[d] = vals[t];
     } else {
         fixmask[vtx] = 0;
@@ -405,7 +405,6 @@ __global__ void k_set_fixed(int n, const int *__restrict__ dofs, const double *_
     }
 }
 
-// x0 (free DOFs) = warm start v_old, fixed DOFs 0
 __global__ void k_masked_copy(int N, const double *__restrict__ vin, const unsigned char *__restrict__ fixmask,
                               double *__restrict__ xout)
 {
@@ -415,7 +414,6 @@ __global__ void k_masked_copy(int N, const double *__restrict__ vin, const unsig
     xout[t] = ((fixmask[vtx] >> c) & 1) ? 0.0 : vin[t];
 }
 
-// xs64 += xl32; xl32 = 0 (mixed mode, before a reliable update)
 __global__ void k_accumulate(int n, double *__restrict__ xs, float *__restrict__ xl)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -424,7 +422,6 @@ __global__ void k_accumulate(int n, double *__restrict__ xs, float *__restrict__
     xl[t] = 0.0f;
 }
 
-// v = xs on free DOFs, vfix on fixed DOFs
 __global__ void k_write_solution(int N, const double *__restrict__ xs, const unsigned char *__restrict__ fixmask,
                                  const double *__restrict__ vfix, double *__restrict__ v)
 {
@@ -434,129 +431,148 @@ __global__ void k_write_solution(int N, const double *__restrict__ xs, const uns
     v[t] = ((fixmask[vtx] >> c) & 1) ? vfix[t] : xs[t];
 }
 
-// reset: set the tolerances and clear the counters; otherwise only clear `done`.
+// reset: set the tolerances and clear the counters (iteration 0 starts with
+// beta = 0); otherwise only clear `done` so the iteration can go on.
 __global__ void k_scal_set(SolverScalars *s, double tol2, double delta2, int reset)
 {
     if (reset) {
         s->tol2 = tol2;
         s->delta2 = delta2;
         s->iters = 0;
+        s->first_iter = 0;
     }
     s->done = 0;
     s->reason = 0;
 }
 
-int grid_rows(int N)
+inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+
+__global__ void k_fill(int n, double v, double *__restrict__ out)
 {
-    int g = (N + 4 * eolc::kRowsPerWG - 1) / (4 * eolc::kRowsPerWG);  // >= 4 passes per workgroup
-    if (g > eolc::kMaxPartials) g = eolc::kMaxPartials;
-    g = (g + 7) & ~7;
-    if (g < 8) g = 8;
-    if (g > eolc::kMaxPartials) g = eolc::kMaxPartials;
-    return g;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = v;
 }
 
-inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+template <class T>
+void launch_iteration(eolc_ctx *c, int k, int G, const T *A, const T *D, T *z, T *const p[2], T *q, T *r, T *x)
+{
+    const int N = c->topo.N;
+    const int64_t nbp = c->topo.nbp;
+    const int pin = k & 1, pout = pin ^ 1;
+    if (c->lpr == 8)
+        hipLaunchKernelGGL((k_pcg_spmv<T, 8>), dim3(G), dim3(WG), 0, c->stream, k, N, c->rowptr, c->col, A, nbp,
+                           c->fixmask, z, p[pin], p[pout], q, c->scal, c->partials);
+    else
+        hipLaunchKernelGGL((k_pcg_spmv<T, 16>), dim3(G), dim3(WG), 0, c->stream, k, N, c->rowptr, c->col, A, nbp,
+                           c->fixmask, z, p[pin], p[pout], q, c->scal, c->partials);
+    hipLaunchKernelGGL((k_pcg_update<T, T>), dim3(G), dim3(WG), 0, c->stream, k, N, D, p[pout], q, x, r, z, c->scal,
+                       c->partials);
+}
+
+void launch_resid64(eolc_ctx *c, int G, const double *xin, const double *b, double *out)
+{
+    const int N = c->topo.N;
+    double *pr = c->partials + 5 * eolc::kMaxPartials;
+    if (c->lpr == 8)
+        hipLaunchKernelGGL((k_resid64<8>), dim3(G), dim3(WG), 0, c->stream, N, c->rowptr, c->col, c->A64, c->topo.nbp,
+                           c->fixmask, xin, b, out, pr);
+    else
+        hipLaunchKernelGGL((k_resid64<16>), dim3(G), dim3(WG), 0, c->stream, N, c->rowptr, c->col, c->A64, c->topo.nbp,
+                           c->fixmask, xin, b, out, pr);
+}
 
 }  // namespace
 
 namespace eolc {
 
+int pcg_grid(int N)
+{
+    int g = (N + 2 * (WG / 16) - 1) / (2 * (WG / 16));   // at least two row passes per workgroup
+    if (g > kPcgGrid) g = kPcgGrid;
+    g = (g + 7) & ~7;
+    if (g < 8) g = 8;
+    return g;
+}
+
 hipError_t pcg_setup(eolc_ctx *c, bool /*mixed*/, int precond_kind)
 {
     int N = c->topo.N;
-    hipLaunchKernelGGL(k_block_jacobi, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, c->diagpos,
-                       c->A64, c->topo.nbp, c->fixmask, precond_kind, c->D64, c->D32);
+    hipLaunchKernelGGL(k_block_jacobi, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, c->diagpos, c->A64,
+                       c->topo.nbp, c->fixmask, precond_kind, c->D64, c->D32);
     return hipGetLastError();
 }
 
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_set_fixed, dim3(blocks_for(n)), dim3(256), 0, c->stream, n, d_dofs, d_vals,
-                       c->fixmask, c->vfix, set);
+    hipLaunchKernelGGL(k_set_fixed, dim3(blocks_for(n)), dim3(256), 0, c->stream, n, d_dofs, d_vals, c->fixmask,
+                       c->vfix, set);
     return hipGetLastError();
 }
 
-// b_red = rhs - A vfix; x0 = masked v_old; r = b_red - A x0; z, rz, beta = 0.
-hipError_t pcg_init(eolc_ctx *c, bool mixed, double tol)
+// b_red = rhs - A vfix (-> bnorm2); x0 = masked v_old; r = b_red - A x0 (-> rmax2);
+// z, r.z, r.r -> PB[0].
+hipError_t pcg_init(eolc_ctx *c, bool mixed)
 {
-    int N = c->topo.N;
-    int G = grid_rows(N);
-    const int64_t nbp = c->topo.nbp;
-    double *rr_slot = &c->scal->bnorm2;
-    hipLaunchKernelGGL(k_resid64, dim3(G), dim3(256), 0, c->stream, N, c->rowptr, c->col, c->A64, nbp,
-                       c->fixmask, c->vfix, c->rhs, c->b64, rr_slot, c->partials, c->counter);
-    hipLaunchKernelGGL(k_masked_copy, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, N, c->v, c->fixmask,
-                       c->xs64);
-    hipLaunchKernelGGL(k_resid64, dim3(G), dim3(256), 0, c->stream, N, c->rowptr, c->col, c->A64, nbp,
-                       c->fixmask, c->xs64, c->b64, c->r64, &c->scal->rmax2, c->partials, c->counter);
+    const int N = c->topo.N;
+    const int G = pcg_grid(N);
+    const double *pr = c->partials + 5 * kMaxPartials;
+    launch_resid64(c, G, c->vfix, c->rhs, c->b64);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, pr, &c->scal->bnorm2);
+    hipLaunchKernelGGL(k_masked_copy, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, N, c->v, c->fixmask, c->xs64);
+    launch_resid64(c, G, c->xs64, c->b64, c->r64);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, pr, &c->scal->rmax2);
     hipError_t e = hipMemsetAsync(mixed ? (void *)c->p32[0] : (void *)c->p64[0], 0,
                                   (size_t)3 * N * (mixed ? sizeof(float) : sizeof(double)), c->stream);
     if (e != hipSuccess) return e;
     if (mixed) {
         e = hipMemsetAsync(c->xl32, 0, (size_t)3 * N * sizeof(float), c->stream);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_precond_init<float>, dim3(G), dim3(256), 0, c->stream, N, c->D32, c->r64, c->r32,
-                           c->z32, 0, c->scal, c->partials, c->counter);
+        hipLaunchKernelGGL(k_precond_init<float>, dim3(G), dim3(WG), 0, c->stream, 0, N, c->D32, c->r64, c->r32,
+                           c->z32, c->partials);
     } else {
-        hipLaunchKernelGGL(k_precond_init<double>, dim3(G), dim3(256), 0, c->stream, N, c->D64, c->r64, c->r64,
-                           c->z64, 0, c->scal, c->partials, c->counter);
-    }
-    (void)tol;
-    return hipGetLastError();
-}
-
-hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int iters, int *parity)
-{
-    int N = c->topo.N;
-    int G = grid_rows(N);
-    const int64_t nbp = c->topo.nbp;
-    for (int it = 0; it < iters; ++it) {
-        int pin = *parity & 1, pout = pin ^ 1;
-        if (mixed) {
-            hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(G), dim3(256), 0, c->stream, N, c->rowptr, c->col, c->A32, nbp,
-                               c->fixmask, c->z32, c->p32[pin], c->p32[pout], c->q32, c->scal, c->partials,
-                               c->counter);
-            hipLaunchKernelGGL((k_pcg_update<float, float>), dim3(G), dim3(256), 0, c->stream, N, c->D32, c->p32[pout],
-                               c->q32, c->xl32, c->r32, c->z32, c->scal, c->partials, c->counter);
-        } else {
-            hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(G), dim3(256), 0, c->stream, N, c->rowptr, c->col, c->A64, nbp,
-                               c->fixmask, c->z64, c->p64[pin], c->p64[pout], c->q64, c->scal, c->partials,
-                               c->counter);
-            hipLaunchKernelGGL((k_pcg_update<double, double>), dim3(G), dim3(256), 0, c->stream, N, c->D64,
-                               c->p64[pout], c->q64, c->xs64, c->r64, c->z64, c->scal, c->partials, c->counter);
-        }
-        *parity = pout;
+        hipLaunchKernelGGL(k_precond_init<double>, dim3(G), dim3(WG), 0, c->stream, 0, N, c->D64, c->r64, c->r64,
+                           c->z64, c->partials);
     }
     return hipGetLastError();
 }
 
-// Mixed mode: fold the FP32 correction into the FP64 solution and replace the
-// recursive residual by the true FP64 one (Clark et al., reliable updates).
-// Stores ||r_true||^2 into scal->rmax2; the caller decides whether to go on.
-hipError_t pcg_reliable_update(eolc_ctx *c, double /*tol*/, int /*parity*/)
+hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters)
 {
-    int N = c->topo.N;
-    int G = grid_rows(N);
+    const int G = pcg_grid(c->topo.N);
+    for (int k = k0; k < k0 + iters; ++k) {
+        if (mixed) launch_iteration<float>(c, k, G, c->A32, c->D32, c->z32, c->p32, c->q32, c->r32, c->xl32);
+        else launch_iteration<double>(c, k, G, c->A64, c->D64, c->z64, c->p64, c->q64, c->r64, c->xs64);
+    }
+    return hipGetLastError();
+}
+
+// Mixed mode: fold the FP32 correction into the FP64 solution and form the true
+// FP64 residual (Clark et al., reliable updates); ||r||^2 -> scal->rmax2.
+hipError_t pcg_reliable_update(eolc_ctx *c)
+{
+    const int N = c->topo.N;
+    const int G = pcg_grid(N);
     hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, 3 * N, c->xs64, c->xl32);
-    hipLaunchKernelGGL(k_resid64, dim3(G), dim3(256), 0, c->stream, N, c->rowptr, c->col, c->A64, c->topo.nbp,
-                       c->fixmask, c->xs64, c->b64, c->r64, &c->scal->rmax2, c->partials, c->counter);
+    launch_resid64(c, G, c->xs64, c->b64, c->r64);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, c->partials + 5 * kMaxPartials,
+                       &c->scal->rmax2);
     return hipGetLastError();
 }
 
-hipError_t pcg_restart_inner(eolc_ctx *c)
+// Continue the inner iteration at index k from the replaced residual, keeping p.
+hipError_t pcg_restart_inner(eolc_ctx *c, int k)
 {
-    int N = c->topo.N;
-    int G = grid_rows(N);
-    hipLaunchKernelGGL(k_precond_init<float>, dim3(G), dim3(256), 0, c->stream, N, c->D32, c->r64, c->r32, c->z32,
-                       1, c->scal, c->partials, c->counter);
+    const int N = c->topo.N;
+    const int G = pcg_grid(N);
+    hipLaunchKernelGGL(k_precond_init<float>, dim3(G), dim3(WG), 0, c->stream, k & 1, N, c->D32, c->r64, c->r32,
+                       c->z32, c->partials);
     return hipGetLastError();
 }
 
 hipError_t pcg_finish(eolc_ctx *c, bool /*mixed*/)
 {
-    int N = c->topo.N;
+    const int N = c->topo.N;
     hipLaunchKernelGGL(k_write_solution, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, N, c->xs64, c->fixmask,
                        c->vfix, c->v);
     return hipGetLastError();
@@ -568,40 +584,52 @@ hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset)
     return hipGetLastError();
 }
 
-// ||rhs - A v||^2 over free DOFs (v = final solution incl. fixed values) -> scal->rr
-hipError_t spmv_true_residual(eolc_ctx *c, double * /*unused*/)
+// ||rhs - A v||^2 over free DOFs (v = final solution incl. fixed values) -> scal->true_rr
+hipError_t spmv_true_residual(eolc_ctx *c)
 {
-    int N = c->topo.N;
-    int G = grid_rows(N);
-    hipLaunchKernelGGL(k_resid64, dim3(G), dim3(256), 0, c->stream, N, c->rowptr, c->col, c->A64, c->topo.nbp,
-                       c->fixmask, c->v, c->rhs, c->r64, &c->scal->rr, c->partials, c->counter);
+    const int N = c->topo.N;
+    const int G = pcg_grid(N);
+    launch_resid64(c, G, c->v, c->rhs, c->r64);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, c->partials + 5 * kMaxPartials,
+                       &c->scal->true_rr);
     return hipGetLastError();
 }
 
-}  // namespace eolc
-
-namespace eolc {
-// Back-to-back launches of the SpMV kernel for timing (scalars reset each launch).
+// Back-to-back launches of the SpMV kernel for timing (iteration index 1, `done`
+// cleared first).  Leaves the solver state undefined.
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int reps, float *ms)
 {
-    int N = c->topo.N;
-    int G = grid_rows(N);
+    const int N = c->topo.N;
+    const int G = pcg_grid(N);
     hipEvent_t a, b;
     hipError_t e = hipEventCreate(&a);
     if (e != hipSuccess) return e;
     e = hipEventCreate(&b);
     if (e != hipSuccess) return e;
     scal_set(c, 0.0, 0.0, 1);
+    hipLaunchKernelGGL(k_fill, dim3(blocks_for(4 * kMaxPartials)), dim3(256), 0, c->stream, 4 * kMaxPartials, 1.0,
+                       c->partials + kMaxPartials);
     hipEventRecord(a, c->stream);
     for (int r = 0; r < reps; ++r) {
-        if (mixed)
-            hipLaunchKernelGGL(k_pcg_spmv<float>, dim3(G), dim3(256), 0, c->stream, N, c->rowptr, c->col, c->A32,
-                               c->topo.nbp, c->fixmask, c->z32, c->p32[0], c->p32[1], c->q32, c->scal, c->partials,
-                               c->counter);
-        else
-            hipLaunchKernelGGL(k_pcg_spmv<double>, dim3(G), dim3(256), 0, c->stream, N, c->rowptr, c->col, c->A64,
-                               c->topo.nbp, c->fixmask, c->z64, c->p64[0], c->p64[1], c->q64, c->scal, c->partials,
-                               c->counter);
+        if (mixed) {
+            if (c->lpr == 8)
+                hipLaunchKernelGGL((k_pcg_spmv<float, 8>), dim3(G), dim3(WG), 0, c->stream, 1, N, c->rowptr, c->col,
+                                   c->A32, c->topo.nbp, c->fixmask, c->z32, c->p32[1], c->p32[0], c->q32, c->scal,
+                                   c->partials);
+            else
+                hipLaunchKernelGGL((k_pcg_spmv<float, 16>), dim3(G), dim3(WG), 0, c->stream, 1, N, c->rowptr, c->col,
+                                   c->A32, c->topo.nbp, c->fixmask, c->z32, c->p32[1], c->p32[0], c->q32, c->scal,
+                                   c->partials);
+        } else {
+            if (c->lpr == 8)
+                hipLaunchKernelGGL((k_pcg_spmv<double, 8>), dim3(G), dim3(WG), 0, c->stream, 1, N, c->rowptr, c->col,
+                                   c->A64, c->topo.nbp, c->fixmask, c->z64, c->p64[1], c->p64[0], c->q64, c->scal,
+                                   c->partials);
+            else
+                hipLaunchKernelGGL((k_pcg_spmv<double, 16>), dim3(G), dim3(WG), 0, c->stream, 1, N, c->rowptr, c->col,
+                                   c->A64, c->topo.nbp, c->fixmask, c->z64, c->p64[1], c->p64[0], c->q64, c->scal,
+                                   c->partials);
+        }
     }
     hipEventRecord(b, c->stream);
     e = hipEventSynchronize(b);
@@ -610,4 +638,5 @@ hipError_t time_spmv(eolc_ctx *c, bool mixed, int reps, float *ms)
     hipEventDestroy(b);
     return e == hipSuccess ? hipGetLastError() : e;
 }
+
 }  // namespace eolc
