@@ -1,11 +1,474 @@
-// Proximity queries (CD / CD2): placeholder until the gfx950 collision kernels land.
+// Host side of the proximity queries: CD2 / CD (Collisions.cpp:11-77) on top of the
+// gfx950 kernels in k_collide.hip.
+//
+//  - the jitter stream is the reference's own: std::mt19937 seeded with 1 feeding
+//    std::uniform_real_distribution<double>(-1, 1) (boxTriCollision.cpp:649-659),
+//    generated once per topology and kept on the device as unit values;
+//  - the cloth edge table is createEdges (boxTriCollision.cpp:141-231): face edges
+//    stable-sorted by the 32-bit hash kmin + (3F + 1) kmax -- which wraps modulo 2^32
+//    for large meshes, and that wrap-around is reproduced -- twins paired when
+//    adjacent with equal hashes.  Topology only; built at set-up;
+//  - box geometry (createBox, boxTriCollision.cpp:400-420) is computed here per call;
+//  - records are assembled in the reference's order: point records, then per box
+//    V2-T1, V1-T2, E2-E1, corner de-duplication and the pos1_ snap.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "collide_common.h"
 #include "eolc_internal.h"
 
-extern "C" int eolc_collide(eolc_ctx *c, double, int, const double *, const double *, int, const double *,
-                            const double *, int, int, int, eolc_collision *, int, int *count)
+using namespace eolc_cd;
+
+namespace eolc {
+hipError_t cd_prep(eolc_ctx *c, double thr, double *xj, double *fn, double *aabb_part, double *aabb);
+hipError_t cd_grid_build(eolc_ctx *c, const CdGrid &gp, const double *xj, unsigned *bcnt, unsigned *bstart,
+                         unsigned *cursor, int *entries, unsigned *nalways, int *always, unsigned *total, bool fill);
+hipError_t cd_query(eolc_ctx *c, const CdGrid &gp, int Q, const double *qpts, const double *qnrm, double thr,
+                    const double *aabb, const double *xj, const double *fn, const unsigned *bstart, const unsigned *bcnt,
+                    const int *entries, const unsigned *nalways, const int *always, int *face, eolc_collision *rec);
+hipError_t cd_v2t1(eolc_ctx *c, double thr, const BoxGeom *box, const double *xj, const double *fn, int *flag,
+                   eolc_collision *rec, int *err);
+hipError_t cd_eolpts(eolc_ctx *c, int P, double thr, const double *pts, const double *nrm, const double *xj, int *flag,
+                     eolc_collision *rec);
+hipError_t cd_ee(eolc_ctx *c, int pass, double thr, const BoxGeom *box, const double *xj, unsigned *counts,
+                 const unsigned *offsets, int *ambig, eolc_collision *out);
+hipError_t cd_compact(eolc_ctx *c, int n, const int *flag, unsigned *u32, unsigned *offs, unsigned *total,
+                      const eolc_collision *rec, eolc_collision *out, bool write);
+hipError_t cd_scan(eolc_ctx *c, unsigned n, const unsigned *in, unsigned *out, unsigned *total);
+}  // namespace eolc
+
+namespace {
+
+#define EOLC_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e__ = (call);                                                         \
+        if (e__ != hipSuccess) {                                                         \
+            c->err = std::string("HIP error ") + hipGetErrorString(e__) + " at " #call;  \
+            return EOLC_ERR_HIP;                                                         \
+        }                                                                                \
+    } while (0)
+
+// createBox (boxTriCollision.cpp:400-420) with createFaceNormals/createVertNormals.
+void make_box(BoxGeom &B, const double *whd, const double *E1 /* column-major 4x4 */)
+{
+    double S[16] = {0}, E[16];
+    S[0] = 0.5 * whd[0]; S[5] = 0.5 * whd[1]; S[10] = 0.5 * whd[2]; S[15] = 1.0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double s = E1[0 * 4 + i] * S[j * 4 + 0];
+            for (int k = 1; k < 4; ++k) s = s + E1[k * 4 + i] * S[j * 4 + k];
+            E[j * 4 + i] = s;
+        }
+    for (int c = 0; c < 14; ++c) {
+        const double vv[4] = {kBoxUnitVerts[c][0], kBoxUnitVerts[c][1], kBoxUnitVerts[c][2], 1.0};
+        for (int i = 0; i < 3; ++i) {
+            double s = E[0 * 4 + i] * vv[0];
+            for (int k = 1; k < 4; ++k) s = s + E[k * 4 + i] * vv[k];
+            B.v[c][i] = s;
+        }
+    }
+    for (int f = 0; f < 24; ++f) face_normal(B.v[kBoxFaces[f][0]], B.v[kBoxFaces[f][1]], B.v[kBoxFaces[f][2]], B.fn[f]);
+    double acc[14][3], ang[14];
+    std::memset(acc, 0, sizeof(acc));
+    std::memset(ang, 0, sizeof(ang));
+    for (int f = 0; f < 24; ++f) {
+        const double *xa = B.v[kBoxFaces[f][0]], *xb = B.v[kBoxFaces[f][1]], *xc = B.v[kBoxFaces[f][2]];
+        double dba[3], dcb[3], dac[3], nd[3], nor[3], t[3];
+        v_sub(xb, xa, dba); v_sub(xc, xb, dcb); v_sub(xa, xc, dac);
+        v_copy(dac, nd); v_neg(nd);
+        v_cross(dba, nd, nor);
+        v_normalized(nor, nor); v_normalized(dba, dba); v_normalized(dcb, dcb); v_normalized(dac, dac);
+        v_copy(dac, t); v_neg(t);
+        const double a1 = std::acos(v_dot(dba, t));
+        v_copy(dba, t); v_neg(t);
+        const double a2 = std::acos(v_dot(dcb, t));
+        v_copy(dcb, t); v_neg(t);
+        const double a3 = std::acos(v_dot(dac, t));
+        const int i0 = kBoxFaces[f][0], i1 = kBoxFaces[f][1], i2 = kBoxFaces[f][2];
+        for (int k = 0; k < 3; ++k) {
+            acc[i0][k] = acc[i0][k] + a1 * nor[k];
+            acc[i1][k] = acc[i1][k] + a2 * nor[k];
+            acc[i2][k] = acc[i2][k] + a3 * nor[k];
+        }
+        ang[i0] += a1; ang[i1] += a2; ang[i2] += a3;
+    }
+    for (int c = 0; c < 14; ++c) {
+        double nn[3] = {acc[c][0] / ang[c], acc[c][1] / ang[c], acc[c][2] / ang[c]};
+        v_normalized(nn, B.vn[c]);
+    }
+    for (int k = 0; k < 12; ++k) {
+        v_copy(B.fn[kBoxEdgeFaces[k][0]], B.en[k][0]);
+        v_copy(B.fn[kBoxEdgeFaces[k][1]], B.en[k][1]);
+        B.eangle[k] = std::acos(v_dot(B.en[k][0], B.en[k][1]));
+    }
+    for (int i = 0; i < 3; ++i) {
+        double lo = B.v[0][i], hi = B.v[0][i];
+        for (int c = 1; c < 14; ++c) { lo = std::min(lo, B.v[c][i]); hi = std::max(hi, B.v[c][i]); }
+        B.aabb[i] = lo; B.aabb[i + 3] = hi;
+    }
+    for (int f = 0; f < 24; ++f) {
+        const double *xa = B.v[kBoxFaces[f][0]], *xb = B.v[kBoxFaces[f][1]], *xc = B.v[kBoxFaces[f][2]];
+        for (int i = 0; i < 3; ++i) {
+            double lo = xa[i], hi = xa[i];
+            lo = std::min(xb[i], lo); lo = std::min(xc[i], lo);
+            hi = std::max(xb[i], hi); hi = std::max(xc[i], hi);
+            B.faabb[f][i] = lo; B.faabb[f][i + 3] = hi;
+        }
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+template <class T> hipError_t dmalloc(DevBuf &b, size_t n)
+{
+    return hipMalloc(&b.p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+}  // namespace
+
+namespace eolc {
+
+// createEdges order and pairing (topology only).  cedges: v0, v1, face0, face1|-1.
+void build_collision_edges(eolc_ctx *c, int F, const int *tri)
+{
+    struct FE {
+        int lv[3];
+        int face;
+        int hash;
+    };
+    const int n = 3 * F;
+    std::vector<FE> tmp((size_t)n);
+    for (int k = 0; k < F; ++k)
+        for (int i = 0; i < 3; ++i) {
+            FE &fe = tmp[3 * (size_t)k + i];
+            fe.lv[0] = i % 3; fe.lv[1] = (i + 1) % 3; fe.lv[2] = (i + 2) % 3;
+            fe.face = k;
+            const int a = tri[3 * k + fe.lv[0]], b = tri[3 * k + fe.lv[1]];
+            const int kmin = std::min(a, b) + 1, kmax = std::max(a, b) + 1;
+            fe.hash = (int)((uint32_t)kmin + (uint32_t)(n + 1) * (uint32_t)kmax);   // int overflow wraps
+        }
+    std::stable_sort(tmp.begin(), tmp.end(), [](const FE &x, const FE &y) { return x.hash < y.hash; });
+    std::vector<int> &E = c->h_cedges;
+    E.clear();
+    int k = 0;
+    while (k < n) {
+        const int f0 = tmp[k].face;
+        const int v0 = tri[3 * f0 + tmp[k].lv[0]], v1 = tri[3 * f0 + tmp[k].lv[1]];
+        if (k < n - 1 && tmp[k].hash == tmp[k + 1].hash) {
+            E.insert(E.end(), {v0, v1, f0, tmp[k + 1].face});
+            k += 2;
+        } else {
+            E.insert(E.end(), {v0, v1, f0, -1});
+            k += 1;
+        }
+    }
+    c->n_cedges = (int)(E.size() / 4);
+}
+
+// unit jitter values: the reference's random stream
+void build_jitter(std::vector<double> &u, int N)
+{
+    std::mt19937 gen;
+    std::uniform_real_distribution<> dis(-1.0, 1.0);
+    gen.seed(1);
+    u.resize(3 * (size_t)N);
+    for (size_t i = 0; i < u.size(); ++i) u[i] = dis(gen);
+}
+
+}  // namespace eolc
+
+namespace {
+
+int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double *nrm, int B, const double *whd,
+                 const double *Eb, int eol_points, int box_npts, int box_nedges, std::vector<eolc_collision> &out)
+{
+    const int N = c->topo.N, F = c->topo.F;
+    DevBuf xj, fn, aabb_part, aabb, qpts, qnrm, qface, qrec;
+    EOLC_HIP(dmalloc<double>(xj, 3 * (size_t)N));
+    EOLC_HIP(dmalloc<double>(fn, 3 * (size_t)F));
+    EOLC_HIP(dmalloc<double>(aabb_part, 6 * 256));
+    EOLC_HIP(dmalloc<double>(aabb, 6));
+    EOLC_HIP(eolc::cd_prep(c, thr, (double *)xj.p, (double *)fn.p, (double *)aabb_part.p, (double *)aabb.p));
+    double hab[6];
+    EOLC_HIP(hipMemcpyAsync(hab, aabb.p, sizeof(hab), hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    // boxes
+    std::vector<BoxGeom> boxes((size_t)B);
+    for (int b = 0; b < B; ++b) make_box(boxes[b], whd + 3 * b, Eb + 16 * b);
+    // ---- uniform grid over the cloth faces
+    CdGrid g;
+    const double L = std::max({hab[3] - hab[0], hab[4] - hab[1], hab[5] - hab[2], 1e-300});
+    g.margin = 5.0 * thr * (1.0 + 1e-6) + 1e-7 * L + 1e-12;
+    const double area = std::max((hab[3] - hab[0]) * (hab[4] - hab[1]), 1e-300);
+    double cell = std::max(2.0 * g.margin, std::sqrt(area / std::max(F, 1)) * 4.0);
+    int dims[3];
+    for (;;) {
+        double tot = 1.0;
+        for (int k = 0; k < 3; ++k) {
+            dims[k] = std::max(1, (int)std::ceil((hab[3 + k] - hab[k] + 2 * g.margin) / cell) + 1);
+            tot *= dims[k];
+        }
+        if (tot <= 1 << 24) break;
+        cell *= 1.5;
+    }
+    for (int k = 0; k < 3; ++k) { g.org[k] = hab[k] - g.margin; g.dim[k] = dims[k]; }
+    g.inv_cell = 1.0 / cell;
+    unsigned tbl = 1;
+    while (tbl < (unsigned)std::max(2 * F, 64)) tbl <<= 1;
+    g.mask = tbl - 1;
+    g.cond_max = 1e6;
+    DevBuf bcnt, bstart, cursor, total, nalways, always, entries;
+    EOLC_HIP(dmalloc<unsigned>(bcnt, tbl));
+    EOLC_HIP(dmalloc<unsigned>(bstart, tbl));
+    EOLC_HIP(dmalloc<unsigned>(cursor, tbl));
+    EOLC_HIP(dmalloc<unsigned>(total, 1));
+    EOLC_HIP(dmalloc<unsigned>(nalways, 1));
+    EOLC_HIP(dmalloc<int>(always, (size_t)F));
+    EOLC_HIP(hipMemsetAsync(bcnt.p, 0, tbl * sizeof(unsigned), c->stream));
+    EOLC_HIP(hipMemsetAsync(cursor.p, 0, tbl * sizeof(unsigned), c->stream));
+    EOLC_HIP(hipMemsetAsync(nalways.p, 0, sizeof(unsigned), c->stream));
+    EOLC_HIP(eolc::cd_grid_build(c, g, (const double *)xj.p, (unsigned *)bcnt.p, (unsigned *)bstart.p,
+                                 (unsigned *)cursor.p, nullptr, (unsigned *)nalways.p, (int *)always.p,
+                                 (unsigned *)total.p, false));
+    unsigned nentries = 0;
+    EOLC_HIP(hipMemcpyAsync(&nentries, total.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    EOLC_HIP(dmalloc<int>(entries, nentries));
+    EOLC_HIP(eolc::cd_grid_build(c, g, (const double *)xj.p, (unsigned *)bcnt.p, (unsigned *)bstart.p,
+                                 (unsigned *)cursor.p, (int *)entries.p, (unsigned *)nalways.p, (int *)always.p,
+                                 (unsigned *)total.p, true));
+    // ---- point / corner queries against the faces
+    const int Q = P + 8 * B;
+    std::vector<double> hq(3 * (size_t)std::max(Q, 1)), hn(3 * (size_t)std::max(Q, 1));
+    for (int i = 0; i < 3 * P; ++i) { hq[i] = pts[i]; hn[i] = nrm[i]; }
+    for (int b = 0; b < B; ++b)
+        for (int i1 = 0; i1 < 8; ++i1)
+            for (int k = 0; k < 3; ++k) {
+                hq[3 * (P + 8 * b + i1) + k] = boxes[b].v[i1][k];
+                hn[3 * (P + 8 * b + i1) + k] = boxes[b].vn[i1][k];
+            }
+    EOLC_HIP(dmalloc<double>(qpts, hq.size()));
+    EOLC_HIP(dmalloc<double>(qnrm, hn.size()));
+    EOLC_HIP(dmalloc<int>(qface, std::max(Q, 1)));
+    EOLC_HIP(dmalloc<eolc_collision>(qrec, std::max(Q, 1)));
+    std::vector<eolc_collision> qres((size_t)std::max(Q, 1));
+    if (Q > 0) {
+        EOLC_HIP(hipMemcpyAsync(qpts.p, hq.data(), hq.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(hipMemcpyAsync(qnrm.p, hn.data(), hn.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(eolc::cd_query(c, g, Q, (const double *)qpts.p, (const double *)qnrm.p, thr, (const double *)aabb.p,
+                                (const double *)xj.p, (const double *)fn.p, (const unsigned *)bstart.p,
+                                (const unsigned *)bcnt.p, (const int *)entries.p, (const unsigned *)nalways.p,
+                                (const int *)always.p, (int *)qface.p, (eolc_collision *)qrec.p));
+        EOLC_HIP(hipMemcpyAsync(qres.data(), qrec.p, sizeof(eolc_collision) * Q, hipMemcpyDeviceToHost, c->stream));
+    }
+    // ---- per-vertex passes (EOL points and V2-T1) share flag/record buffers
+    DevBuf flag, vrec, u32, offs, vout, err;
+    EOLC_HIP(dmalloc<int>(flag, (size_t)N));
+    EOLC_HIP(dmalloc<eolc_collision>(vrec, (size_t)N));
+    EOLC_HIP(dmalloc<unsigned>(u32, (size_t)N));
+    EOLC_HIP(dmalloc<unsigned>(offs, (size_t)N));
+    EOLC_HIP(dmalloc<int>(err, 1));
+    EOLC_HIP(hipMemsetAsync(err.p, 0, sizeof(int), c->stream));
+    auto per_vertex_records = [&](std::vector<eolc_collision> &dst) -> int {
+        EOLC_HIP(eolc::cd_compact(c, N, (const int *)flag.p, (unsigned *)u32.p, (unsigned *)offs.p,
+                                  (unsigned *)total.p, nullptr, nullptr, false));
+        unsigned cnt = 0;
+        EOLC_HIP(hipMemcpyAsync(&cnt, total.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(hipStreamSynchronize(c->stream));
+        dst.resize(cnt);
+        if (cnt) {
+            DevBuf o;
+            EOLC_HIP(dmalloc<eolc_collision>(o, cnt));
+            EOLC_HIP(eolc::cd_compact(c, N, (const int *)flag.p, (unsigned *)u32.p, (unsigned *)offs.p,
+                                      (unsigned *)total.p, (const eolc_collision *)vrec.p, (eolc_collision *)o.p, true));
+            EOLC_HIP(hipMemcpyAsync(dst.data(), o.p, sizeof(eolc_collision) * cnt, hipMemcpyDeviceToHost, c->stream));
+            EOLC_HIP(hipStreamSynchronize(c->stream));
+        }
+        return EOLC_OK;
+    };
+    const double snap = 0.1 * thr;
+    auto snap_all = [&](std::vector<eolc_collision> &v, size_t from) {
+        for (size_t k = from; k < v.size(); ++k)
+            for (int i = 0; i < 3; ++i) v[k].pos1_[i] = v[k].pos1[i] - snap * v[k].nor1[i];
+    };
+    // ---- pointTriCollision
+    DevBuf dpts, dnrm;
+    if (eol_points && P > 0) {
+        EOLC_HIP(dmalloc<double>(dpts, 3 * (size_t)P));
+        EOLC_HIP(dmalloc<double>(dnrm, 3 * (size_t)P));
+        EOLC_HIP(hipMemcpyAsync(dpts.p, pts, 3 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(hipMemcpyAsync(dnrm.p, nrm, 3 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(eolc::cd_eolpts(c, P, thr, (const double *)dpts.p, (const double *)dnrm.p, (const double *)xj.p,
+                                 (int *)flag.p, (eolc_collision *)vrec.p));
+        std::vector<eolc_collision> ev;
+        int rc = per_vertex_records(ev);
+        if (rc) return rc;
+        out.insert(out.end(), ev.begin(), ev.end());
+    } else {
+        EOLC_HIP(hipStreamSynchronize(c->stream));
+    }
+    for (int i1 = 0; i1 < P; ++i1) {
+        eolc_collision r = qres[i1];
+        if (r.count1 == 0) continue;
+        r.verts1[0] = i1;
+        out.push_back(r);
+    }
+    snap_all(out, 0);
+    // ---- boxTriCollision per box
+    DevBuf dbox, ecnt, eoff, eamb;
+    EOLC_HIP(dmalloc<BoxGeom>(dbox, 1));
+    const int E = c->n_cedges;
+    EOLC_HIP(dmalloc<unsigned>(ecnt, (size_t)E));
+    EOLC_HIP(dmalloc<unsigned>(eoff, (size_t)E));
+    EOLC_HIP(dmalloc<int>(eamb, (size_t)E));
+    std::vector<double> hx, hxj;   // host copies, only if an acos decision must be re-made
+    const size_t base_after_points = out.size();
+    for (int b = 0; b < B; ++b) {
+        std::vector<eolc_collision> clst;
+        EOLC_HIP(hipMemcpyAsync(dbox.p, &boxes[b], sizeof(BoxGeom), hipMemcpyHostToDevice, c->stream));
+        // V2-T1
+        EOLC_HIP(eolc::cd_v2t1(c, thr, (const BoxGeom *)dbox.p, (const double *)xj.p, (const double *)fn.p,
+                               (int *)flag.p, (eolc_collision *)vrec.p, (int *)err.p));
+        {
+            std::vector<eolc_collision> vv;
+            int rc = per_vertex_records(vv);
+            if (rc) return rc;
+            clst.insert(clst.end(), vv.begin(), vv.end());
+        }
+        // V1-T2
+        for (int i1 = 0; i1 < 8; ++i1) {
+            eolc_collision r = qres[P + 8 * b + i1];
+            if (r.count1 == 0) continue;
+            r.verts1[0] = i1;
+            r.nedge1 = 3;
+            for (int i = 0; i < 3; ++i) r.edge1[i] = kBoxVertEdges[i1][i];
+            clst.push_back(r);
+        }
+        // E2-E1
+        if (E > 0) {
+            EOLC_HIP(eolc::cd_ee(c, 0, thr, (const BoxGeom *)dbox.p, (const double *)xj.p, (unsigned *)ecnt.p, nullptr,
+                                 (int *)eamb.p, nullptr));
+            EOLC_HIP(eolc::cd_scan(c, (unsigned)E, (const unsigned *)ecnt.p, (unsigned *)eoff.p, (unsigned *)total.p));
+            unsigned ne = 0;
+            EOLC_HIP(hipMemcpyAsync(&ne, total.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+            std::vector<int> amb((size_t)E);
+            EOLC_HIP(hipMemcpyAsync(amb.data(), eamb.p, sizeof(int) * E, hipMemcpyDeviceToHost, c->stream));
+            EOLC_HIP(hipStreamSynchronize(c->stream));
+            std::vector<eolc_collision> ee(ne);
+            if (ne) {
+                DevBuf o;
+                EOLC_HIP(dmalloc<eolc_collision>(o, ne));
+                EOLC_HIP(eolc::cd_ee(c, 1, thr, (const BoxGeom *)dbox.p, (const double *)xj.p, (unsigned *)ecnt.p,
+                                     (const unsigned *)eoff.p, (int *)eamb.p, (eolc_collision *)o.p));
+                EOLC_HIP(hipMemcpyAsync(ee.data(), o.p, sizeof(eolc_collision) * ne, hipMemcpyDeviceToHost, c->stream));
+                EOLC_HIP(hipStreamSynchronize(c->stream));
+            }
+            // re-decide edges whose angle tests were within the band, with the C library acos
+            bool any = std::any_of(amb.begin(), amb.end(), [](int m) { return m != 0; });
+            if (any) {
+                if (hx.empty()) {
+                    hx.resize(3 * (size_t)N);
+                    hxj.resize(3 * (size_t)N);
+                    EOLC_HIP(hipMemcpy(hx.data(), c->x, hx.size() * sizeof(double), hipMemcpyDeviceToHost));
+                    EOLC_HIP(hipMemcpy(hxj.data(), xj.p, hxj.size() * sizeof(double), hipMemcpyDeviceToHost));
+                }
+                std::vector<eolc_collision> fixed;
+                size_t pos = 0;
+                for (int k2 = 0; k2 < E; ++k2) {
+                    // records of edge k2 are contiguous and ordered by k2
+                    size_t end = pos;
+                    while (end < ee.size() && ee[end].edge2 == k2) ++end;
+                    if (amb[k2]) {
+                        eolc_collision tmp[12];
+                        int mask = 0;
+                        int n = ee_edge(k2, thr, boxes[b], c->h_cedges.data(), hx.data(), hxj.data(),
+                                        c->topo_tri.data(), tmp, &mask, 0.0);
+                        fixed.insert(fixed.end(), tmp, tmp + n);
+                    } else {
+                        fixed.insert(fixed.end(), ee.begin() + pos, ee.begin() + end);
+                    }
+                    pos = end;
+                }
+                ee.swap(fixed);
+                c->cd_host_decisions += 1;
+            }
+            clst.insert(clst.end(), ee.begin(), ee.end());
+        }
+        // corner de-duplication (boxTriCollision.cpp:1020-1052)
+        for (int i1 = 0; i1 < 8; ++i1) {
+            int kmin = -1;
+            double dmin = 1e9;
+            for (size_t k = 0; k < clst.size(); ++k)
+                if (clst[k].count1 == 1 && clst[k].verts1[0] == i1) {
+                    double dx[3];
+                    v_sub(clst[k].pos2, boxes[b].v[i1], dx);
+                    double d = v_dot(dx, dx);
+                    if (d < dmin) { kmin = (int)k; dmin = d; }
+                }
+            if (kmin != -1) {
+                std::vector<size_t> dl;
+                for (size_t k = 0; k < clst.size(); ++k)
+                    if (clst[k].count1 == 1 && clst[k].verts1[0] == i1 && (int)k != kmin) dl.push_back(k);
+                for (size_t kd : dl) {
+                    clst[kd] = clst.back();
+                    clst.pop_back();
+                }
+            }
+        }
+        snap_all(clst, 0);
+        out.insert(out.end(), clst.begin(), clst.end());
+        if (eol_points) {
+            // CD: box feature indices shifted per obstacle (Collisions.cpp:38-49)
+            const size_t start = out.size() - clst.size();
+            for (size_t k = start; k < out.size(); ++k) {
+                eolc_collision &r = out[k];
+                const int base = P + b * box_npts + b * box_nedges;
+                if (r.count1 == 1 && r.count2 == 3) r.verts1[0] = base + r.verts1[0];
+                for (int e = 0; e < r.nedge1; ++e) r.edge1[e] = base + (box_npts + r.edge1[e]);
+            }
+        }
+    }
+    (void)base_after_points;
+    int herr = 0;
+    EOLC_HIP(hipMemcpy(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        c->err = "collide: a cloth vertex with index >= face count reached the face-normal lookup of V2-T1 "
+                 "(boxTriCollision.cpp:731 indexes face normals by vertex index)";
+        return EOLC_ERR_INVALID;
+    }
+    return EOLC_OK;
+}
+
+}  // namespace
+
+extern "C" int eolc_collide(eolc_ctx *c, double threshold, int P, const double *pts, const double *nrm, int B,
+                            const double *whd, const double *E, int eol_points, int box_npts, int box_nedges,
+                            eolc_collision *out, int cap, int *count)
 {
     if (!c) return EOLC_ERR_INVALID;
-    if (count) *count = 0;
-    c->err = "collide: not available yet";
-    return EOLC_ERR_INVALID;
+    if (!count) { c->err = "collide: null count"; return EOLC_ERR_INVALID; }
+    *count = 0;
+    if (!c->have_topo || !c->have_state) { c->err = "collide: topology and state must be set first"; return EOLC_ERR_INVALID; }
+    if (!(threshold > 0.0) || P < 0 || B < 0 || (P && (!pts || !nrm)) || (B && (!whd || !E)) || cap < 0) {
+        c->err = "collide: bad arguments";
+        return EOLC_ERR_INVALID;
+    }
+    EOLC_HIP(hipSetDevice(c->device));
+    std::vector<eolc_collision> recs;
+    int rc = collide_impl(c, threshold, P, pts, nrm, B, whd, E, eol_points, box_npts, box_nedges, recs);
+    if (rc) return rc;
+    *count = (int)recs.size();
+    if ((int)recs.size() > cap || (!out && !recs.empty())) {
+        c->err = "collide: output capacity too small";
+        return EOLC_ERR_OVERFLOW;
+    }
+    if (!recs.empty()) std::memcpy(out, recs.data(), recs.size() * sizeof(eolc_collision));
+    return EOLC_OK;
 }

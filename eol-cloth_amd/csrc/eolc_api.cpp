@@ -68,7 +68,7 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->p64[0]); dfree(c->p64[1]);
     dfree(c->r32); dfree(c->z32); dfree(c->q32); dfree(c->xl32);
     dfree(c->p32[0]); dfree(c->p32[1]);
-    dfree(c->cedges);
+    dfree(c->cedges); dfree(c->cd_unit);
     c->alloc_bytes = 0;
 }
 
@@ -293,6 +293,13 @@ int eolc_set_topology(eolc_ctx *c, int N, const double *X, int F, const int *tri
     for (double **p : {&c->b64, &c->xs64, &c->r64, &c->z64, &c->q64, &c->p64[0], &c->p64[1]})
         EOLC_HIP(dalloc(c, p, n3));
     for (float **p : {&c->r32, &c->z32, &c->q32, &c->xl32, &c->p32[0], &c->p32[1]}) EOLC_HIP(dalloc(c, p, n3));
+    // proximity-query tables (createEdges order, jitter stream)
+    c->topo_tri.assign(tri, tri + 3 * (size_t)F);
+    eolc::build_collision_edges(c, F, tri);
+    EOLC_HIP(upload(c, &c->cedges, c->h_cedges));
+    std::vector<double> unit;
+    eolc::build_jitter(unit, N);
+    EOLC_HIP(upload(c, &c->cd_unit, unit));
     c->have_topo = true;
     return EOLC_OK;
 }

@@ -95,10 +95,13 @@ struct eolc_ctx {
     // host pinned copy of the scalars
     eolc::SolverScalars *hscal = nullptr;
     size_t alloc_bytes = 0;
-    // collision edge table (topology only, createEdges order)
+    // collision data (topology only): createEdges table, unit jitter, host triangles
     int *cedges = nullptr;      // 4 per edge: v0, v1, face0, face1 (-1)
     int n_cedges = 0;
     std::vector<int> h_cedges;
+    std::vector<int> topo_tri;
+    double *cd_unit = nullptr;  // 3N jitter values in (-1, 1)
+    int cd_host_decisions = 0;  // angle tests re-decided on the host (diagnostic)
 };
 
 namespace eolc {
@@ -120,4 +123,6 @@ hipError_t spmv_true_residual(eolc_ctx *c);
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int reps, float *ms);
+void build_collision_edges(eolc_ctx *c, int F, const int *tri);
+void build_jitter(std::vector<double> &u, int N);
 }  // namespace eolc
