@@ -41,10 +41,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def spmv_bytes(N, nb, s):
-    """Algorithmic bytes of one k_pcg_spmv launch: block values (9 s per block),
-    column indices, row pointers, z and p_old read, p_new and q written, fixed mask."""
-    return (9 * s + 4) * nb + 4 * (N + 1) + 4 * s * 3 * N + N
+def spmv_bytes(N, nb, s, diff):
+    """Algorithmic bytes of one k_pcg_spmv launch over the nb stored blocks: block values
+    (9 scalars, plus the mass scalar in the FP32 difference form), column indices,
+    per-row length and row index, z and p_old read, p_new and q written, fixed mask."""
+    per_block = (10 if diff else 9) * s + 4
+    return per_block * nb + 8 * N + 4 * s * 3 * N + N
 
 
 def run_path(ctx, sc, mode, steps, warmup, maxit):
@@ -100,6 +102,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 block-Jacobi comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-spmv", action="store_true",
+                    help="counter runs: assemble once, 10 launches of each SpMV kernel and a 1 GiB "
+                         "device copy for calibration, then exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the multi-GPU layout is described in "
@@ -115,6 +120,21 @@ def main():
     ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
     ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
     log(f"topology set-up {time.perf_counter() - t:.1f} s")
+
+    if args.profile_spmv:
+        ctx.upload_state(sc.x0, sc.v0)
+        ctx.forces_fill(GRAV, H)
+        for prec in (32, 64):
+            ms = ctx.time_spmv(prec, 10)
+            log(f"spmv fp{prec}: {ms * 1e3:.1f} us per launch")
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy of 1 GiB done")
+        ctx.close()
+        return
 
     head_mode, head_name = eolc.SOLVE_MIXED_BJ, "mixed-bj"
     t_head, st_head = run_path(ctx, sc, head_mode, args.steps, args.warmup, args.maxit)
@@ -134,7 +154,7 @@ def main():
     nnz = C.c_int64(0)
     eolc.lib().eolc_download_csr(ctx._h, eolc.MAT_MDK, None, None, None, 0, C.byref(nnz))
     nb = nnz.value // 9
-    traffic = spmv_bytes(sc.N, nb, 4)
+    traffic = spmv_bytes(sc.N, nb, 4, True)
     achieved = traffic / (spmv32_ms * 1e-3)
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": achieved / PEAK_HBM, "traffic": None, "kernel": "k_pcg_spmv<float>",

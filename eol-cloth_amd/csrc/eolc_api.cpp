@@ -57,12 +57,11 @@ template <class T> void dfree(T *&p)
 void free_topology_buffers(eolc_ctx *c)
 {
     dfree(c->X); dfree(c->tri); dfree(c->hinge);
-    dfree(c->rowptr); dfree(c->col); dfree(c->diagpos);
+    dfree(c->slice_off); dfree(c->srow); dfree(c->slen); dfree(c->scol); dfree(c->diagpos);
     dfree(c->cptr); dfree(c->clist); dfree(c->fptr); dfree(c->flist);
-    dfree(c->mptr); dfree(c->mlist);
     dfree(c->stageK); dfree(c->stageF); dfree(c->stageM);
     dfree(c->x); dfree(c->v); dfree(c->f); dfree(c->rhs);
-    dfree(c->A64); dfree(c->A32); dfree(c->Mval); dfree(c->D64); dfree(c->D32);
+    dfree(c->A64); dfree(c->S32); dfree(c->M32); dfree(c->entry_is_diag); dfree(c->Mval); dfree(c->D64); dfree(c->D32);
     dfree(c->fixmask); dfree(c->vfix);
     dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
     dfree(c->p64[0]); dfree(c->p64[1]);
@@ -123,10 +122,7 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
         t.rowptr[i + 1] = (int)t.col.size();
     }
     t.nb = (int64_t)t.col.size();
-    t.nbp = (t.nb + 63) & ~(int64_t)63;
-    t.diagpos.resize(N);
-    for (int i = 0; i < N; ++i) t.diagpos[i] = find_block(t, i, i);
-    // block contributions in reference insertion order (slot order)
+    // block contributions in reference insertion order (slot order), per CSR block
     const int64_t nslots = 6 * (int64_t)F + 10 * (int64_t)H;
     if (2 * nslots + 1 > (int64_t)INT32_MAX) return "too many element slots for 32-bit indices";
     std::vector<int> cnt(t.nb + 1, 0);
@@ -150,14 +146,75 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
         for (int p = 0; p < 10; ++p)
             visit(6 * (int64_t)F + 10 * (int64_t)h + p, hinge[4 * h + kHingePairs[p][0]],
                   hinge[4 * h + kHingePairs[p][1]]);
-    t.cptr.assign(t.nb + 1, 0);
-    for (int64_t b = 0; b < t.nb; ++b) t.cptr[b + 1] = t.cptr[b] + cnt[b + 1];
-    t.clist.assign(t.cptr[t.nb], 0);
-    std::vector<int> fill(t.cptr.begin(), t.cptr.end() - 1);
-    for (int64_t slot = 0; slot < nslots; ++slot) {
-        int b1 = slotblk[2 * slot], b2 = slotblk[2 * slot + 1];
-        t.clist[fill[b1]++] = (int)(slot << 1);
-        if (b2 >= 0) t.clist[fill[b2]++] = (int)((slot << 1) | 1);
+    std::vector<int> bptr(t.nb + 1, 0);
+    for (int64_t b = 0; b < t.nb; ++b) bptr[b + 1] = bptr[b] + cnt[b + 1];
+    std::vector<int> blist(bptr[t.nb], 0);
+    {
+        std::vector<int> fill(bptr.begin(), bptr.end() - 1);
+        for (int64_t slot = 0; slot < nslots; ++slot) {
+            int b1 = slotblk[2 * slot], b2 = slotblk[2 * slot + 1];
+            blist[fill[b1]++] = (int)(slot << 1);
+            if (b2 >= 0) blist[fill[b2]++] = (int)((slot << 1) | 1);
+        }
+    }
+    t.has_tri.assign(t.nb, 0);
+    for (int64_t b = 0; b < t.nb; ++b)
+        for (int e = bptr[b]; e < bptr[b + 1]; ++e)
+            if ((blist[e] >> 1) < 6 * F) { t.has_tri[b] = 1; break; }
+    // SELL-C-sigma layout
+    const int C = eolc::kSlice;
+    t.nslices = (N + C - 1) / C;
+    t.slice_off.assign(t.nslices + 1, 0);
+    t.srow.assign(N, 0);
+    t.slen.assign(N, 0);
+    for (int sl = 0; sl < t.nslices; ++sl) {
+        const int r0 = sl * C, r1 = std::min(N, r0 + C);
+        std::vector<int> rows;
+        for (int r = r0; r < r1; ++r) rows.push_back(r);
+        std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) {
+            return (t.rowptr[a + 1] - t.rowptr[a]) > (t.rowptr[b + 1] - t.rowptr[b]);
+        });
+        int W = 0;
+        for (int l = 0; l < (int)rows.size(); ++l) {
+            t.srow[r0 + l] = rows[l];
+            t.slen[r0 + l] = t.rowptr[rows[l] + 1] - t.rowptr[rows[l]];
+            W = std::max(W, t.slen[r0 + l]);
+        }
+        if ((int64_t)t.slice_off[sl] + (int64_t)W * C > (int64_t)INT32_MAX) return "too many SELL entries";
+        t.slice_off[sl + 1] = t.slice_off[sl] + W * C;
+    }
+    t.nbs = t.slice_off[t.nslices];
+    t.scol.assign(t.nbs, 0);
+    t.sellpos.assign(t.nb, -1);
+    t.sdiag.assign(N, -1);
+    std::vector<int> entry_block(t.nbs, -1);
+    for (int sl = 0; sl < t.nslices; ++sl) {
+        const int r0 = sl * C, r1 = std::min(N, r0 + C);
+        for (int l = 0; l < r1 - r0; ++l) {
+            const int row = t.srow[r0 + l];
+            for (int k = 0; k < t.slen[r0 + l]; ++k) {
+                const int e = t.slice_off[sl] + k * C + l;
+                const int b = t.rowptr[row] + k;
+                t.scol[e] = t.col[b];
+                t.sellpos[b] = e;
+                entry_block[e] = b;
+                if (t.col[b] == row) t.sdiag[row] = e;
+            }
+            // padding entries point at the row itself (never read as values)
+            for (int k = t.slen[r0 + l]; k < (t.slice_off[sl + 1] - t.slice_off[sl]) / C; ++k)
+                t.scol[t.slice_off[sl] + k * C + l] = row;
+        }
+    }
+    // contributor lists in SELL entry order (padding entries: empty)
+    t.cptr.assign(t.nbs + 1, 0);
+    for (int64_t e = 0; e < t.nbs; ++e) {
+        const int b = entry_block[e];
+        t.cptr[e + 1] = t.cptr[e] + (b >= 0 ? bptr[b + 1] - bptr[b] : 0);
+    }
+    t.clist.assign(t.cptr[t.nbs], 0);
+    for (int64_t e = 0; e < t.nbs; ++e) {
+        const int b = entry_block[e];
+        if (b >= 0) std::copy(blist.begin() + bptr[b], blist.begin() + bptr[b + 1], t.clist.begin() + t.cptr[e]);
     }
     // vertex force slots: faces (3f+v), then hinges (3F + 4h + v)
     t.fptr.assign(N + 1, 0);
@@ -253,6 +310,8 @@ int eolc_set_topology(eolc_ctx *c, int N, const double *X, int F, const int *tri
             return fail(c, EOLC_ERR_DEGENERATE,
                         "set_topology: face " + std::to_string(f) + " has non-positive material area (must be CCW)");
     }
+    if (eolc::pcg_grid(N) > eolc::kMaxPartials)
+        return fail(c, EOLC_ERR_INVALID, "set_topology: mesh too large for the PCG reduction buffers");
     Topology t;
     std::string msg = build_topology(t, N, F, tri, H, hinge);
     if (!msg.empty()) return fail(c, EOLC_ERR_INVALID, "set_topology: " + msg);
@@ -265,9 +324,11 @@ int eolc_set_topology(eolc_ctx *c, int N, const double *X, int F, const int *tri
     EOLC_HIP(upload(c, &c->X, Xv));
     EOLC_HIP(upload(c, &c->tri, triv));
     EOLC_HIP(upload(c, &c->hinge, hv));
-    EOLC_HIP(upload(c, &c->rowptr, T.rowptr));
-    EOLC_HIP(upload(c, &c->col, T.col));
-    EOLC_HIP(upload(c, &c->diagpos, T.diagpos));
+    EOLC_HIP(upload(c, &c->slice_off, T.slice_off));
+    EOLC_HIP(upload(c, &c->srow, T.srow));
+    EOLC_HIP(upload(c, &c->slen, T.slen));
+    EOLC_HIP(upload(c, &c->scol, T.scol));
+    EOLC_HIP(upload(c, &c->diagpos, T.sdiag));
     EOLC_HIP(upload(c, &c->cptr, T.cptr));
     EOLC_HIP(upload(c, &c->clist, T.clist));
     EOLC_HIP(upload(c, &c->fptr, T.fptr));
@@ -281,9 +342,15 @@ int eolc_set_topology(eolc_ctx *c, int N, const double *X, int F, const int *tri
     EOLC_HIP(dalloc(c, &c->v, n3));
     EOLC_HIP(dalloc(c, &c->f, n3));
     EOLC_HIP(dalloc(c, &c->rhs, n3));
-    EOLC_HIP(dalloc(c, &c->A64, 9 * (size_t)T.nbp));
-    EOLC_HIP(dalloc(c, &c->A32, 9 * (size_t)T.nbp));
-    EOLC_HIP(dalloc(c, &c->Mval, (size_t)T.nb));
+    EOLC_HIP(dalloc(c, &c->A64, 9 * (size_t)T.nbs));
+    EOLC_HIP(dalloc(c, &c->S32, 9 * (size_t)T.nbs));
+    EOLC_HIP(dalloc(c, &c->M32, (size_t)T.nbs));
+    {
+        std::vector<int> isd((size_t)T.nbs, 0);
+        for (int i = 0; i < N; ++i) isd[T.sdiag[i]] = 1;
+        EOLC_HIP(upload(c, &c->entry_is_diag, isd));
+    }
+    EOLC_HIP(dalloc(c, &c->Mval, (size_t)T.nbs));
     EOLC_HIP(dalloc(c, &c->D64, 9 * (size_t)N));
     EOLC_HIP(dalloc(c, &c->D32, 9 * (size_t)N));
     EOLC_HIP(dalloc(c, &c->fixmask, ((size_t)N + 3) & ~(size_t)3));
@@ -401,22 +468,12 @@ int eolc_download_csr(eolc_ctx *c, int which, int64_t *rowptr, int *col, double 
     EOLC_HIP(hipSetDevice(c->device));
     const Topology &T = c->topo;
     const int N = T.N;
-    // blocks present: MDK = all; M = blocks with at least one triangle slot
-    std::vector<char> keep(T.nb, 1);
-    if (which == EOLC_MAT_M) {
-        const int ntri = 6 * T.F;
-        for (int64_t b = 0; b < T.nb; ++b) {
-            char k = 0;
-            for (int e = T.cptr[b]; e < T.cptr[b + 1]; ++e)
-                if ((T.clist[e] >> 1) < ntri) { k = 1; break; }
-            keep[b] = k;
-        }
-    }
+    // blocks present: MDK = all; M = blocks with at least one triangle contribution
     int64_t need = 0;
-    for (int64_t b = 0; b < T.nb; ++b) need += keep[b] ? 9 : 0;
+    for (int64_t b = 0; b < T.nb; ++b) need += (which == EOLC_MAT_MDK || T.has_tri[b]) ? 9 : 0;
     *nnz = need;
     if (!rowptr || !col || !val || cap < need) return fail(c, EOLC_ERR_OVERFLOW, "download_csr: capacity too small");
-    std::vector<double> A((size_t)9 * T.nbp), m((size_t)T.nb);
+    std::vector<double> A((size_t)9 * T.nbs), m((size_t)T.nbs);
     EOLC_HIP(hipMemcpyAsync(A.data(), c->A64, A.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipMemcpyAsync(m.data(), c->Mval, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
@@ -425,11 +482,12 @@ int eolc_download_csr(eolc_ctx *c, int which, int64_t *rowptr, int *col, double 
         for (int a = 0; a < 3; ++a) {
             rowptr[3 * i + a] = p;
             for (int k = T.rowptr[i]; k < T.rowptr[i + 1]; ++k) {
-                if (!keep[k]) continue;
+                if (which == EOLC_MAT_M && !T.has_tri[k]) continue;
                 const int j = T.col[k];
+                const size_t e = (size_t)T.sellpos[k];
                 for (int cc = 0; cc < 3; ++cc) {
                     col[p] = 3 * j + cc;
-                    val[p] = (which == EOLC_MAT_MDK) ? A[(size_t)(3 * a + cc) * T.nbp + k] : (a == cc ? m[k] : 0.0);
+                    val[p] = (which == EOLC_MAT_MDK) ? A[(size_t)(3 * a + cc) * T.nbs + e] : (a == cc ? m[e] : 0.0);
                     ++p;
                 }
             }

@@ -13,7 +13,7 @@ namespace eolc {
 // 16 lanes per block row, 256 threads per workgroup.
 constexpr int kPcgWG = 1024;        // threads per workgroup of the PCG row kernels
 constexpr int kPcgGrid = 512;       // maximum workgroups of the PCG row kernels
-constexpr int kMaxPartials = 2048;  // stride of one partial-sum array
+constexpr int kMaxPartials = 16384; // stride of one partial-sum array (max PCG workgroups)
 
 // Device-side solver scalars.  Written only by the last workgroup of a
 // reduction kernel (or by a one-thread set-up kernel); read by every
@@ -35,18 +35,29 @@ struct SolverScalars {
     int first_iter;   // iteration that starts with beta = 0
 };
 
+// 3x3 block matrix in sliced ELL form (SELL-C-sigma, C = 64 = one wavefront):
+// rows are grouped in slices of 64 consecutive vertices; inside a slice the rows are
+// ordered by decreasing length, so at every column position k the rows that still
+// have a block form a prefix of the wavefront.  Entry (slice s, position k, lane l)
+// lives at slice_off[s] + 64 k + l; storage position 64 s + l holds row srow[..]
+// with slen[..] blocks.  The CSR form (rowptr/col, columns sorted) is kept on the
+// host for downloads.
+constexpr int kSlice = 64;
+
 struct Topology {
     int N = 0, F = 0, H = 0;
-    int64_t nb = 0;          // 3x3 blocks
-    int64_t nbp = 0;         // padded stride of the SoA value arrays
-    std::vector<int> rowptr, col, diagpos;
-    // gather maps (element slots -> blocks, element forces -> vertices)
-    std::vector<int> cptr;   // nb + 1
+    int64_t nb = 0;          // 3x3 blocks (CSR)
+    std::vector<int> rowptr, col;
+    // SELL layout
+    int nslices = 0;
+    int64_t nbs = 0;         // SELL entries including padding (stride of the SoA value arrays)
+    std::vector<int> slice_off, srow, slen, scol, sellpos, sdiag;
+    // gather maps in SELL entry order (element slots -> entries, element forces -> vertices)
+    std::vector<int> cptr;   // nbs + 1
     std::vector<int> clist;  // (slot << 1) | transposed
+    std::vector<char> has_tri;  // per CSR block: at least one triangle contribution (M pattern)
     std::vector<int> fptr;   // N + 1
     std::vector<int> flist;  // force-vector slot (3F tri slots, then 4H hinge slots)
-    std::vector<int> mptr;   // nb + 1, triangle slots only (mass scalars)
-    std::vector<int> mlist;  // slot of a triangle block (6f + p), transpose irrelevant
 };
 
 template <class T> struct DArr {
@@ -59,7 +70,6 @@ template <class T> struct DArr {
 struct eolc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    int lpr = 16;               // lanes per block row in the PCG SpMV (8 or 16)
     std::string err;
     int flags = 0;
     // material
@@ -70,16 +80,17 @@ struct eolc_ctx {
 
     // ---- device buffers
     double *X = nullptr; int *tri = nullptr; int *hinge = nullptr;
-    int *rowptr = nullptr, *col = nullptr, *diagpos = nullptr;
+    int *slice_off = nullptr, *srow = nullptr, *slen = nullptr, *scol = nullptr, *diagpos = nullptr;
     int *cptr = nullptr, *clist = nullptr, *fptr = nullptr, *flist = nullptr;
-    int *mptr = nullptr, *mlist = nullptr;
     double *stageK = nullptr;   // (6F + 10H) x 9
     double *stageF = nullptr;   // (3F + 4H) x 3
     double *stageM = nullptr;   // 6F mass scalars
     double *x = nullptr, *v = nullptr, *f = nullptr, *rhs = nullptr;
-    double *A64 = nullptr;      // 9 x nbp (SoA)
-    float *A32 = nullptr;       // 9 x nbp (SoA)
-    double *Mval = nullptr;     // nb mass scalars (M block = m I3)
+    double *A64 = nullptr;      // 9 x nbs (SoA, SELL entry order)
+    float *S32 = nullptr;       // 9 x nbs: dampingB h^2 K off the diagonal, 0 on it
+    float *M32 = nullptr;       // nbs mass scalars (FP32)
+    int *entry_is_diag = nullptr;  // nbs
+    double *Mval = nullptr;     // nbs mass scalars (M block = m I3)
     double *D64 = nullptr;      // 9 x N  block-Jacobi inverse (SoA)
     float *D32 = nullptr;
     unsigned char *fixmask = nullptr;  // N, bit k = DOF k fixed

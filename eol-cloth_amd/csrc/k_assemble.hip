@@ -6,11 +6,14 @@
 // (faces in face order, then hinges in edge order), so each block is summed in
 // the same order Eigen's collapseDuplicates folds duplicates -- without atomics.
 //
-//   k_assemble : one thread per block; MDK block = sum of its staged slots (a slot
+//   k_assemble : one thread per SELL entry (padding entries have no slots and
+//                come out 0); MDK block = sum of its staged slots (a slot
 //                flagged "transposed" is the (w,v) image of an element's (v,w)
 //                block); the mass scalar sums the triangle slots.  Writes the FP64
-//                SoA values and, for the mixed solver, an FP32 copy.
-//   k_rhs      : one thread per vertex; f_i = sum of its element force slots
+//                SoA values and, for the mixed solver, the FP32 difference form:
+//                S_ij = dampingB h^2 K_ij off the diagonal (0 on it) and m_ij, with
+//                which (A p)_i = sum_j m_ij p_j + sum_j S_ij (p_j - p_i).
+//   k_rhs      : one thread per vertex (SELL storage order); f_i = sum of its element force slots
 //                (faces, then hinges when the bending force is included) and
 //                rhs_i = sum_j m_ij v_j + h f_i, the right-hand side M v + h f of
 //                Cloth::solve (Cloth.cpp:345).
@@ -19,49 +22,55 @@
 namespace {
 
 __global__ void __launch_bounds__(256)
-k_assemble(int64_t nb, int64_t nbp, int ntrislots, const int *__restrict__ cptr, const int *__restrict__ clist,
-           const double *__restrict__ stageK, const double *__restrict__ stageM, double *__restrict__ A64,
-           float *__restrict__ A32, double *__restrict__ Mval)
+k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *__restrict__ clist,
+           const int *__restrict__ entry_is_diag, const double *__restrict__ stageK, const double *__restrict__ stageM,
+           double *__restrict__ A64, float *__restrict__ S32, float *__restrict__ M32, double *__restrict__ Mval)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    double acc[9];
+    if (b >= nbs) return;
+    double acc[9], stf[9];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) acc[c] = 0.0;
+    for (int c = 0; c < 9; ++c) { acc[c] = 0.0; stf[c] = 0.0; }
     double m = 0.0;
     const int e0 = cptr[b], e1 = cptr[b + 1];
     for (int e = e0; e < e1; ++e) {
         const int code = clist[e];
         const int slot = code >> 1;
         const double *sk = stageK + (size_t)9 * slot;
-        if (code & 1) {
+        const double mm = slot < ntrislots ? stageM[slot] : 0.0;
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
+        for (int j = 0; j < 3; ++j)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) acc[3 * j + k] += sk[3 * k + j];
-        } else {
-#pragma unroll
-            for (int c = 0; c < 9; ++c) acc[c] += sk[c];
-        }
-        if (slot < ntrislots) m += stageM[slot];
+            for (int k = 0; k < 3; ++k) {
+                const double kv = (code & 1) ? sk[3 * k + j] : sk[3 * j + k];
+                // the triplet value of the reference: M_vw + dampingB h^2 K_vw (hinges: no mass)
+                acc[3 * j + k] += (slot < ntrislots ? (j == k ? mm : 0.0) + kv : kv);
+                stf[3 * j + k] += kv;
+            }
+        m += mm;
     }
 #pragma unroll
-    for (int c = 0; c < 9; ++c) A64[c * nbp + b] = acc[c];
-    if (A32) {
+    for (int c = 0; c < 9; ++c) A64[c * nbs + b] = acc[c];
+    if (S32) {
+        const bool diag = entry_is_diag[b] != 0;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) A32[c * nbp + b] = (float)acc[c];
+        for (int c = 0; c < 9; ++c) S32[c * nbs + b] = diag ? 0.0f : (float)stf[c];
+        M32[b] = (float)m;
     }
     Mval[b] = m;
 }
 
+// one thread per SELL storage position (row srow[g])
 __global__ void __launch_bounds__(256)
 k_rhs(int N, int ntriforce, int include_bending, double h, const int *__restrict__ fptr,
-      const int *__restrict__ flist, const double *__restrict__ stageF, const int *__restrict__ rowptr,
-      const int *__restrict__ col, const double *__restrict__ Mval, const double *__restrict__ v,
-      double *__restrict__ f, double *__restrict__ rhs)
+      const int *__restrict__ flist, const double *__restrict__ stageF, const int *__restrict__ slice_off,
+      const int *__restrict__ srow, const int *__restrict__ slen, const int *__restrict__ scol,
+      const double *__restrict__ Mval, const double *__restrict__ v, double *__restrict__ f, double *__restrict__ rhs)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    const int i = srow[g], len = slen[g];
+    const int base = slice_off[g / eolc::kSlice] + (g % eolc::kSlice);
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     for (int e = fptr[i]; e < fptr[i + 1]; ++e) {
         const int slot = flist[e];
@@ -70,10 +79,12 @@ k_rhs(int N, int ntriforce, int include_bending, double h, const int *__restrict
         f0 += sf[0]; f1 += sf[1]; f2 += sf[2];
     }
     f[3 * i] = f0; f[3 * i + 1] = f1; f[3 * i + 2] = f2;
+    // M v with the columns in increasing order (SELL keeps the CSR order within a row)
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-        const int j = col[k];
-        const double m = Mval[k];
+    for (int k = 0; k < len; ++k) {
+        const int e = base + k * eolc::kSlice;
+        const int j = scol[e];
+        const double m = Mval[e];
         y0 += m * v[3 * j]; y1 += m * v[3 * j + 1]; y2 += m * v[3 * j + 2];
     }
     rhs[3 * i] = y0 + h * f0;
@@ -89,10 +100,11 @@ namespace eolc {
 
 hipError_t launch_assemble(eolc_ctx *c, bool want32)
 {
-    const int64_t nb = c->topo.nb;
+    const int64_t nb = c->topo.nbs;
     if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, c->topo.nbp, 6 * c->topo.F,
-                       c->cptr, c->clist, c->stageK, c->stageM, c->A64, want32 ? c->A32 : nullptr, c->Mval);
+    hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
+                       c->entry_is_diag, c->stageK, c->stageM, c->A64, want32 ? c->S32 : nullptr,
+                       want32 ? c->M32 : nullptr, c->Mval);
     return hipGetLastError();
 }
 
@@ -101,7 +113,7 @@ hipError_t launch_rhs(eolc_ctx *c, double h)
     const int N = c->topo.N;
     hipLaunchKernelGGL(k_rhs, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, 3 * c->topo.F,
                        (c->flags & EOLC_OPT_INCLUDE_BENDING_FORCE) ? 1 : 0, h, c->fptr, c->flist, c->stageF,
-                       c->rowptr, c->col, c->Mval, c->v, c->f, c->rhs);
+                       c->slice_off, c->srow, c->slen, c->scol, c->Mval, c->v, c->f, c->rhs);
     return hipGetLastError();
 }
 

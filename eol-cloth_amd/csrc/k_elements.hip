@@ -7,9 +7,10 @@
 //          G  = sum_v (R x_v) d_v^T, E = G - I, S = A (2 mu E + lambda tr(E) I),
 //          f_v = -R^T S d_v + (rho A / 3) g,
 //          K_vw = A [2 mu (d_v.d_w) R^T R + lambda (R^T d_v)(R^T d_w)^T].
-//        Writes the MDK contribution M_vw + dampingB h^2 K_vw of the six local
-//        blocks (aa, bb, cc, ab, ac, bc -- the order of fillxMI/fillxxMI,
-//        Forces.cpp:103-125), the mass scalars and f_v.
+//        Writes dampingB h^2 K_vw of the six local blocks (aa, bb, cc, ab, ac, bc --
+//        the order of fillxMI/fillxxMI, Forces.cpp:103-125), their mass scalars
+//        (M_vw = m I3; the assembly adds them back, giving the reference's
+//        M_vw + dampingB h^2 K_vw triplet values) and f_v.
 // hinge: edgeBasedF (Forces.cpp:685-910) -- ComputeBending
 //        (ComputeBending.cpp:16-738): W = w (1 - n0^.n1^), with
 //        n0 = (x1-x0)x(x2-x0), n1 = (x0-x1)x(x3-x1), w = 3/2 beta |X1-X0|^2/(A0+A1).
@@ -136,8 +137,7 @@ k_face_eval(int F, const int *__restrict__ tri, const double *__restrict__ x, co
             for (int k = 0; k < 3; ++k) {
                 // (Rd_v,j * Rd_w,k) is formed first so that diagonal blocks come out exactly symmetric
                 double kk = A * (2.0 * mu * dd * RtR[j][k] + lam * (comp(Rd[vv], j) * comp(Rd[ww], k)));
-                double mm = (j == k) ? mass : 0.0;
-                sk[9 * p + 3 * j + k] = mm + s * kk;
+                sk[9 * p + 3 * j + k] = s * kk;
             }
     }
 }

@@ -17,13 +17,17 @@
 // of the kernel that decides to stop) turns later launches into no-ops, so the host
 // enqueues chunks of iterations and synchronises only between chunks.
 //
-// Workgroup -> rows: launch w is mapped to logical chunk (w % 8) * (G / 8) + w / 8,
+// Workgroup -> work: launch w is mapped to logical workgroup (w % 8) * (G / 8) + w / 8,
 // so each of the 8 XCDs (which receive workgroups round-robin) streams one
-// contiguous eighth of the rows and keeps its neighbour gathers in its own L2.
+// contiguous eighth of the matrix and keeps its neighbour gathers in its own L2.
 //
-// Matrix values are stored SoA: A[c * nbp + k] is entry c (row-major 3x3) of
-// block k; the LPR lanes of a row read consecutive blocks, fully coalesced.
-// Products and row sums are accumulated in FP64 also when A, p, q are FP32.
+// The matrix is in SELL-C-sigma form (C = 64, see eolc_internal.h): each wavefront
+// of the SpMV owns one 64-row slice, one thread per block row; at column position
+// k the 64 lanes read 64 consecutive entries of each of the 9 SoA value arrays
+// (A[c * nbs + e]) and of the column array, fully coalesced, and because rows are
+// sorted by length inside the slice the active lanes are always a prefix, so the
+// padding is never fetched.  Products and row sums are accumulated in FP64 also
+// when A, p, q are FP32.
 #include <type_traits>
 
 #include "eolc_internal.h"
@@ -50,6 +54,9 @@ __device__ __forceinline__ void wg_rows(int N, int *begin, int *end)
     *begin = b < N ? b : N;
     *end = e < N ? e : N;
 }
+
+// slice owned by this wavefront (16 per workgroup)
+__device__ __forceinline__ int wave_slice() { return logical_wg() * (WG / 64) + (threadIdx.x >> 6); }
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -117,27 +124,24 @@ __device__ __forceinline__ void load3(const T *__restrict__ a, int i, double &x,
     x = (double)a[3 * i]; y = (double)a[3 * i + 1]; z = (double)a[3 * i + 2];
 }
 
-// sum over the LPR lanes of a row group
-template <int LPR>
-__device__ __forceinline__ double row_sum(double v)
-{
-#pragma unroll
-    for (int m = LPR / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, LPR);
-    return v;
-}
-
 // Partials layout (doubles, stride kMaxPartials): PA at slot 0, PB[parity] (r.z, r.r)
 // at slots 1 + 2 parity and 2 + 2 parity, residual-norm partials at slot 5.
 __device__ __forceinline__ double *PA(double *p) { return p; }
 __device__ __forceinline__ double *PB(double *p, int parity) { return p + (1 + 2 * parity) * eolc::kMaxPartials; }
 
 // ---------------------------------------------------------------------------
-template <class T, int LPR>
-__global__ void __launch_bounds__(WG)
-k_pcg_spmv(int k, int N, const int *__restrict__ rowptr, const int *__restrict__ col, const T *__restrict__ A,
-           int64_t nbp, const unsigned char *__restrict__ fixmask, const T *__restrict__ z,
-           const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, SolverScalars *scal,
-           double *partials)
+// DIFF = false (FP64 path): q = A p with the assembled blocks of MDK.
+// DIFF = true  (FP32 path): q_i = sum_j m_ij p_j + sum_j S_ij (p_j - p_i), S = dampingB h^2 K
+// with zero diagonal blocks.  The stiffness then annihilates translations exactly
+// and its FP32 rounding is relative to the local differences p_j - p_i, not to |p|:
+// an assembled FP32 copy of M + h^2 K loses the mass (~1e-8) under the rounding of
+// the stiffness (~1e-2 x 6e-8) in the smooth, slowly converging modes.
+template <class T, bool DIFF>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const int *__restrict__ srow,
+           const int *__restrict__ slen, const int *__restrict__ scol, const T *__restrict__ A,
+           const T *__restrict__ Mm, int64_t nbs, const unsigned char *__restrict__ fixmask, const T *__restrict__ z,
+           const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, SolverScalars *scal, double *partials)
 {
     __shared__ double lds[2 * (WG / 64) + 2];
     if (scal->done) return;
@@ -158,49 +162,50 @@ k_pcg_spmv(int k, int N, const int *__restrict__ rowptr, const int *__restrict__
         return;
     }
     const double beta = restart_beta0 ? 0.0 : rzk / prev[0];
-    const int lane = threadIdx.x % LPR;
-    const int sub = threadIdx.x / LPR;
-    constexpr int RPP = WG / LPR;   // rows per pass
-    int rb, re;
-    wg_rows(N, &rb, &re);
+    const T betaT = (T)beta;   // p_new = z + beta p is formed in T, identically everywhere
+    const int sl = wave_slice();
+    const int lane = threadIdx.x & 63;
+    const int g = sl * eolc::kSlice + lane;
     double dot = 0.0;
-    for (int base = rb; base < re; base += RPP) {
-        const int i = base + sub;
+    if (sl < nslices && g < N) {
+        const int row = srow[g], len = slen[g];
+        const int base = slice_off[sl] + lane;
+        const T *__restrict__ a = A + base;
+        const int *__restrict__ cl = scol + base;
+        const T ui0 = fma(betaT, pold[3 * row], z[3 * row]);
+        const T ui1 = fma(betaT, pold[3 * row + 1], z[3 * row + 1]);
+        const T ui2 = fma(betaT, pold[3 * row + 2], z[3 * row + 2]);
         double y0 = 0, y1 = 0, y2 = 0;
-        if (i < re) {
-            const int kb = rowptr[i], ke = rowptr[i + 1];
-            for (int kk = kb + lane; kk < ke; kk += LPR) {
-                const int j = col[kk];
-                double zj0, zj1, zj2, pj0, pj1, pj2;
-                load3(z, j, zj0, zj1, zj2);
-                load3(pold, j, pj0, pj1, pj2);
-                // p_new_j formed exactly as row j forms (and stores) it below
-                const double u0 = (double)(T)fma(beta, pj0, zj0);
-                const double u1 = (double)(T)fma(beta, pj1, zj1);
-                const double u2 = (double)(T)fma(beta, pj2, zj2);
-                const T *a = A + kk;
-                y0 += (double)a[0 * nbp] * u0 + (double)a[1 * nbp] * u1 + (double)a[2 * nbp] * u2;
-                y1 += (double)a[3 * nbp] * u0 + (double)a[4 * nbp] * u1 + (double)a[5 * nbp] * u2;
-                y2 += (double)a[6 * nbp] * u0 + (double)a[7 * nbp] * u1 + (double)a[8 * nbp] * u2;
+#pragma unroll 1
+        for (int kk = 0; kk < len; ++kk) {
+            const int e = kk * eolc::kSlice;
+            const int j = cl[e];
+            const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
+            const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
+            const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+            if constexpr (DIFF) {
+                const T m = Mm[base + e];
+                const T d0 = u0 - ui0, d1 = u1 - ui1, d2 = u2 - ui2;
+                const T t0 = a[0 * nbs + e] * d0 + a[1 * nbs + e] * d1 + a[2 * nbs + e] * d2;
+                const T t1 = a[3 * nbs + e] * d0 + a[4 * nbs + e] * d1 + a[5 * nbs + e] * d2;
+                const T t2 = a[6 * nbs + e] * d0 + a[7 * nbs + e] * d1 + a[8 * nbs + e] * d2;
+                y0 += (double)fma(m, u0, t0);
+                y1 += (double)fma(m, u1, t1);
+                y2 += (double)fma(m, u2, t2);
+            } else {
+                y0 += (double)a[0 * nbs + e] * u0 + (double)a[1 * nbs + e] * u1 + (double)a[2 * nbs + e] * u2;
+                y1 += (double)a[3 * nbs + e] * u0 + (double)a[4 * nbs + e] * u1 + (double)a[5 * nbs + e] * u2;
+                y2 += (double)a[6 * nbs + e] * u0 + (double)a[7 * nbs + e] * u1 + (double)a[8 * nbs + e] * u2;
             }
         }
-        y0 = row_sum<LPR>(y0);
-        y1 = row_sum<LPR>(y1);
-        y2 = row_sum<LPR>(y2);
-        if (i < re && lane == 0) {
-            const unsigned char fm = fixmask[i];
-            if (fm & 1) y0 = 0;
-            if (fm & 2) y1 = 0;
-            if (fm & 4) y2 = 0;
-            double zi0, zi1, zi2, pi0, pi1, pi2;
-            load3(z, i, zi0, zi1, zi2);
-            load3(pold, i, pi0, pi1, pi2);
-            const T u0 = (T)fma(beta, pi0, zi0), u1 = (T)fma(beta, pi1, zi1), u2 = (T)fma(beta, pi2, zi2);
-            pnew[3 * i] = u0; pnew[3 * i + 1] = u1; pnew[3 * i + 2] = u2;
-            const T q0 = (T)y0, q1 = (T)y1, q2 = (T)y2;
-            q[3 * i] = q0; q[3 * i + 1] = q1; q[3 * i + 2] = q2;
-            dot += ((double)u0 * (double)q0 + (double)u1 * (double)q1) + (double)u2 * (double)q2;
-        }
+        const unsigned char fm = fixmask[row];
+        if (fm & 1) y0 = 0;
+        if (fm & 2) y1 = 0;
+        if (fm & 4) y2 = 0;
+        pnew[3 * row] = ui0; pnew[3 * row + 1] = ui1; pnew[3 * row + 2] = ui2;
+        const T q0 = (T)y0, q1 = (T)y1, q2 = (T)y2;
+        q[3 * row] = q0; q[3 * row + 1] = q1; q[3 * row + 2] = q2;
+        dot = ((double)ui0 * (double)q0 + (double)ui1 * (double)q1) + (double)ui2 * (double)q2;
     }
     double v[1] = {dot};
     write_partials<1>(v, PA(partials), lds);
@@ -213,7 +218,7 @@ k_pcg_spmv(int k, int N, const int *__restrict__ rowptr, const int *__restrict__
 // ---------------------------------------------------------------------------
 // TX is the solution accumulator (double in FP64 mode, float in mixed mode).
 template <class T, class TX>
-__global__ void __launch_bounds__(WG)
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, const T *__restrict__ q,
              TX *__restrict__ x, T *__restrict__ r, T *__restrict__ z, SolverScalars *scal, double *partials)
 {
@@ -231,27 +236,24 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
         return;
     }
     const double alpha = rz[0] / pq[0];
+    const T alphaT = (T)alpha;
     int rb, re;
     wg_rows(N, &rb, &re);
     double srz = 0.0, srr = 0.0;
     for (int i = rb + threadIdx.x; i < re; i += WG) {
-        double p0, p1, p2, q0, q1, q2, r0, r1, r2;
-        load3(pnew, i, p0, p1, p2);
-        load3(q, i, q0, q1, q2);
-        load3(r, i, r0, r1, r2);
-        x[3 * i] += (TX)(alpha * p0);
-        x[3 * i + 1] += (TX)(alpha * p1);
-        x[3 * i + 2] += (TX)(alpha * p2);
-        const T rn0 = (T)fma(-alpha, q0, r0), rn1 = (T)fma(-alpha, q1, r1), rn2 = (T)fma(-alpha, q2, r2);
-        r[3 * i] = rn0; r[3 * i + 1] = rn1; r[3 * i + 2] = rn2;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[3 * i + c] += (TX)(alphaT * pnew[3 * i + c]);
+        const T r0 = fma(-alphaT, q[3 * i], r[3 * i]);
+        const T r1 = fma(-alphaT, q[3 * i + 1], r[3 * i + 1]);
+        const T r2 = fma(-alphaT, q[3 * i + 2], r[3 * i + 2]);
+        r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
         const T *d = D + i;
-        const double s0 = rn0, s1 = rn1, s2 = rn2;
-        const T z0 = (T)((double)d[0 * N] * s0 + (double)d[1 * N] * s1 + (double)d[2 * N] * s2);
-        const T z1 = (T)((double)d[3 * N] * s0 + (double)d[4 * N] * s1 + (double)d[5 * N] * s2);
-        const T z2 = (T)((double)d[6 * N] * s0 + (double)d[7 * N] * s1 + (double)d[8 * N] * s2);
+        const T z0 = d[0 * N] * r0 + d[1 * N] * r1 + d[2 * N] * r2;
+        const T z1 = d[3 * N] * r0 + d[4 * N] * r1 + d[5 * N] * r2;
+        const T z2 = d[6 * N] * r0 + d[7 * N] * r1 + d[8 * N] * r2;
         z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
-        srz += (s0 * (double)z0 + s1 * (double)z1) + s2 * (double)z2;
-        srr += (s0 * s0 + s1 * s1) + s2 * s2;
+        srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
+        srr += ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2;
     }
     double v[2] = {srz, srr};
     write_partials<2>(v, PB(partials, (k + 1) & 1), lds);
@@ -265,47 +267,40 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
 // ---------------------------------------------------------------------------
 // out = b - A x (FP64), masked at fixed DOFs; b may be null (0).  Partial
 // ||out||^2 -> partials (one array of G).
-template <int LPR>
-__global__ void __launch_bounds__(WG)
-k_resid64(int N, const int *__restrict__ rowptr, const int *__restrict__ col, const double *__restrict__ A,
-          int64_t nbp, const unsigned char *__restrict__ fixmask, const double *__restrict__ xin,
-          const double *__restrict__ b, double *__restrict__ out, double *partials)
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_resid64(int N, int nslices, const int *__restrict__ slice_off, const int *__restrict__ srow,
+          const int *__restrict__ slen, const int *__restrict__ scol, const double *__restrict__ A, int64_t nbs,
+          const unsigned char *__restrict__ fixmask, const double *__restrict__ xin, const double *__restrict__ b,
+          double *__restrict__ out, double *partials)
 {
     __shared__ double lds[2 * (WG / 64) + 2];
-    const int lane = threadIdx.x % LPR;
-    const int sub = threadIdx.x / LPR;
-    constexpr int RPP = WG / LPR;
-    int rb, re;
-    wg_rows(N, &rb, &re);
+    const int sl = wave_slice();
+    const int lane = threadIdx.x & 63;
+    const int g = sl * eolc::kSlice + lane;
     double acc = 0.0;
-    for (int base = rb; base < re; base += RPP) {
-        const int i = base + sub;
+    if (sl < nslices && g < N) {
+        const int row = srow[g], len = slen[g];
+        const double *__restrict__ a = A + slice_off[sl] + lane;
+        const int *__restrict__ cl = scol + slice_off[sl] + lane;
         double y0 = 0, y1 = 0, y2 = 0;
-        if (i < re) {
-            const int kb = rowptr[i], ke = rowptr[i + 1];
-            for (int kk = kb + lane; kk < ke; kk += LPR) {
-                const int j = col[kk];
-                const double u0 = xin[3 * j], u1 = xin[3 * j + 1], u2 = xin[3 * j + 2];
-                const double *a = A + kk;
-                y0 += a[0 * nbp] * u0 + a[1 * nbp] * u1 + a[2 * nbp] * u2;
-                y1 += a[3 * nbp] * u0 + a[4 * nbp] * u1 + a[5 * nbp] * u2;
-                y2 += a[6 * nbp] * u0 + a[7 * nbp] * u1 + a[8 * nbp] * u2;
-            }
+#pragma unroll 1
+        for (int kk = 0; kk < len; ++kk) {
+            const int e = kk * eolc::kSlice;
+            const int j = cl[e];
+            const double u0 = xin[3 * j], u1 = xin[3 * j + 1], u2 = xin[3 * j + 2];
+            y0 += a[0 * nbs + e] * u0 + a[1 * nbs + e] * u1 + a[2 * nbs + e] * u2;
+            y1 += a[3 * nbs + e] * u0 + a[4 * nbs + e] * u1 + a[5 * nbs + e] * u2;
+            y2 += a[6 * nbs + e] * u0 + a[7 * nbs + e] * u1 + a[8 * nbs + e] * u2;
         }
-        y0 = row_sum<LPR>(y0);
-        y1 = row_sum<LPR>(y1);
-        y2 = row_sum<LPR>(y2);
-        if (i < re && lane == 0) {
-            double o0 = (b ? b[3 * i] : 0.0) - y0;
-            double o1 = (b ? b[3 * i + 1] : 0.0) - y1;
-            double o2 = (b ? b[3 * i + 2] : 0.0) - y2;
-            const unsigned char fm = fixmask[i];
-            if (fm & 1) o0 = 0;
-            if (fm & 2) o1 = 0;
-            if (fm & 4) o2 = 0;
-            out[3 * i] = o0; out[3 * i + 1] = o1; out[3 * i + 2] = o2;
-            acc += (o0 * o0 + o1 * o1) + o2 * o2;
-        }
+        double o0 = (b ? b[3 * row] : 0.0) - y0;
+        double o1 = (b ? b[3 * row + 1] : 0.0) - y1;
+        double o2 = (b ? b[3 * row + 2] : 0.0) - y2;
+        const unsigned char fm = fixmask[row];
+        if (fm & 1) o0 = 0;
+        if (fm & 2) o1 = 0;
+        if (fm & 4) o2 = 0;
+        out[3 * row] = o0; out[3 * row + 1] = o1; out[3 * row + 2] = o2;
+        acc = (o0 * o0 + o1 * o1) + o2 * o2;
     }
     double v[1] = {acc};
     write_partials<1>(v, partials, lds);
@@ -350,7 +345,7 @@ k_precond_init(int parity, int N, const T *__restrict__ D, const double *r64, T 
 // Block-Jacobi inverse of the diagonal blocks, with fixed DOFs replaced by
 // identity rows/columns.  kind 0: 3x3 blocks; kind 1: diagonal only (Jacobi).
 __global__ void __launch_bounds__(256)
-k_block_jacobi(int N, const int *__restrict__ diagpos, const double *__restrict__ A, int64_t nbp,
+k_block_jacobi(int N, const int *__restrict__ diagpos, const double *__restrict__ A, int64_t nbs,
                const unsigned char *__restrict__ fixmask, int kind, double *__restrict__ D64,
                float *__restrict__ D32)
 {
@@ -362,7 +357,7 @@ k_block_jacobi(int N, const int *__restrict__ diagpos, const double *__restrict_
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
         int r = c / 3, s = c % 3;
-        double val = A[c * nbp + k];
+        double val = A[c * nbs + k];
         if (kind == 1 && r != s) val = 0.0;
         if (((fm >> r) & 1) || ((fm >> s) & 1)) val = (r == s) ? 1.0 : 0.0;
         a[c] = val;
@@ -454,31 +449,23 @@ __global__ void k_fill(int n, double v, double *__restrict__ out)
 }
 
 template <class T>
-void launch_iteration(eolc_ctx *c, int k, int G, const T *A, const T *D, T *z, T *const p[2], T *q, T *r, T *x)
+void launch_iteration(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const T *D, T *z, T *const p[2], T *q, T *r,
+                      T *x)
 {
-    const int N = c->topo.N;
-    const int64_t nbp = c->topo.nbp;
+    const eolc::Topology &t = c->topo;
     const int pin = k & 1, pout = pin ^ 1;
-    if (c->lpr == 8)
-        hipLaunchKernelGGL((k_pcg_spmv<T, 8>), dim3(G), dim3(WG), 0, c->stream, k, N, c->rowptr, c->col, A, nbp,
-                           c->fixmask, z, p[pin], p[pout], q, c->scal, c->partials);
-    else
-        hipLaunchKernelGGL((k_pcg_spmv<T, 16>), dim3(G), dim3(WG), 0, c->stream, k, N, c->rowptr, c->col, A, nbp,
-                           c->fixmask, z, p[pin], p[pout], q, c->scal, c->partials);
-    hipLaunchKernelGGL((k_pcg_update<T, T>), dim3(G), dim3(WG), 0, c->stream, k, N, D, p[pout], q, x, r, z, c->scal,
+    constexpr bool diff = std::is_same<T, float>::value;
+    hipLaunchKernelGGL((k_pcg_spmv<T, diff>), dim3(G), dim3(WG), 0, c->stream, k, t.N, t.nslices, c->slice_off, c->srow,
+                       c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, p[pin], p[pout], q, c->scal, c->partials);
+    hipLaunchKernelGGL((k_pcg_update<T, T>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r, z, c->scal,
                        c->partials);
 }
 
 void launch_resid64(eolc_ctx *c, int G, const double *xin, const double *b, double *out)
 {
-    const int N = c->topo.N;
-    double *pr = c->partials + 5 * eolc::kMaxPartials;
-    if (c->lpr == 8)
-        hipLaunchKernelGGL((k_resid64<8>), dim3(G), dim3(WG), 0, c->stream, N, c->rowptr, c->col, c->A64, c->topo.nbp,
-                           c->fixmask, xin, b, out, pr);
-    else
-        hipLaunchKernelGGL((k_resid64<16>), dim3(G), dim3(WG), 0, c->stream, N, c->rowptr, c->col, c->A64, c->topo.nbp,
-                           c->fixmask, xin, b, out, pr);
+    const eolc::Topology &t = c->topo;
+    hipLaunchKernelGGL(k_resid64, dim3(G), dim3(WG), 0, c->stream, t.N, t.nslices, c->slice_off, c->srow, c->slen,
+                       c->scol, c->A64, t.nbs, c->fixmask, xin, b, out, c->partials + 5 * eolc::kMaxPartials);
 }
 
 }  // namespace
@@ -487,10 +474,9 @@ namespace eolc {
 
 int pcg_grid(int N)
 {
-    int g = (N + 2 * (WG / 16) - 1) / (2 * (WG / 16));   // at least two row passes per workgroup
-    if (g > kPcgGrid) g = kPcgGrid;
+    const int nslices = (N + kSlice - 1) / kSlice;
+    int g = (nslices + WG / 64 - 1) / (WG / 64);
     g = (g + 7) & ~7;
-    if (g < 8) g = 8;
     return g;
 }
 
@@ -498,7 +484,7 @@ hipError_t pcg_setup(eolc_ctx *c, bool /*mixed*/, int precond_kind)
 {
     int N = c->topo.N;
     hipLaunchKernelGGL(k_block_jacobi, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, c->diagpos, c->A64,
-                       c->topo.nbp, c->fixmask, precond_kind, c->D64, c->D32);
+                       c->topo.nbs, c->fixmask, precond_kind, c->D64, c->D32);
     return hipGetLastError();
 }
 
@@ -541,8 +527,11 @@ hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters)
 {
     const int G = pcg_grid(c->topo.N);
     for (int k = k0; k < k0 + iters; ++k) {
-        if (mixed) launch_iteration<float>(c, k, G, c->A32, c->D32, c->z32, c->p32, c->q32, c->r32, c->xl32);
-        else launch_iteration<double>(c, k, G, c->A64, c->D64, c->z64, c->p64, c->q64, c->r64, c->xs64);
+        if (mixed)
+            launch_iteration<float>(c, k, G, c->S32, c->M32, c->D32, c->z32, c->p32, c->q32, c->r32, c->xl32);
+        else
+            launch_iteration<double>(c, k, G, c->A64, (const double *)nullptr, c->D64, c->z64, c->p64, c->q64, c->r64,
+                                     c->xs64);
     }
     return hipGetLastError();
 }
@@ -610,26 +599,16 @@ hipError_t time_spmv(eolc_ctx *c, bool mixed, int reps, float *ms)
     hipLaunchKernelGGL(k_fill, dim3(blocks_for(4 * kMaxPartials)), dim3(256), 0, c->stream, 4 * kMaxPartials, 1.0,
                        c->partials + kMaxPartials);
     hipEventRecord(a, c->stream);
+    const eolc::Topology &t = c->topo;
     for (int r = 0; r < reps; ++r) {
-        if (mixed) {
-            if (c->lpr == 8)
-                hipLaunchKernelGGL((k_pcg_spmv<float, 8>), dim3(G), dim3(WG), 0, c->stream, 1, N, c->rowptr, c->col,
-                                   c->A32, c->topo.nbp, c->fixmask, c->z32, c->p32[1], c->p32[0], c->q32, c->scal,
-                                   c->partials);
-            else
-                hipLaunchKernelGGL((k_pcg_spmv<float, 16>), dim3(G), dim3(WG), 0, c->stream, 1, N, c->rowptr, c->col,
-                                   c->A32, c->topo.nbp, c->fixmask, c->z32, c->p32[1], c->p32[0], c->q32, c->scal,
-                                   c->partials);
-        } else {
-            if (c->lpr == 8)
-                hipLaunchKernelGGL((k_pcg_spmv<double, 8>), dim3(G), dim3(WG), 0, c->stream, 1, N, c->rowptr, c->col,
-                                   c->A64, c->topo.nbp, c->fixmask, c->z64, c->p64[1], c->p64[0], c->q64, c->scal,
-                                   c->partials);
-            else
-                hipLaunchKernelGGL((k_pcg_spmv<double, 16>), dim3(G), dim3(WG), 0, c->stream, 1, N, c->rowptr, c->col,
-                                   c->A64, c->topo.nbp, c->fixmask, c->z64, c->p64[1], c->p64[0], c->q64, c->scal,
-                                   c->partials);
-        }
+        if (mixed)
+            hipLaunchKernelGGL((k_pcg_spmv<float, true>), dim3(G), dim3(WG), 0, c->stream, 1, N, t.nslices,
+                               c->slice_off, c->srow, c->slen, c->scol, c->S32, c->M32, t.nbs, c->fixmask, c->z32,
+                               c->p32[1], c->p32[0], c->q32, c->scal, c->partials);
+        else
+            hipLaunchKernelGGL((k_pcg_spmv<double, false>), dim3(G), dim3(WG), 0, c->stream, 1, N, t.nslices,
+                               c->slice_off, c->srow, c->slen, c->scol, c->A64, (const double *)nullptr, t.nbs,
+                               c->fixmask, c->z64, c->p64[1], c->p64[0], c->q64, c->scal, c->partials);
     }
     hipEventRecord(b, c->stream);
     e = hipEventSynchronize(b);

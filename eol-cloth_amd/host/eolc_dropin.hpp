@@ -1,0 +1,167 @@
+// Drop-in C++ classes with the reference's interfaces for solver "none", built
+// on the eolc C ABI.  Header-only; link against libeolc.so.
+//
+//   eolc::Forces             Forces.h:26-45   -- fill(mesh, material, grav, h) -> f, M, MDK
+//   eolc::GeneralizedSolver  GeneralizedSolver.h:12-32 -- velocitySolve(...) for solver "none"
+//   eolc::CD / eolc::CD2     Collisions.h:9-11 -- proximity queries into btc::Collision-like records
+//
+// The reference's types (ArcSim Mesh, Eigen vectors and sparse matrices) are not
+// available here, so the mesh is passed as the flat arrays those types hold
+// (MeshArrays) and matrices come back as scalar CSR (CsrMatrix).  A reference
+// build replaces the bodies of its Forces::fill / velocitySolve / CD2 by calls to
+// these classes after copying Node::x, Vert::u, Face::v[] and Edge::n[]/adjf[] into
+// a MeshArrays (see INTEGRATION.md for the exact glue).
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/eolc.h"
+
+namespace eolc {
+
+struct Material {           // Cloth.h:28-35
+    double density = 0.05, e = 50.0, nu = 0.01, beta = 1e-5, dampingA = 0.0, dampingB = 1.0;
+};
+
+struct MeshArrays {         // what Forces::fill reads from the ArcSim mesh
+    std::vector<double> x;  // 3N  Node::x
+    std::vector<double> v;  // 3N  Node::v
+    std::vector<double> X;  // 2N  Vert::u (material space)
+    std::vector<int> tri;   // 3F  Face::v[k]->node->index
+    std::vector<int> hinge; // 4H  (n[0], n[1], apex of adjf[0], apex of adjf[1]) of interior edges
+    int N() const { return (int)(x.size() / 3); }
+};
+
+struct CsrMatrix {
+    int n = 0;
+    std::vector<int64_t> rowptr;
+    std::vector<int> col;
+    std::vector<double> val;
+};
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string &m) : std::runtime_error(m), code(code) {}
+    int code;
+};
+
+// One GPU context shared by the classes below (one per host thread).
+class Device {
+public:
+    explicit Device(int device = 0)
+    {
+        int rc = eolc_create(device, &ctx_);
+        if (rc != EOLC_OK) throw Error(rc, "eolc_create failed");
+    }
+    ~Device() { eolc_destroy(ctx_); }
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+    eolc_ctx *ctx() const { return ctx_; }
+    void check(int rc) const
+    {
+        if (rc != EOLC_OK) throw Error(rc, eolc_last_error(ctx_));
+    }
+    // call after every remesh (topology change); cheap to skip otherwise
+    void setTopology(const MeshArrays &m)
+    {
+        check(eolc_set_topology(ctx_, m.N(), m.X.data(), (int)(m.tri.size() / 3), m.tri.data(),
+                                (int)(m.hinge.size() / 4), m.hinge.data()));
+    }
+
+private:
+    eolc_ctx *ctx_ = nullptr;
+};
+
+// Forces: f, M and MDK = M + dampingB h^2 K of the current state (Forces.cpp:912-930).
+class Forces {
+public:
+    explicit Forces(Device &d) : dev_(d) {}
+    std::vector<double> f;
+    CsrMatrix M, MDK;
+    int EoL_cutoff = 0;
+
+    void fill(const MeshArrays &mesh, const Material &mat, const double grav[3], double h, bool downloadMatrices = true)
+    {
+        dev_.check(eolc_set_material(dev_.ctx(), mat.density, mat.e, mat.nu, mat.beta, mat.dampingA, mat.dampingB));
+        dev_.check(eolc_upload_state(dev_.ctx(), mesh.x.data(), mesh.v.data()));
+        dev_.check(eolc_forces_fill(dev_.ctx(), grav, h));
+        f.resize(mesh.x.size());
+        dev_.check(eolc_download_forces(dev_.ctx(), f.data()));
+        EoL_cutoff = (int)mesh.x.size();
+        if (downloadMatrices) {
+            download(EOLC_MAT_M, M);
+            download(EOLC_MAT_MDK, MDK);
+        }
+    }
+
+private:
+    void download(int which, CsrMatrix &A)
+    {
+        int64_t nnz = 0;
+        int rc = eolc_download_csr(dev_.ctx(), which, nullptr, nullptr, nullptr, 0, &nnz);
+        if (rc != EOLC_ERR_OVERFLOW && rc != EOLC_OK) dev_.check(rc);
+        A.n = (int)f.size();
+        A.rowptr.resize(A.n + 1);
+        A.col.resize(nnz);
+        A.val.resize(nnz);
+        dev_.check(eolc_download_csr(dev_.ctx(), which, A.rowptr.data(), A.col.data(), A.val.data(), nnz, &nnz));
+    }
+    Device &dev_;
+};
+
+// GeneralizedSolver::velocitySolve for solver "none" (GeneralizedSolver.cpp:107-172):
+// MDK v = M v_old + h f with v[fixed] = 0.99 v_old[fixed] + scripted (the QP's
+// single-DOF equality rows, Constraints.cpp:114-119).  Uses the MDK, M and f of the
+// last Forces::fill on the same Device; v holds v_old on entry and the solution on exit.
+class GeneralizedSolver {
+public:
+    explicit GeneralizedSolver(Device &d) : dev_(d) {}
+    int mode = EOLC_SOLVE_FP64_BJ;
+    double tol = 1e-8;
+    int maxit = 100000;
+    eolc_solve_stats stats{};
+
+    bool velocitySolve(const std::vector<int> &fixedDofs, const std::vector<double> &fixedVals, std::vector<double> &v)
+    {
+        int rc = eolc_velocity_solve(dev_.ctx(), (int)fixedDofs.size(), fixedDofs.data(), fixedVals.data(), tol, maxit,
+                                     mode, &stats);
+        std::vector<double> x(v.size());
+        dev_.check(eolc_download_state(dev_.ctx(), x.data(), v.data()));
+        return rc == EOLC_OK;
+    }
+
+private:
+    Device &dev_;
+};
+
+struct Obstacles {          // Obstacles / Points / Box (Points.h, Box.h)
+    double threshold = 5e-3;
+    std::vector<double> pts, nrm;   // 3P each
+    std::vector<double> whd;        // 3B box dimensions
+    std::vector<double> E;          // 16B column-major 4x4 frames
+};
+
+// CD2 (Collisions.cpp:55-77) and CD (Collisions.cpp:11-53): append to cls.
+inline void collide(Device &dev, const Obstacles &obs, bool eolPoints, std::vector<eolc_collision> &cls)
+{
+    const int P = (int)(obs.pts.size() / 3), B = (int)(obs.whd.size() / 3);
+    int count = 0;
+    std::vector<eolc_collision> out(1024);
+    for (;;) {
+        int rc = eolc_collide(dev.ctx(), obs.threshold, P, obs.pts.data(), obs.nrm.data(), B, obs.whd.data(),
+                              obs.E.data(), eolPoints ? 1 : 0, 8, 12, out.data(), (int)out.size(), &count);
+        if (rc == EOLC_ERR_OVERFLOW && count > (int)out.size()) {
+            out.resize(count);
+            continue;
+        }
+        dev.check(rc);
+        break;
+    }
+    cls.insert(cls.end(), out.begin(), out.begin() + count);
+}
+inline void CD2(Device &dev, const Obstacles &obs, std::vector<eolc_collision> &cls) { collide(dev, obs, false, cls); }
+inline void CD(Device &dev, const Obstacles &obs, std::vector<eolc_collision> &cls) { collide(dev, obs, true, cls); }
+
+}  // namespace eolc
