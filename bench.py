@@ -128,11 +128,11 @@ def main():
             ms = ctx.time_spmv(prec, 10)
             log(f"spmv fp{prec}: {ms * 1e3:.1f} us per launch")
         import torch
-        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
-        b.copy_(a)
+        torch.mul(a, 2.0, out=b)   # one elementwise kernel: reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
-        log("copy of 1 GiB done")
+        log("calibration kernel: 1 GiB read + 1 GiB written")
         ctx.close()
         return
 

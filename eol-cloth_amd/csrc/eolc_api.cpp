@@ -68,7 +68,23 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->r32); dfree(c->z32); dfree(c->q32); dfree(c->xl32);
     dfree(c->p32[0]); dfree(c->p32[1]);
     dfree(c->cedges); dfree(c->cd_unit);
+    eolc::LearnedState &L = c->lp;
+    dfree(L.tpos); dfree(L.w); dfree(L.wT); dfree(L.X);
+    for (auto &p : L.h) dfree(p);
+    for (auto &p : L.a) dfree(p);
+    for (auto &p : L.dpre) dfree(p);
+    dfree(L.o); dfree(L.dO); dfree(L.dinp); dfree(L.gpart); dfree(L.dpart); dfree(L.scal);
+    dfree(L.e); dfree(L.r); dfree(L.z); dfree(L.Az);
+    L.N = 0;
+    L.nbs = 0;
     c->alloc_bytes = 0;
+}
+
+void free_learned_weights(eolc_ctx *c)
+{
+    eolc::LearnedState &L = c->lp;
+    dfree(L.theta); dfree(L.grad); dfree(L.adam_m); dfree(L.adam_v); dfree(L.wq);
+    L.have_weights = false;
 }
 
 // Local blocks of a face (fillxMI / fillxxMI order) and of a hinge (fillxB / fillxxB).
@@ -247,6 +263,86 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b)
     return (double)ms;
 }
 
+// Per-topology buffers of the learned preconditioner, (re)allocated when the
+// topology changed since the last use.  tpos maps every SELL entry (i, j) to the
+// SELL entry of (j, i) (the pattern is symmetric); padding maps to -1.
+int learned_buffers(eolc_ctx *c)
+{
+    eolc::LearnedState &L = c->lp;
+    const Topology &t = c->topo;
+    if (L.N == t.N && L.nbs == t.nbs && L.tpos) return EOLC_OK;
+    const size_t N = (size_t)t.N;
+    std::vector<int> tpos((size_t)t.nbs, -1);
+    for (int i = 0; i < t.N; ++i)
+        for (int k = t.rowptr[i]; k < t.rowptr[i + 1]; ++k) {
+            const int k2 = find_block(t, t.col[k], i);
+            if (k2 < 0) return fail(c, EOLC_ERR_INVALID, "learned: block pattern is not symmetric");
+            tpos[(size_t)t.sellpos[k]] = t.sellpos[k2];
+        }
+    EOLC_HIP(upload(c, &L.tpos, tpos));
+    EOLC_HIP(dalloc(c, &L.w, (size_t)t.nbs));
+    EOLC_HIP(dalloc(c, &L.wT, (size_t)t.nbs));
+    EOLC_HIP(dalloc(c, &L.X, N * eolc::kLpFeat));
+    for (auto &p : L.h) EOLC_HIP(dalloc(c, &p, N * eolc::kLpHidden));
+    for (auto &p : L.a) EOLC_HIP(dalloc(c, &p, N * eolc::kLpHidden));
+    for (auto &p : L.dpre) EOLC_HIP(dalloc(c, &p, N * eolc::kLpHidden));
+    EOLC_HIP(dalloc(c, &L.o, N * eolc::kLpOut));
+    EOLC_HIP(dalloc(c, &L.dO, N * eolc::kLpOut));
+    EOLC_HIP(dalloc(c, &L.dinp, N * 2 * eolc::kLpHidden));
+    const size_t nblk = (N + 255) / 256;
+    int Gw = 0, rows = 0;
+    eolc::learned_wgrad_shape(t.N, &Gw, &rows);
+    L.gpart_n = std::max(nblk * (eolc::kLpHidden * eolc::kLpOut + eolc::kLpOut),
+                         (size_t)Gw * eolc::kLpLayerSize);
+    EOLC_HIP(dalloc(c, &L.gpart, L.gpart_n));
+    EOLC_HIP(dalloc(c, &L.dpart, nblk));
+    EOLC_HIP(dalloc(c, &L.scal, 8));
+    EOLC_HIP(hipMemset(L.scal, 0, 8 * sizeof(double)));
+    for (double **p : {&L.e, &L.r, &L.z, &L.Az}) EOLC_HIP(dalloc(c, p, 3 * N));
+    L.N = t.N;
+    L.nbs = t.nbs;
+    return EOLC_OK;
+}
+
+// Fixed DOFs -> device mask (and values) for the duration of one call.
+struct FixedScope {
+    eolc_ctx *c = nullptr;
+    int n = 0;
+    int *d_dofs = nullptr;
+    double *d_vals = nullptr;
+    int set(eolc_ctx *ctx, int nf, const int *dofs, const double *vals)
+    {
+        eolc_ctx *c = ctx;
+        this->c = ctx;
+        const int N = c->topo.N;
+        for (int i = 0; i < nf; ++i)
+            if (dofs[i] < 0 || dofs[i] >= 3 * N || (vals && !std::isfinite(vals[i])))
+                return fail(c, EOLC_ERR_INVALID, "fixed DOF index out of range or value non-finite");
+        if (nf <= 0) return EOLC_OK;
+        std::vector<double> zeros;
+        if (!vals) {
+            zeros.assign(nf, 0.0);
+            vals = zeros.data();
+        }
+        EOLC_HIP(hipMalloc((void **)&d_dofs, sizeof(int) * nf));
+        EOLC_HIP(hipMalloc((void **)&d_vals, sizeof(double) * nf));
+        EOLC_HIP(hipMemcpyAsync(d_dofs, dofs, sizeof(int) * nf, hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(hipMemcpyAsync(d_vals, vals, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+        n = nf;
+        EOLC_HIP(eolc::set_fixed(c, nf, d_dofs, d_vals, 1));
+        EOLC_HIP(hipStreamSynchronize(c->stream));   // host copy of vals may be a temporary
+        return EOLC_OK;
+    }
+    ~FixedScope()
+    {
+        if (!c) return;
+        if (n > 0) (void)eolc::set_fixed(c, n, d_dofs, d_vals, 0);
+        (void)hipStreamSynchronize(c->stream);
+        if (d_dofs) (void)hipFree(d_dofs);
+        if (d_vals) (void)hipFree(d_vals);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -285,6 +381,7 @@ void eolc_destroy(eolc_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_topology_buffers(c);
+    free_learned_weights(c);
     dfree(c->scal); dfree(c->partials); dfree(c->counter);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -502,6 +599,11 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
 {
     const int N = c->topo.N;
     const bool mixed = (mode == EOLC_SOLVE_MIXED_BJ || mode == EOLC_SOLVE_MIXED_LEARNED);
+    if (mode == EOLC_SOLVE_MIXED_LEARNED) {
+        if (!c->lp.have_weights) return fail(c, EOLC_ERR_INVALID, "velocity_solve: learned mode without weights");
+        const int rc = learned_buffers(c);
+        if (rc) return rc;
+    }
     // fixed DOFs -> device mask and values
     for (int i = 0; i < n_fixed; ++i)
         if (fixed_dof[i] < 0 || fixed_dof[i] >= 3 * N || !std::isfinite(fixed_val[i]))
@@ -517,6 +619,17 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     }
     const int precond = (mode == EOLC_SOLVE_FP64_JACOBI) ? 1 : 0;
     EOLC_HIP(eolc::pcg_setup(c, mixed, precond));
+    // Learned mode: P replaces the FP32 block-Jacobi blocks; after twice the
+    // iterations of the last converged block-Jacobi solve on this context (maxit
+    // when there was none) the solve falls back to block-Jacobi from its current
+    // iterate.
+    bool learned = (mode == EOLC_SOLVE_MIXED_LEARNED);
+    int used_fallback = 0;
+    int cap = maxit;
+    if (learned) {
+        EOLC_HIP(eolc::learned_precond(c));
+        if (c->bj_iters_hint > 0) cap = std::min(maxit, 2 * c->bj_iters_hint);
+    }
     const double delta = mixed ? 0.1 : 0.0;
     EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
     EOLC_HIP(eolc::pcg_init(c, mixed));
@@ -524,7 +637,16 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     int chunk = 8;
     double relres = NAN;
     for (;;) {
-        const int todo = std::min(chunk, maxit - k);
+        if (learned && k >= cap) {
+            // fallback: block-Jacobi blocks, true residual of the current iterate, beta = 0
+            learned = false;
+            used_fallback = 1;
+            EOLC_HIP(eolc::pcg_setup(c, mixed, 0));
+            EOLC_HIP(eolc::pcg_reliable_update(c));
+            EOLC_HIP(eolc::pcg_restart_inner(c, k));
+            EOLC_HIP(eolc::scal_restart(c, k));
+        }
+        const int todo = std::min(chunk, (learned ? cap : maxit) - k);
         if (todo <= 0) { status = EOLC_ERR_MAXIT; break; }
         EOLC_HIP(eolc::pcg_chunk(c, mixed, k, todo));
         EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
@@ -534,6 +656,12 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         relres = s.bnorm2 > 0 ? std::sqrt(s.rr / s.bnorm2) : 0.0;
         if (chunk < 256) chunk *= 2;
         if (!s.done) continue;
+        if (s.reason == 2 && learned) {
+            // breakdown under the learned blocks (P is SPD, so only from an
+            // indefinite MDK): the block-Jacobi fallback reports it
+            cap = k;
+            continue;
+        }
         if (s.reason == 2) { status = EOLC_ERR_NOT_SPD; break; }
         if (s.reason == 4) { status = EOLC_ERR_NONFINITE; break; }
         if (!mixed) { status = EOLC_OK; break; }
@@ -547,9 +675,13 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         if (!std::isfinite(s2.rmax2)) { status = EOLC_ERR_NONFINITE; break; }
         if (s2.rmax2 <= tol * tol * s2.bnorm2) { status = EOLC_OK; break; }
         if (k >= maxit) { status = EOLC_ERR_MAXIT; break; }
+        if (learned && k >= cap) continue;   // the fallback restarts from here
         EOLC_HIP(eolc::pcg_restart_inner(c, k));
         EOLC_HIP(eolc::scal_set(c, 0, 0, 0));
     }
+    if (status == EOLC_OK && !learned && used_fallback == 0 &&
+        (mode == EOLC_SOLVE_FP64_BJ || mode == EOLC_SOLVE_MIXED_BJ))
+        c->bj_iters_hint = k;
     if (mixed && status != EOLC_OK && status != EOLC_ERR_NOT_SPD) {
         // fold the last FP32 correction in before reporting
         EOLC_HIP(eolc::pcg_reliable_update(c));
@@ -565,7 +697,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         st->iters = c->hscal->iters;
         st->refinements = refinements;
         st->status = status;
-        st->used_fallback = 0;
+        st->used_fallback = used_fallback;
         st->relres = relres;
         st->true_relres = c->hscal->bnorm2 > 0 ? std::sqrt(c->hscal->true_rr / c->hscal->bnorm2) : 0.0;
     }
@@ -640,6 +772,120 @@ int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, co
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (st) *st = local;
     return rc;
+}
+
+// ---- learned block preconditioner ------------------------------------------
+int eolc_learned_set_weights(eolc_ctx *c, const float *theta, int n)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!theta || n != eolc::kLpParams) return fail(c, EOLC_ERR_INVALID, "learned_set_weights: need EOLC_LEARNED_NPARAMS values");
+    for (int k = 0; k < n; ++k)
+        if (!std::isfinite(theta[k])) return fail(c, EOLC_ERR_NONFINITE, "learned_set_weights: non-finite parameter");
+    EOLC_HIP(hipSetDevice(c->device));
+    eolc::LearnedState &L = c->lp;
+    if (!L.theta) {
+        EOLC_HIP(dalloc(c, &L.theta, (size_t)eolc::kLpParams));
+        EOLC_HIP(dalloc(c, &L.grad, (size_t)eolc::kLpParams));
+        EOLC_HIP(dalloc(c, &L.adam_m, (size_t)eolc::kLpParams));
+        EOLC_HIP(dalloc(c, &L.adam_v, (size_t)eolc::kLpParams));
+        EOLC_HIP(dalloc(c, &L.wq, (size_t)eolc::kLqSize));
+    }
+    EOLC_HIP(hipMemcpyAsync(L.theta, theta, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    EOLC_HIP(hipMemsetAsync(L.adam_m, 0, sizeof(float) * n, c->stream));
+    EOLC_HIP(hipMemsetAsync(L.adam_v, 0, sizeof(float) * n, c->stream));
+    EOLC_HIP(hipMemsetAsync(L.grad, 0, sizeof(float) * n, c->stream));
+    EOLC_HIP(eolc::learned_quantize(c));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    L.adam_t = 0;
+    L.have_weights = true;
+    return EOLC_OK;
+}
+
+int eolc_learned_get_weights(eolc_ctx *c, float *theta, int n)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!theta || n != eolc::kLpParams) return fail(c, EOLC_ERR_INVALID, "learned_get_weights: need EOLC_LEARNED_NPARAMS values");
+    if (!c->lp.have_weights) return fail(c, EOLC_ERR_INVALID, "learned_get_weights: no weights set");
+    EOLC_HIP(hipSetDevice(c->device));
+    EOLC_HIP(hipMemcpyAsync(theta, c->lp.theta, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    return EOLC_OK;
+}
+
+static int learned_ready(eolc_ctx *c, const char *who, int n_fixed, const int *fixed_dof)
+{
+    if (!c->have_forces) return fail(c, EOLC_ERR_INVALID, std::string(who) + ": call forces_fill first");
+    if (!c->lp.have_weights) return fail(c, EOLC_ERR_INVALID, std::string(who) + ": no weights set");
+    if (n_fixed < 0 || (n_fixed && !fixed_dof)) return fail(c, EOLC_ERR_INVALID, std::string(who) + ": bad fixed DOFs");
+    EOLC_HIP(hipSetDevice(c->device));
+    return learned_buffers(c);
+}
+
+int eolc_learned_infer(eolc_ctx *c, int n_fixed, const int *fixed_dof, float *out, float *P)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    int rc = learned_ready(c, "learned_infer", n_fixed, fixed_dof);
+    if (rc) return rc;
+    FixedScope fs;
+    if ((rc = fs.set(c, n_fixed, fixed_dof, nullptr))) return rc;
+    const size_t N = (size_t)c->topo.N;
+    EOLC_HIP(eolc::learned_precond(c));
+    std::vector<float> soa(P ? 9 * N : 0);
+    if (out) EOLC_HIP(hipMemcpyAsync(out, c->lp.o, sizeof(float) * 6 * N, hipMemcpyDeviceToHost, c->stream));
+    if (P) EOLC_HIP(hipMemcpyAsync(soa.data(), c->D32, sizeof(float) * 9 * N, hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    if (P)
+        for (size_t i = 0; i < N; ++i)
+            for (int k = 0; k < 9; ++k) P[9 * i + k] = soa[(size_t)k * N + i];
+    return EOLC_OK;
+}
+
+static int learned_upload_error(eolc_ctx *c, const double *e)
+{
+    const size_t N = (size_t)c->topo.N;
+    for (size_t k = 0; k < 3 * N; ++k)
+        if (!std::isfinite(e[k])) return fail(c, EOLC_ERR_NONFINITE, "learned: non-finite error vector");
+    EOLC_HIP(hipMemcpyAsync(c->lp.e, e, sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+    EOLC_HIP(eolc::learned_mask_error(c));
+    return EOLC_OK;
+}
+
+int eolc_learned_loss_grad(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *e, double *loss, float *grad)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!e) return fail(c, EOLC_ERR_INVALID, "learned_loss_grad: null error vector");
+    int rc = learned_ready(c, "learned_loss_grad", n_fixed, fixed_dof);
+    if (rc) return rc;
+    FixedScope fs;
+    if ((rc = fs.set(c, n_fixed, fixed_dof, nullptr))) return rc;
+    if ((rc = learned_upload_error(c, e))) return rc;
+    EOLC_HIP(eolc::learned_loss_grad(c));
+    double sc[8];
+    EOLC_HIP(hipMemcpyAsync(sc, c->lp.scal, sizeof(sc), hipMemcpyDeviceToHost, c->stream));
+    if (grad) EOLC_HIP(hipMemcpyAsync(grad, c->lp.grad, sizeof(float) * eolc::kLpParams, hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    if (loss) *loss = sc[4];
+    if (!std::isfinite(sc[4]) || !(sc[1] > 0.0)) return fail(c, EOLC_ERR_NONFINITE, "learned_loss_grad: r.e <= 0 or non-finite loss");
+    return EOLC_OK;
+}
+
+int eolc_learned_train(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *e, int steps, double lr,
+                       double *losses)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!e || steps <= 0 || !(lr > 0.0)) return fail(c, EOLC_ERR_INVALID, "learned_train: bad arguments");
+    int rc = learned_ready(c, "learned_train", n_fixed, fixed_dof);
+    if (rc) return rc;
+    FixedScope fs;
+    if ((rc = fs.set(c, n_fixed, fixed_dof, nullptr))) return rc;
+    if ((rc = learned_upload_error(c, e))) return rc;
+    for (int s = 0; s < steps; ++s) {
+        EOLC_HIP(eolc::learned_loss_grad(c));
+        if (losses) EOLC_HIP(hipMemcpyAsync(&losses[s], c->lp.scal + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(eolc::learned_adam(c, (float)lr));
+    }
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    return EOLC_OK;
 }
 
 int eolc_time_spmv(eolc_ctx *c, int precision, int reps, double *avg_ms)

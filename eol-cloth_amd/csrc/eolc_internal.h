@@ -65,6 +65,45 @@ template <class T> struct DArr {
     size_t n = 0;
 };
 
+// Learned block preconditioner (message-passing network over the vertex graph).
+constexpr int kLpFeat = 16, kLpHidden = 64, kLpOut = 6, kLpLayers = 3;
+// flat parameter vector: W_in[16][64], b_in[64], {W_l[128][64], b_l[64]} x 3, W_out[64][6], b_out[6]
+constexpr int kLpOffIn = 0;
+constexpr int kLpLayerSize = 2 * kLpHidden * kLpHidden + kLpHidden;   // 8256
+constexpr int kLpOffLayer0 = kLpFeat * kLpHidden + kLpHidden;          // 1088
+constexpr int kLpOffOut = kLpOffLayer0 + kLpLayers * kLpLayerSize;     // 25856
+constexpr int kLpParams = kLpOffOut + kLpHidden * kLpOut + kLpOut;     // 26246
+// bfloat16 copies used by the matrix units (unsigned short): W_in^T[64][16],
+// W_l^T[64][128] x 3 (forward), W_l[128][64] x 3 (backward), W_out[64][6]
+constexpr int kLqInT = 0;
+constexpr int kLqLayerT = kLpHidden * kLpFeat;                             // 1024
+constexpr int kLqLayer = kLqLayerT + kLpLayers * 2 * kLpHidden * kLpHidden; // 25600
+constexpr int kLqOut = kLqLayer + kLpLayers * 2 * kLpHidden * kLpHidden;    // 50176
+constexpr int kLqSize = kLqOut + kLpHidden * kLpOut;                        // 50560
+constexpr int kLpGradWG = 256;   // maximum workgroups of the weight-gradient kernels
+
+struct LearnedState {
+    bool have_weights = false;
+    int N = 0;                   // vertex count the buffers were sized for
+    int64_t nbs = 0;
+    int adam_t = 0;
+    float *theta = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;   // kLpParams
+    unsigned short *wq = nullptr;                                                     // kLqSize
+    int *tpos = nullptr;         // nbs: SELL entry of the transposed block, -1 for padding
+    float *w = nullptr, *wT = nullptr;   // nbs edge weights (row-normalised) and their transpose
+    unsigned short *X = nullptr;         // N x 16
+    unsigned short *h[kLpLayers + 1] = {nullptr, nullptr, nullptr, nullptr};   // N x 64
+    unsigned short *a[kLpLayers] = {nullptr, nullptr, nullptr};                // N x 64
+    unsigned short *dpre[2] = {nullptr, nullptr};                              // N x 64
+    float *o = nullptr, *dO = nullptr;   // N x 6
+    float *dinp = nullptr;               // N x 128
+    float *gpart = nullptr;              // weight-gradient partials
+    size_t gpart_n = 0;
+    double *dpart = nullptr;             // FP64 per-workgroup partial sums (ceil(N / 256))
+    double *scal = nullptr;              // 8: sum log t, r.e, z.r, z.Az, loss
+    double *e = nullptr, *r = nullptr, *z = nullptr, *Az = nullptr;   // 3N training vectors
+};
+
 }  // namespace eolc
 
 struct eolc_ctx {
@@ -113,6 +152,9 @@ struct eolc_ctx {
     std::vector<int> topo_tri;
     double *cd_unit = nullptr;  // 3N jitter values in (-1, 1)
     int cd_host_decisions = 0;  // angle tests re-decided on the host (diagnostic)
+    // learned preconditioner
+    eolc::LearnedState lp;
+    int bj_iters_hint = 0;      // iterations of the last converged block-Jacobi solve
 };
 
 namespace eolc {
@@ -134,6 +176,15 @@ hipError_t spmv_true_residual(eolc_ctx *c);
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int reps, float *ms);
+// learned preconditioner (k_learned.hip); buffers allocated by the API layer
+void learned_wgrad_shape(int N, int *G, int *rows);   // weight-gradient workgroups, rows per workgroup
+hipError_t learned_quantize(eolc_ctx *c);
+hipError_t learned_forward(eolc_ctx *c);
+hipError_t learned_precond(eolc_ctx *c);               // forward + P -> D32 (fixmask set)
+hipError_t learned_loss_grad(eolc_ctx *c);             // lp.e set and masked; loss -> lp.scal[4]
+hipError_t learned_adam(eolc_ctx *c, float lr);
+hipError_t learned_mask_error(eolc_ctx *c);            // lp.e <- e with fixed DOFs zeroed
+hipError_t scal_restart(eolc_ctx *c, int k);           // iteration k starts with beta = 0
 void build_collision_edges(eolc_ctx *c, int F, const int *tri);
 void build_jitter(std::vector<double> &u, int N);
 }  // namespace eolc

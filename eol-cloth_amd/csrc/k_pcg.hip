@@ -440,6 +440,14 @@ __global__ void k_scal_set(SolverScalars *s, double tol2, double delta2, int res
     s->reason = 0;
 }
 
+// The iteration k starts over with beta = 0 (preconditioner changed).
+__global__ void k_scal_restart(SolverScalars *s, int k)
+{
+    s->first_iter = k;
+    s->done = 0;
+    s->reason = 0;
+}
+
 inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
 __global__ void k_fill(int n, double v, double *__restrict__ out)
@@ -564,6 +572,12 @@ hipError_t pcg_finish(eolc_ctx *c, bool /*mixed*/)
     const int N = c->topo.N;
     hipLaunchKernelGGL(k_write_solution, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, N, c->xs64, c->fixmask,
                        c->vfix, c->v);
+    return hipGetLastError();
+}
+
+hipError_t scal_restart(eolc_ctx *c, int k)
+{
+    hipLaunchKernelGGL(k_scal_restart, dim3(1), dim3(1), 0, c->stream, c->scal, k);
     return hipGetLastError();
 }
 

@@ -27,8 +27,11 @@ EXPORTED_SYMBOLS = (
     "eolc_create", "eolc_destroy", "eolc_last_error", "eolc_set_topology", "eolc_set_material",
     "eolc_set_options", "eolc_upload_state", "eolc_download_state", "eolc_forces_fill",
     "eolc_download_forces", "eolc_download_csr", "eolc_velocity_solve", "eolc_step", "eolc_collide",
-    "eolc_time_spmv",
+    "eolc_time_spmv", "eolc_learned_set_weights", "eolc_learned_get_weights", "eolc_learned_infer",
+    "eolc_learned_loss_grad", "eolc_learned_train",
 )
+
+LEARNED_NPARAMS = 26246
 
 
 class SolveStats(C.Structure):
@@ -91,6 +94,13 @@ def lib():
                                    C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(CollisionRec), C.c_int, C.POINTER(C.c_int)]
         L.eolc_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.eolc_learned_set_weights.argtypes = [vp, C.c_void_p, C.c_int]
+        L.eolc_learned_get_weights.argtypes = [vp, C.c_void_p, C.c_int]
+        L.eolc_learned_infer.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.eolc_learned_loss_grad.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p,
+                                             C.POINTER(C.c_double), C.c_void_p]
+        L.eolc_learned_train.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                         C.c_void_p]
         _lib = L
     return _lib
 
@@ -230,3 +240,37 @@ class Context:
                 continue
             self._check(rc)
             return [buf[i] for i in range(cnt.value)]
+
+    # ---- learned block preconditioner
+    def learned_set_weights(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32).reshape(-1)
+        return self._check(lib().eolc_learned_set_weights(self._h, _ptr(th), th.size))
+
+    def learned_get_weights(self):
+        th = np.zeros(LEARNED_NPARAMS, dtype=np.float32)
+        self._check(lib().eolc_learned_get_weights(self._h, _ptr(th), th.size))
+        return th
+
+    def learned_infer(self, fixed_dof=()):
+        """(o[N, 6], P[N, 3, 3]) of the current system."""
+        fd = _i(fixed_dof)
+        o = np.zeros((self.N, 6), dtype=np.float32)
+        P = np.zeros((self.N, 3, 3), dtype=np.float32)
+        self._check(lib().eolc_learned_infer(self._h, fd.size, _ptr(fd) if fd.size else None,
+                                             _ptr(o), _ptr(P)))
+        return o, P
+
+    def learned_loss_grad(self, e, fixed_dof=()):
+        fd, e = _i(fixed_dof), _d(e)
+        loss = C.c_double(0)
+        g = np.zeros(LEARNED_NPARAMS, dtype=np.float32)
+        self._check(lib().eolc_learned_loss_grad(self._h, fd.size, _ptr(fd) if fd.size else None,
+                                                 _ptr(e), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def learned_train(self, e, steps, lr, fixed_dof=()):
+        fd, e = _i(fixed_dof), _d(e)
+        losses = np.zeros(int(steps))
+        self._check(lib().eolc_learned_train(self._h, fd.size, _ptr(fd) if fd.size else None,
+                                             _ptr(e), int(steps), float(lr), _ptr(losses)))
+        return losses

@@ -47,8 +47,10 @@ typedef enum {
 #define EOLC_SOLVE_FP64_BJ       0  /* FP64 3x3 block-Jacobi PCG */
 #define EOLC_SOLVE_MIXED_BJ      1  /* FP32 matrix/vectors, FP64 dots and true residual,
                                        reliable updates; block-Jacobi */
-#define EOLC_SOLVE_MIXED_LEARNED 2  /* as MIXED_BJ with the learned block correction;
-                                       falls back to MIXED_BJ at the iteration cap */
+#define EOLC_SOLVE_MIXED_LEARNED 2  /* as MIXED_BJ with the learned blocks P_i (weights
+                                       from eolc_learned_set_weights); falls back to
+                                       block-Jacobi after twice the iterations of the last
+                                       converged block-Jacobi solve on the context */
 #define EOLC_SOLVE_FP64_JACOBI   3  /* FP64 scalar-Jacobi PCG (Eigen's default) */
 
 /* matrices (eolc_download_csr) */
@@ -140,6 +142,39 @@ int eolc_step(eolc_ctx *ctx, const double grav[3], double h, int n_fixed_nodes,
 int eolc_collide(eolc_ctx *ctx, double threshold, int P, const double *pts, const double *nrm,
                  int B, const double *whd, const double *E, int eol_points, int box_npts,
                  int box_nedges, eolc_collision *out, int cap, int *count);
+
+/* ---- learned block preconditioner ----------------------------------------------
+ * No counterpart in the reference (its solver "none" aborts; Eigen's CG there uses
+ * a scalar-Jacobi preconditioner).  A message-passing network over the vertex
+ * graph of the current MDK emits per-vertex corrections T_i (lower-triangular,
+ * 0.5 tanh of 6 outputs) of the block-Jacobi factor:
+ *     P_i = G_i G_i^T,  G_i = chol(D_i^-1) (I + T_i)
+ * (SPD for every output; block-Jacobi at output 0).  Parameters: a flat float
+ * vector of EOLC_LEARNED_NPARAMS in the order W_in[16][64], b_in[64],
+ * {W_l[128][64], b_l[64]} for l = 0..2, W_out[64][6], b_out[6].
+ * All calls below use the MDK of the last forces_fill / step and treat the DOFs in
+ * fixed_dof as removed (identity rows/columns, as the solve does). */
+#define EOLC_LEARNED_NPARAMS 26246
+
+/* Set / read the parameters.  Setting them resets the Adam state. */
+int eolc_learned_set_weights(eolc_ctx *ctx, const float *theta, int n);
+int eolc_learned_get_weights(eolc_ctx *ctx, float *theta, int n);
+
+/* Network outputs o (6 per vertex, may be NULL) and blocks P (9 per vertex,
+ * row-major, may be NULL) for the current system. */
+int eolc_learned_infer(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, float *out, float *P);
+
+/* Training loss 1 - (z.r)^2 / ((z.A z)(r.e)), r = A e, z = P r, for an error
+ * vector e (3N, fixed entries ignored) and its gradient (EOLC_LEARNED_NPARAMS
+ * floats, may be NULL) with respect to the parameters. */
+int eolc_learned_loss_grad(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, const double *e,
+                           double *loss, float *grad);
+
+/* `steps` Adam steps (lr, beta1 0.9, beta2 0.999) on the current system and e;
+ * losses[k] (may be NULL) is the loss before step k.  The Adam state persists
+ * across calls until the weights are set again. */
+int eolc_learned_train(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, const double *e, int steps,
+                       double lr, double *losses);
 
 /* ---- measurement -------------------------------------------------------------
  * Average device time (HIP events on the context's stream) of one launch of the

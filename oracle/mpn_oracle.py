@@ -25,8 +25,9 @@ initial A-norm error.  Pinned by central finite differences in FP64
 (tests/test_learned_oracle.py) and used to check the GPU gradients.
 
 `quant=True` rounds the tensors the GPU stores in bfloat16 (weights used by the
-matrix units, activations, aggregates, features) the way the GPU does (round to
-nearest even), so the comparison with the GPU isolates FP32-vs-FP64 accumulation.
+matrix units, activations, aggregates, features and the backward pre-activation
+gradients) the way the GPU does (round to nearest even), so the comparison with
+the GPU isolates FP32-vs-FP64 accumulation.
 """
 import numpy as np
 
@@ -63,17 +64,10 @@ def nparams():
 
 
 def init_params(seed):
-    """Deterministic initialisation (splitmix64 stream), output layer zero."""
-    from eolc import scene
-    th = []
-    for name, shp in param_shapes():
-        n = int(np.prod(shp))
-        if name.startswith("b") or name == "W_out":
-            th.append(np.zeros(n))
-        else:
-            fan_in = shp[0]
-            th.append(scene.gaussian(seed + len(th), n, np.sqrt(2.0 / fan_in)))
-    return np.concatenate(th)
+    """Deterministic initialisation (splitmix64 stream), output layer zero; the
+    same values the library's host helper produces (as FP64)."""
+    from eolc import learned
+    return learned.init_weights(seed).astype(np.float64)
 
 
 def blocks(A, N):
@@ -169,16 +163,21 @@ def precond_from_outputs(o, B, N, fixmask):
     return P, aux
 
 
+def loss_from_outputs(o, A, B, N, fixmask, r, e):
+    """Loss for given network outputs o (N x 6); also z, A z and the factors."""
+    P, aux = precond_from_outputs(o, B, N, fixmask)
+    z = np.einsum("nij,nj->ni", P, r.reshape(N, 3)).reshape(-1)
+    Az = A @ z
+    zr, zAz, re = z @ r, z @ Az, r @ e
+    return 1.0 - zr * zr / (zAz * re), (P, aux, z, Az, zr, zAz, re)
+
+
 def loss_and_grad(theta, A, mass, N, fixmask, r, e, quant=False):
     """Loss and its gradient with respect to all parameters (FP64 backprop)."""
     X, W, B = features(A, mass, N, fixmask)
     o, cache, rows = forward(theta, X, W, N, quant)
-    P, aux = precond_from_outputs(o, B, N, fixmask)
+    loss, (P, aux, z, Az, zr, zAz, re) = loss_from_outputs(o, A, B, N, fixmask, r, e)
     r3 = r.reshape(N, 3)
-    z = np.einsum("nij,nj->ni", P, r3).reshape(-1)
-    Az = A @ z
-    zr, zAz, re = z @ r, z @ Az, r @ e
-    loss = 1.0 - zr * zr / (zAz * re)
     dz = -2.0 * zr / (zAz * re) * (r - (zr / zAz) * Az)
     dz3 = dz.reshape(N, 3)
     dO = np.zeros((N, NOUT))
@@ -196,6 +195,8 @@ def loss_and_grad(theta, A, mass, N, fixmask, r, e, quant=False):
     q = bf16 if quant else (lambda a: np.asarray(a, dtype=np.float64))
     Wm = {k: q(v) if k.startswith("W") else v for k, v in p.items()}
     g = {k: np.zeros_like(v) for k, v in p.items()}
+    if quant:
+        dO = dO.astype(np.float32).astype(np.float64)
     h = cache[f"h{NLAYER}"]
     g["W_out"] = h.T @ dO
     g["b_out"] = dO.sum(0)
@@ -203,7 +204,7 @@ def loss_and_grad(theta, A, mass, N, fixmask, r, e, quant=False):
     ri, rj, rw = rows
     for l in reversed(range(NLAYER)):
         hl1 = cache[f"h{l + 1}"]
-        dpre = dh * (hl1 > 0)
+        dpre = q(dh * (hl1 > 0))
         inp = np.hstack([cache[f"h{l}"], cache[f"a{l}"]])
         g[f"W{l}"] = inp.T @ dpre
         g[f"b{l}"] = dpre.sum(0)
@@ -211,7 +212,7 @@ def loss_and_grad(theta, A, mass, N, fixmask, r, e, quant=False):
         dh = dinp[:, :HID].copy()
         dagg = dinp[:, HID:]
         np.add.at(dh, rj, rw[:, None] * dagg[ri])
-    dpre = dh * (cache["h0"] > 0)
+    dpre = q(dh * (cache["h0"] > 0))
     g["W_in"] = cache["X"].T @ dpre
     g["b_in"] = dpre.sum(0)
     grad = np.concatenate([g[name].reshape(-1) for name, _ in param_shapes()])

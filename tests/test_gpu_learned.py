@@ -1,0 +1,187 @@
+"""Learned block preconditioner on the GPU against the FP64 oracle (mpn_oracle).
+
+Tolerances: the GPU stores features, activations, aggregates, backward
+pre-activation gradients and weight copies in bfloat16 and accumulates in FP32;
+the oracle is run with the same bfloat16 roundings (quant=True) in FP64, so the
+remaining differences come from FP32 accumulation and from values that fall on
+the other side of a bfloat16 rounding boundary.  Stated bars: network outputs
+within 1e-2 of the oracle's (relative to max |o|); the FP64 loss of the GPU's own
+outputs within 1e-9 relative of the oracle's loss of those outputs; the
+end-to-end loss within 1e-4 relative; gradient within 3e-2 relative (2-norm) per
+parameter tensor.  At zero output the blocks equal the block-Jacobi inverse to
+FP32 rounding (1e-6 relative)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import eolc
+import mpn_oracle as MO
+from conftest import GRAV, H
+from eolc import learned, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(ctx, sc):
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    ctx.upload_state(sc.x0, sc.v0)
+    ctx.forces_fill(GRAV, H)
+
+
+def reduced_system(ctx, sc):
+    """MDK with the fixed DOFs' rows/columns replaced by the identity's (pattern kept),
+    the per-vertex mass and the fixed mask, as the library sees them."""
+    A = ctx.download_csr(eolc.MAT_MDK).tocoo()
+    M = ctx.download_csr(eolc.MAT_M).tocsr()
+    N = sc.N
+    fd = learned.fixed_dofs(sc.fixed)
+    fixed = np.zeros(3 * N, bool)
+    fixed[fd] = True
+    val = A.data.copy()
+    hit = fixed[A.row] | fixed[A.col]
+    val[hit] = np.where(A.row[hit] == A.col[hit], 1.0, 0.0)
+    Ar = sp.csr_matrix((val, (A.row, A.col)), shape=A.shape)
+    mass = np.array([M[3 * i, 3 * i] for i in range(N)])
+    fixmask = np.zeros(N, dtype=np.int64)
+    fixmask[sc.fixed] = 7
+    return Ar, mass, fixmask, fd
+
+
+def test_zero_output_is_block_jacobi(ctx):
+    sc = scene.Scene(20, sigma_x=1e-3, sigma_v=0.05)
+    print("learned: zero output vs block-Jacobi, 20x20", flush=True)
+    setup(ctx, sc)
+    Ar, mass, fixmask, fd = reduced_system(ctx, sc)
+    ctx.learned_set_weights(learned.init_weights(3))
+    o, P = ctx.learned_infer(fd)
+    assert np.all(o == 0.0)
+    B = MO.blocks(Ar, sc.N)
+    for i in range(sc.N):
+        if fixmask[i]:
+            assert np.all(P[i] == 0.0)
+            continue
+        Di = np.linalg.inv(B[(i, i)])
+        assert np.max(np.abs(P[i] - Di)) <= 1e-6 * np.max(np.abs(Di))
+
+
+def random_theta(seed):
+    """Initial weights plus a non-zero output layer (outputs of order 1)."""
+    th = learned.init_weights(seed).astype(np.float64)
+    rng = np.random.default_rng(seed)
+    off = learned.nparams() - (learned.HID * learned.NOUT + learned.NOUT)
+    th[off:] = rng.normal(scale=0.05, size=th.size - off)
+    th[:off] += rng.normal(scale=0.01, size=off)
+    return th.astype(np.float32)
+
+
+@pytest.mark.parametrize("n", [20, 33])
+def test_forward_matches_oracle(ctx, n):
+    sc = scene.Scene(n, sigma_x=1e-3, sigma_v=0.05)
+    print(f"learned: forward parity {n}x{n}", flush=True)
+    setup(ctx, sc)
+    Ar, mass, fixmask, fd = reduced_system(ctx, sc)
+    th = random_theta(11)
+    ctx.learned_set_weights(th)
+    o, P = ctx.learned_infer(fd)
+    X, W, B = MO.features(Ar, mass, sc.N, fixmask)
+    oq, _, _ = MO.forward(th.astype(np.float64), X, W, sc.N, quant=True)
+    err = np.max(np.abs(o - oq)) / np.max(np.abs(oq))
+    print(f"  max|o| {np.max(np.abs(oq)):.3e} rel err vs quantised oracle {err:.3e}", flush=True)
+    assert err < 1e-2
+    # the blocks built from the GPU outputs
+    Po, _ = MO.precond_from_outputs(o.astype(np.float64), B, sc.N, fixmask)
+    sP = np.max(np.abs(Po))
+    assert np.max(np.abs(P - Po)) <= 1e-6 * sP
+
+
+def test_loss_and_gradient_match_oracle(ctx):
+    sc = scene.Scene(20, sigma_x=1e-3, sigma_v=0.05)
+    print("learned: loss/gradient parity 20x20", flush=True)
+    setup(ctx, sc)
+    Ar, mass, fixmask, fd = reduced_system(ctx, sc)
+    th = random_theta(5)
+    ctx.learned_set_weights(th)
+    rng = np.random.default_rng(9)
+    e = rng.normal(size=3 * sc.N)
+    e[fd] = 0.0
+    loss, g = ctx.learned_loss_grad(e, fd)
+    o_gpu, _ = ctx.learned_infer(fd)
+    r = Ar @ e
+    lo, go, _ = MO.loss_and_grad(th.astype(np.float64), Ar, mass, sc.N, fixmask, r, e, quant=True)
+    B = MO.blocks(Ar, sc.N)
+    l_same, _ = MO.loss_from_outputs(o_gpu.astype(np.float64), Ar, B, sc.N, fixmask, r, e)
+    print(f"  loss gpu {loss:.12f} oracle(gpu outputs) {l_same:.12f} oracle {lo:.12f}", flush=True)
+    assert abs(loss - l_same) <= 1e-9 * abs(l_same)
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    k = 0
+    for name, shp in MO.param_shapes():
+        m = int(np.prod(shp))
+        gg, gr = g[k:k + m].astype(np.float64), go[k:k + m]
+        k += m
+        rel = np.linalg.norm(gg - gr) / max(np.linalg.norm(gr), 1e-30)
+        print(f"  grad {name:6s} |g| {np.linalg.norm(gr):.3e} rel err {rel:.3e}", flush=True)
+        assert rel < 3e-2, name
+
+
+def test_training_lowers_loss_and_is_deterministic(ctx):
+    sc = scene.Scene(24, sigma_x=1e-3, sigma_v=0.05)
+    print("learned: 40 Adam steps on one 24x24 system, twice", flush=True)
+    setup(ctx, sc)
+    fd = learned.fixed_dofs(sc.fixed)
+    rng = np.random.default_rng(2)
+    e = rng.normal(size=3 * sc.N)
+    e[fd] = 0.0
+    out = []
+    for _ in range(2):
+        ctx.learned_set_weights(learned.init_weights(4))
+        losses = ctx.learned_train(e, 40, 1e-3, fd)
+        out.append((losses, ctx.learned_get_weights()))
+    losses = out[0][0]
+    print(f"  loss {losses[0]:.5f} -> {losses[-1]:.5f}", flush=True)
+    assert np.all(np.isfinite(losses)) and 0.0 < losses[0] <= 1.0
+    assert losses[-1] < losses[0]
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_learned_solve_reaches_tolerance_and_falls_back(ctx):
+    sc = scene.Scene(32, sigma_x=1e-3, sigma_v=0.05)
+    print("learned: solve with fallback 32x32", flush=True)
+    setup(ctx, sc)
+    fd = learned.fixed_dofs(sc.fixed)
+    vals = 0.99 * sc.v0[fd]
+    ctx.learned_set_weights(random_theta(8))
+    # a loose block-Jacobi solve leaves a small iteration hint ...
+    rc, st_bj = ctx.velocity_solve(fd, vals, tol=1e-2, mode=eolc.SOLVE_MIXED_BJ)
+    hint = st_bj.iters
+    ctx.upload_state(sc.x0, sc.v0)
+    ctx.forces_fill(GRAV, H)
+    # ... so a tight learned solve passes 2x that and finishes on block-Jacobi
+    rc, st = ctx.velocity_solve(fd, vals, tol=1e-9, mode=eolc.SOLVE_MIXED_LEARNED)
+    print(f"  hint {hint}, learned solve {st.as_dict()}", flush=True)
+    assert rc == eolc.OK and st.used_fallback == 1 and st.true_relres <= 1e-9
+    _, v = ctx.download_state()
+    ctx.upload_state(sc.x0, sc.v0)
+    ctx.forces_fill(GRAV, H)
+    rc, st64 = ctx.velocity_solve(fd, vals, tol=1e-12, mode=eolc.SOLVE_FP64_BJ)
+    _, v64 = ctx.download_state()
+    assert np.max(np.abs(v - v64)) <= 1e-6 * np.max(np.abs(v64))
+    # with a generous hint the learned blocks finish the solve themselves
+    ctx.upload_state(sc.x0, sc.v0)
+    ctx.forces_fill(GRAV, H)
+    rc, st2 = ctx.velocity_solve(fd, vals, tol=1e-9, mode=eolc.SOLVE_MIXED_LEARNED)
+    print(f"  learned solve after a tight block-Jacobi hint {st2.as_dict()}", flush=True)
+    assert rc == eolc.OK and st2.used_fallback == 0 and st2.true_relres <= 1e-9
+
+
+def test_learned_mode_requires_weights():
+    c = eolc.Context(0)
+    try:
+        sc = scene.Scene(8)
+        setup(c, sc)
+        rc, st = c.velocity_solve(learned.fixed_dofs(sc.fixed), np.zeros(12), mode=eolc.SOLVE_MIXED_LEARNED,
+                                  allow=(eolc.ERR_INVALID,))
+        assert rc == eolc.ERR_INVALID and "weights" in c.last_error()
+    finally:
+        c.close()
