@@ -9,13 +9,21 @@ rhs + velocity solve to 1e-8 relative residual (FP64, reduced system) + x += h v
 all on the device with the state resident in HBM.  Each step's solve is checked
 against the tolerance.
 
-The headline path is the mixed-precision PCG (FP32 matrix and vectors, FP64 dots
-and true residual with reliable updates); FP64 3x3 block-Jacobi PCG runs on the
-same initial state for comparison.  The noisy initial state makes the exact
-bending Hessian indefinite, so both paths assemble the SPD-projected hinge
-Hessians (EOLC_OPT_SPD_PROJECT_BENDING).
+The headline path is the learned-preconditioned mixed-precision PCG (FP32 matrix
+and vectors, FP64 dots and true residual with reliable updates, P_i from the
+message-passing network with weights eol-cloth_amd/eolc/weights/mpn_cloth.npy,
+block-Jacobi fallback); mixed block-Jacobi PCG and FP64 3x3 block-Jacobi PCG run
+from the same initial state for comparison.  Before timing, the SpMV launch
+variant of each precision is chosen by the library's autotuner on the first
+matrix.  The noisy initial state makes the exact bending Hessian indefinite, so
+every path assembles the SPD-projected hinge Hessians
+(EOLC_OPT_SPD_PROJECT_BENDING).
 
-Prints one JSON line (rank 0) with roofline and cpu_baseline objects.
+Prints one JSON line (rank 0) with roofline and cpu_baseline objects.  The
+roofline's `traffic` (HBM bytes per SpMV launch) is read from
+profiles/pmc_spmv_cloth1024.json, the calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE
+summary of `bench.py --profile-spmv`, when it was measured for the same kernel
+variant and size.
 """
 import argparse
 import json
@@ -92,6 +100,68 @@ def cpu_baseline(sc, iters_per_step, threads):
             "fill_s": t1 - t0, "iter_ms": t_iter * 1e3}
 
 
+WEIGHTS = os.path.join(ROOT, "eol-cloth_amd", "eolc", "weights", "mpn_cloth.npy")
+TRAFFIC = os.path.join(ROOT, "profiles", "pmc_spmv_cloth1024.json")
+
+
+def load_weights():
+    from eolc import learned
+    if os.path.exists(WEIGHTS):
+        th = np.load(WEIGHTS, allow_pickle=False).astype(np.float32)
+        if th.size == eolc.LEARNED_NPARAMS:
+            return th, os.path.relpath(WEIGHTS, ROOT)
+    return learned.init_weights(7), "initial weights (zero output = block-Jacobi)"
+
+
+def measured_traffic(kernel, variant, n):
+    """HBM bytes per launch from the committed counter summary, if it matches."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for k in t.get("kernels", []):
+        if k.get("kernel") == kernel and k.get("variant") == variant and k.get("n") == n:
+            return k.get("hbm_bytes")
+    return None
+
+
+def run_batched(steps, warmup, maxit):
+    """The many-cloth workload: 256 cloths, sides uniform in {16..128} quads, E
+    log-uniform in [1e2, 1e4] Pa, beta log-uniform in [1e-6, 1e-4], the single-cloth
+    noise, seed 1237; one persistent launch per solve, graph-captured step, solve
+    workgroup size from the batch autotuner.  Steps/s of the whole batch."""
+    bs = scene.BatchScene()
+    t = time.perf_counter()
+    ctx = eolc.Context(0)
+    bs.setup(ctx)
+    ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    setup_s = time.perf_counter() - t
+    best, ms = ctx.batch_autotune(GRAV, H, tol=TOL, maxit=maxit)
+    log(f"batched: {bs.B} cloths, {bs.N_real} vertices, set-up {setup_s:.1f} s, "
+        f"autotune variant {best} ms {np.round(ms, 2).tolist()}")
+    for _ in range(warmup):
+        ctx.batch_step(GRAV, H, tol=TOL, maxit=maxit)
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(steps):
+        rc, st = ctx.batch_step(GRAV, H, tol=TOL, maxit=maxit)
+        bad = [s.as_dict() for s in st if not (s.true_relres <= TOL)]
+        if rc != eolc.OK or bad:
+            raise RuntimeError(f"batched solve missed the tolerance: {bad[:1]}")
+        its.append([s.iters for s in st])
+    dt = (time.perf_counter() - t0) / steps
+    ctx.close()
+    its = np.array(its)
+    log(f"batched: {1 / dt:.3f} steps/s, iterations per cloth min {its.min()} median "
+        f"{int(np.median(its))} max {its.max()}")
+    return {"workload": f"{bs.B} cloths, sides {int(bs.sides.min())}..{int(bs.sides.max())} quads, "
+                        f"{bs.N_real} vertices", "steps_per_s": 1 / dt, "ms_per_step": dt * 1e3,
+            "cloth_steps_per_s": bs.B / dt, "iters_min": int(its.min()), "iters_median": int(np.median(its)),
+            "iters_max": int(its.max()), "solve_variant": best,
+            "variant_ms": [round(float(m), 3) for m in ms]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +172,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 block-Jacobi comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched many-cloth workload")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="counter runs: assemble once, 10 launches of each SpMV kernel and a 1 GiB "
                          "device copy for calibration, then exit")
@@ -121,44 +192,66 @@ def main():
     ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
     log(f"topology set-up {time.perf_counter() - t:.1f} s")
 
+    ctx.upload_state(sc.x0, sc.v0)
+    ctx.forces_fill(GRAV, H)
+    tuned = {}
+    for prec in (32, 64):
+        best, ms = ctx.autotune(prec, reps=10)
+        tuned[prec] = {"variant": best, "us": [round(float(m) * 1e3, 1) for m in ms]}
+        log(f"autotune fp{prec}: variant {best}, us per launch {tuned[prec]['us']}")
+
     if args.profile_spmv:
-        ctx.upload_state(sc.x0, sc.v0)
-        ctx.forces_fill(GRAV, H)
+        # every SpMV variant of both precisions (10 launches each, after the 10 of
+        # the autotuner), then the calibration copy: 1 GiB read + 1 GiB written
         for prec in (32, 64):
-            ms = ctx.time_spmv(prec, 10)
-            log(f"spmv fp{prec}: {ms * 1e3:.1f} us per launch")
-        import torch
-        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.mul(a, 2.0, out=b)   # one elementwise kernel: reads 1 GiB, writes 1 GiB
-        torch.cuda.synchronize()
-        log("calibration kernel: 1 GiB read + 1 GiB written")
+            for var in range(eolc.SPMV_VARIANTS):
+                ctx.set_spmv_variant(prec, var)
+                ms = ctx.time_spmv(prec, 10)
+                log(f"spmv fp{prec} variant {var}: {ms * 1e3:.1f} us per launch")
+        ms = ctx.stream_copy(1 << 30, 3)
+        log(f"calibration copy: 1 GiB read + 1 GiB written, {ms:.3f} ms per launch "
+            f"({2 * (1 << 30) / (ms * 1e-3) / 1e9:.0f} GB/s)")
         ctx.close()
         return
 
-    head_mode, head_name = eolc.SOLVE_MIXED_BJ, "mixed-bj"
-    t_head, st_head = run_path(ctx, sc, head_mode, args.steps, args.warmup, args.maxit)
-    log(f"{head_name}: {1 / t_head:.4f} steps/s, iters {[s['iters'] for s in st_head]}")
-    spmv32_ms = ctx.time_spmv(32, 20)
+    theta, wsrc = load_weights()
+    ctx.learned_set_weights(theta)
+    # the block-Jacobi runs go first: the learned solve falls back to block-Jacobi
+    # after twice the iterations of the last block-Jacobi solve on the context
     out_fp64 = None
     if not args.no_fp64:
         t64, st64 = run_path(ctx, sc, eolc.SOLVE_FP64_BJ, args.steps, args.warmup, args.maxit)
         log(f"fp64-bj: {1 / t64:.4f} steps/s, iters {[s['iters'] for s in st64]}")
         spmv64_ms = ctx.time_spmv(64, 20)
         out_fp64 = {"steps_per_s": 1 / t64, "iters": [s["iters"] for s in st64],
-                    "ms_per_step": t64 * 1e3, "spmv_ms": spmv64_ms,
-                    "speedup_of_headline": t64 / t_head}
+                    "ms_per_step": t64 * 1e3, "spmv_ms": spmv64_ms}
+    tbj, stbj = run_path(ctx, sc, eolc.SOLVE_MIXED_BJ, args.steps, args.warmup, args.maxit)
+    log(f"mixed-bj: {1 / tbj:.4f} steps/s, iters {[s['iters'] for s in stbj]}")
+    out_mbj = {"steps_per_s": 1 / tbj, "iters": [s["iters"] for s in stbj], "ms_per_step": tbj * 1e3}
+    head_mode, head_name = eolc.SOLVE_MIXED_LEARNED, "mixed-learned"
+    t_head, st_head = run_path(ctx, sc, head_mode, args.steps, args.warmup, args.maxit)
+    log(f"{head_name}: {1 / t_head:.4f} steps/s, iters {[s['iters'] for s in st_head]}, "
+        f"fallback {[s['used_fallback'] for s in st_head]}")
+    spmv32_ms = ctx.time_spmv(32, 20)
+    if out_fp64:
+        out_fp64["speedup_of_headline"] = t64 / t_head
+    out_mbj["speedup_of_headline"] = tbj / t_head
     nb = None
     # nb from the matrix pattern size: 9 scalars per block
     import ctypes as C
     nnz = C.c_int64(0)
     eolc.lib().eolc_download_csr(ctx._h, eolc.MAT_MDK, None, None, None, 0, C.byref(nnz))
     nb = nnz.value // 9
-    traffic = spmv_bytes(sc.N, nb, 4, True)
-    achieved = traffic / (spmv32_ms * 1e-3)
+    # algorithmic bytes over the SELL storage actually streamed (padding excluded)
+    alg = spmv_bytes(sc.N, nb, 4, True)
+    achieved = alg / (spmv32_ms * 1e-3)
+    kname = "k_pcg_spmv<float, true>"
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM, "traffic": None, "kernel": "k_pcg_spmv<float>",
-            "bytes_per_launch": traffic, "avg_launch_ms": spmv32_ms}
+            "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname, tuned[32]["variant"], args.n),
+            "kernel": kname, "variant": tuned[32]["variant"], "algorithmic_bytes": alg,
+            "avg_launch_ms": spmv32_ms}
+    ctx.close()
+    batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit)
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -174,17 +267,20 @@ def main():
         "config": {"workload": f"cloth {args.n}x{args.n} quads, noise 1e-3 m, one step = "
                                f"assemble + solve to 1e-8 + update", "path": head_name,
                    "vertices": sc.N, "triangles": sc.F, "hinges": sc.H, "blocks": nb,
-                   "bending_hessian": "spd-projected"},
+                   "bending_hessian": "spd-projected", "weights": wsrc},
         "iters": [s["iters"] for s in st_head],
+        "used_fallback": [s["used_fallback"] for s in st_head],
         "true_relres": [s["true_relres"] for s in st_head],
         "phase_ms": {k: float(np.mean([s[k] for s in st_head])) for k in
                      ("ms_assembly", "ms_rhs", "ms_solve", "ms_update")},
         "fp64_bj": out_fp64,
+        "mixed_bj": out_mbj,
+        "autotune": tuned,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(res), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

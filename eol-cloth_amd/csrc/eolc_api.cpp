@@ -77,6 +77,13 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(L.e); dfree(L.r); dfree(L.z); dfree(L.Az);
     L.N = 0;
     L.nbs = 0;
+    eolc::BatchState &S = c->batch;
+    dfree(S.d_inst); dfree(S.d_order); dfree(S.d_user_fixed); dfree(S.slice_part); dfree(S.stats); dfree(S.dstats);
+    if (S.graph) (void)hipGraphExecDestroy(S.graph);
+    S.graph = nullptr;
+    S.B = 0;
+    S.inst.clear(); S.order.clear(); S.fixed_nodes.clear(); S.user_fixed.clear();
+    dfree(c->e_face); dfree(c->beta_hinge);
     c->alloc_bytes = 0;
 }
 
@@ -100,7 +107,10 @@ int find_block(const Topology &t, int i, int j)
 }
 
 // Builds the BSR pattern and the gather maps.  Returns an error message or "".
-std::string build_topology(Topology &t, int N, int F, const int *tri, int H, const int *hinge)
+// seg: optional segment starts (ascending, multiples of 64, first 0); rows are
+// never reordered across a segment boundary (batched instances).
+std::string build_topology(Topology &t, int N, int F, const int *tri, int H, const int *hinge,
+                           const std::vector<int> *seg = nullptr)
 {
     t.N = N; t.F = F; t.H = H;
     for (int i = 0; i < 3 * F; ++i)
@@ -177,25 +187,35 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
     for (int64_t b = 0; b < t.nb; ++b)
         for (int e = bptr[b]; e < bptr[b + 1]; ++e)
             if ((blist[e] >> 1) < 6 * F) { t.has_tri[b] = 1; break; }
-    // SELL-C-sigma layout
+    // SELL-C-sigma layout: rows sorted by decreasing length inside windows of
+    // sigma = kSellSigma rows (never across a segment boundary), then cut into
+    // slices of C = 64.  A window of 4 slices separates the two row lengths of a
+    // checkerboard grid, so every lane of a slice has the same trip count.
     const int C = eolc::kSlice;
     t.nslices = (N + C - 1) / C;
     t.slice_off.assign(t.nslices + 1, 0);
     t.srow.assign(N, 0);
     t.slen.assign(N, 0);
+    std::vector<int> bounds = seg ? *seg : std::vector<int>{0};
+    bounds.push_back(N);
+    for (size_t si = 0; si + 1 < bounds.size(); ++si) {
+        for (int w0 = bounds[si]; w0 < bounds[si + 1]; w0 += eolc::kSellSigma) {
+            const int w1 = std::min(bounds[si + 1], w0 + eolc::kSellSigma);
+            std::vector<int> rows;
+            for (int r = w0; r < w1; ++r) rows.push_back(r);
+            std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) {
+                return (t.rowptr[a + 1] - t.rowptr[a]) > (t.rowptr[b + 1] - t.rowptr[b]);
+            });
+            for (int l = 0; l < (int)rows.size(); ++l) {
+                t.srow[w0 + l] = rows[l];
+                t.slen[w0 + l] = t.rowptr[rows[l] + 1] - t.rowptr[rows[l]];
+            }
+        }
+    }
     for (int sl = 0; sl < t.nslices; ++sl) {
         const int r0 = sl * C, r1 = std::min(N, r0 + C);
-        std::vector<int> rows;
-        for (int r = r0; r < r1; ++r) rows.push_back(r);
-        std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) {
-            return (t.rowptr[a + 1] - t.rowptr[a]) > (t.rowptr[b + 1] - t.rowptr[b]);
-        });
         int W = 0;
-        for (int l = 0; l < (int)rows.size(); ++l) {
-            t.srow[r0 + l] = rows[l];
-            t.slen[r0 + l] = t.rowptr[rows[l] + 1] - t.rowptr[rows[l]];
-            W = std::max(W, t.slen[r0 + l]);
-        }
+        for (int g = r0; g < r1; ++g) W = std::max(W, t.slen[g]);
         if ((int64_t)t.slice_off[sl] + (int64_t)W * C > (int64_t)INT32_MAX) return "too many SELL entries";
         t.slice_off[sl + 1] = t.slice_off[sl] + W * C;
     }
@@ -343,6 +363,12 @@ struct FixedScope {
     }
 };
 
+void drop_batch_graph(eolc_ctx *c)
+{
+    if (c->batch.graph) (void)hipGraphExecDestroy(c->batch.graph);
+    c->batch.graph = nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -390,7 +416,16 @@ void eolc_destroy(eolc_ctx *c)
 
 const char *eolc_last_error(const eolc_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const int *tri, int H, const int *hinge,
+                             const std::vector<int> *seg);
+
 int eolc_set_topology(eolc_ctx *c, int N, const double *X, int F, const int *tri, int H, const int *hinge)
+{
+    return set_topology_impl(c, N, X, F, tri, H, hinge, nullptr);
+}
+
+static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const int *tri, int H, const int *hinge,
+                             const std::vector<int> *seg)
 {
     if (!c) return EOLC_ERR_INVALID;
     if (N <= 0 || F < 0 || H < 0 || !X || (F && !tri) || (H && !hinge))
@@ -410,7 +445,7 @@ int eolc_set_topology(eolc_ctx *c, int N, const double *X, int F, const int *tri
     if (eolc::pcg_grid(N) > eolc::kMaxPartials)
         return fail(c, EOLC_ERR_INVALID, "set_topology: mesh too large for the PCG reduction buffers");
     Topology t;
-    std::string msg = build_topology(t, N, F, tri, H, hinge);
+    std::string msg = build_topology(t, N, F, tri, H, hinge, seg);
     if (!msg.empty()) return fail(c, EOLC_ERR_INVALID, "set_topology: " + msg);
     free_topology_buffers(c);
     c->have_topo = c->have_state = c->have_forces = false;
@@ -478,6 +513,7 @@ int eolc_set_material(eolc_ctx *c, double density, double e, double nu, double b
     c->rho = density; c->E = e; c->nu = nu; c->beta = beta; c->dampA = dampingA; c->dampB = dampingB;
     c->have_mat = true;
     c->have_forces = false;
+    drop_batch_graph(c);   // the captured step holds the old scalars
     return EOLC_OK;
 }
 
@@ -488,6 +524,7 @@ int eolc_set_options(eolc_ctx *c, int flags)
         return fail(c, EOLC_ERR_INVALID, "set_options: unknown flag");
     c->flags = flags;
     c->have_forces = false;
+    drop_batch_graph(c);   // the captured step chose its kernels by the old flags
     return EOLC_OK;
 }
 
@@ -674,7 +711,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         relres = s2.bnorm2 > 0 ? std::sqrt(s2.rmax2 / s2.bnorm2) : 0.0;
         if (!std::isfinite(s2.rmax2)) { status = EOLC_ERR_NONFINITE; break; }
         if (s2.rmax2 <= tol * tol * s2.bnorm2) { status = EOLC_OK; break; }
-        if (k >= maxit) { status = EOLC_ERR_MAXIT; break; }
+        if (k >= maxit || refinements > eolc::kMaxRefinements) { status = EOLC_ERR_MAXIT; break; }
         if (learned && k >= cap) continue;   // the fallback restarts from here
         EOLC_HIP(eolc::pcg_restart_inner(c, k));
         EOLC_HIP(eolc::scal_set(c, 0, 0, 0));
@@ -888,6 +925,288 @@ int eolc_learned_train(eolc_ctx *c, int n_fixed, const int *fixed_dof, const dou
     return EOLC_OK;
 }
 
+// ---- batched cloths -------------------------------------------------------------
+static void batch_invalidate_graph(eolc_ctx *c) { drop_batch_graph(c); }
+
+int eolc_set_batch_topology(eolc_ctx *c, int B, const int *Nb, const double *X, const int *Fb, const int *tri,
+                            const int *Hb, const int *hinge, const int *nfixb, const int *fixed, int *vert_off)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (B <= 0 || !Nb || !X || !Fb || !Hb || !nfixb)
+        return fail(c, EOLC_ERR_INVALID, "set_batch_topology: bad sizes or null arrays");
+    std::vector<int> off(B + 1, 0);
+    int64_t Ftot = 0, Htot = 0, Ptot = 0, Ntot = 0;
+    for (int b = 0; b < B; ++b) {
+        if (Nb[b] <= 0 || Fb[b] < 0 || Hb[b] < 0 || nfixb[b] < 0)
+            return fail(c, EOLC_ERR_INVALID, "set_batch_topology: negative or zero instance size");
+        const int64_t padded = ((int64_t)Nb[b] + eolc::kSlice - 1) / eolc::kSlice * eolc::kSlice;
+        Ntot += padded;
+        if (Ntot > INT32_MAX / 3) return fail(c, EOLC_ERR_INVALID, "set_batch_topology: too many vertices");
+        off[b + 1] = (int)Ntot;
+        Ftot += Fb[b];
+        Htot += Hb[b];
+        Ptot += nfixb[b];
+    }
+    if ((Ftot && !tri) || (Htot && !hinge) || (Ptot && !fixed))
+        return fail(c, EOLC_ERR_INVALID, "set_batch_topology: null element or fixed arrays");
+    if (Ftot > INT32_MAX / 3 || Htot > INT32_MAX / 4)
+        return fail(c, EOLC_ERR_INVALID, "set_batch_topology: too many elements");
+    std::vector<double> Xc(2 * (size_t)Ntot, 0.0);
+    std::vector<int> tc(3 * (size_t)Ftot), hc(4 * (size_t)Htot);
+    std::vector<int> user_fixed, all_fixed;
+    int64_t xr = 0, fr = 0, hr = 0, pr = 0;
+    for (int b = 0; b < B; ++b) {
+        for (int64_t k = 0; k < 2 * (int64_t)Nb[b]; ++k) Xc[2 * (size_t)off[b] + k] = X[2 * xr + k];
+        for (int64_t k = 0; k < 3 * (int64_t)Fb[b]; ++k) {
+            const int v = tri[3 * fr + k];
+            if (v < 0 || v >= Nb[b]) return fail(c, EOLC_ERR_INVALID, "set_batch_topology: triangle index out of range");
+            tc[3 * fr + k] = v + off[b];
+        }
+        for (int64_t k = 0; k < 4 * (int64_t)Hb[b]; ++k) {
+            const int v = hinge[4 * hr + k];
+            if (v < 0 || v >= Nb[b]) return fail(c, EOLC_ERR_INVALID, "set_batch_topology: hinge index out of range");
+            hc[4 * hr + k] = v + off[b];
+        }
+        for (int k = 0; k < nfixb[b]; ++k) {
+            const int v = fixed[pr + k];
+            if (v < 0 || v >= Nb[b]) return fail(c, EOLC_ERR_INVALID, "set_batch_topology: fixed node out of range");
+            user_fixed.push_back(v + off[b]);
+        }
+        for (int v = off[b] + Nb[b]; v < off[b + 1]; ++v) all_fixed.push_back(v);   // padding
+        xr += Nb[b];
+        fr += Fb[b];
+        hr += Hb[b];
+        pr += nfixb[b];
+    }
+    all_fixed.insert(all_fixed.end(), user_fixed.begin(), user_fixed.end());
+    std::vector<int> seg(off.begin(), off.end() - 1);
+    int rc = set_topology_impl(c, (int)Ntot, Xc.data(), (int)Ftot, tc.data(), (int)Htot, hc.data(), &seg);
+    if (rc) return rc;
+    eolc::BatchState &S = c->batch;
+    S.B = B;
+    S.inst.assign(B, eolc::BatchInst{});
+    for (int b = 0; b < B; ++b) {
+        eolc::BatchInst &I = S.inst[b];
+        I.v0 = off[b];
+        I.v1 = off[b + 1];
+        I.s0 = off[b] / eolc::kSlice;
+        I.s1 = off[b + 1] / eolc::kSlice;
+        I.n_real = Nb[b];
+    }
+    S.order.resize(B);
+    for (int b = 0; b < B; ++b) S.order[b] = b;
+    std::stable_sort(S.order.begin(), S.order.end(), [&](int a, int b2) {
+        return (S.inst[a].v1 - S.inst[a].v0) > (S.inst[b2].v1 - S.inst[b2].v0);
+    });
+    S.fixed_nodes = all_fixed;
+    S.user_fixed = user_fixed;
+    EOLC_HIP(dalloc(c, &S.d_inst, (size_t)B));
+    EOLC_HIP(hipMemcpy(S.d_inst, S.inst.data(), sizeof(eolc::BatchInst) * B, hipMemcpyHostToDevice));
+    EOLC_HIP(upload(c, &S.d_order, S.order));
+    EOLC_HIP(upload(c, &S.d_user_fixed, S.user_fixed));
+    EOLC_HIP(dalloc(c, &S.slice_part, 4 * (size_t)c->topo.nslices));
+    EOLC_HIP(dalloc(c, &S.stats, 4 * (size_t)B));
+    EOLC_HIP(dalloc(c, &S.dstats, 4 * (size_t)B));
+    int *d_all = nullptr;
+    EOLC_HIP(upload(c, &d_all, all_fixed));
+    EOLC_HIP(eolc::batch_set_fixmask(c, d_all, (int)all_fixed.size()));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    dfree(d_all);
+    batch_invalidate_graph(c);
+    if (vert_off)
+        for (int b = 0; b <= B; ++b) vert_off[b] = off[b];
+    return EOLC_OK;
+}
+
+int eolc_set_batch_material(eolc_ctx *c, const double *E, const double *beta)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    eolc::BatchState &S = c->batch;
+    if (S.B <= 0) return fail(c, EOLC_ERR_INVALID, "set_batch_material: no batch topology");
+    if (!E || !beta) return fail(c, EOLC_ERR_INVALID, "set_batch_material: null arrays");
+    for (int b = 0; b < S.B; ++b)
+        if (!(E[b] > 0.0) || !(beta[b] >= 0.0) || !std::isfinite(E[b]) || !std::isfinite(beta[b]))
+            return fail(c, EOLC_ERR_INVALID, "set_batch_material: E must be > 0 and beta >= 0");
+    EOLC_HIP(hipSetDevice(c->device));
+    const Topology &t = c->topo;
+    // element -> instance: faces and hinges are stored instance by instance, and
+    // every element's first vertex lies in its instance's vertex range
+    std::vector<double> ef(t.F), bh(t.H);
+    auto inst_of = [&](int v) {
+        int lo = 0, hi = S.B - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (S.inst[mid].v0 <= v) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    };
+    for (int f = 0; f < t.F; ++f) ef[f] = E[inst_of(c->topo_tri[3 * f])];
+    std::vector<int> hh(4 * (size_t)t.H);
+    if (t.H) EOLC_HIP(hipMemcpy(hh.data(), c->hinge, sizeof(int) * 4 * t.H, hipMemcpyDeviceToHost));
+    for (int h = 0; h < t.H; ++h) bh[h] = beta[inst_of(hh[4 * h])];
+    EOLC_HIP(upload(c, &c->e_face, ef));
+    EOLC_HIP(upload(c, &c->beta_hinge, bh));
+    c->have_mat = true;
+    c->have_forces = false;
+    batch_invalidate_graph(c);
+    return EOLC_OK;
+}
+
+// One batched step: forces, rhs, fixed values, persistent solve, x += h v.
+static int batch_step_enqueue(eolc_ctx *c, const double grav[3], double h, double tol, int maxit, int variant)
+{
+    EOLC_HIP(eolc::launch_face_eval(c, h, grav));
+    EOLC_HIP(eolc::launch_hinge_eval(c, h));
+    EOLC_HIP(eolc::launch_assemble(c, true));
+    EOLC_HIP(eolc::launch_rhs(c, h));
+    EOLC_HIP(eolc::batch_solve_async(c, tol, maxit, variant));
+    EOLC_HIP(eolc::launch_update_x(c, h));
+    return EOLC_OK;
+}
+
+static int batch_ready(eolc_ctx *c, const double grav[3], double h, double tol, int maxit)
+{
+    if (c->batch.B <= 0) return fail(c, EOLC_ERR_INVALID, "batch: no batch topology");
+    if (!c->have_mat || !c->have_state || !c->e_face) return fail(c, EOLC_ERR_INVALID, "batch: material and state must be set");
+    if (!grav || !(h > 0.0) || !(tol > 0.0) || maxit <= 0) return fail(c, EOLC_ERR_INVALID, "batch: bad arguments");
+    return EOLC_OK;
+}
+
+static int batch_collect(eolc_ctx *c, eolc_solve_stats *stats, double ms)
+{
+    eolc::BatchState &S = c->batch;
+    std::vector<int> is(4 * (size_t)S.B);
+    std::vector<double> ds(4 * (size_t)S.B);
+    EOLC_HIP(hipMemcpy(is.data(), S.stats, sizeof(int) * 4 * S.B, hipMemcpyDeviceToHost));
+    EOLC_HIP(hipMemcpy(ds.data(), S.dstats, sizeof(double) * 4 * S.B, hipMemcpyDeviceToHost));
+    int worst = EOLC_OK;
+    for (int b = 0; b < S.B; ++b) {
+        if (is[4 * b + 2] != EOLC_OK && worst == EOLC_OK) worst = is[4 * b + 2];
+        if (stats) {
+            eolc_solve_stats &st = stats[b];
+            std::memset(&st, 0, sizeof(st));
+            st.iters = is[4 * b];
+            st.refinements = is[4 * b + 1];
+            st.status = is[4 * b + 2];
+            st.relres = ds[4 * b + 2];
+            st.true_relres = ds[4 * b] > 0 ? std::sqrt(ds[4 * b + 1] / ds[4 * b]) : 0.0;
+            st.ms_solve = ms;
+        }
+    }
+    if (worst != EOLC_OK) c->err = "batch_step: at least one instance did not converge (see its stats)";
+    return worst;
+}
+
+int eolc_batch_step(eolc_ctx *c, const double grav[3], double h, double tol, int maxit, int flags,
+                    eolc_solve_stats *stats)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    int rc = batch_ready(c, grav, h, tol, maxit);
+    if (rc) return rc;
+    EOLC_HIP(hipSetDevice(c->device));
+    eolc::BatchState &S = c->batch;
+    hipEvent_t e0, e1;
+    EOLC_HIP(hipEventCreate(&e0));
+    EOLC_HIP(hipEventCreate(&e1));
+    EOLC_HIP(hipEventRecord(e0, c->stream));
+    if (flags & EOLC_BATCH_EAGER) {
+        rc = batch_step_enqueue(c, grav, h, tol, maxit, S.variant);
+    } else {
+        const bool same = S.graph && S.graph_h == h && S.graph_tol == tol && S.graph_maxit == maxit &&
+                          S.graph_variant == S.variant && S.graph_grav[0] == grav[0] && S.graph_grav[1] == grav[1] &&
+                          S.graph_grav[2] == grav[2];
+        if (!same) {
+            batch_invalidate_graph(c);
+            hipGraph_t g = nullptr;
+            EOLC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = batch_step_enqueue(c, grav, h, tol, maxit, S.variant);
+            hipError_t ec = hipStreamEndCapture(c->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            EOLC_HIP(ec);
+            hipError_t ei = hipGraphInstantiate(&S.graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            EOLC_HIP(ei);
+            S.graph_h = h;
+            S.graph_tol = tol;
+            S.graph_maxit = maxit;
+            S.graph_variant = S.variant;
+            for (int k = 0; k < 3; ++k) S.graph_grav[k] = grav[k];
+        }
+        EOLC_HIP(hipGraphLaunch(S.graph, c->stream));
+    }
+    if (rc) return rc;
+    EOLC_HIP(hipEventRecord(e1, c->stream));
+    EOLC_HIP(hipEventSynchronize(e1));
+    const double ms = elapsed_ms(e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->have_forces = true;
+    c->h_last = h;
+    return batch_collect(c, stats, ms);
+}
+
+int eolc_batch_autotune(eolc_ctx *c, const double grav[3], double h, double tol, int maxit, int *best, double *ms)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    int rc = batch_ready(c, grav, h, tol, maxit);
+    if (rc) return rc;
+    EOLC_HIP(hipSetDevice(c->device));
+    eolc::BatchState &S = c->batch;
+    const size_t n3 = 3 * (size_t)c->topo.N;
+    double *x0 = nullptr, *v0 = nullptr;
+    EOLC_HIP(hipMalloc((void **)&x0, n3 * sizeof(double)));
+    EOLC_HIP(hipMalloc((void **)&v0, n3 * sizeof(double)));
+    EOLC_HIP(hipMemcpyAsync(x0, c->x, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    EOLC_HIP(hipMemcpyAsync(v0, c->v, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    // forces once, then the solve of the same system with each workgroup size
+    EOLC_HIP(eolc::launch_face_eval(c, h, grav));
+    EOLC_HIP(eolc::launch_hinge_eval(c, h));
+    EOLC_HIP(eolc::launch_assemble(c, true));
+    EOLC_HIP(eolc::launch_rhs(c, h));
+    hipEvent_t e0, e1;
+    EOLC_HIP(hipEventCreate(&e0));
+    EOLC_HIP(hipEventCreate(&e1));
+    int arg = S.variant;
+    double tb = 0.0;
+    for (int v = 0; v < eolc::kBatchVariants; ++v) {
+        EOLC_HIP(hipMemcpyAsync(c->v, v0, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        EOLC_HIP(hipEventRecord(e0, c->stream));
+        EOLC_HIP(eolc::batch_solve_async(c, tol, maxit, v));
+        EOLC_HIP(hipEventRecord(e1, c->stream));
+        EOLC_HIP(hipEventSynchronize(e1));
+        const double t = elapsed_ms(e0, e1);
+        if (ms) ms[v] = t;
+        if (v == 0 || t < tb) {
+            tb = t;
+            arg = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    EOLC_HIP(hipMemcpyAsync(c->x, x0, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    EOLC_HIP(hipMemcpyAsync(c->v, v0, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(x0);
+    (void)hipFree(v0);
+    S.variant = arg;
+    batch_invalidate_graph(c);
+    c->have_forces = false;
+    if (best) *best = arg;
+    return EOLC_OK;
+}
+
+int eolc_set_batch_variant(eolc_ctx *c, int variant)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (variant < 0 || variant >= eolc::kBatchVariants) return fail(c, EOLC_ERR_INVALID, "set_batch_variant: bad variant");
+    c->batch.variant = variant;
+    batch_invalidate_graph(c);
+    return EOLC_OK;
+}
+
 int eolc_time_spmv(eolc_ctx *c, int precision, int reps, double *avg_ms)
 {
     if (!c) return EOLC_ERR_INVALID;
@@ -895,8 +1214,51 @@ int eolc_time_spmv(eolc_ctx *c, int precision, int reps, double *avg_ms)
         return fail(c, EOLC_ERR_INVALID, "time_spmv: bad arguments or no assembled matrix");
     EOLC_HIP(hipSetDevice(c->device));
     float ms = 0.f;
-    EOLC_HIP(eolc::time_spmv(c, precision == 32, reps, &ms));
+    EOLC_HIP(eolc::time_spmv(c, precision == 32, -1, reps, &ms));
     *avg_ms = ms / reps;
+    return EOLC_OK;
+}
+
+int eolc_stream_copy(eolc_ctx *c, int64_t bytes, int reps, double *avg_ms)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (bytes < 16 || reps <= 0 || !avg_ms) return fail(c, EOLC_ERR_INVALID, "stream_copy: bad arguments");
+    EOLC_HIP(hipSetDevice(c->device));
+    float ms = 0.f;
+    EOLC_HIP(eolc::stream_copy(c, (size_t)bytes, reps, &ms));
+    *avg_ms = ms / reps;
+    return EOLC_OK;
+}
+
+int eolc_autotune(eolc_ctx *c, int precision, int reps, int *best, double *ms)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_forces || (precision != 32 && precision != 64) || reps <= 0)
+        return fail(c, EOLC_ERR_INVALID, "autotune: bad arguments or no assembled matrix");
+    EOLC_HIP(hipSetDevice(c->device));
+    int arg = 0;
+    double tbest = 0.0;
+    for (int v = 0; v < eolc::kSpmvVariants; ++v) {
+        float t = 0.f;
+        EOLC_HIP(eolc::time_spmv(c, precision == 32, v, reps, &t));
+        const double avg = (double)t / reps;
+        if (ms) ms[v] = avg;
+        if (v == 0 || avg < tbest) {
+            tbest = avg;
+            arg = v;
+        }
+    }
+    c->spmv_variant[precision == 32 ? 0 : 1] = arg;
+    if (best) *best = arg;
+    return EOLC_OK;
+}
+
+int eolc_set_spmv_variant(eolc_ctx *c, int precision, int variant)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if ((precision != 32 && precision != 64) || variant < 0 || variant >= eolc::kSpmvVariants)
+        return fail(c, EOLC_ERR_INVALID, "set_spmv_variant: bad precision or variant");
+    c->spmv_variant[precision == 32 ? 0 : 1] = variant;
     return EOLC_OK;
 }
 

@@ -43,6 +43,7 @@ struct SolverScalars {
 // with slen[..] blocks.  The CSR form (rowptr/col, columns sorted) is kept on the
 // host for downloads.
 constexpr int kSlice = 64;
+constexpr int kSellSigma = 256;   // rows sorted by length inside windows of 4 slices
 
 struct Topology {
     int N = 0, F = 0, H = 0;
@@ -63,6 +64,40 @@ struct Topology {
 template <class T> struct DArr {
     T *p = nullptr;
     size_t n = 0;
+};
+
+// Batched cloths: instance b owns vertices [vert_off[b], vert_off[b+1]) (a multiple of
+// 64 each, so every SELL slice belongs to one instance; the padding vertices are
+// isolated and fixed) and SELL slices [vert_off[b] / 64, vert_off[b+1] / 64).
+struct BatchInst {
+    int v0, v1;      // vertex range (padded)
+    int s0, s1;      // slice range
+    int n_real;      // vertices before padding
+    int pad_[3];
+};
+constexpr int kBatchVariants = 3;   // persistent-solve workgroup sizes 256, 512, 1024
+// Reliable updates allowed per solve: a residual that has left FP32's range (or
+// keeps reporting convergence of the inner iteration without progress) ends the
+// solve with EOLC_ERR_MAXIT instead of looping.
+constexpr int kMaxRefinements = 1000;
+
+struct BatchState {
+    int B = 0;
+    std::vector<BatchInst> inst;
+    std::vector<int> order;          // launch order: largest instance first
+    std::vector<int> fixed_nodes;    // padded vertex indices with all DOFs fixed (incl. padding)
+    std::vector<int> user_fixed;     // the caller's fixed nodes (scripted: v = 0.99 v_old)
+    BatchInst *d_inst = nullptr;
+    int *d_order = nullptr;
+    int *d_user_fixed = nullptr;
+    double *slice_part = nullptr;    // per-slice FP64 partial sums (4 x nslices)
+    int *stats = nullptr;            // per instance: iters, refinements, status, spare
+    double *dstats = nullptr;        // per instance: bnorm2, true_rr, relres, spare
+    int variant = 2;                 // workgroup size choice
+    hipGraphExec_t graph = nullptr;  // captured step
+    double graph_h = 0.0, graph_tol = 0.0;
+    double graph_grav[3] = {0, 0, 0};
+    int graph_maxit = 0, graph_variant = -1;
 };
 
 // Learned block preconditioner (message-passing network over the vertex graph).
@@ -154,6 +189,12 @@ struct eolc_ctx {
     int cd_host_decisions = 0;  // angle tests re-decided on the host (diagnostic)
     // learned preconditioner
     eolc::LearnedState lp;
+    int spmv_variant[2] = {0, 0};   // SpMV launch variant for FP32 [0] and FP64 [1] solves
+    // per-element material (batched cloths): Young's modulus per face, bending
+    // stiffness per hinge; null = the context's scalar material
+    double *e_face = nullptr;
+    double *beta_hinge = nullptr;
+    eolc::BatchState batch;
     int bj_iters_hint = 0;      // iterations of the last converged block-Jacobi solve
 };
 
@@ -175,7 +216,9 @@ hipError_t pcg_finish(eolc_ctx *c, bool mixed);
 hipError_t spmv_true_residual(eolc_ctx *c);
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
-hipError_t time_spmv(eolc_ctx *c, bool mixed, int reps, float *ms);
+constexpr int kSpmvVariants = 9;
+hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms);
+hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms);
 // learned preconditioner (k_learned.hip); buffers allocated by the API layer
 void learned_wgrad_shape(int N, int *G, int *rows);   // weight-gradient workgroups, rows per workgroup
 hipError_t learned_quantize(eolc_ctx *c);
@@ -185,6 +228,9 @@ hipError_t learned_loss_grad(eolc_ctx *c);             // lp.e set and masked; l
 hipError_t learned_adam(eolc_ctx *c, float lr);
 hipError_t learned_mask_error(eolc_ctx *c);            // lp.e <- e with fixed DOFs zeroed
 hipError_t scal_restart(eolc_ctx *c, int k);           // iteration k starts with beta = 0
+// batched cloths (k_batch.hip)
+hipError_t batch_set_fixmask(eolc_ctx *c, const int *d_nodes, int n);
+hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant);
 void build_collision_edges(eolc_ctx *c, int F, const int *tri);
 void build_jitter(std::vector<double> &u, int N);
 }  // namespace eolc

@@ -46,12 +46,13 @@ __device__ __forceinline__ double comp(V3 a, int k) { return k == 0 ? a.x : (k =
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_face_eval(int F, const int *__restrict__ tri, const double *__restrict__ x, const double *__restrict__ X,
-            double e, double nu, double rho, double gx, double gy, double gz, double s,
-            double *__restrict__ stageK, double *__restrict__ stageF, double *__restrict__ stageM,
+            double e_all, const double *__restrict__ e_face, double nu, double rho, double gx, double gy, double gz,
+            double s, double *__restrict__ stageK, double *__restrict__ stageF, double *__restrict__ stageM,
             int *__restrict__ bad)
 {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
+    const double e = e_face ? e_face[f] : e_all;   // per-face Young's modulus (batched cloths)
     const int ia = tri[3 * f], ib = tri[3 * f + 1], ic = tri[3 * f + 2];
     const V3 xa = ld3(x + 3 * ia), xb = ld3(x + 3 * ib), xc = ld3(x + 3 * ic);
     const double Xax = X[2 * ia], Xay = X[2 * ia + 1];
@@ -225,14 +226,14 @@ __device__ __forceinline__ void hinge_hess_block(const HingeGeom &G, int i, int 
 
 __global__ void __launch_bounds__(256)
 k_hinge_eval(int H, int F, const int *__restrict__ hinge, const double *__restrict__ x,
-             const double *__restrict__ X, double beta, double s, double *__restrict__ stageK,
-             double *__restrict__ stageF)
+             const double *__restrict__ X, double beta_all, const double *__restrict__ beta_hinge, double s,
+             double *__restrict__ stageK, double *__restrict__ stageF)
 {
     int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= H) return;
     int q[4] = {hinge[4 * h], hinge[4 * h + 1], hinge[4 * h + 2], hinge[4 * h + 3]};
     HingeGeom G;
-    hinge_geom(x, X, q, beta, G);
+    hinge_geom(x, X, q, beta_hinge ? beta_hinge[h] : beta_all, G);
     // force f_j = w sum_q g_q x a_qj
     double *sf = stageF + (size_t)9 * F + (size_t)12 * h;
 #pragma unroll
@@ -263,8 +264,8 @@ constexpr int kProjWG = 64;
 
 __global__ void __launch_bounds__(kProjWG)
 k_hinge_eval_proj(int H, int F, const int *__restrict__ hinge, const double *__restrict__ x,
-                  const double *__restrict__ X, double beta, double s, double *__restrict__ stageK,
-                  double *__restrict__ stageF, int *__restrict__ sweeps_out)
+                  const double *__restrict__ X, double beta_all, const double *__restrict__ beta_hinge, double s,
+                  double *__restrict__ stageK, double *__restrict__ stageF, int *__restrict__ sweeps_out)
 {
     __shared__ double Aw[81][kProjWG];
     __shared__ double Vw[81][kProjWG];
@@ -274,7 +275,7 @@ k_hinge_eval_proj(int H, int F, const int *__restrict__ hinge, const double *__r
     if (!active) h = H - 1;   // keep the lane busy on valid data; its output is not written
     int q[4] = {hinge[4 * h], hinge[4 * h + 1], hinge[4 * h + 2], hinge[4 * h + 3]};
     HingeGeom G;
-    hinge_geom(x, X, q, beta, G);
+    hinge_geom(x, X, q, beta_hinge ? beta_hinge[h] : beta_all, G);
     if (active) {
         double *sf = stageF + (size_t)9 * F + (size_t)12 * h;
 #pragma unroll
@@ -387,8 +388,8 @@ hipError_t launch_face_eval(eolc_ctx *c, double h, const double *grav)
     if (F == 0) return hipSuccess;
     const double s = c->dampB * h * h;
     hipLaunchKernelGGL(k_face_eval, dim3(blocks_for(F)), dim3(256), 0, c->stream, F, c->tri, c->x, c->X, c->E,
-                       c->nu, c->rho, grav[0], grav[1], grav[2], s, c->stageK, c->stageF, c->stageM,
-                       (int *)(c->counter + 1));
+                       (const double *)c->e_face, c->nu, c->rho, grav[0], grav[1], grav[2], s, c->stageK, c->stageF,
+                       c->stageM, (int *)(c->counter + 1));
     return hipGetLastError();
 }
 
@@ -399,10 +400,11 @@ hipError_t launch_hinge_eval(eolc_ctx *c, double h)
     const double s = c->dampB * h * h;
     if (c->flags & EOLC_OPT_SPD_PROJECT_BENDING) {
         hipLaunchKernelGGL(k_hinge_eval_proj, dim3(blocks_for(H, kProjWG)), dim3(kProjWG), 0, c->stream, H, c->topo.F,
-                           c->hinge, c->x, c->X, c->beta, s, c->stageK, c->stageF, (int *)(c->counter + 2));
+                           c->hinge, c->x, c->X, c->beta, (const double *)c->beta_hinge, s, c->stageK, c->stageF,
+                           (int *)(c->counter + 2));
     } else {
         hipLaunchKernelGGL(k_hinge_eval, dim3(blocks_for(H)), dim3(256), 0, c->stream, H, c->topo.F, c->hinge, c->x,
-                           c->X, c->beta, s, c->stageK, c->stageF);
+                           c->X, c->beta, (const double *)c->beta_hinge, s, c->stageK, c->stageF);
     }
     return hipGetLastError();
 }

@@ -34,6 +34,10 @@
 
 using eolc::SolverScalars;
 
+// Every multiply-add below is written out (fma where fused): no contraction is
+// left to the compiler, so all launch variants of a kernel round identically.
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int WG = eolc::kPcgWG;
@@ -136,13 +140,75 @@ __device__ __forceinline__ double *PB(double *p, int parity) { return p + (1 + 2
 // and its FP32 rounding is relative to the local differences p_j - p_i, not to |p|:
 // an assembled FP32 copy of M + h^2 K loses the mass (~1e-8) under the rounding of
 // the stiffness (~1e-2 x 6e-8) in the smooth, slowly converging modes.
+//
+// Launch variants (same arithmetic in the same order, so bitwise-identical results;
+// chosen per matrix by eolc_autotune):
+//   U   entries per loop trip (1 or 2): all loads of a trip are issued together
+//   PF  the column index of the next trip is loaded one trip ahead, so the gathers of
+//       p and z do not wait for a dependent load
+//   NT  matrix values and columns are read with non-temporal loads (streamed past the
+//       caches, which then keep the gathered vectors)
+//   W   waves per SIMD the register allocation targets
+template <class T, bool NT>
+__device__ __forceinline__ T ldm(const T *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+// One block of the product, accumulated into (y0, y1, y2):
+//   DIFF:  y += m u + S (u - ui)   (FP32 terms, FP64 sums)
+//   else:  y += A u               (FP64)
 template <class T, bool DIFF>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__device__ __forceinline__ void block_apply(T a0, T a1, T a2, T a3, T a4, T a5, T a6, T a7, T a8, T m, T u0, T u1,
+                                            T u2, T ui0, T ui1, T ui2, double &y0, double &y1, double &y2)
+{
+    if constexpr (DIFF) {
+        const T d0 = u0 - ui0, d1 = u1 - ui1, d2 = u2 - ui2;
+        const T t0 = fma(a2, d2, fma(a1, d1, a0 * d0));
+        const T t1 = fma(a5, d2, fma(a4, d1, a3 * d0));
+        const T t2 = fma(a8, d2, fma(a7, d1, a6 * d0));
+        y0 += (double)fma(m, u0, t0);
+        y1 += (double)fma(m, u1, t1);
+        y2 += (double)fma(m, u2, t2);
+    } else {
+        y0 += fma((double)a2, (double)u2, fma((double)a1, (double)u1, (double)a0 * (double)u0));
+        y1 += fma((double)a5, (double)u2, fma((double)a4, (double)u1, (double)a3 * (double)u0));
+        y2 += fma((double)a8, (double)u2, fma((double)a7, (double)u1, (double)a6 * (double)u0));
+    }
+}
+
+template <class T, bool DIFF>
+struct SpmvEntry {
+    T a[9];
+    T m;
+    T u0, u1, u2;
+    template <bool NT>
+    __device__ __forceinline__ void load(const T *__restrict__ A, const T *__restrict__ Mm, int64_t nbs, int e, int j,
+                                         T betaT, const T *__restrict__ z, const T *__restrict__ pold)
+    {
+        u0 = fma(betaT, pold[3 * j], z[3 * j]);
+        u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
+        u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) a[c] = ldm<T, NT>(A + c * nbs + e);
+        if constexpr (DIFF) m = ldm<T, NT>(Mm + e);
+    }
+    __device__ __forceinline__ void accumulate(T ui0, T ui1, T ui2, double &y0, double &y1, double &y2) const
+    {
+        block_apply<T, DIFF>(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], m, u0, u1, u2, ui0, ui1, ui2, y0,
+                             y1, y2);
+    }
+};
+
+template <class T, bool DIFF, int U, bool PF, bool NT, int W>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const int *__restrict__ srow,
            const int *__restrict__ slen, const int *__restrict__ scol, const T *__restrict__ A,
            const T *__restrict__ Mm, int64_t nbs, const unsigned char *__restrict__ fixmask, const T *__restrict__ z,
            const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, SolverScalars *scal, double *partials)
 {
+    static_assert(U == 1 || U == 2, "entries per trip");
     __shared__ double lds[2 * (WG / 64) + 2];
     if (scal->done) return;
     double cur[2];
@@ -168,41 +234,80 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
     const int g = sl * eolc::kSlice + lane;
     double dot = 0.0;
     if (sl < nslices && g < N) {
-        const int row = srow[g], len = slen[g];
+        const int row = srow[g], len = slen[g];   // len >= 1: every row holds its diagonal block
         const int base = slice_off[sl] + lane;
         const T *__restrict__ a = A + base;
+        const T *__restrict__ mm = DIFF ? Mm + base : nullptr;
         const int *__restrict__ cl = scol + base;
-        const T ui0 = fma(betaT, pold[3 * row], z[3 * row]);
-        const T ui1 = fma(betaT, pold[3 * row + 1], z[3 * row + 1]);
-        const T ui2 = fma(betaT, pold[3 * row + 2], z[3 * row + 2]);
+        T ui0 = fma(betaT, pold[3 * row], z[3 * row]);
+        T ui1 = fma(betaT, pold[3 * row + 1], z[3 * row + 1]);
+        T ui2 = fma(betaT, pold[3 * row + 2], z[3 * row + 2]);
+        if constexpr (!DIFF) {
+            // the own row's p is not needed inside the loop: store it now, reload after
+            pnew[3 * row] = ui0; pnew[3 * row + 1] = ui1; pnew[3 * row + 2] = ui2;
+        }
         double y0 = 0, y1 = 0, y2 = 0;
+        int kk = 0;
+        if constexpr (U == 1) {
+            int jn = ldm<int, NT>(cl);
 #pragma unroll 1
-        for (int kk = 0; kk < len; ++kk) {
-            const int e = kk * eolc::kSlice;
-            const int j = cl[e];
-            const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
-            const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
-            const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
-            if constexpr (DIFF) {
-                const T m = Mm[base + e];
-                const T d0 = u0 - ui0, d1 = u1 - ui1, d2 = u2 - ui2;
-                const T t0 = a[0 * nbs + e] * d0 + a[1 * nbs + e] * d1 + a[2 * nbs + e] * d2;
-                const T t1 = a[3 * nbs + e] * d0 + a[4 * nbs + e] * d1 + a[5 * nbs + e] * d2;
-                const T t2 = a[6 * nbs + e] * d0 + a[7 * nbs + e] * d1 + a[8 * nbs + e] * d2;
-                y0 += (double)fma(m, u0, t0);
-                y1 += (double)fma(m, u1, t1);
-                y2 += (double)fma(m, u2, t2);
-            } else {
-                y0 += (double)a[0 * nbs + e] * u0 + (double)a[1 * nbs + e] * u1 + (double)a[2 * nbs + e] * u2;
-                y1 += (double)a[3 * nbs + e] * u0 + (double)a[4 * nbs + e] * u1 + (double)a[5 * nbs + e] * u2;
-                y2 += (double)a[6 * nbs + e] * u0 + (double)a[7 * nbs + e] * u1 + (double)a[8 * nbs + e] * u2;
+            for (; kk < len; ++kk) {
+                const int e = kk * eolc::kSlice;
+                int j;
+                if constexpr (PF) {
+                    j = jn;
+                    const int kn = kk + 1 < len ? kk + 1 : kk;   // clamped: always a valid entry
+                    jn = ldm<int, NT>(cl + kn * eolc::kSlice);
+                } else {
+                    j = ldm<int, NT>(cl + e);
+                }
+                const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
+                const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
+                const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                const T *__restrict__ ae = a + e;
+                block_apply<T, DIFF>(ldm<T, NT>(ae), ldm<T, NT>(ae + nbs), ldm<T, NT>(ae + 2 * nbs),
+                                     ldm<T, NT>(ae + 3 * nbs), ldm<T, NT>(ae + 4 * nbs), ldm<T, NT>(ae + 5 * nbs),
+                                     ldm<T, NT>(ae + 6 * nbs), ldm<T, NT>(ae + 7 * nbs), ldm<T, NT>(ae + 8 * nbs),
+                                     DIFF ? ldm<T, NT>(mm + e) : T(0), u0, u1, u2, ui0, ui1, ui2, y0, y1, y2);
+            }
+        } else {
+            int jn0 = ldm<int, NT>(cl), jn1 = len > 1 ? ldm<int, NT>(cl + eolc::kSlice) : 0;
+#pragma unroll 1
+            for (; kk + 1 < len; kk += 2) {
+                const int e = kk * eolc::kSlice;
+                int j0, j1;
+                if constexpr (PF) {
+                    j0 = jn0;
+                    j1 = jn1;
+                    const int k0 = kk + 2 < len ? kk + 2 : kk, k1 = kk + 3 < len ? kk + 3 : kk;
+                    jn0 = ldm<int, NT>(cl + k0 * eolc::kSlice);
+                    jn1 = ldm<int, NT>(cl + k1 * eolc::kSlice);
+                } else {
+                    j0 = ldm<int, NT>(cl + e);
+                    j1 = ldm<int, NT>(cl + e + eolc::kSlice);
+                }
+                SpmvEntry<T, DIFF> x0, x1;
+                x0.template load<NT>(a, mm, nbs, e, j0, betaT, z, pold);
+                x1.template load<NT>(a, mm, nbs, e + eolc::kSlice, j1, betaT, z, pold);
+                x0.accumulate(ui0, ui1, ui2, y0, y1, y2);
+                x1.accumulate(ui0, ui1, ui2, y0, y1, y2);
+            }
+            if (kk < len) {
+                const int e = kk * eolc::kSlice;
+                SpmvEntry<T, DIFF> x0;
+                x0.template load<NT>(a, mm, nbs, e, ldm<int, NT>(cl + e), betaT, z, pold);
+                x0.accumulate(ui0, ui1, ui2, y0, y1, y2);
             }
         }
         const unsigned char fm = fixmask[row];
         if (fm & 1) y0 = 0;
         if (fm & 2) y1 = 0;
         if (fm & 4) y2 = 0;
-        pnew[3 * row] = ui0; pnew[3 * row + 1] = ui1; pnew[3 * row + 2] = ui2;
+        if constexpr (DIFF) {
+            pnew[3 * row] = ui0; pnew[3 * row + 1] = ui1; pnew[3 * row + 2] = ui2;
+        } else {
+            ui0 = pnew[3 * row]; ui1 = pnew[3 * row + 1]; ui2 = pnew[3 * row + 2];
+        }
         const T q0 = (T)y0, q1 = (T)y1, q2 = (T)y2;
         q[3 * row] = q0; q[3 * row + 1] = q1; q[3 * row + 2] = q2;
         dot = ((double)ui0 * (double)q0 + (double)ui1 * (double)q1) + (double)ui2 * (double)q2;
@@ -242,15 +347,15 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
     double srz = 0.0, srr = 0.0;
     for (int i = rb + threadIdx.x; i < re; i += WG) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) x[3 * i + c] += (TX)(alphaT * pnew[3 * i + c]);
+        for (int c = 0; c < 3; ++c) x[3 * i + c] = (TX)fma(alphaT, pnew[3 * i + c], (T)x[3 * i + c]);
         const T r0 = fma(-alphaT, q[3 * i], r[3 * i]);
         const T r1 = fma(-alphaT, q[3 * i + 1], r[3 * i + 1]);
         const T r2 = fma(-alphaT, q[3 * i + 2], r[3 * i + 2]);
         r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
         const T *d = D + i;
-        const T z0 = d[0 * N] * r0 + d[1 * N] * r1 + d[2 * N] * r2;
-        const T z1 = d[3 * N] * r0 + d[4 * N] * r1 + d[5 * N] * r2;
-        const T z2 = d[6 * N] * r0 + d[7 * N] * r1 + d[8 * N] * r2;
+        const T z0 = fma(d[2 * N], r2, fma(d[1 * N], r1, d[0 * N] * r0));
+        const T z1 = fma(d[5 * N], r2, fma(d[4 * N], r1, d[3 * N] * r0));
+        const T z2 = fma(d[8 * N], r2, fma(d[7 * N], r1, d[6 * N] * r0));
         z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
         srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
         srr += ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2;
@@ -288,9 +393,9 @@ k_resid64(int N, int nslices, const int *__restrict__ slice_off, const int *__re
             const int e = kk * eolc::kSlice;
             const int j = cl[e];
             const double u0 = xin[3 * j], u1 = xin[3 * j + 1], u2 = xin[3 * j + 2];
-            y0 += a[0 * nbs + e] * u0 + a[1 * nbs + e] * u1 + a[2 * nbs + e] * u2;
-            y1 += a[3 * nbs + e] * u0 + a[4 * nbs + e] * u1 + a[5 * nbs + e] * u2;
-            y2 += a[6 * nbs + e] * u0 + a[7 * nbs + e] * u1 + a[8 * nbs + e] * u2;
+            block_apply<double, false>(a[0 * nbs + e], a[1 * nbs + e], a[2 * nbs + e], a[3 * nbs + e],
+                                       a[4 * nbs + e], a[5 * nbs + e], a[6 * nbs + e], a[7 * nbs + e],
+                                       a[8 * nbs + e], 0.0, u0, u1, u2, 0.0, 0.0, 0.0, y0, y1, y2);
         }
         double o0 = (b ? b[3 * row] : 0.0) - y0;
         double o1 = (b ? b[3 * row + 1] : 0.0) - y1;
@@ -450,10 +555,52 @@ __global__ void k_scal_restart(SolverScalars *s, int k)
 
 inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
+// Streaming copy (16-byte loads and stores, grid stride): the known-size kernel
+// that calibrates the memory counters and measures achievable HBM bandwidth.
+__global__ void __launch_bounds__(256) k_stream_copy(int64_t n16, const uint4 *__restrict__ src, uint4 *__restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 __global__ void k_fill(int n, double v, double *__restrict__ out)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) out[t] = v;
+}
+
+// SpMV launch variants (see k_pcg_spmv); the order is the one eolc_autotune reports.
+#define EOLC_SPMV_VARIANT_LIST(X) \
+    X(0, 1, false, false, 8)      \
+    X(1, 1, true, false, 8)       \
+    X(2, 1, true, true, 8)        \
+    X(3, 1, false, true, 8)       \
+    X(4, 1, true, true, 6)        \
+    X(5, 2, true, true, 6)        \
+    X(6, 2, true, false, 6)       \
+    X(7, 2, true, true, 4)        \
+    X(8, 1, true, true, 4)
+
+template <class T>
+void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm, const T *z, const T *pold, T *pnew,
+                 T *q)
+{
+    const eolc::Topology &t = c->topo;
+    constexpr bool diff = std::is_same<T, float>::value;
+#define EOLC_SPMV_CASE(id, U, PF, NT, W)                                                                         \
+    case id:                                                                                                     \
+        hipLaunchKernelGGL((k_pcg_spmv<T, diff, U, PF, NT, W>), dim3(G), dim3(WG), 0, c->stream, k, t.N,          \
+                           t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, pold, \
+                           pnew, q, c->scal, c->partials);                                                        \
+        break;
+    switch (variant) {
+        EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_CASE)
+    default:
+        hipLaunchKernelGGL((k_pcg_spmv<T, diff, 1, false, false, 8>), dim3(G), dim3(WG), 0, c->stream, k, t.N,
+                           t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, pold, pnew,
+                           q, c->scal, c->partials);
+    }
+#undef EOLC_SPMV_CASE
 }
 
 template <class T>
@@ -463,8 +610,7 @@ void launch_iteration(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const 
     const eolc::Topology &t = c->topo;
     const int pin = k & 1, pout = pin ^ 1;
     constexpr bool diff = std::is_same<T, float>::value;
-    hipLaunchKernelGGL((k_pcg_spmv<T, diff>), dim3(G), dim3(WG), 0, c->stream, k, t.N, t.nslices, c->slice_off, c->srow,
-                       c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, p[pin], p[pout], q, c->scal, c->partials);
+    launch_spmv<T>(c, c->spmv_variant[diff ? 0 : 1], k, G, A, Mm, z, p[pin], p[pout], q);
     hipLaunchKernelGGL((k_pcg_update<T, T>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r, z, c->scal,
                        c->partials);
 }
@@ -600,29 +746,64 @@ hipError_t spmv_true_residual(eolc_ctx *c)
 
 // Back-to-back launches of the SpMV kernel for timing (iteration index 1, `done`
 // cleared first).  Leaves the solver state undefined.
-hipError_t time_spmv(eolc_ctx *c, bool mixed, int reps, float *ms)
+hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms)
+{
+    const int64_t n16 = (int64_t)(bytes / 16);
+    uint4 *a = nullptr, *b = nullptr;
+    hipError_t e = hipMalloc((void **)&a, n16 * 16);
+    if (e != hipSuccess) return e;
+    e = hipMalloc((void **)&b, n16 * 16);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return e;
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int grid = 256 * 8 * 4;   // 8 workgroups per CU on 256 CUs, 4x over
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, n16, a, b);   // first touch
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, n16, a, b);
+    (void)hipEventRecord(e1, c->stream);
+    e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return e;
+}
+
+// Back-to-back launches of one SpMV variant (-1: the context's current choice) for
+// timing (iteration index 1, `done` cleared first).  Leaves the solver state undefined.
+hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms)
 {
     const int N = c->topo.N;
     const int G = pcg_grid(N);
+    if (variant < 0) variant = c->spmv_variant[mixed ? 0 : 1];
     hipEvent_t a, b;
     hipError_t e = hipEventCreate(&a);
     if (e != hipSuccess) return e;
     e = hipEventCreate(&b);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+        hipEventDestroy(a);
+        return e;
+    }
     scal_set(c, 0.0, 0.0, 1);
     hipLaunchKernelGGL(k_fill, dim3(blocks_for(4 * kMaxPartials)), dim3(256), 0, c->stream, 4 * kMaxPartials, 1.0,
                        c->partials + kMaxPartials);
+    // one untimed launch (first-touch, instruction cache)
+    if (mixed)
+        launch_spmv<float>(c, variant, 1, G, c->S32, c->M32, c->z32, c->p32[1], c->p32[0], c->q32);
+    else
+        launch_spmv<double>(c, variant, 1, G, c->A64, (const double *)nullptr, c->z64, c->p64[1], c->p64[0], c->q64);
     hipEventRecord(a, c->stream);
-    const eolc::Topology &t = c->topo;
     for (int r = 0; r < reps; ++r) {
         if (mixed)
-            hipLaunchKernelGGL((k_pcg_spmv<float, true>), dim3(G), dim3(WG), 0, c->stream, 1, N, t.nslices,
-                               c->slice_off, c->srow, c->slen, c->scol, c->S32, c->M32, t.nbs, c->fixmask, c->z32,
-                               c->p32[1], c->p32[0], c->q32, c->scal, c->partials);
+            launch_spmv<float>(c, variant, 1, G, c->S32, c->M32, c->z32, c->p32[1], c->p32[0], c->q32);
         else
-            hipLaunchKernelGGL((k_pcg_spmv<double, false>), dim3(G), dim3(WG), 0, c->stream, 1, N, t.nslices,
-                               c->slice_off, c->srow, c->slen, c->scol, c->A64, (const double *)nullptr, t.nbs,
-                               c->fixmask, c->z64, c->p64[1], c->p64[0], c->q64, c->scal, c->partials);
+            launch_spmv<double>(c, variant, 1, G, c->A64, (const double *)nullptr, c->z64, c->p64[1], c->p64[0],
+                                c->q64);
     }
     hipEventRecord(b, c->stream);
     e = hipEventSynchronize(b);

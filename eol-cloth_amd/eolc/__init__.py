@@ -28,8 +28,15 @@ EXPORTED_SYMBOLS = (
     "eolc_set_options", "eolc_upload_state", "eolc_download_state", "eolc_forces_fill",
     "eolc_download_forces", "eolc_download_csr", "eolc_velocity_solve", "eolc_step", "eolc_collide",
     "eolc_time_spmv", "eolc_learned_set_weights", "eolc_learned_get_weights", "eolc_learned_infer",
-    "eolc_learned_loss_grad", "eolc_learned_train",
+    "eolc_learned_loss_grad", "eolc_learned_train", "eolc_autotune", "eolc_set_spmv_variant",
+    "eolc_set_batch_topology", "eolc_set_batch_material", "eolc_batch_step", "eolc_batch_autotune",
+    "eolc_set_batch_variant", "eolc_stream_copy",
 )
+
+BATCH_EAGER = 0x1
+BATCH_VARIANTS = 3
+
+SPMV_VARIANTS = 9
 
 LEARNED_NPARAMS = 26246
 
@@ -94,6 +101,15 @@ def lib():
                                    C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(CollisionRec), C.c_int, C.POINTER(C.c_int)]
         L.eolc_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.eolc_set_batch_topology.argtypes = [vp, C.c_int] + [C.c_void_p] * 9
+        L.eolc_set_batch_material.argtypes = [vp, C.c_void_p, C.c_void_p]
+        L.eolc_batch_step.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p]
+        L.eolc_batch_autotune.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                          C.c_void_p]
+        L.eolc_set_batch_variant.argtypes = [vp, C.c_int]
+        L.eolc_stream_copy.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+        L.eolc_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_void_p]
+        L.eolc_set_spmv_variant.argtypes = [vp, C.c_int, C.c_int]
         L.eolc_learned_set_weights.argtypes = [vp, C.c_void_p, C.c_int]
         L.eolc_learned_get_weights.argtypes = [vp, C.c_void_p, C.c_int]
         L.eolc_learned_infer.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -240,6 +256,62 @@ class Context:
                 continue
             self._check(rc)
             return [buf[i] for i in range(cnt.value)]
+
+    def stream_copy(self, nbytes, reps=10):
+        """Average ms of one copy-kernel launch reading and writing nbytes each."""
+        ms = C.c_double(0)
+        self._check(lib().eolc_stream_copy(self._h, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
+
+    def autotune(self, precision, reps=10):
+        """(best variant, average ms of every SpMV variant) on the current matrix."""
+        best = C.c_int(0)
+        ms = np.zeros(SPMV_VARIANTS)
+        self._check(lib().eolc_autotune(self._h, int(precision), int(reps), C.byref(best), _ptr(ms)))
+        return best.value, ms
+
+    def set_spmv_variant(self, precision, variant):
+        return self._check(lib().eolc_set_spmv_variant(self._h, int(precision), int(variant)))
+
+    # ---- batched cloths
+    def set_batch_topology(self, Xs, tris, hinges, fixeds):
+        """Per-instance lists of arrays (local indices).  Returns vert_off (B+1)."""
+        B = len(Xs)
+        Nb = _i([np.asarray(x).reshape(-1, 2).shape[0] for x in Xs])
+        Fb = _i([np.asarray(t).reshape(-1, 3).shape[0] for t in tris])
+        Hb = _i([np.asarray(h).reshape(-1, 4).shape[0] for h in hinges])
+        Pb = _i([np.asarray(f).reshape(-1).size for f in fixeds])
+        X = _d(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in Xs]))
+        T = _i(np.concatenate([np.asarray(t).reshape(-1) for t in tris]))
+        Hh = _i(np.concatenate([np.asarray(h).reshape(-1) for h in hinges]))
+        P = _i(np.concatenate([np.asarray(f).reshape(-1) for f in fixeds]))
+        off = np.zeros(B + 1, dtype=np.int32)
+        self._check(lib().eolc_set_batch_topology(self._h, B, _ptr(Nb), _ptr(X), _ptr(Fb), _ptr(T), _ptr(Hb),
+                                                  _ptr(Hh), _ptr(Pb), _ptr(P), _ptr(off)))
+        self.N = int(off[-1])
+        self.B = B
+        return off
+
+    def set_batch_material(self, E, beta):
+        E, beta = _d(E), _d(beta)
+        return self._check(lib().eolc_set_batch_material(self._h, _ptr(E), _ptr(beta)))
+
+    def batch_step(self, grav, h, tol=1e-8, maxit=100000, eager=False, allow=()):
+        st = (SolveStats * self.B)()
+        rc = lib().eolc_batch_step(self._h, _d(grav), float(h), float(tol), int(maxit),
+                                   BATCH_EAGER if eager else 0, st)
+        self._check(rc, allow=allow)
+        return rc, [st[b] for b in range(self.B)]
+
+    def batch_autotune(self, grav, h, tol=1e-8, maxit=100000):
+        best = C.c_int(0)
+        ms = np.zeros(BATCH_VARIANTS)
+        self._check(lib().eolc_batch_autotune(self._h, _d(grav), float(h), float(tol), int(maxit), C.byref(best),
+                                              _ptr(ms)))
+        return best.value, ms
+
+    def set_batch_variant(self, variant):
+        return self._check(lib().eolc_set_batch_variant(self._h, int(variant)))
 
     # ---- learned block preconditioner
     def learned_set_weights(self, theta):

@@ -144,3 +144,44 @@ def obstacles(seed=1236, P=64, B=4):
         M[2, 3] = 0.0
         E.append(M)
     return pts, nrm, whd, E
+
+
+class BatchScene:
+    """Many cloths of uneven resolution and material: per instance the side is
+    uniform in {side_lo..side_hi} quads, Young's modulus log-uniform in [E_lo, E_hi],
+    bending stiffness log-uniform in [beta_lo, beta_hi] (splitmix64 stream `seed`),
+    each with the single-cloth initial state (noise sigma_x, sigma_v; noise seeds
+    seed * 1000 + 2b and seed * 1000 + 2b + 1) and its two top corners fixed."""
+
+    def __init__(self, B=256, side_lo=16, side_hi=128, E_lo=1e2, E_hi=1e4, beta_lo=1e-6, beta_hi=1e-4,
+                 sigma_x=1e-3, sigma_v=0.05, seed=1237):
+        u = uniform(seed, 3 * B).reshape(B, 3)
+        self.B = int(B)
+        self.sides = (side_lo + np.floor(u[:, 0] * (side_hi - side_lo + 1))).astype(int)
+        self.sides = np.minimum(self.sides, side_hi)
+        self.E = np.exp(np.log(E_lo) + u[:, 1] * (np.log(E_hi) - np.log(E_lo)))
+        self.beta = np.exp(np.log(beta_lo) + u[:, 2] * (np.log(beta_hi) - np.log(beta_lo)))
+        self.inst = [Scene(int(s), sigma_x, sigma_v, seed_x=seed * 1000 + 2 * b, seed_v=seed * 1000 + 2 * b + 1)
+                     for b, s in enumerate(self.sides)]
+        self.N_real = int(sum(sc.N for sc in self.inst))
+
+    def setup(self, ctx, density=0.1, nu=0.3):
+        """Topology, material and initial state on a context; returns vert_off."""
+        off = ctx.set_batch_topology([sc.X for sc in self.inst], [sc.tri for sc in self.inst],
+                                     [sc.hinge for sc in self.inst], [sc.fixed for sc in self.inst])
+        ctx.set_material(density, 1.0, nu, 1e-5, 0.0, 1.0)
+        ctx.set_batch_material(self.E, self.beta)
+        self.off = off
+        x, v = self.pack([sc.x0 for sc in self.inst]), self.pack([sc.v0 for sc in self.inst])
+        ctx.upload_state(x, v)
+        return off
+
+    def pack(self, per_inst):
+        """Concatenate per-instance 3N arrays into the padded numbering."""
+        out = np.zeros(3 * int(self.off[-1]))
+        for b, a in enumerate(per_inst):
+            out[3 * self.off[b]: 3 * self.off[b] + a.size] = a
+        return out
+
+    def unpack(self, a):
+        return [a[3 * self.off[b]: 3 * self.off[b] + 3 * sc.N] for b, sc in enumerate(self.inst)]

@@ -176,11 +176,64 @@ int eolc_learned_loss_grad(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, con
 int eolc_learned_train(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, const double *e, int steps,
                        double lr, double *losses);
 
+/* ---- batched cloths (no counterpart in the reference) ---------------------------
+ * B independent cloths in one context, solved together.  Instance b has Nb[b]
+ * vertices, Fb[b] faces, Hb[b] hinges and nfixb[b] fixed nodes (all DOFs fixed,
+ * v = 0.99 v_old as in eolc_step); X, tri, hinge and fixed are the instances'
+ * arrays one after the other with instance-local vertex indices.  Each instance
+ * is padded to a multiple of 64 vertices with isolated fixed vertices: state
+ * arrays (eolc_upload_state / eolc_download_state) use the padded numbering,
+ * instance b at vertices [vert_off[b], vert_off[b+1]) (vert_off: B+1 outputs).
+ * Replaces eolc_set_topology; density, Poisson ratio and damping come from
+ * eolc_set_material, Young's modulus and bending stiffness per instance from
+ * eolc_set_batch_material. */
+int eolc_set_batch_topology(eolc_ctx *ctx, int B, const int *Nb, const double *X, const int *Fb,
+                            const int *tri, const int *Hb, const int *hinge, const int *nfixb,
+                            const int *fixed, int *vert_off);
+int eolc_set_batch_material(eolc_ctx *ctx, const double *E, const double *beta);
+
+/* One step of every instance (forces, rhs, velocity solve to tol per instance,
+ * x += h v).  The solve is one persistent launch: one workgroup per instance runs
+ * that instance's mixed-precision PCG (FP32 SpMV and block-Jacobi, FP64 dots,
+ * FP64 true residual at reliable updates) and leaves when the instance has
+ * converged.  The whole step is captured in a HIP graph on first use and replayed
+ * (re-captured when h, tol, maxit, gravity or the launch variant change) unless
+ * flags has EOLC_BATCH_EAGER.  stats (B entries, may be NULL) per instance; the
+ * return value is the first non-OK instance status. */
+#define EOLC_BATCH_EAGER 0x1
+int eolc_batch_step(eolc_ctx *ctx, const double grav[3], double h, double tol, int maxit, int flags,
+                    eolc_solve_stats *stats);
+
+/* Autotuner of the batched solve: times the persistent solve with workgroup sizes
+ * 256, 512 and 1024 (ms: 3 entries, may be NULL) on the current state's system
+ * and keeps the fastest.  The state is left unchanged.  Results do not depend on
+ * the workgroup size (reductions are per 64-row slice, then in slice order). */
+#define EOLC_BATCH_VARIANTS 3
+int eolc_batch_autotune(eolc_ctx *ctx, const double grav[3], double h, double tol, int maxit, int *best,
+                        double *ms);
+int eolc_set_batch_variant(eolc_ctx *ctx, int variant);
+
 /* ---- measurement -------------------------------------------------------------
  * Average device time (HIP events on the context's stream) of one launch of the
  * PCG SpMV kernel (precision 64 or 32) over `reps` back-to-back launches on the
  * current matrix.  For roofline reporting; leaves the solver state undefined. */
 int eolc_time_spmv(eolc_ctx *ctx, int precision, int reps, double *avg_ms);
+
+/* Average device time of one launch of a streaming copy kernel (16-byte loads and
+ * stores) reading and writing `bytes` each, over `reps` launches: the known-size
+ * kernel for counter calibration and the achievable-bandwidth reference. */
+int eolc_stream_copy(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
+
+/* ---- launch autotuner (no counterpart in the reference) -----------------------
+ * Times each SpMV launch variant (entries per loop trip, column-index prefetch,
+ * non-temporal matrix loads, waves per SIMD) for `reps` launches on the current
+ * matrix and keeps the fastest for later solves of that precision (32 or 64) on
+ * this context.  All variants give bitwise-identical results.  ms (may be NULL)
+ * receives EOLC_SPMV_VARIANTS average times; *best (may be NULL) the choice.
+ * eolc_set_spmv_variant sets the choice directly. */
+#define EOLC_SPMV_VARIANTS 9
+int eolc_autotune(eolc_ctx *ctx, int precision, int reps, int *best, double *ms);
+int eolc_set_spmv_variant(eolc_ctx *ctx, int precision, int variant);
 
 #ifdef __cplusplus
 }

@@ -121,3 +121,26 @@ def test_iteration_cap_reports_maxit(ctx):
     rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-14, maxit=5, mode=eolc.SOLVE_FP64_BJ,
                       allow=(eolc.ERR_MAXIT,))
     assert rc == eolc.ERR_MAXIT and st.iters == 5
+
+
+def test_spmv_variants_bitwise_identical_and_autotune(ctx):
+    """Every SpMV launch variant gives the same bits (same arithmetic, same order);
+    the autotuner times them all and keeps one."""
+    sc = scene.Scene(40, sigma_x=1e-3, sigma_v=0.05)
+    for prec, mode in ((32, eolc.SOLVE_MIXED_BJ), (64, eolc.SOLVE_FP64_BJ)):
+        out = []
+        for var in range(eolc.SPMV_VARIANTS):
+            print(f"spmv variant {var} precision {prec}", flush=True)
+            setup(ctx, sc)
+            ctx.set_spmv_variant(prec, var)
+            rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-9, maxit=20000, mode=mode)
+            assert rc == 0
+            out.append((ctx.download_state(), st.iters))
+        for (x, v), it in out[1:]:
+            assert it == out[0][1]
+            assert np.array_equal(x, out[0][0][0]) and np.array_equal(v, out[0][0][1])
+        setup(ctx, sc)
+        ctx.forces_fill(GRAV, H)
+        best, ms = ctx.autotune(prec, reps=5)
+        print(f"autotune fp{prec}: best {best}, ms {np.round(ms * 1e3, 1)} us", flush=True)
+        assert 0 <= best < eolc.SPMV_VARIANTS and np.all(ms > 0) and ms[best] == ms.min()
