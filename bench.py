@@ -50,11 +50,12 @@ def log(*a):
 
 
 def spmv_bytes(N, nb, s, diff):
-    """Algorithmic bytes of one k_pcg_spmv launch over the nb stored blocks: block values
-    (9 scalars, plus the mass scalar in the FP32 difference form), column indices,
-    per-row length and row index, z and p_old read, p_new and q written, fixed mask."""
-    per_block = (10 if diff else 9) * s + 4
-    return per_block * nb + 8 * N + 4 * s * 3 * N + N
+    """Algorithmic bytes of one k_pcg_spmv launch: 9 scalars and a column index per
+    block read (the FP32 difference form skips the N diagonal blocks and reads one
+    row-mass scalar per row instead), per-row length and row index, z and p_old
+    read, p_new and q written, fixed mask."""
+    blocks = nb - N if diff else nb
+    return (9 * s + 4) * blocks + (4 * N if diff else 0) + 8 * N + 4 * s * 3 * N + N
 
 
 def run_path(ctx, sc, mode, steps, warmup, maxit):
@@ -173,6 +174,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-cloth workload")
+    ap.add_argument("--profile-solve", action="store_true",
+                    help="kernel-trace runs: one step of each solve mode capped at 400 PCG iterations, then exit")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="counter runs: assemble once, 10 launches of each SpMV kernel and a 1 GiB "
                          "device copy for calibration, then exit")
@@ -199,6 +202,14 @@ def main():
         best, ms = ctx.autotune(prec, reps=10)
         tuned[prec] = {"variant": best, "us": [round(float(m) * 1e3, 1) for m in ms]}
         log(f"autotune fp{prec}: variant {best}, us per launch {tuned[prec]['us']}")
+
+    if args.profile_solve:
+        for mode, name in ((eolc.SOLVE_MIXED_BJ, "mixed-bj"), (eolc.SOLVE_FP64_BJ, "fp64-bj")):
+            ctx.upload_state(sc.x0, sc.v0)
+            rc, st = ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=400, mode=mode, allow=(eolc.ERR_MAXIT,))
+            log(f"{name}: {st.iters} iterations, solve {st.ms_solve:.1f} ms, assembly {st.ms_assembly:.1f} ms")
+        ctx.close()
+        return
 
     if args.profile_spmv:
         # every SpMV variant of both precisions (10 launches each, after the 10 of

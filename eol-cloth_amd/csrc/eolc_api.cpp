@@ -189,8 +189,7 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
             if ((blist[e] >> 1) < 6 * F) { t.has_tri[b] = 1; break; }
     // SELL-C-sigma layout: rows sorted by decreasing length inside windows of
     // sigma = kSellSigma rows (never across a segment boundary), then cut into
-    // slices of C = 64.  A window of 4 slices separates the two row lengths of a
-    // checkerboard grid, so every lane of a slice has the same trip count.
+    // slices of C = 64.
     const int C = eolc::kSlice;
     t.nslices = (N + C - 1) / C;
     t.slice_off.assign(t.nslices + 1, 0);
@@ -224,18 +223,31 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
     t.sellpos.assign(t.nb, -1);
     t.sdiag.assign(N, -1);
     std::vector<int> entry_block(t.nbs, -1);
+    // Within a row the entries keep the CSR (increasing column) order except the
+    // diagonal block, which is placed last: the FP32 difference-form product stops
+    // one entry early and never reads it.
     for (int sl = 0; sl < t.nslices; ++sl) {
         const int r0 = sl * C, r1 = std::min(N, r0 + C);
         for (int l = 0; l < r1 - r0; ++l) {
             const int row = t.srow[r0 + l];
-            for (int k = 0; k < t.slen[r0 + l]; ++k) {
+            int k = 0, bdiag = -1;
+            for (int b = t.rowptr[row]; b < t.rowptr[row + 1]; ++b) {
+                if (t.col[b] == row) {
+                    bdiag = b;
+                    continue;
+                }
                 const int e = t.slice_off[sl] + k * C + l;
-                const int b = t.rowptr[row] + k;
                 t.scol[e] = t.col[b];
                 t.sellpos[b] = e;
                 entry_block[e] = b;
-                if (t.col[b] == row) t.sdiag[row] = e;
+                ++k;
             }
+            if (bdiag < 0 || k != t.slen[r0 + l] - 1) return "row without its diagonal block";
+            const int e = t.slice_off[sl] + k * C + l;
+            t.scol[e] = row;
+            t.sellpos[bdiag] = e;
+            entry_block[e] = bdiag;
+            t.sdiag[row] = e;
             // padding entries point at the row itself (never read as values)
             for (int k = t.slen[r0 + l]; k < (t.slice_off[sl + 1] - t.slice_off[sl]) / C; ++k)
                 t.scol[t.slice_off[sl] + k * C + l] = row;
@@ -476,7 +488,7 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(dalloc(c, &c->rhs, n3));
     EOLC_HIP(dalloc(c, &c->A64, 9 * (size_t)T.nbs));
     EOLC_HIP(dalloc(c, &c->S32, 9 * (size_t)T.nbs));
-    EOLC_HIP(dalloc(c, &c->M32, (size_t)T.nbs));
+    EOLC_HIP(dalloc(c, &c->M32, (size_t)N));
     {
         std::vector<int> isd((size_t)T.nbs, 0);
         for (int i = 0; i < N; ++i) isd[T.sdiag[i]] = 1;
@@ -667,7 +679,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         EOLC_HIP(eolc::learned_precond(c));
         if (c->bj_iters_hint > 0) cap = std::min(maxit, 2 * c->bj_iters_hint);
     }
-    const double delta = mixed ? 0.1 : 0.0;
+    const double delta = mixed ? c->mixed_delta : 0.0;
     EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
     EOLC_HIP(eolc::pcg_init(c, mixed));
     int k = 0, refinements = 0, status = EOLC_ERR_MAXIT;
@@ -1216,6 +1228,15 @@ int eolc_time_spmv(eolc_ctx *c, int precision, int reps, double *avg_ms)
     float ms = 0.f;
     EOLC_HIP(eolc::time_spmv(c, precision == 32, -1, reps, &ms));
     *avg_ms = ms / reps;
+    return EOLC_OK;
+}
+
+int eolc_set_mixed_delta(eolc_ctx *c, double delta)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!(delta > 0.0 && delta < 1.0)) return fail(c, EOLC_ERR_INVALID, "set_mixed_delta: need 0 < delta < 1");
+    c->mixed_delta = delta;
+    drop_batch_graph(c);
     return EOLC_OK;
 }
 

@@ -43,7 +43,12 @@ struct SolverScalars {
 // with slen[..] blocks.  The CSR form (rowptr/col, columns sorted) is kept on the
 // host for downloads.
 constexpr int kSlice = 64;
-constexpr int kSellSigma = 256;   // rows sorted by length inside windows of 4 slices
+#ifndef EOLC_SELL_SIGMA
+#define EOLC_SELL_SIGMA 64
+#endif
+constexpr int kSellSigma = EOLC_SELL_SIGMA;   // rows sorted by length inside windows of this many rows
+// (64: one slice; a window of 4 slices evens the trip counts of a checkerboard grid
+// but spreads each slice's gathers wider and measured 5% slower)
 
 struct Topology {
     int N = 0, F = 0, H = 0;
@@ -161,8 +166,8 @@ struct eolc_ctx {
     double *stageM = nullptr;   // 6F mass scalars
     double *x = nullptr, *v = nullptr, *f = nullptr, *rhs = nullptr;
     double *A64 = nullptr;      // 9 x nbs (SoA, SELL entry order)
-    float *S32 = nullptr;       // 9 x nbs: dampingB h^2 K off the diagonal, 0 on it
-    float *M32 = nullptr;       // nbs mass scalars (FP32)
+    float *S32 = nullptr;       // 9 x nbs: dampingB h^2 K_ij + m_ij I off the diagonal, 0 on it
+    float *M32 = nullptr;       // N lumped row masses sum_j m_ij (FP32, written by k_rhs)
     int *entry_is_diag = nullptr;  // nbs
     double *Mval = nullptr;     // nbs mass scalars (M block = m I3)
     double *D64 = nullptr;      // 9 x N  block-Jacobi inverse (SoA)
@@ -190,6 +195,7 @@ struct eolc_ctx {
     // learned preconditioner
     eolc::LearnedState lp;
     int spmv_variant[2] = {0, 0};   // SpMV launch variant for FP32 [0] and FP64 [1] solves
+    double mixed_delta = 0.5;       // reliable-update trigger of the mixed solves
     // per-element material (batched cloths): Young's modulus per face, bending
     // stiffness per hinge; null = the context's scalar material
     double *e_face = nullptr;

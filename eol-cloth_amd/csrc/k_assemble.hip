@@ -10,13 +10,15 @@
 //                come out 0); MDK block = sum of its staged slots (a slot
 //                flagged "transposed" is the (w,v) image of an element's (v,w)
 //                block); the mass scalar sums the triangle slots.  Writes the FP64
-//                SoA values and, for the mixed solver, the FP32 difference form:
-//                S_ij = dampingB h^2 K_ij off the diagonal (0 on it) and m_ij, with
-//                which (A p)_i = sum_j m_ij p_j + sum_j S_ij (p_j - p_i).
+//                SoA values and, for the mixed solver, the FP32 difference form
+//                S'_ij = dampingB h^2 K_ij + m_ij I off the diagonal (0 on it), with
+//                which (A p)_i = Mbar_i p_i + sum_{j != i} S'_ij (p_j - p_i),
+//                Mbar_i = sum_j m_ij (since sum_j m_ij p_j = Mbar_i p_i +
+//                sum_{j != i} m_ij (p_j - p_i)).
 //   k_rhs      : one thread per vertex (SELL storage order); f_i = sum of its element force slots
-//                (faces, then hinges when the bending force is included) and
+//                (faces, then hinges when the bending force is included),
 //                rhs_i = sum_j m_ij v_j + h f_i, the right-hand side M v + h f of
-//                Cloth::solve (Cloth.cpp:345).
+//                Cloth::solve (Cloth.cpp:345), and the FP32 row mass Mbar_i.
 #include "eolc_internal.h"
 
 namespace {
@@ -24,7 +26,7 @@ namespace {
 __global__ void __launch_bounds__(256)
 k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *__restrict__ clist,
            const int *__restrict__ entry_is_diag, const double *__restrict__ stageK, const double *__restrict__ stageM,
-           double *__restrict__ A64, float *__restrict__ S32, float *__restrict__ M32, double *__restrict__ Mval)
+           double *__restrict__ A64, float *__restrict__ S32, double *__restrict__ Mval)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbs) return;
@@ -54,8 +56,7 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
     if (S32) {
         const bool diag = entry_is_diag[b] != 0;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) S32[c * nbs + b] = diag ? 0.0f : (float)stf[c];
-        M32[b] = (float)m;
+        for (int c = 0; c < 9; ++c) S32[c * nbs + b] = diag ? 0.0f : (float)(stf[c] + (c % 4 == 0 ? m : 0.0));
     }
     Mval[b] = m;
 }
@@ -65,7 +66,8 @@ __global__ void __launch_bounds__(256)
 k_rhs(int N, int ntriforce, int include_bending, double h, const int *__restrict__ fptr,
       const int *__restrict__ flist, const double *__restrict__ stageF, const int *__restrict__ slice_off,
       const int *__restrict__ srow, const int *__restrict__ slen, const int *__restrict__ scol,
-      const double *__restrict__ Mval, const double *__restrict__ v, double *__restrict__ f, double *__restrict__ rhs)
+      const double *__restrict__ Mval, const double *__restrict__ v, double *__restrict__ f, double *__restrict__ rhs,
+      float *__restrict__ Mrow)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= N) return;
@@ -80,13 +82,15 @@ k_rhs(int N, int ntriforce, int include_bending, double h, const int *__restrict
     }
     f[3 * i] = f0; f[3 * i + 1] = f1; f[3 * i + 2] = f2;
     // M v with the columns in increasing order (SELL keeps the CSR order within a row)
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0, msum = 0.0;
     for (int k = 0; k < len; ++k) {
         const int e = base + k * eolc::kSlice;
         const int j = scol[e];
         const double m = Mval[e];
         y0 += m * v[3 * j]; y1 += m * v[3 * j + 1]; y2 += m * v[3 * j + 2];
+        msum += m;
     }
+    if (Mrow) Mrow[i] = (float)msum;
     rhs[3 * i] = y0 + h * f0;
     rhs[3 * i + 1] = y1 + h * f1;
     rhs[3 * i + 2] = y2 + h * f2;
@@ -103,8 +107,7 @@ hipError_t launch_assemble(eolc_ctx *c, bool want32)
     const int64_t nb = c->topo.nbs;
     if (nb == 0) return hipSuccess;
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
-                       c->entry_is_diag, c->stageK, c->stageM, c->A64, want32 ? c->S32 : nullptr,
-                       want32 ? c->M32 : nullptr, c->Mval);
+                       c->entry_is_diag, c->stageK, c->stageM, c->A64, want32 ? c->S32 : nullptr, c->Mval);
     return hipGetLastError();
 }
 
@@ -113,7 +116,7 @@ hipError_t launch_rhs(eolc_ctx *c, double h)
     const int N = c->topo.N;
     hipLaunchKernelGGL(k_rhs, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, 3 * c->topo.F,
                        (c->flags & EOLC_OPT_INCLUDE_BENDING_FORCE) ? 1 : 0, h, c->fptr, c->flist, c->stageF,
-                       c->slice_off, c->srow, c->slen, c->scol, c->Mval, c->v, c->f, c->rhs);
+                       c->slice_off, c->srow, c->slen, c->scol, c->Mval, c->v, c->f, c->rhs, c->M32);
     return hipGetLastError();
 }
 

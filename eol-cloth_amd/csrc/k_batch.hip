@@ -8,7 +8,8 @@
 // different XCDs, which receive workgroups round-robin) and runs that instance's
 // whole mixed-precision PCG -- the velocity solve of GeneralizedSolver::velocitySolve
 // for solver "none" (GeneralizedSolver.cpp:107-172) -- to its own tolerance:
-//   FP32 difference-form SpMV and block-Jacobi, FP64 dot products, FP64 true
+//   FP32 difference-form SpMV (row mass plus S'_ij (p_j - p_i), the diagonal
+//   entry skipped) and block-Jacobi, FP64 dot products, FP64 true
 //   residual at every reliable update (delta), iterate accumulated in FP64.
 // No grid-wide synchronisation exists: the instances are independent, a workgroup
 // only touches its own rows, and an instance that has converged simply leaves
@@ -239,27 +240,24 @@ __global__ void __launch_bounds__(WGS) k_batch_pcg(BatchArgs a)
         // p_new = z + beta p_old (own rows; neighbours formed on the fly), q = A p_new
         for (int sl = I.s0 + wave; sl < I.s1; sl += NW) {
             const int g = sl * SL + lane;
-            const int row = a.srow[g], len = a.slen[g];
+            const int row = a.srow[g], len = a.slen[g] - 1;   // the diagonal (last entry) is skipped
             const int base = a.slice_off[sl] + lane;
             const float ui0 = fmaf(betaT, pold[3 * row], a.z32[3 * row]);
             const float ui1 = fmaf(betaT, pold[3 * row + 1], a.z32[3 * row + 1]);
             const float ui2 = fmaf(betaT, pold[3 * row + 2], a.z32[3 * row + 2]);
-            double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+            const float mb = a.M32[row];
+            double y0 = (double)(mb * ui0), y1 = (double)(mb * ui1), y2 = (double)(mb * ui2);
             for (int kk = 0; kk < len; ++kk) {
                 const int e = base + kk * SL;
                 const int j = a.scol[e];
                 const float u0 = fmaf(betaT, pold[3 * j], a.z32[3 * j]);
                 const float u1 = fmaf(betaT, pold[3 * j + 1], a.z32[3 * j + 1]);
                 const float u2 = fmaf(betaT, pold[3 * j + 2], a.z32[3 * j + 2]);
-                const float m = a.M32[e];
                 const float d0 = u0 - ui0, d1 = u1 - ui1, d2 = u2 - ui2;
                 const float *s = a.S32 + e;
-                const float t0 = fmaf(s[2 * nbs], d2, fmaf(s[1 * nbs], d1, s[0 * nbs] * d0));
-                const float t1 = fmaf(s[5 * nbs], d2, fmaf(s[4 * nbs], d1, s[3 * nbs] * d0));
-                const float t2 = fmaf(s[8 * nbs], d2, fmaf(s[7 * nbs], d1, s[6 * nbs] * d0));
-                y0 += (double)fmaf(m, u0, t0);
-                y1 += (double)fmaf(m, u1, t1);
-                y2 += (double)fmaf(m, u2, t2);
+                y0 += (double)fmaf(s[2 * nbs], d2, fmaf(s[1 * nbs], d1, s[0 * nbs] * d0));
+                y1 += (double)fmaf(s[5 * nbs], d2, fmaf(s[4 * nbs], d1, s[3 * nbs] * d0));
+                y2 += (double)fmaf(s[8 * nbs], d2, fmaf(s[7 * nbs], d1, s[6 * nbs] * d0));
             }
             const unsigned fm = a.fixmask[row];
             if (fm & 1) y0 = 0.0;
@@ -392,7 +390,7 @@ hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant)
     a.v = c->v; a.xs64 = c->xs64; a.r64 = c->r64; a.b64 = c->b64;
     a.r32 = c->r32; a.z32 = c->z32; a.q32 = c->q32; a.xl32 = c->xl32; a.p32a = c->p32[0]; a.p32b = c->p32[1];
     a.part = S.slice_part;
-    a.tol2 = tol * tol; a.delta2 = 0.1 * 0.1; a.maxit = maxit;
+    a.tol2 = tol * tol; a.delta2 = c->mixed_delta * c->mixed_delta; a.maxit = maxit;
     a.stats = S.stats; a.dstats = S.dstats;
     switch (variant) {
     case 0: launch_batch<256>(c, a); break;

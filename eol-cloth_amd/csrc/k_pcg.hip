@@ -135,11 +135,14 @@ __device__ __forceinline__ double *PB(double *p, int parity) { return p + (1 + 2
 
 // ---------------------------------------------------------------------------
 // DIFF = false (FP64 path): q = A p with the assembled blocks of MDK.
-// DIFF = true  (FP32 path): q_i = sum_j m_ij p_j + sum_j S_ij (p_j - p_i), S = dampingB h^2 K
-// with zero diagonal blocks.  The stiffness then annihilates translations exactly
-// and its FP32 rounding is relative to the local differences p_j - p_i, not to |p|:
-// an assembled FP32 copy of M + h^2 K loses the mass (~1e-8) under the rounding of
-// the stiffness (~1e-2 x 6e-8) in the smooth, slowly converging modes.
+// DIFF = true  (FP32 path): q_i = Mbar_i p_i + sum_{j != i} S'_ij (p_j - p_i) with
+// S'_ij = dampingB h^2 K_ij + m_ij I and the row mass Mbar_i = sum_j m_ij (exact
+// algebra: sum_j m_ij p_j = Mbar_i p_i + sum_{j != i} m_ij (p_j - p_i)).  The
+// stiffness then annihilates translations exactly and its FP32 rounding is relative
+// to the local differences p_j - p_i, not to |p|; the smooth, slowly converging modes
+// see the mass through Mbar_i.  An assembled FP32 copy of M + h^2 K instead loses the
+// mass (~1e-8) under the rounding of the stiffness (~1e-2 x 6e-8).  Each row's
+// diagonal block is its last SELL entry, so the FP32 loop stops one entry early.
 //
 // Launch variants (same arithmetic in the same order, so bitwise-identical results;
 // chosen per matrix by eolc_autotune):
@@ -157,20 +160,17 @@ __device__ __forceinline__ T ldm(const T *p)
 }
 
 // One block of the product, accumulated into (y0, y1, y2):
-//   DIFF:  y += m u + S (u - ui)   (FP32 terms, FP64 sums)
-//   else:  y += A u               (FP64)
+//   DIFF:  y += S' (u - ui)   (FP32 terms, FP64 sums)
+//   else:  y += A u          (FP64)
 template <class T, bool DIFF>
-__device__ __forceinline__ void block_apply(T a0, T a1, T a2, T a3, T a4, T a5, T a6, T a7, T a8, T m, T u0, T u1,
-                                            T u2, T ui0, T ui1, T ui2, double &y0, double &y1, double &y2)
+__device__ __forceinline__ void block_apply(T a0, T a1, T a2, T a3, T a4, T a5, T a6, T a7, T a8, T u0, T u1, T u2,
+                                            T ui0, T ui1, T ui2, double &y0, double &y1, double &y2)
 {
     if constexpr (DIFF) {
         const T d0 = u0 - ui0, d1 = u1 - ui1, d2 = u2 - ui2;
-        const T t0 = fma(a2, d2, fma(a1, d1, a0 * d0));
-        const T t1 = fma(a5, d2, fma(a4, d1, a3 * d0));
-        const T t2 = fma(a8, d2, fma(a7, d1, a6 * d0));
-        y0 += (double)fma(m, u0, t0);
-        y1 += (double)fma(m, u1, t1);
-        y2 += (double)fma(m, u2, t2);
+        y0 += (double)fma(a2, d2, fma(a1, d1, a0 * d0));
+        y1 += (double)fma(a5, d2, fma(a4, d1, a3 * d0));
+        y2 += (double)fma(a8, d2, fma(a7, d1, a6 * d0));
     } else {
         y0 += fma((double)a2, (double)u2, fma((double)a1, (double)u1, (double)a0 * (double)u0));
         y1 += fma((double)a5, (double)u2, fma((double)a4, (double)u1, (double)a3 * (double)u0));
@@ -181,23 +181,21 @@ __device__ __forceinline__ void block_apply(T a0, T a1, T a2, T a3, T a4, T a5, 
 template <class T, bool DIFF>
 struct SpmvEntry {
     T a[9];
-    T m;
     T u0, u1, u2;
     template <bool NT>
-    __device__ __forceinline__ void load(const T *__restrict__ A, const T *__restrict__ Mm, int64_t nbs, int e, int j,
-                                         T betaT, const T *__restrict__ z, const T *__restrict__ pold)
+    __device__ __forceinline__ void load(const T *__restrict__ A, int64_t nbs, int e, int j, T betaT,
+                                         const T *__restrict__ z, const T *__restrict__ pold)
     {
         u0 = fma(betaT, pold[3 * j], z[3 * j]);
         u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
         u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
 #pragma unroll
         for (int c = 0; c < 9; ++c) a[c] = ldm<T, NT>(A + c * nbs + e);
-        if constexpr (DIFF) m = ldm<T, NT>(Mm + e);
     }
     __device__ __forceinline__ void accumulate(T ui0, T ui1, T ui2, double &y0, double &y1, double &y2) const
     {
-        block_apply<T, DIFF>(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], m, u0, u1, u2, ui0, ui1, ui2, y0,
-                             y1, y2);
+        block_apply<T, DIFF>(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], u0, u1, u2, ui0, ui1, ui2, y0, y1,
+                             y2);
     }
 };
 
@@ -234,10 +232,12 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
     const int g = sl * eolc::kSlice + lane;
     double dot = 0.0;
     if (sl < nslices && g < N) {
-        const int row = srow[g], len = slen[g];   // len >= 1: every row holds its diagonal block
+        const int row = srow[g];
+        // every row holds its diagonal block as the last entry; the FP32 difference
+        // form skips it (its S' block is 0 and the row mass is applied below)
+        const int len = slen[g] - (DIFF ? 1 : 0);
         const int base = slice_off[sl] + lane;
         const T *__restrict__ a = A + base;
-        const T *__restrict__ mm = DIFF ? Mm + base : nullptr;
         const int *__restrict__ cl = scol + base;
         T ui0 = fma(betaT, pold[3 * row], z[3 * row]);
         T ui1 = fma(betaT, pold[3 * row + 1], z[3 * row + 1]);
@@ -247,6 +247,12 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
             pnew[3 * row] = ui0; pnew[3 * row + 1] = ui1; pnew[3 * row + 2] = ui2;
         }
         double y0 = 0, y1 = 0, y2 = 0;
+        if constexpr (DIFF) {
+            const T mb = Mm[row];
+            y0 = (double)(mb * ui0);
+            y1 = (double)(mb * ui1);
+            y2 = (double)(mb * ui2);
+        }
         int kk = 0;
         if constexpr (U == 1) {
             int jn = ldm<int, NT>(cl);
@@ -268,7 +274,7 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
                 block_apply<T, DIFF>(ldm<T, NT>(ae), ldm<T, NT>(ae + nbs), ldm<T, NT>(ae + 2 * nbs),
                                      ldm<T, NT>(ae + 3 * nbs), ldm<T, NT>(ae + 4 * nbs), ldm<T, NT>(ae + 5 * nbs),
                                      ldm<T, NT>(ae + 6 * nbs), ldm<T, NT>(ae + 7 * nbs), ldm<T, NT>(ae + 8 * nbs),
-                                     DIFF ? ldm<T, NT>(mm + e) : T(0), u0, u1, u2, ui0, ui1, ui2, y0, y1, y2);
+                                     u0, u1, u2, ui0, ui1, ui2, y0, y1, y2);
             }
         } else {
             int jn0 = ldm<int, NT>(cl), jn1 = len > 1 ? ldm<int, NT>(cl + eolc::kSlice) : 0;
@@ -287,15 +293,15 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
                     j1 = ldm<int, NT>(cl + e + eolc::kSlice);
                 }
                 SpmvEntry<T, DIFF> x0, x1;
-                x0.template load<NT>(a, mm, nbs, e, j0, betaT, z, pold);
-                x1.template load<NT>(a, mm, nbs, e + eolc::kSlice, j1, betaT, z, pold);
+                x0.template load<NT>(a, nbs, e, j0, betaT, z, pold);
+                x1.template load<NT>(a, nbs, e + eolc::kSlice, j1, betaT, z, pold);
                 x0.accumulate(ui0, ui1, ui2, y0, y1, y2);
                 x1.accumulate(ui0, ui1, ui2, y0, y1, y2);
             }
             if (kk < len) {
                 const int e = kk * eolc::kSlice;
                 SpmvEntry<T, DIFF> x0;
-                x0.template load<NT>(a, mm, nbs, e, ldm<int, NT>(cl + e), betaT, z, pold);
+                x0.template load<NT>(a, nbs, e, ldm<int, NT>(cl + e), betaT, z, pold);
                 x0.accumulate(ui0, ui1, ui2, y0, y1, y2);
             }
         }
@@ -395,7 +401,7 @@ k_resid64(int N, int nslices, const int *__restrict__ slice_off, const int *__re
             const double u0 = xin[3 * j], u1 = xin[3 * j + 1], u2 = xin[3 * j + 2];
             block_apply<double, false>(a[0 * nbs + e], a[1 * nbs + e], a[2 * nbs + e], a[3 * nbs + e],
                                        a[4 * nbs + e], a[5 * nbs + e], a[6 * nbs + e], a[7 * nbs + e],
-                                       a[8 * nbs + e], 0.0, u0, u1, u2, 0.0, 0.0, 0.0, y0, y1, y2);
+                                       a[8 * nbs + e], u0, u1, u2, 0.0, 0.0, 0.0, y0, y1, y2);
         }
         double o0 = (b ? b[3 * row] : 0.0) - y0;
         double o1 = (b ? b[3 * row + 1] : 0.0) - y1;
@@ -555,12 +561,23 @@ __global__ void k_scal_restart(SolverScalars *s, int k)
 
 inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
-// Streaming copy (16-byte loads and stores, grid stride): the known-size kernel
-// that calibrates the memory counters and measures achievable HBM bandwidth.
-__global__ void __launch_bounds__(256) k_stream_copy(int64_t n16, const uint4 *__restrict__ src, uint4 *__restrict__ dst)
+// Streaming copy (16-byte non-temporal loads and stores, four in flight per
+// thread, grid stride): the known-size kernel that calibrates the memory counters
+// and measures achievable HBM bandwidth.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_stream_copy(int64_t n16, const u32x4 *__restrict__ src, u32x4 *__restrict__ dst)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        u32x4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(v[u], dst + i + u * stride);
+    }
+    for (; i < n16; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 __global__ void k_fill(int n, double v, double *__restrict__ out)
@@ -749,7 +766,7 @@ hipError_t spmv_true_residual(eolc_ctx *c)
 hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms)
 {
     const int64_t n16 = (int64_t)(bytes / 16);
-    uint4 *a = nullptr, *b = nullptr;
+    u32x4 *a = nullptr, *b = nullptr;
     hipError_t e = hipMalloc((void **)&a, n16 * 16);
     if (e != hipSuccess) return e;
     e = hipMalloc((void **)&b, n16 * 16);
@@ -760,7 +777,7 @@ hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms)
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const int grid = 256 * 8 * 4;   // 8 workgroups per CU on 256 CUs, 4x over
+    const int grid = 256 * 8;   // 8 workgroups of 256 threads per CU on 256 CUs
     hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, n16, a, b);   // first touch
     (void)hipEventRecord(e0, c->stream);
     for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, n16, a, b);

@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "eolc_time_spmv", "eolc_learned_set_weights", "eolc_learned_get_weights", "eolc_learned_infer",
     "eolc_learned_loss_grad", "eolc_learned_train", "eolc_autotune", "eolc_set_spmv_variant",
     "eolc_set_batch_topology", "eolc_set_batch_material", "eolc_batch_step", "eolc_batch_autotune",
-    "eolc_set_batch_variant", "eolc_stream_copy",
+    "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta",
 )
 
 BATCH_EAGER = 0x1
@@ -107,6 +107,7 @@ def lib():
         L.eolc_batch_autotune.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int),
                                           C.c_void_p]
         L.eolc_set_batch_variant.argtypes = [vp, C.c_int]
+        L.eolc_set_mixed_delta.argtypes = [vp, C.c_double]
         L.eolc_stream_copy.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.eolc_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_void_p]
         L.eolc_set_spmv_variant.argtypes = [vp, C.c_int, C.c_int]
@@ -256,6 +257,9 @@ class Context:
                 continue
             self._check(rc)
             return [buf[i] for i in range(cnt.value)]
+
+    def set_mixed_delta(self, delta):
+        return self._check(lib().eolc_set_mixed_delta(self._h, float(delta)))
 
     def stream_copy(self, nbytes, reps=10):
         """Average ms of one copy-kernel launch reading and writing nbytes each."""

@@ -53,6 +53,11 @@ typedef enum {
                                        converged block-Jacobi solve on the context */
 #define EOLC_SOLVE_FP64_JACOBI   3  /* FP64 scalar-Jacobi PCG (Eigen's default) */
 
+/* Reliable-update trigger of the mixed modes (default 0.5): the FP32 correction is
+ * folded into the FP64 solution and the residual replaced by the FP64 true residual
+ * whenever the recursive residual has dropped by this factor since the last update. */
+int eolc_set_mixed_delta(eolc_ctx *ctx, double delta);
+
 /* matrices (eolc_download_csr) */
 #define EOLC_MAT_M   0
 #define EOLC_MAT_MDK 1
