@@ -114,15 +114,32 @@ def load_weights():
     return learned.init_weights(7), "initial weights (zero output = block-Jacobi)"
 
 
-def measured_traffic(kernel, variant, n):
-    """HBM bytes per launch from the committed counter summary, if it matches."""
+# SpMV launch variants: (entries per trip, column prefetch, non-temporal loads,
+# waves per SIMD, symmetric layout), in the library's order
+SPMV_VARIANTS = [(1, False, False, 8, False), (1, True, False, 8, False), (1, True, True, 8, False),
+                 (1, False, True, 8, False), (1, True, True, 6, False), (2, True, True, 6, False),
+                 (2, True, False, 6, False), (2, True, True, 4, False), (1, True, True, 4, False),
+                 (1, False, False, 8, True), (1, False, False, 6, True), (1, False, True, 8, True)]
+
+
+def spmv_kernel_name(fp32, variant):
+    u, pf, nt, w, sym = SPMV_VARIANTS[variant]
+    b = lambda x: "true" if x else "false"
+    t = "float, true" if fp32 else "double, false"
+    return f"k_pcg_spmv<{t}, {u}, {b(pf)}, {b(nt)}, {w}, {b(sym and fp32)}>"
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n x n from the committed counter summary."""
+    if n != 1024:
+        return None
     try:
         with open(TRAFFIC) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
     for k in t.get("kernels", []):
-        if k.get("kernel") == kernel and k.get("variant") == variant and k.get("n") == n:
+        if k.get("kernel_template") == kernel:
             return k.get("hbm_bytes")
     return None
 
@@ -197,6 +214,9 @@ def main():
 
     ctx.upload_state(sc.x0, sc.v0)
     ctx.forces_fill(GRAV, H)
+    copy_ms = ctx.stream_copy(1 << 30, 10)
+    copy_gbps = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    log(f"copy kernel: 1 GiB read + 1 GiB written in {copy_ms:.3f} ms ({copy_gbps:.0f} GB/s)")
     tuned = {}
     for prec in (32, 64):
         best, ms = ctx.autotune(prec, reps=10)
@@ -256,11 +276,11 @@ def main():
     # algorithmic bytes over the SELL storage actually streamed (padding excluded)
     alg = spmv_bytes(sc.N, nb, 4, True)
     achieved = alg / (spmv32_ms * 1e-3)
-    kname = "k_pcg_spmv<float, true>"
+    kname = spmv_kernel_name(True, tuned[32]["variant"])
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname, tuned[32]["variant"], args.n),
+            "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname, args.n),
             "kernel": kname, "variant": tuned[32]["variant"], "algorithmic_bytes": alg,
-            "avg_launch_ms": spmv32_ms}
+            "avg_launch_ms": spmv32_ms, "copy_kernel_GBps": copy_gbps}
     ctx.close()
     batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit)
     cpu = None

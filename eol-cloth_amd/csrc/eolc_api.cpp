@@ -62,6 +62,7 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->stageK); dfree(c->stageF); dfree(c->stageM);
     dfree(c->x); dfree(c->v); dfree(c->f); dfree(c->rhs);
     dfree(c->A64); dfree(c->S32); dfree(c->M32); dfree(c->entry_is_diag); dfree(c->Mval); dfree(c->D64); dfree(c->D32);
+    dfree(c->supper); dfree(c->uoff); dfree(c->loff); dfree(c->lrec); dfree(c->e2u); dfree(c->S32u);
     dfree(c->fixmask); dfree(c->vfix);
     dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
     dfree(c->p64[0]); dfree(c->p64[1]);
@@ -223,25 +224,31 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
     t.sellpos.assign(t.nb, -1);
     t.sdiag.assign(N, -1);
     std::vector<int> entry_block(t.nbs, -1);
-    // Within a row the entries keep the CSR (increasing column) order except the
-    // diagonal block, which is placed last: the FP32 difference-form product stops
-    // one entry early and never reads it.
+    // Entry order within a row: the upper blocks (column > row) in increasing column
+    // order, then the lower blocks (column < row) in increasing column order, then
+    // the diagonal block last.  The FP32 difference-form product stops one entry
+    // early and never reads the diagonal, and the symmetric FP32 layout below sums
+    // in exactly this order.
+    t.supper.assign(N, 0);
     for (int sl = 0; sl < t.nslices; ++sl) {
         const int r0 = sl * C, r1 = std::min(N, r0 + C);
         for (int l = 0; l < r1 - r0; ++l) {
             const int row = t.srow[r0 + l];
             int k = 0, bdiag = -1;
-            for (int b = t.rowptr[row]; b < t.rowptr[row + 1]; ++b) {
-                if (t.col[b] == row) {
-                    bdiag = b;
-                    continue;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int b = t.rowptr[row]; b < t.rowptr[row + 1]; ++b) {
+                    if (t.col[b] == row) {
+                        bdiag = b;
+                        continue;
+                    }
+                    if ((pass == 0) != (t.col[b] > row)) continue;
+                    const int e = t.slice_off[sl] + k * C + l;
+                    t.scol[e] = t.col[b];
+                    t.sellpos[b] = e;
+                    entry_block[e] = b;
+                    ++k;
+                    if (pass == 0) t.supper[r0 + l] = k;
                 }
-                const int e = t.slice_off[sl] + k * C + l;
-                t.scol[e] = t.col[b];
-                t.sellpos[b] = e;
-                entry_block[e] = b;
-                ++k;
-            }
             if (bdiag < 0 || k != t.slen[r0 + l] - 1) return "row without its diagonal block";
             const int e = t.slice_off[sl] + k * C + l;
             t.scol[e] = row;
@@ -251,6 +258,68 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
             // padding entries point at the row itself (never read as values)
             for (int k = t.slen[r0 + l]; k < (t.slice_off[sl + 1] - t.slice_off[sl]) / C; ++k)
                 t.scol[t.slice_off[sl] + k * C + l] = row;
+        }
+    }
+    // Symmetric FP32 layout: each off-diagonal block stored once, with its upper
+    // row.  Per slice, upper records (column + 9 values, 40 B) at
+    // uoff[s] + 64 k + lane for k < supper, and lower records (column, upper slot of
+    // the transposed block) at loff[s] + 64 k + lane for k < slen - 1 - supper.
+    t.uoff.assign(t.nslices + 1, 0);
+    t.loff.assign(t.nslices + 1, 0);
+    for (int sl = 0; sl < t.nslices; ++sl) {
+        const int r0 = sl * C, r1 = std::min(N, r0 + C);
+        int Wu = 0, Wl = 0;
+        for (int g = r0; g < r1; ++g) {
+            Wu = std::max(Wu, t.supper[g]);
+            Wl = std::max(Wl, t.slen[g] - 1 - t.supper[g]);
+        }
+        t.uoff[sl + 1] = t.uoff[sl] + Wu * C;
+        t.loff[sl + 1] = t.loff[sl] + Wl * C;
+    }
+    t.nup = t.uoff[t.nslices];
+    t.nlo = t.loff[t.nslices];
+    t.e2u.assign(t.nbs, -1);
+    t.ucol.assign(t.nup, 0);
+    t.lrec.assign(2 * (size_t)t.nlo, 0);
+    std::vector<int> uposb(t.nb, -1);
+    for (int sl = 0; sl < t.nslices; ++sl) {
+        const int r0 = sl * C, r1 = std::min(N, r0 + C);
+        for (int l = 0; l < r1 - r0; ++l) {
+            const int row = t.srow[r0 + l];
+            const int wu = (t.uoff[sl + 1] - t.uoff[sl]) / C;
+            for (int k = 0; k < wu; ++k) {
+                const int u = t.uoff[sl] + k * C + l;
+                if (k < t.supper[r0 + l]) {
+                    const int e = t.slice_off[sl] + k * C + l;
+                    t.e2u[e] = u;
+                    t.ucol[u] = t.scol[e];
+                    uposb[entry_block[e]] = u;
+                } else {
+                    t.ucol[u] = row;   // padding
+                }
+            }
+        }
+    }
+    for (int sl = 0; sl < t.nslices; ++sl) {
+        const int r0 = sl * C, r1 = std::min(N, r0 + C);
+        for (int l = 0; l < r1 - r0; ++l) {
+            const int row = t.srow[r0 + l];
+            const int nl = t.slen[r0 + l] - 1 - t.supper[r0 + l];
+            const int wl = (t.loff[sl + 1] - t.loff[sl]) / C;
+            for (int k = 0; k < wl; ++k) {
+                const int lo = t.loff[sl] + k * C + l;
+                if (k < nl) {
+                    const int e = t.slice_off[sl] + (t.supper[r0 + l] + k) * C + l;
+                    const int j = t.scol[e];
+                    const int bt = find_block(t, j, row);
+                    if (bt < 0 || uposb[bt] < 0) return "block pattern is not symmetric";
+                    t.lrec[2 * (size_t)lo] = j;
+                    t.lrec[2 * (size_t)lo + 1] = uposb[bt];
+                } else {
+                    t.lrec[2 * (size_t)lo] = row;   // padding: never read
+                    t.lrec[2 * (size_t)lo + 1] = 0;
+                }
+            }
         }
     }
     // contributor lists in SELL entry order (padding entries: empty)
@@ -473,6 +542,11 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(upload(c, &c->slen, T.slen));
     EOLC_HIP(upload(c, &c->scol, T.scol));
     EOLC_HIP(upload(c, &c->diagpos, T.sdiag));
+    EOLC_HIP(upload(c, &c->supper, T.supper));
+    EOLC_HIP(upload(c, &c->uoff, T.uoff));
+    EOLC_HIP(upload(c, &c->loff, T.loff));
+    EOLC_HIP(upload(c, &c->lrec, T.lrec));
+    EOLC_HIP(upload(c, &c->e2u, T.e2u));
     EOLC_HIP(upload(c, &c->cptr, T.cptr));
     EOLC_HIP(upload(c, &c->clist, T.clist));
     EOLC_HIP(upload(c, &c->fptr, T.fptr));
@@ -489,14 +563,15 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(dalloc(c, &c->A64, 9 * (size_t)T.nbs));
     EOLC_HIP(dalloc(c, &c->S32, 9 * (size_t)T.nbs));
     EOLC_HIP(dalloc(c, &c->M32, (size_t)N));
+    EOLC_HIP(dalloc(c, &c->S32u, 10 * (size_t)std::max<int64_t>(T.nup, 1)));
     {
         std::vector<int> isd((size_t)T.nbs, 0);
         for (int i = 0; i < N; ++i) isd[T.sdiag[i]] = 1;
         EOLC_HIP(upload(c, &c->entry_is_diag, isd));
     }
     EOLC_HIP(dalloc(c, &c->Mval, (size_t)T.nbs));
-    EOLC_HIP(dalloc(c, &c->D64, 9 * (size_t)N));
-    EOLC_HIP(dalloc(c, &c->D32, 9 * (size_t)N));
+    EOLC_HIP(dalloc(c, &c->D64, eolc::kDsym * (size_t)N));
+    EOLC_HIP(dalloc(c, &c->D32, eolc::kDsym * (size_t)N));
     EOLC_HIP(dalloc(c, &c->fixmask, ((size_t)N + 3) & ~(size_t)3));
     EOLC_HIP(dalloc(c, &c->vfix, n3));
     EOLC_HIP(hipMemset(c->fixmask, 0, ((size_t)N + 3) & ~(size_t)3));
@@ -683,7 +758,13 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
     EOLC_HIP(eolc::pcg_init(c, mixed));
     int k = 0, refinements = 0, status = EOLC_ERR_MAXIT;
+    // Iterations are enqueued in chunks; the host synchronises once per chunk.  A
+    // chunk that overruns the stop point costs one no-op launch pair per extra
+    // iteration, so the next chunk is sized at 80% of the iterations the observed
+    // convergence rate predicts to the current threshold (8 .. 1024).
     int chunk = 8;
+    int k_prev = 0;
+    double rr_prev = -1.0;
     double relres = NAN;
     for (;;) {
         if (learned && k >= cap) {
@@ -703,8 +784,20 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         const eolc::SolverScalars s = *c->hscal;
         k = s.iters;
         relres = s.bnorm2 > 0 ? std::sqrt(s.rr / s.bnorm2) : 0.0;
-        if (chunk < 256) chunk *= 2;
+        {
+            const double thr = std::max(s.tol2 * s.bnorm2, s.delta2 * s.rmax2);
+            int next = std::min(2 * chunk, 1024);
+            if (rr_prev > 0.0 && k > k_prev && s.rr > thr && s.rr < rr_prev) {
+                const double rate = std::log(s.rr / rr_prev) / (k - k_prev);   // log reduction per iteration
+                next = (int)std::min(1024.0, std::max(8.0, 0.8 * std::log(thr / s.rr) / rate));
+            }
+            k_prev = k;
+            rr_prev = s.rr;
+            chunk = next;
+        }
         if (!s.done) continue;
+        rr_prev = -1.0;   // the residual is replaced below: measure the rate afresh
+        chunk = 16;
         if (s.reason == 2 && learned) {
             // breakdown under the learned blocks (P is SPD, so only from an
             // indefinite MDK): the block-Jacobi fallback reports it
@@ -879,13 +972,14 @@ int eolc_learned_infer(eolc_ctx *c, int n_fixed, const int *fixed_dof, float *ou
     if ((rc = fs.set(c, n_fixed, fixed_dof, nullptr))) return rc;
     const size_t N = (size_t)c->topo.N;
     EOLC_HIP(eolc::learned_precond(c));
-    std::vector<float> soa(P ? 9 * N : 0);
+    std::vector<float> soa(P ? eolc::kDsym * N : 0);
     if (out) EOLC_HIP(hipMemcpyAsync(out, c->lp.o, sizeof(float) * 6 * N, hipMemcpyDeviceToHost, c->stream));
-    if (P) EOLC_HIP(hipMemcpyAsync(soa.data(), c->D32, sizeof(float) * 9 * N, hipMemcpyDeviceToHost, c->stream));
+    if (P)
+        EOLC_HIP(hipMemcpyAsync(soa.data(), c->D32, sizeof(float) * eolc::kDsym * N, hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
     if (P)
         for (size_t i = 0; i < N; ++i)
-            for (int k = 0; k < 9; ++k) P[9 * i + k] = soa[(size_t)k * N + i];
+            for (int k = 0; k < 9; ++k) P[9 * i + k] = soa[(size_t)eolc::kSymIdx[k] * N + i];
     return EOLC_OK;
 }
 

@@ -58,6 +58,13 @@ struct Topology {
     int nslices = 0;
     int64_t nbs = 0;         // SELL entries including padding (stride of the SoA value arrays)
     std::vector<int> slice_off, srow, slen, scol, sellpos, sdiag;
+    // symmetric FP32 layout (each off-diagonal block once, with its upper row)
+    std::vector<int> supper;        // per storage position: number of upper entries
+    std::vector<int> uoff, loff;    // per slice: first upper / lower record
+    int64_t nup = 0, nlo = 0;       // records including padding
+    std::vector<int> e2u;           // per SELL entry: its upper record or -1
+    std::vector<int> ucol;          // per upper record: column
+    std::vector<int> lrec;          // per lower record: column, upper record of the transposed block
     // gather maps in SELL entry order (element slots -> entries, element forces -> vertices)
     std::vector<int> cptr;   // nbs + 1
     std::vector<int> clist;  // (slot << 1) | transposed
@@ -65,6 +72,23 @@ struct Topology {
     std::vector<int> fptr;   // N + 1
     std::vector<int> flist;  // force-vector slot (3F tri slots, then 4H hinge slots)
 };
+
+// Preconditioner blocks are symmetric by construction (block-Jacobi inverse by the
+// adjugate, learned G G^T) and stored as their 6 upper entries, SoA over vertices:
+// D[k * N + i], k = 00, 01, 02, 11, 12, 22.
+constexpr int kDsym = 6;
+constexpr int kSymIdx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};   // row-major 3x3 -> stored entry
+
+// z = D r for one vertex, each row as fma(d2, r2, fma(d1, r1, d0 r0)).
+template <class T>
+__host__ __device__ __forceinline__ void apply_sym3(const T *__restrict__ d, int64_t N, T r0, T r1, T r2, T &z0, T &z1,
+                                                    T &z2)
+{
+    const T d00 = d[0], d01 = d[N], d02 = d[2 * N], d11 = d[3 * N], d12 = d[4 * N], d22 = d[5 * N];
+    z0 = fma(d02, r2, fma(d01, r1, d00 * r0));
+    z1 = fma(d12, r2, fma(d11, r1, d01 * r0));
+    z2 = fma(d22, r2, fma(d12, r1, d02 * r0));
+}
 
 template <class T> struct DArr {
     T *p = nullptr;
@@ -168,9 +192,12 @@ struct eolc_ctx {
     double *A64 = nullptr;      // 9 x nbs (SoA, SELL entry order)
     float *S32 = nullptr;       // 9 x nbs: dampingB h^2 K_ij + m_ij I off the diagonal, 0 on it
     float *M32 = nullptr;       // N lumped row masses sum_j m_ij (FP32, written by k_rhs)
+    // symmetric FP32 layout: upper records {column bits, S'_ij (9)} and lower records
+    int *supper = nullptr, *uoff = nullptr, *loff = nullptr, *lrec = nullptr, *e2u = nullptr;
+    float *S32u = nullptr;      // 10 x nup
     int *entry_is_diag = nullptr;  // nbs
     double *Mval = nullptr;     // nbs mass scalars (M block = m I3)
-    double *D64 = nullptr;      // 9 x N  block-Jacobi inverse (SoA)
+    double *D64 = nullptr;      // 6 x N  block-Jacobi inverse (symmetric, SoA)
     float *D32 = nullptr;
     unsigned char *fixmask = nullptr;  // N, bit k = DOF k fixed
     double *vfix = nullptr;     // 3N fixed values (0 where free)
@@ -222,7 +249,7 @@ hipError_t pcg_finish(eolc_ctx *c, bool mixed);
 hipError_t spmv_true_residual(eolc_ctx *c);
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
-constexpr int kSpmvVariants = 9;
+constexpr int kSpmvVariants = 12;
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms);
 hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms);
 // learned preconditioner (k_learned.hip); buffers allocated by the API layer

@@ -26,7 +26,8 @@ namespace {
 __global__ void __launch_bounds__(256)
 k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *__restrict__ clist,
            const int *__restrict__ entry_is_diag, const double *__restrict__ stageK, const double *__restrict__ stageM,
-           double *__restrict__ A64, float *__restrict__ S32, double *__restrict__ Mval)
+           double *__restrict__ A64, float *__restrict__ S32, const int *__restrict__ e2u,
+           const int *__restrict__ scol, float *__restrict__ S32u, double *__restrict__ Mval)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbs) return;
@@ -55,8 +56,19 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
     for (int c = 0; c < 9; ++c) A64[c * nbs + b] = acc[c];
     if (S32) {
         const bool diag = entry_is_diag[b] != 0;
+        float sv[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) S32[c * nbs + b] = diag ? 0.0f : (float)(stf[c] + (c % 4 == 0 ? m : 0.0));
+        for (int c = 0; c < 9; ++c) {
+            sv[c] = diag ? 0.0f : (float)(stf[c] + (c % 4 == 0 ? m : 0.0));
+            S32[c * nbs + b] = sv[c];
+        }
+        const int u = e2u[b];
+        if (u >= 0) {   // upper block: its record in the symmetric layout
+            float *rec = S32u + (size_t)10 * u;
+            rec[0] = __int_as_float(scol[b]);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) rec[1 + c] = sv[c];
+        }
     }
     Mval[b] = m;
 }
@@ -107,7 +119,8 @@ hipError_t launch_assemble(eolc_ctx *c, bool want32)
     const int64_t nb = c->topo.nbs;
     if (nb == 0) return hipSuccess;
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
-                       c->entry_is_diag, c->stageK, c->stageM, c->A64, want32 ? c->S32 : nullptr, c->Mval);
+                       c->entry_is_diag, c->stageK, c->stageM, c->A64, want32 ? c->S32 : nullptr, c->e2u, c->scol,
+                       c->S32u, c->Mval);
     return hipGetLastError();
 }
 

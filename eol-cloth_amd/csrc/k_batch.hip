@@ -136,9 +136,10 @@ __device__ void inst_precond(const eolc::BatchInst &I, const int *__restrict__ s
         r32[3 * i] = r0; r32[3 * i + 1] = r1; r32[3 * i + 2] = r2;
         const float *d = D + i;
         const double s0 = r0, s1 = r1, s2 = r2;
-        const float z0 = (float)((double)d[0 * N] * s0 + (double)d[1 * N] * s1 + (double)d[2 * N] * s2);
-        const float z1 = (float)((double)d[3 * N] * s0 + (double)d[4 * N] * s1 + (double)d[5 * N] * s2);
-        const float z2 = (float)((double)d[6 * N] * s0 + (double)d[7 * N] * s1 + (double)d[8 * N] * s2);
+        const double e00 = d[0], e01 = d[N], e02 = d[2 * N], e11 = d[3 * N], e12 = d[4 * N], e22 = d[5 * N];
+        const float z0 = (float)fma(e02, s2, fma(e01, s1, e00 * s0));
+        const float z1 = (float)fma(e12, s2, fma(e11, s1, e01 * s0));
+        const float z2 = (float)fma(e22, s2, fma(e12, s1, e02 * s0));
         z32[3 * i] = z0; z32[3 * i + 1] = z1; z32[3 * i + 2] = z2;
         const double v[2] = {(s0 * (double)z0 + s1 * (double)z1) + s2 * (double)z2, (s0 * s0 + s1 * s1) + s2 * s2};
         slice_partial<2>(v, sl, part, nslices);
@@ -286,10 +287,8 @@ __global__ void __launch_bounds__(WGS) k_batch_pcg(BatchArgs a)
             const float r1 = fmaf(-alphaT, a.q32[3 * i + 1], a.r32[3 * i + 1]);
             const float r2 = fmaf(-alphaT, a.q32[3 * i + 2], a.r32[3 * i + 2]);
             a.r32[3 * i] = r0; a.r32[3 * i + 1] = r1; a.r32[3 * i + 2] = r2;
-            const float *d = a.D32 + i;
-            const float z0 = fmaf(d[2 * N], r2, fmaf(d[1 * N], r1, d[0 * N] * r0));
-            const float z1 = fmaf(d[5 * N], r2, fmaf(d[4 * N], r1, d[3 * N] * r0));
-            const float z2 = fmaf(d[8 * N], r2, fmaf(d[7 * N], r1, d[6 * N] * r0));
+            float z0, z1, z2;
+            eolc::apply_sym3<float>(a.D32 + i, N, r0, r1, r2, z0, z1, z2);
             a.z32[3 * i] = z0; a.z32[3 * i + 1] = z1; a.z32[3 * i + 2] = z2;
             const double v[2] = {((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2,
                                  ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2};

@@ -259,24 +259,31 @@ k_hinge_eval(int H, int F, const int *__restrict__ hinge, const double *__restri
 // SPD-projected variant.  Q = orthonormal basis (12x9) of the complement of the
 // translations: contrasts c1 = (1,-1,0,0)/sqrt2, c2 = (1,1,-2,0)/sqrt6,
 // c3 = (1,1,1,-3)/sqrt12 (each times I3).  K9 = Q^T K Q is diagonalised with
-// cyclic Jacobi in LDS, negative eigenvalues are zeroed and K+ = Q V L+ V^T Q^T.
-constexpr int kProjWG = 64;
+// cyclic Jacobi (rotation order (0,1), (0,2), ..., (7,8); at most 30 sweeps, until
+// the off-diagonal mass falls below 1e-30 of the Frobenius mass), negative
+// eigenvalues are zeroed and K+ = Q V L+ V^T Q^T.
+// K9 is kept as its 45 upper entries and V as 81 values in registers; every index is
+// a compile-time constant (all loops over matrix entries are unrolled), so nothing
+// goes through LDS (one wave per SIMD, no LDS occupancy limit, no scratch).
+namespace proj {
+constexpr double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637, r12 = 0.28867513459481288225;
+constexpr double cw[3][4] = {{r2, -r2, 0, 0}, {r6, r6, -2 * r6, 0}, {r12, r12, r12, -3 * r12}};
+constexpr int ix(int P, int Q) { return P <= Q ? P * 9 - P * (P - 1) / 2 + (Q - P) : Q * 9 - Q * (Q - 1) / 2 + (P - Q); }
+}  // namespace proj
 
-__global__ void __launch_bounds__(kProjWG)
-k_hinge_eval_proj(int H, int F, const int *__restrict__ hinge, const double *__restrict__ x,
-                  const double *__restrict__ X, double beta_all, const double *__restrict__ beta_hinge, double s,
-                  double *__restrict__ stageK, double *__restrict__ stageF, int *__restrict__ sweeps_out)
+__global__ void __launch_bounds__(64)
+k_hinge_eval_projr(int H, int F, const int *__restrict__ hinge, const double *__restrict__ x,
+                   const double *__restrict__ X, double beta_all, const double *__restrict__ beta_hinge, double s,
+                   double *__restrict__ stageK, double *__restrict__ stageF)
 {
-    __shared__ double Aw[81][kProjWG];
-    __shared__ double Vw[81][kProjWG];
-    const int t = threadIdx.x;
-    int h = blockIdx.x * blockDim.x + t;
-    bool active = h < H;
-    if (!active) h = H - 1;   // keep the lane busy on valid data; its output is not written
+    using proj::cw;
+    using proj::ix;
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
     int q[4] = {hinge[4 * h], hinge[4 * h + 1], hinge[4 * h + 2], hinge[4 * h + 3]};
     HingeGeom G;
     hinge_geom(x, X, q, beta_hinge ? beta_hinge[h] : beta_all, G);
-    if (active) {
+    {
         double *sf = stageF + (size_t)9 * F + (size_t)12 * h;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -284,86 +291,109 @@ k_hinge_eval_proj(int H, int F, const int *__restrict__ hinge, const double *__r
             sf[3 * j] = G.w * gj.x; sf[3 * j + 1] = G.w * gj.y; sf[3 * j + 2] = G.w * gj.z;
         }
     }
-    // contrast weights cw[c][vertex]
-    const double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637, r12 = 0.28867513459481288225;
-    const double cw[3][4] = {{r2, -r2, 0, 0}, {r6, r6, -2 * r6, 0}, {r12, r12, r12, -3 * r12}};
-    // K9[(c,a),(d,b)] = sum_{i,j} cw[c][i] cw[d][j] K_ij[a][b], K = -w Hess
-    for (int e = 0; e < 81; ++e) Aw[e][t] = 0.0;
+    double A[45];
+#pragma unroll
+    for (int e = 0; e < 45; ++e) A[e] = 0.0;
+#pragma unroll
     for (int i = 0; i < 4; ++i)
+#pragma unroll
         for (int j = i; j < 4; ++j) {
             double blk[9];
             hinge_hess_block(G, i, j, blk);
             if (i == j) { blk[3] = blk[1]; blk[6] = blk[2]; blk[7] = blk[5]; }
+#pragma unroll
             for (int c = 0; c < 3; ++c)
+#pragma unroll
                 for (int dd = 0; dd < 3; ++dd) {
-                    double wgt = cw[c][i] * cw[dd][j];
-                    double wgt2 = (i != j) ? cw[c][j] * cw[dd][i] : 0.0;
+                    const double wgt = cw[c][i] * cw[dd][j];
+                    const double wgt2 = (i != j) ? cw[c][j] * cw[dd][i] : 0.0;
                     if (wgt == 0.0 && wgt2 == 0.0) continue;
+#pragma unroll
                     for (int a = 0; a < 3; ++a)
+#pragma unroll
                         for (int b = 0; b < 3; ++b) {
-                            // block (i,j) contributes K_ij[a][b]; block (j,i) = K_ij^T contributes K_ij[b][a]
-                            double val = wgt * blk[3 * a + b] + wgt2 * blk[3 * b + a];
-                            Aw[(3 * c + a) * 9 + 3 * dd + b][t] += -G.w * val;
+                            const int P = 3 * c + a, Qc = 3 * dd + b;
+                            if (P > Qc) continue;   // upper triangle (the matrix is symmetric)
+                            const double val = wgt * blk[3 * a + b] + wgt2 * blk[3 * b + a];
+                            A[ix(P, Qc)] += -G.w * val;
                         }
                 }
         }
-    for (int e = 0; e < 81; ++e) Vw[e][t] = (e % 10 == 0) ? 1.0 : 0.0;
-    // cyclic Jacobi
+    double V[81];
+#pragma unroll
+    for (int e = 0; e < 81; ++e) V[e] = (e % 10 == 0) ? 1.0 : 0.0;
     double fro = 0.0;
-    for (int e = 0; e < 81; ++e) fro += Aw[e][t] * Aw[e][t];
-    int sweep = 0;
-    for (; sweep < 30; ++sweep) {
+#pragma unroll
+    for (int P = 0; P < 9; ++P)
+#pragma unroll
+        for (int Qc = 0; Qc < 9; ++Qc) fro += A[ix(P, Qc)] * A[ix(P, Qc)];
+    for (int sweep = 0; sweep < 30; ++sweep) {
         double off = 0.0;
+#pragma unroll
         for (int p = 0; p < 9; ++p)
-            for (int r = p + 1; r < 9; ++r) off += Aw[p * 9 + r][t] * Aw[p * 9 + r][t];
+#pragma unroll
+            for (int r = p + 1; r < 9; ++r) off += A[ix(p, r)] * A[ix(p, r)];
         if (!(off > 1e-30 * fro)) break;
+#pragma unroll
         for (int p = 0; p < 8; ++p)
+#pragma unroll
             for (int r = p + 1; r < 9; ++r) {
-                double apr = Aw[p * 9 + r][t];
-                if (apr == 0.0) continue;
-                double app = Aw[p * 9 + p][t], arr = Aw[r * 9 + r][t];
-                double theta = (arr - app) / (2.0 * apr);
-                double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                double c = 1.0 / sqrt(tt * tt + 1.0), sn = tt * c;
-                for (int k = 0; k < 9; ++k) {  // columns p, r
-                    double akp = Aw[k * 9 + p][t], akr = Aw[k * 9 + r][t];
-                    Aw[k * 9 + p][t] = c * akp - sn * akr;
-                    Aw[k * 9 + r][t] = sn * akp + c * akr;
-                }
-                for (int k = 0; k < 9; ++k) {  // rows p, r
-                    double apk = Aw[p * 9 + k][t], ark = Aw[r * 9 + k][t];
-                    Aw[p * 9 + k][t] = c * apk - sn * ark;
-                    Aw[r * 9 + k][t] = sn * apk + c * ark;
-                }
-                for (int k = 0; k < 9; ++k) {
-                    double vkp = Vw[k * 9 + p][t], vkr = Vw[k * 9 + r][t];
-                    Vw[k * 9 + p][t] = c * vkp - sn * vkr;
-                    Vw[k * 9 + r][t] = sn * vkp + c * vkr;
+                const double apr = A[ix(p, r)];
+                if (apr != 0.0) {
+                    const double app = A[ix(p, p)], arr = A[ix(r, r)];
+                    const double theta = (arr - app) / (2.0 * apr);
+                    const double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    const double c = 1.0 / sqrt(tt * tt + 1.0), sn = tt * c;
+                    // columns p, r then rows p, r of the symmetric matrix
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        if (k == p || k == r) continue;
+                        const double akp = A[ix(k, p)], akr = A[ix(k, r)];
+                        A[ix(k, p)] = c * akp - sn * akr;
+                        A[ix(k, r)] = sn * akp + c * akr;
+                    }
+                    // the 2x2 block {p, r}: column step, then row step
+                    const double cpp = c * app - sn * apr, cpr = sn * app + c * apr;
+                    const double crp = c * apr - sn * arr, crr = sn * apr + c * arr;
+                    A[ix(p, p)] = c * cpp - sn * crp;
+                    A[ix(p, r)] = c * cpr - sn * crr;
+                    A[ix(r, r)] = sn * cpr + c * crr;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        const double vkp = V[k * 9 + p], vkr = V[k * 9 + r];
+                        V[k * 9 + p] = c * vkp - sn * vkr;
+                        V[k * 9 + r] = sn * vkp + c * vkr;
+                    }
                 }
             }
     }
-    if (active && t == 0 && sweeps_out) atomicMax(sweeps_out, sweep);
-    // K9+ = V diag(max(lambda, 0)) V^T, stored back into Aw
     double lam[9];
-    for (int k = 0; k < 9; ++k) lam[k] = fmax(Aw[k * 9 + k][t], 0.0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) lam[k] = fmax(A[ix(k, k)], 0.0);
+    // K9+ = V diag(lam) V^T (upper triangle, into A)
+#pragma unroll
     for (int a = 0; a < 9; ++a)
+#pragma unroll
         for (int b = a; b < 9; ++b) {
             double acc = 0.0;
-            for (int k = 0; k < 9; ++k) acc += Vw[a * 9 + k][t] * lam[k] * Vw[b * 9 + k][t];
-            Aw[a * 9 + b][t] = acc;
-            Aw[b * 9 + a][t] = acc;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc += V[a * 9 + k] * lam[k] * V[b * 9 + k];
+            A[ix(a, b)] = acc;
         }
-    if (!active) return;
-    // K+_ij = sum_{c,d} cw[c][i] cw[d][j] K9+[(c,.),(d,.)]
     const int pv[10] = {0, 1, 2, 3, 0, 0, 0, 1, 1, 2}, pw[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
     double *sk = stageK + (size_t)54 * F + (size_t)90 * h;
+#pragma unroll
     for (int p = 0; p < 10; ++p) {
         const int i = pv[p], j = pw[p];
+#pragma unroll
         for (int a = 0; a < 3; ++a)
+#pragma unroll
             for (int b = 0; b < 3; ++b) {
                 double acc = 0.0;
+#pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    for (int dd = 0; dd < 3; ++dd) acc += cw[c][i] * cw[dd][j] * Aw[(3 * c + a) * 9 + 3 * dd + b][t];
+#pragma unroll
+                    for (int dd = 0; dd < 3; ++dd) acc += cw[c][i] * cw[dd][j] * A[ix(3 * c + a, 3 * dd + b)];
                 sk[9 * p + 3 * a + b] = s * acc;
             }
     }
@@ -399,9 +429,8 @@ hipError_t launch_hinge_eval(eolc_ctx *c, double h)
     if (H == 0) return hipSuccess;
     const double s = c->dampB * h * h;
     if (c->flags & EOLC_OPT_SPD_PROJECT_BENDING) {
-        hipLaunchKernelGGL(k_hinge_eval_proj, dim3(blocks_for(H, kProjWG)), dim3(kProjWG), 0, c->stream, H, c->topo.F,
-                           c->hinge, c->x, c->X, c->beta, (const double *)c->beta_hinge, s, c->stageK, c->stageF,
-                           (int *)(c->counter + 2));
+        hipLaunchKernelGGL(k_hinge_eval_projr, dim3(blocks_for(H, 64)), dim3(64), 0, c->stream, H, c->topo.F,
+                           c->hinge, c->x, c->X, c->beta, (const double *)c->beta_hinge, s, c->stageK, c->stageF);
     } else {
         hipLaunchKernelGGL(k_hinge_eval, dim3(blocks_for(H)), dim3(256), 0, c->stream, H, c->topo.F, c->hinge, c->x,
                            c->X, c->beta, (const double *)c->beta_hinge, s, c->stageK, c->stageF);

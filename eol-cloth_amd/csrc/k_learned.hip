@@ -387,7 +387,7 @@ k_lp_out(int N, const unsigned short *__restrict__ h, const unsigned short *__re
     for (int k = 0; k < NO; ++k) o[(size_t)NO * i + k] = acc[k] + b[k];
 }
 
-// P blocks (natural vertex order).  Inference: P -> D32 (SoA, c * N + i).
+// P blocks (natural vertex order).  Inference: P -> D32 (6 upper entries, SoA).
 // Training (r != null): z = P r and partial sums of z.r.
 __global__ void __launch_bounds__(RT)
 k_lp_precond(int N, const int *__restrict__ diagpos, const double *__restrict__ A, int64_t nbs,
@@ -406,8 +406,9 @@ k_lp_precond(int N, const int *__restrict__ diagpos, const double *__restrict__ 
         double P[9];
         lp_P(b, fm, P);
         if (D32) {
+            const int up[eolc::kDsym] = {0, 1, 2, 4, 5, 8};   // P is symmetric (G G^T)
 #pragma unroll
-            for (int c = 0; c < 9; ++c) D32[(size_t)c * N + i] = (float)P[c];
+            for (int k = 0; k < eolc::kDsym; ++k) D32[(size_t)k * N + i] = (float)P[up[k]];
         }
         if (r) {
             const double r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];

@@ -199,12 +199,24 @@ struct SpmvEntry {
     }
 };
 
-template <class T, bool DIFF, int U, bool PF, bool NT, int W>
+// Symmetric FP32 layout (SYM, difference form only): every off-diagonal block is
+// stored once, as a 40-byte record {column, S'_ij} with its upper row; a lower entry
+// (column < row) is a record {column, upper record of (j, i)} and applies that block
+// transposed, read through the caches (row j was streamed shortly before).  Same
+// entries, same order, same values (the assembled blocks satisfy K_ji = K_ij^T
+// exactly) as the full layout, so the result is bitwise identical.
+struct SymLayout {
+    const int *supper, *uoff, *loff, *lrec;
+    const float *S32u;
+};
+
+template <class T, bool DIFF, int U, bool PF, bool NT, int W, bool SYM>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const int *__restrict__ srow,
            const int *__restrict__ slen, const int *__restrict__ scol, const T *__restrict__ A,
            const T *__restrict__ Mm, int64_t nbs, const unsigned char *__restrict__ fixmask, const T *__restrict__ z,
-           const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, SolverScalars *scal, double *partials)
+           const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, SolverScalars *scal, double *partials,
+           SymLayout sym)
 {
     static_assert(U == 1 || U == 2, "entries per trip");
     __shared__ double lds[2 * (WG / 64) + 2];
@@ -254,7 +266,40 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
             y2 = (double)(mb * ui2);
         }
         int kk = 0;
-        if constexpr (U == 1) {
+        if constexpr (DIFF && SYM) {
+            static_assert(std::is_same<T, float>::value, "symmetric layout is FP32");
+            const int nu = sym.supper[g];
+            const int nl = len - nu;
+            const float *__restrict__ ur = sym.S32u + (size_t)10 * (sym.uoff[sl] + lane);
+            typedef float f2v __attribute__((ext_vector_type(2)));
+#pragma unroll 1
+            for (kk = 0; kk < nu; ++kk) {
+                const f2v *rec = reinterpret_cast<const f2v *>(ur + (size_t)640 * kk);
+                const f2v r0 = ldm<f2v, NT>(rec), r1 = ldm<f2v, NT>(rec + 1), r2 = ldm<f2v, NT>(rec + 2),
+                          r3 = ldm<f2v, NT>(rec + 3), r4 = ldm<f2v, NT>(rec + 4);
+                const int j = __float_as_int(r0.x);
+                const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
+                const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
+                const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                block_apply<T, DIFF>(r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x, r4.y, u0, u1, u2, ui0, ui1, ui2,
+                                     y0, y1, y2);
+            }
+            typedef int i2v __attribute__((ext_vector_type(2)));
+            const i2v *__restrict__ lr = reinterpret_cast<const i2v *>(sym.lrec) + (sym.loff[sl] + lane);
+#pragma unroll 1
+            for (kk = 0; kk < nl; ++kk) {
+                const i2v jt = ldm<i2v, NT>(lr + (size_t)64 * kk);
+                const int j = jt.x;
+                // block (j, i), record {column, 9 values}: apply its transpose
+                const f2v *rec = reinterpret_cast<const f2v *>(sym.S32u + (size_t)10 * jt.y);
+                const f2v r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3], r4 = rec[4];
+                const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
+                const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
+                const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                block_apply<T, DIFF>(r0.y, r2.x, r3.y, r1.x, r2.y, r4.x, r1.y, r3.x, r4.y, u0, u1, u2, ui0, ui1, ui2,
+                                     y0, y1, y2);
+            }
+        } else if constexpr (U == 1) {
             int jn = ldm<int, NT>(cl);
 #pragma unroll 1
             for (; kk < len; ++kk) {
@@ -358,10 +403,8 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
         const T r1 = fma(-alphaT, q[3 * i + 1], r[3 * i + 1]);
         const T r2 = fma(-alphaT, q[3 * i + 2], r[3 * i + 2]);
         r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
-        const T *d = D + i;
-        const T z0 = fma(d[2 * N], r2, fma(d[1 * N], r1, d[0 * N] * r0));
-        const T z1 = fma(d[5 * N], r2, fma(d[4 * N], r1, d[3 * N] * r0));
-        const T z2 = fma(d[8 * N], r2, fma(d[7 * N], r1, d[6 * N] * r0));
+        T z0, z1, z2;
+        eolc::apply_sym3<T>(D + i, N, r0, r1, r2, z0, z1, z2);
         z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
         srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
         srr += ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2;
@@ -442,9 +485,10 @@ k_precond_init(int parity, int N, const T *__restrict__ D, const double *r64, T 
         r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
         const T *d = D + i;
         const double s0 = r0, s1 = r1, s2 = r2;
-        const T z0 = (T)((double)d[0 * N] * s0 + (double)d[1 * N] * s1 + (double)d[2 * N] * s2);
-        const T z1 = (T)((double)d[3 * N] * s0 + (double)d[4 * N] * s1 + (double)d[5 * N] * s2);
-        const T z2 = (T)((double)d[6 * N] * s0 + (double)d[7 * N] * s1 + (double)d[8 * N] * s2);
+        const double e00 = d[0], e01 = d[N], e02 = d[2 * N], e11 = d[3 * N], e12 = d[4 * N], e22 = d[5 * N];
+        const T z0 = (T)fma(e02, s2, fma(e01, s1, e00 * s0));
+        const T z1 = (T)fma(e12, s2, fma(e11, s1, e01 * s0));
+        const T z2 = (T)fma(e22, s2, fma(e12, s1, e02 * s0));
         z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
         rz += (s0 * (double)z0 + s1 * (double)z1) + s2 * (double)z2;
         rr += (s0 * s0 + s1 * s1) + s2 * s2;
@@ -489,8 +533,12 @@ k_block_jacobi(int N, const int *__restrict__ diagpos, const double *__restrict_
     for (int c = 0; c < 9; ++c) {
         int r = c / 3, s = c % 3;
         if (((fm >> r) & 1) || ((fm >> s) & 1)) o[c] = 0.0;   // z stays 0 on fixed DOFs
-        D64[c * N + i] = o[c];
-        D32[c * N + i] = (float)o[c];
+    }
+    const int up[eolc::kDsym] = {0, 1, 2, 4, 5, 8};           // upper entries (the inverse is symmetric)
+#pragma unroll
+    for (int k = 0; k < eolc::kDsym; ++k) {
+        D64[k * N + i] = o[up[k]];
+        D32[k * N + i] = (float)o[up[k]];
     }
 }
 
@@ -587,16 +635,21 @@ __global__ void k_fill(int n, double v, double *__restrict__ out)
 }
 
 // SpMV launch variants (see k_pcg_spmv); the order is the one eolc_autotune reports.
-#define EOLC_SPMV_VARIANT_LIST(X) \
-    X(0, 1, false, false, 8)      \
-    X(1, 1, true, false, 8)       \
-    X(2, 1, true, true, 8)        \
-    X(3, 1, false, true, 8)       \
-    X(4, 1, true, true, 6)        \
-    X(5, 2, true, true, 6)        \
-    X(6, 2, true, false, 6)       \
-    X(7, 2, true, true, 4)        \
-    X(8, 1, true, true, 4)
+// The symmetric-layout variants (last column) apply to the FP32 difference form;
+// the FP64 product has no such layout and runs its full-layout kernel for them.
+#define EOLC_SPMV_VARIANT_LIST(X)   \
+    X(0, 1, false, false, 8, false) \
+    X(1, 1, true, false, 8, false)  \
+    X(2, 1, true, true, 8, false)   \
+    X(3, 1, false, true, 8, false)  \
+    X(4, 1, true, true, 6, false)   \
+    X(5, 2, true, true, 6, false)   \
+    X(6, 2, true, false, 6, false)  \
+    X(7, 2, true, true, 4, false)   \
+    X(8, 1, true, true, 4, false)   \
+    X(9, 1, false, false, 8, true)  \
+    X(10, 1, false, false, 6, true) \
+    X(11, 1, false, true, 8, true)
 
 template <class T>
 void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm, const T *z, const T *pold, T *pnew,
@@ -604,18 +657,19 @@ void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm
 {
     const eolc::Topology &t = c->topo;
     constexpr bool diff = std::is_same<T, float>::value;
-#define EOLC_SPMV_CASE(id, U, PF, NT, W)                                                                         \
+    const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u};
+#define EOLC_SPMV_CASE(id, U, PF, NT, W, SYM)                                                                    \
     case id:                                                                                                     \
-        hipLaunchKernelGGL((k_pcg_spmv<T, diff, U, PF, NT, W>), dim3(G), dim3(WG), 0, c->stream, k, t.N,          \
-                           t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, pold, \
-                           pnew, q, c->scal, c->partials);                                                        \
+        hipLaunchKernelGGL((k_pcg_spmv<T, diff, U, PF, NT, W, SYM && diff>), dim3(G), dim3(WG), 0, c->stream, k, \
+                           t.N, t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z,  \
+                           pold, pnew, q, c->scal, c->partials, sym);                                             \
         break;
     switch (variant) {
         EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_CASE)
     default:
-        hipLaunchKernelGGL((k_pcg_spmv<T, diff, 1, false, false, 8>), dim3(G), dim3(WG), 0, c->stream, k, t.N,
+        hipLaunchKernelGGL((k_pcg_spmv<T, diff, 1, false, false, 8, false>), dim3(G), dim3(WG), 0, c->stream, k, t.N,
                            t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, pold, pnew,
-                           q, c->scal, c->partials);
+                           q, c->scal, c->partials, sym);
     }
 #undef EOLC_SPMV_CASE
 }

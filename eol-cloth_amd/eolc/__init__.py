@@ -36,7 +36,7 @@ EXPORTED_SYMBOLS = (
 BATCH_EAGER = 0x1
 BATCH_VARIANTS = 3
 
-SPMV_VARIANTS = 9
+SPMV_VARIANTS = 12
 
 LEARNED_NPARAMS = 26246
 

@@ -231,12 +231,13 @@ int eolc_stream_copy(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
 
 /* ---- launch autotuner (no counterpart in the reference) -----------------------
  * Times each SpMV launch variant (entries per loop trip, column-index prefetch,
- * non-temporal matrix loads, waves per SIMD) for `reps` launches on the current
+ * non-temporal matrix loads, waves per SIMD, and for FP32 the symmetric layout that
+ * stores each off-diagonal block once) for `reps` launches on the current
  * matrix and keeps the fastest for later solves of that precision (32 or 64) on
  * this context.  All variants give bitwise-identical results.  ms (may be NULL)
  * receives EOLC_SPMV_VARIANTS average times; *best (may be NULL) the choice.
  * eolc_set_spmv_variant sets the choice directly. */
-#define EOLC_SPMV_VARIANTS 9
+#define EOLC_SPMV_VARIANTS 12
 int eolc_autotune(eolc_ctx *ctx, int precision, int reps, int *best, double *ms);
 int eolc_set_spmv_variant(eolc_ctx *ctx, int precision, int variant);
 

@@ -76,3 +76,42 @@ def test_degenerate_face_is_reported(ctx):
     with pytest.raises(eolc.EolcError) as e:
         ctx.set_topology(sc.X, tri, sc.hinge)
     assert e.value.code == eolc.ERR_DEGENERATE
+
+
+def test_spd_projection_matches_eigh(ctx):
+    """EOLC_OPT_SPD_PROJECT_BENDING against a textbook restatement: per hinge,
+    K9 = Q^T K Q (Q: orthonormal contrast basis of the complement of the
+    translations), K+ = Q V max(L, 0) V^T Q^T with numpy's eigh, K from the oracle's
+    bending Hessian.  The assembled difference MDK(projected) - MDK(exact) must equal
+    h^2 sum_h (K+_h - K_h) to 1e-10 of the largest hinge Hessian entry."""
+    import eolc
+    from eolc import scene
+    sc = scene.Scene(5, sigma_x=0.06, sigma_v=0.0, seed_x=99)
+    x = sc.x0.reshape(-1, 3)
+    mats = []
+    for flags in (0, eolc.OPT_SPD_PROJECT_BENDING):
+        ctx.set_topology(sc.X, sc.tri, sc.hinge)
+        ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+        ctx.set_options(flags)
+        ctx.upload_state(sc.x0, sc.v0)
+        ctx.forces_fill(GRAV, H)
+        mats.append(ctx.download_csr(eolc.MAT_MDK).toarray())
+    ctx.set_options(0)
+    diff = mats[1] - mats[0]
+    r2, r6, r12 = 1 / np.sqrt(2), 1 / np.sqrt(6), 1 / np.sqrt(12)
+    cw = np.array([[r2, -r2, 0, 0], [r6, r6, -2 * r6, 0], [r12, r12, r12, -3 * r12]])
+    Q = np.kron(cw.T, np.eye(3))              # 12 x 9, orthonormal columns
+    ref = np.zeros_like(diff)
+    kmax, nneg = 0.0, 0
+    for q in sc.hinge:
+        _, _, K = O.bending(x[q[0]], x[q[1]], x[q[2]], x[q[3]], sc.X[q[0]], sc.X[q[1]], sc.X[q[2]], sc.X[q[3]], 1e-5)
+        w, V = np.linalg.eigh(Q.T @ K @ Q)
+        nneg += int(np.sum(w < 0))
+        Kp = Q @ (V * np.maximum(w, 0.0)) @ V.T @ Q.T
+        dofs = (3 * np.asarray(q)[:, None] + np.arange(3)).reshape(-1)
+        ref[np.ix_(dofs, dofs)] += H * H * (Kp - K)
+        kmax = max(kmax, H * H * np.abs(K).max())
+    print(f"projection: {len(sc.hinge)} hinges, {nneg} negative eigenvalues clamped, "
+          f"max|diff - ref| / max|h^2 K| = {np.abs(diff - ref).max() / kmax:.2e}", flush=True)
+    assert nneg > 0
+    assert np.abs(diff - ref).max() <= 1e-10 * kmax
