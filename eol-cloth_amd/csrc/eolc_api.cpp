@@ -760,11 +760,13 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     int k = 0, refinements = 0, status = EOLC_ERR_MAXIT;
     // Iterations are enqueued in chunks; the host synchronises once per chunk.  A
     // chunk that overruns the stop point costs one no-op launch pair per extra
-    // iteration, so the next chunk is sized at 80% of the iterations the observed
-    // convergence rate predicts to the current threshold (8 .. 1024).
-    int chunk = 8;
-    int k_prev = 0;
-    double rr_prev = -1.0;
+    // iteration, so the next chunk is sized at 60% of the iterations that the mean
+    // convergence rate since the start of the current inner cycle predicts to the
+    // current threshold (8 .. 128).  The CG residual is not monotone and drops in
+    // bursts, so the rate is measured over the whole cycle and the chunk is capped.
+    int chunk = 16;
+    int k_cycle = 0;
+    double rr_cycle = -1.0;
     double relres = NAN;
     for (;;) {
         if (learned && k >= cap) {
@@ -775,6 +777,8 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
             EOLC_HIP(eolc::pcg_reliable_update(c));
             EOLC_HIP(eolc::pcg_restart_inner(c, k));
             EOLC_HIP(eolc::scal_restart(c, k));
+            rr_cycle = -1.0;
+            k_cycle = k;
         }
         const int todo = std::min(chunk, (learned ? cap : maxit) - k);
         if (todo <= 0) { status = EOLC_ERR_MAXIT; break; }
@@ -784,19 +788,19 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         const eolc::SolverScalars s = *c->hscal;
         k = s.iters;
         relres = s.bnorm2 > 0 ? std::sqrt(s.rr / s.bnorm2) : 0.0;
+        if (rr_cycle < 0.0) rr_cycle = s.rmax2;   // the (true) residual the cycle started from
         {
             const double thr = std::max(s.tol2 * s.bnorm2, s.delta2 * s.rmax2);
-            int next = std::min(2 * chunk, 1024);
-            if (rr_prev > 0.0 && k > k_prev && s.rr > thr && s.rr < rr_prev) {
-                const double rate = std::log(s.rr / rr_prev) / (k - k_prev);   // log reduction per iteration
-                next = (int)std::min(1024.0, std::max(8.0, 0.8 * std::log(thr / s.rr) / rate));
+            int next = 32;
+            if (rr_cycle > 0.0 && k > k_cycle && s.rr > thr && s.rr < rr_cycle) {
+                const double rate = std::log(s.rr / rr_cycle) / (k - k_cycle);   // mean log reduction per iteration
+                next = (int)std::min(128.0, std::max(8.0, 0.6 * std::log(thr / s.rr) / rate));
             }
-            k_prev = k;
-            rr_prev = s.rr;
             chunk = next;
         }
         if (!s.done) continue;
-        rr_prev = -1.0;   // the residual is replaced below: measure the rate afresh
+        rr_cycle = -1.0;   // the residual is replaced below: a new cycle starts at k
+        k_cycle = k;
         chunk = 16;
         if (s.reason == 2 && learned) {
             // breakdown under the learned blocks (P is SPD, so only from an

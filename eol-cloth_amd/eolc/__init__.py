@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "eolc_time_spmv", "eolc_learned_set_weights", "eolc_learned_get_weights", "eolc_learned_infer",
     "eolc_learned_loss_grad", "eolc_learned_train", "eolc_autotune", "eolc_set_spmv_variant",
     "eolc_set_batch_topology", "eolc_set_batch_material", "eolc_batch_step", "eolc_batch_autotune",
-    "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta",
+    "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta", "eolc_collision_constraints",
 )
 
 BATCH_EAGER = 0x1
@@ -108,6 +108,9 @@ def lib():
                                           C.c_void_p]
         L.eolc_set_batch_variant.argtypes = [vp, C.c_int]
         L.eolc_set_mixed_delta.argtypes = [vp, C.c_double]
+        L.eolc_collision_constraints.argtypes = [vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_int),
+                                                 C.POINTER(C.c_int64)]
         L.eolc_stream_copy.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.eolc_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_void_p]
         L.eolc_set_spmv_variant.argtypes = [vp, C.c_int, C.c_int]
@@ -350,3 +353,25 @@ class Context:
         self._check(lib().eolc_learned_train(self._h, fd.size, _ptr(fd) if fd.size else None,
                                              _ptr(e), int(steps), float(lr), _ptr(losses)))
         return losses
+
+
+def collision_constraints(records, N, eol=None):
+    """Constraints::fill collision rows from CD2 records: scipy CSR A_ineq (rows x 3N).
+    Host-only (no GPU context needed)."""
+    import scipy.sparse as sp
+    n = len(records)
+    buf = (CollisionRec * max(n, 1))(*records) if n else (CollisionRec * 1)()
+    e = None if eol is None else np.ascontiguousarray(eol, dtype=np.uint8)
+    rows, nnz = C.c_int(0), C.c_int64(0)
+    rc = lib().eolc_collision_constraints(None, buf, n, int(N), _ptr(e), None, None, None, 0, 0, C.byref(rows),
+                                          C.byref(nnz))
+    if rc not in (OK, ERR_OVERFLOW):
+        raise EolcError(rc, "collision_constraints failed")
+    rp = np.zeros(rows.value + 1, dtype=np.int64)
+    col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+    val = np.zeros(max(nnz.value, 1))
+    rc = lib().eolc_collision_constraints(None, buf, n, int(N), _ptr(e), _ptr(rp), _ptr(col), _ptr(val), rows.value,
+                                          nnz.value, C.byref(rows), C.byref(nnz))
+    if rc != OK:
+        raise EolcError(rc, "collision_constraints failed")
+    return sp.csr_matrix((val[: nnz.value], col[: nnz.value], rp), shape=(rows.value, 3 * int(N)))

@@ -4,6 +4,7 @@
 //   eolc::Forces             Forces.h:26-45   -- fill(mesh, material, grav, h) -> f, M, MDK
 //   eolc::GeneralizedSolver  GeneralizedSolver.h:12-32 -- velocitySolve(...) for solver "none"
 //   eolc::CD / eolc::CD2     Collisions.h:9-11 -- proximity queries into btc::Collision-like records
+//   eolc::fillConstraints    Constraints.cpp:114-119, 417-514 -- A_ineq rows from the records, fixed rows
 //
 // The reference's types (ArcSim Mesh, Eigen vectors and sparse matrices) are not
 // available here, so the mesh is passed as the flat arrays those types hold
@@ -163,5 +164,46 @@ inline void collide(Device &dev, const Obstacles &obs, bool eolPoints, std::vect
 }
 inline void CD2(Device &dev, const Obstacles &obs, std::vector<eolc_collision> &cls) { collide(dev, obs, false, cls); }
 inline void CD(Device &dev, const Obstacles &obs, std::vector<eolc_collision> &cls) { collide(dev, obs, true, cls); }
+
+// Constraints::fill (Constraints.cpp:122-514) without the EOL rows: the collision
+// inequality rows from CD2 records (A_ineq v >= 0, b_ineq = 0) and the fixed-DOF
+// equality rows of addFixed (Constraints.cpp:114-119): one row per fixed DOF with
+// coefficient 1 and b = (1 - 0.01) v + scripted velocity.
+struct QpConstraints {
+    CsrMatrix Aineq, Aeq;
+    std::vector<double> bineq, beq;
+};
+
+inline QpConstraints fillConstraints(const std::vector<eolc_collision> &cls, int N, const std::vector<int> &fixedNodes,
+                                     const std::vector<double> &v, const std::vector<double> &scripted = {},
+                                     const std::vector<unsigned char> &eol = {})
+{
+    QpConstraints q;
+    int rows = 0;
+    int64_t nnz = 0;
+    const unsigned char *e = eol.empty() ? nullptr : eol.data();
+    int rc = eolc_collision_constraints(nullptr, cls.data(), (int)cls.size(), N, e, nullptr, nullptr, nullptr, 0, 0,
+                                        &rows, &nnz);
+    if (rc != EOLC_OK && rc != EOLC_ERR_OVERFLOW) throw Error(rc, "collision_constraints failed");
+    q.Aineq.n = rows;
+    q.Aineq.rowptr.assign(rows + 1, 0);
+    q.Aineq.col.resize(nnz);
+    q.Aineq.val.resize(nnz);
+    rc = eolc_collision_constraints(nullptr, cls.data(), (int)cls.size(), N, e, q.Aineq.rowptr.data(),
+                                    q.Aineq.col.data(), q.Aineq.val.data(), rows, nnz, &rows, &nnz);
+    if (rc != EOLC_OK) throw Error(rc, "collision_constraints failed");
+    q.bineq.assign(rows, 0.0);
+    q.Aeq.n = 3 * (int)fixedNodes.size();
+    q.Aeq.rowptr.assign(q.Aeq.n + 1, 0);
+    for (size_t i = 0; i < fixedNodes.size(); ++i)
+        for (int k = 0; k < 3; ++k) {
+            const int r = 3 * (int)i + k, dof = 3 * fixedNodes[i] + k;
+            q.Aeq.col.push_back(dof);
+            q.Aeq.val.push_back(1.0);
+            q.Aeq.rowptr[r + 1] = r + 1;
+            q.beq.push_back((1 - 0.01) * v[dof] + (scripted.empty() ? 0.0 : scripted[3 * i + k]));
+        }
+    return q;
+}
 
 }  // namespace eolc

@@ -148,6 +148,19 @@ int eolc_collide(eolc_ctx *ctx, double threshold, int P, const double *pts, cons
                  int B, const double *whd, const double *E, int eol_points, int box_npts,
                  int box_nedges, eolc_collision *out, int cap, int *count);
 
+/* ---- Constraints::fill, collision rows (Constraints.cpp:417-470) --------------
+ * Inequality rows A_ineq v >= 0 of the QP from CD2 records, in record order:
+ * (count1, count2) = (3, 1): -nor1 on the DOFs of verts2[0]; (2, 2): -nor2 *
+ * weights2[j] on verts2[j], j < 2; (1, 3): -nor2 * weights2[j] on verts2[j], j < 3;
+ * other records give no row; rows touching an EOL node (eol[v] != 0; eol may be
+ * NULL) are skipped.  Scalar CSR over 3N columns, columns increasing within a row,
+ * duplicates summed (what setFromTriplets builds); b_ineq is zero.  Host-only: ctx
+ * may be NULL (it only receives the error message).  On EOLC_ERR_OVERFLOW, *rows
+ * and *nnz are the required sizes; pass NULL arrays and zero capacities to query. */
+int eolc_collision_constraints(eolc_ctx *ctx, const eolc_collision *recs, int n, int N,
+                               const unsigned char *eol, int64_t *rowptr, int *col, double *val,
+                               int cap_rows, int64_t cap_nnz, int *rows, int64_t *nnz);
+
 /* ---- learned block preconditioner ----------------------------------------------
  * No counterpart in the reference (its solver "none" aborts; Eigen's CG there uses
  * a scalar-Jacobi preconditioner).  A message-passing network over the vertex

@@ -89,6 +89,22 @@ def test_cd2_parity_small(ctx):
     assert set(kinds) >= {(1, 3), (3, 1), (2, 2)}          # every record type occurs
 
 
+def test_constraint_rows_from_gpu_records(ctx):
+    """Constraints::fill collision rows (Constraints.cpp:417-470) from the GPU's CD2
+    records equal the restated loop applied to the oracle's records, exactly."""
+    from test_constraints import restated
+    sc, pts, nrm, whd, E = thin_box_scene(96, 1e-3)
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.upload_state(sc.x0, np.zeros_like(sc.x0))
+    g = ctx.collide(1e-3, pts, nrm, whd, E)
+    o = oracle_cd(1e-3, pts, nrm, whd, E, sc.x0, sc.tri)
+    A = eolc.collision_constraints(g, sc.N)
+    R = restated(o, sc.N)
+    print(f"constraint rows {A.shape[0]} from {len(g)} records, nnz {A.nnz}", flush=True)
+    assert A.shape == R.shape and A.shape[0] > 0
+    assert (abs(A - R)).max() == 0.0
+
+
 def test_cd_eol_points_parity(ctx):
     sc, pts, nrm, whd, E = thin_box_scene(64, 1e-3)
     # points right on the cloth so the EOL vertex-point branch fires
