@@ -857,10 +857,15 @@ hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms)
     if (e != hipSuccess) return e;
     e = hipEventCreate(&b);
     if (e != hipSuccess) {
-        hipEventDestroy(a);
+        (void)hipEventDestroy(a);
         return e;
     }
-    scal_set(c, 0.0, 0.0, 1);
+    e = scal_set(c, 0.0, 0.0, 1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return e;
+    }
     hipLaunchKernelGGL(k_fill, dim3(blocks_for(4 * kMaxPartials)), dim3(256), 0, c->stream, 4 * kMaxPartials, 1.0,
                        c->partials + kMaxPartials);
     // one untimed launch (first-touch, instruction cache)
@@ -868,19 +873,19 @@ hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms)
         launch_spmv<float>(c, variant, 1, G, c->S32, c->M32, c->z32, c->p32[1], c->p32[0], c->q32);
     else
         launch_spmv<double>(c, variant, 1, G, c->A64, (const double *)nullptr, c->z64, c->p64[1], c->p64[0], c->q64);
-    hipEventRecord(a, c->stream);
-    for (int r = 0; r < reps; ++r) {
+    e = hipEventRecord(a, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
         if (mixed)
             launch_spmv<float>(c, variant, 1, G, c->S32, c->M32, c->z32, c->p32[1], c->p32[0], c->q32);
         else
             launch_spmv<double>(c, variant, 1, G, c->A64, (const double *)nullptr, c->z64, c->p64[1], c->p64[0],
                                 c->q64);
     }
-    hipEventRecord(b, c->stream);
-    e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     return e == hipSuccess ? hipGetLastError() : e;
 }
 
