@@ -10,10 +10,12 @@ all on the device with the state resident in HBM.  Each step's solve is checked
 against the tolerance.
 
 The headline path is the learned-preconditioned mixed-precision PCG (FP32 matrix
-and vectors, FP64 dots and true residual with reliable updates, P_i from the
-message-passing network with weights eol-cloth_amd/eolc/weights/mpn_cloth.npy,
-block-Jacobi fallback); mixed block-Jacobi PCG and FP64 3x3 block-Jacobi PCG run
-from the same initial state for comparison.  Before timing, the SpMV launch
+and vectors, FP64 dots and true residual with reliable updates, fine blocks P_i from
+the message-passing network with weights eol-cloth_amd/eolc/weights/mpn_cloth.npy,
+plus the coarse levels of the additive multilevel preconditioner
+(EOLC_SOLVE_MULTILEVEL), block-Jacobi fallback).  FP64 3x3 block-Jacobi PCG (the
+baseline), mixed block-Jacobi PCG, and both block-Jacobi precisions with the coarse
+levels run from the same initial state, so the gain of each ingredient is stated.  Before timing, the SpMV launch
 variant of each precision is chosen by the library's autotuner on the first
 matrix.  The noisy initial state makes the exact bending Hessian indefinite, so
 every path assembles the SPD-projected hinge Hessians
@@ -191,6 +193,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-cloth workload")
+    ap.add_argument("--no-ml", action="store_true",
+                    help="headline without the multilevel coarse levels (and skip the multilevel lines)")
     ap.add_argument("--profile-solve", action="store_true",
                     help="kernel-trace runs: one step of each solve mode capped at 400 PCG iterations, then exit")
     ap.add_argument("--profile-spmv", action="store_true",
@@ -223,8 +227,14 @@ def main():
         tuned[prec] = {"variant": best, "us": [round(float(m) * 1e3, 1) for m in ms]}
         log(f"autotune fp{prec}: variant {best}, us per launch {tuned[prec]['us']}")
 
+    t = time.perf_counter()
+    ml_counts = ctx.ml_info()
+    ml_build_s = time.perf_counter() - t
+    log(f"multilevel hierarchy: aggregates per level {ml_counts} ({ml_build_s:.2f} s)")
+
     if args.profile_solve:
-        for mode, name in ((eolc.SOLVE_MIXED_BJ, "mixed-bj"), (eolc.SOLVE_FP64_BJ, "fp64-bj")):
+        for mode, name in ((eolc.SOLVE_MIXED_BJ, "mixed-bj"), (eolc.SOLVE_FP64_BJ, "fp64-bj"),
+                           (eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL, "mixed-bj-ml")):
             ctx.upload_state(sc.x0, sc.v0)
             rc, st = ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=400, mode=mode, allow=(eolc.ERR_MAXIT,))
             log(f"{name}: {st.iters} iterations, solve {st.ms_solve:.1f} ms, assembly {st.ms_assembly:.1f} ms")
@@ -259,7 +269,23 @@ def main():
     tbj, stbj = run_path(ctx, sc, eolc.SOLVE_MIXED_BJ, args.steps, args.warmup, args.maxit)
     log(f"mixed-bj: {1 / tbj:.4f} steps/s, iters {[s['iters'] for s in stbj]}")
     out_mbj = {"steps_per_s": 1 / tbj, "iters": [s["iters"] for s in stbj], "ms_per_step": tbj * 1e3}
-    head_mode, head_name = eolc.SOLVE_MIXED_LEARNED, "mixed-learned"
+    extra = {}
+    if not args.no_ml:
+        # the coarse levels alone, in both precisions (the mixed one also sets the
+        # learned mode's fallback cap for the multilevel case)
+        for name, mode in (("fp64_bj_ml", eolc.SOLVE_FP64_BJ | eolc.SOLVE_MULTILEVEL),
+                           ("mixed_bj_ml", eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL)):
+            if name == "fp64_bj_ml" and args.no_fp64:
+                continue
+            tt, st = run_path(ctx, sc, mode, args.steps, args.warmup, args.maxit)
+            log(f"{name}: {1 / tt:.4f} steps/s, iters {[s['iters'] for s in st]}")
+            extra[name] = {"steps_per_s": 1 / tt, "iters": [s["iters"] for s in st], "ms_per_step": tt * 1e3,
+                           "phase_ms": {k: float(np.mean([s[k] for s in st])) for k in
+                                        ("ms_assembly", "ms_solve")}}
+    if args.no_ml:
+        head_mode, head_name = eolc.SOLVE_MIXED_LEARNED, "mixed-learned"
+    else:
+        head_mode, head_name = eolc.SOLVE_MIXED_LEARNED | eolc.SOLVE_MULTILEVEL, "mixed-learned-multilevel"
     t_head, st_head = run_path(ctx, sc, head_mode, args.steps, args.warmup, args.maxit)
     log(f"{head_name}: {1 / t_head:.4f} steps/s, iters {[s['iters'] for s in st_head]}, "
         f"fallback {[s['used_fallback'] for s in st_head]}")
@@ -267,6 +293,8 @@ def main():
     if out_fp64:
         out_fp64["speedup_of_headline"] = t64 / t_head
     out_mbj["speedup_of_headline"] = tbj / t_head
+    for v in extra.values():
+        v["speedup_of_headline"] = (1 / v["steps_per_s"]) / t_head
     nb = None
     # nb from the matrix pattern size: 9 scalars per block
     import ctypes as C
@@ -298,7 +326,8 @@ def main():
         "config": {"workload": f"cloth {args.n}x{args.n} quads, noise 1e-3 m, one step = "
                                f"assemble + solve to 1e-8 + update", "path": head_name,
                    "vertices": sc.N, "triangles": sc.F, "hinges": sc.H, "blocks": nb,
-                   "bending_hessian": "spd-projected", "weights": wsrc},
+                   "bending_hessian": "spd-projected", "weights": wsrc,
+                   "multilevel_aggregates": None if args.no_ml else ml_counts},
         "iters": [s["iters"] for s in st_head],
         "used_fallback": [s["used_fallback"] for s in st_head],
         "true_relres": [s["true_relres"] for s in st_head],
@@ -306,6 +335,8 @@ def main():
                      ("ms_assembly", "ms_rhs", "ms_solve", "ms_update")},
         "fp64_bj": out_fp64,
         "mixed_bj": out_mbj,
+        **extra,
+        "multilevel_build_s": ml_build_s,
         "autotune": tuned,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -85,6 +85,14 @@ void free_topology_buffers(eolc_ctx *c)
     S.B = 0;
     S.inst.clear(); S.order.clear(); S.fixed_nodes.clear(); S.user_fixed.clear();
     dfree(c->e_face); dfree(c->beta_hinge);
+    eolc::MLState &ml = c->ml;
+    dfree(ml.hord); dfree(ml.ptr1); dfree(ml.agg1); dfree(ml.gpos); dfree(ml.cptr); dfree(ml.par); dfree(ml.lam);
+    dfree(ml.V); dfree(ml.Dinv); dfree(ml.R); dfree(ml.C);
+    ml.built = false;
+    ml.lv = eolc::MLLevels{};
+    ml.counts.clear();
+    ml.maps.clear();
+    c->bj_iters_hint = c->bj_iters_hint_ml = 0;
     c->alloc_bytes = 0;
 }
 
@@ -405,6 +413,59 @@ int learned_buffers(eolc_ctx *c)
     return EOLC_OK;
 }
 
+// Aggregation hierarchy of the multilevel preconditioner for the current topology
+// (built once, on first use).
+int ml_prepare(eolc_ctx *c)
+{
+    eolc::MLState &m = c->ml;
+    if (m.built) return EOLC_OK;
+    const Topology &t = c->topo;
+    std::vector<int> hord, ptr1, agg1, cptr, par;
+    std::vector<unsigned char> lam;
+    eolc::MLLevels lv{};
+    const std::string msg = eolc::ml_build(t, m.maps, hord, ptr1, agg1, cptr, par, lam, lv);
+    if (!msg.empty()) return fail(c, EOLC_ERR_INVALID, msg);
+    m.lv = lv;
+    m.counts.assign(lv.n, lv.n + lv.L + 1);
+    if (lv.L > 0) {
+        std::vector<int> gpos((size_t)t.N);
+        for (int g = 0; g < t.N; ++g) gpos[t.srow[g]] = g;
+        EOLC_HIP(upload(c, &m.hord, hord));
+        EOLC_HIP(upload(c, &m.ptr1, ptr1));
+        EOLC_HIP(upload(c, &m.agg1, agg1));
+        EOLC_HIP(upload(c, &m.gpos, gpos));
+        EOLC_HIP(upload(c, &m.cptr, cptr));
+        EOLC_HIP(upload(c, &m.par, par));
+        EOLC_HIP(upload(c, &m.lam, lam));
+        const size_t na = (size_t)lv.off[lv.L + 1];
+        EOLC_HIP(dalloc(c, &m.V, (size_t)9 * lv.L * na));
+        EOLC_HIP(dalloc(c, &m.Dinv, 6 * na));
+        EOLC_HIP(dalloc(c, &m.R, 3 * na));
+        EOLC_HIP(dalloc(c, &m.C, 3 * na));
+    }
+    m.built = true;
+    return EOLC_OK;
+}
+
+// The multilevel flag is on for exactly one call.
+struct MlScope {
+    eolc_ctx *c;
+    MlScope(eolc_ctx *ctx, bool on) : c(ctx) { c->ml_on = on && c->ml.lv.L > 0; }
+    ~MlScope() { c->ml_on = false; }
+};
+
+bool valid_mode(int mode)
+{
+    const int base = mode & ~EOLC_SOLVE_MULTILEVEL;
+    return base >= EOLC_SOLVE_FP64_BJ && base <= EOLC_SOLVE_FP64_JACOBI;
+}
+
+bool mixed_mode(int mode)
+{
+    const int base = mode & ~EOLC_SOLVE_MULTILEVEL;
+    return base == EOLC_SOLVE_MIXED_BJ || base == EOLC_SOLVE_MIXED_LEARNED;
+}
+
 // Fixed DOFs -> device mask (and values) for the duration of one call.
 struct FixedScope {
     eolc_ctx *c = nullptr;
@@ -722,6 +783,14 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
                       int maxit, int mode, eolc_solve_stats *st)
 {
     const int N = c->topo.N;
+    const bool multilevel = (mode & EOLC_SOLVE_MULTILEVEL) != 0;
+    mode &= ~EOLC_SOLVE_MULTILEVEL;
+    if (multilevel) {
+        const int rc = ml_prepare(c);
+        if (rc) return rc;
+    }
+    MlScope mls(c, multilevel);
+    int &bj_hint = multilevel ? c->bj_iters_hint_ml : c->bj_iters_hint;
     const bool mixed = (mode == EOLC_SOLVE_MIXED_BJ || mode == EOLC_SOLVE_MIXED_LEARNED);
     if (mode == EOLC_SOLVE_MIXED_LEARNED) {
         if (!c->lp.have_weights) return fail(c, EOLC_ERR_INVALID, "velocity_solve: learned mode without weights");
@@ -752,7 +821,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     int cap = maxit;
     if (learned) {
         EOLC_HIP(eolc::learned_precond(c));
-        if (c->bj_iters_hint > 0) cap = std::min(maxit, 2 * c->bj_iters_hint);
+        if (bj_hint > 0) cap = std::min(maxit, 2 * bj_hint);
     }
     const double delta = mixed ? c->mixed_delta : 0.0;
     EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
@@ -827,7 +896,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     }
     if (status == EOLC_OK && !learned && used_fallback == 0 &&
         (mode == EOLC_SOLVE_FP64_BJ || mode == EOLC_SOLVE_MIXED_BJ))
-        c->bj_iters_hint = k;
+        bj_hint = k;
     if (mixed && status != EOLC_OK && status != EOLC_ERR_NOT_SPD) {
         // fold the last FP32 correction in before reporting
         EOLC_HIP(eolc::pcg_reliable_update(c));
@@ -860,11 +929,76 @@ int eolc_velocity_solve(eolc_ctx *c, int n_fixed, const int *fixed_dof, const do
     if (!c->have_forces) return fail(c, EOLC_ERR_INVALID, "velocity_solve: call forces_fill first");
     if (n_fixed < 0 || (n_fixed && (!fixed_dof || !fixed_val)) || !(tol > 0.0) || maxit <= 0)
         return fail(c, EOLC_ERR_INVALID, "velocity_solve: bad arguments");
-    if (mode < EOLC_SOLVE_FP64_BJ || mode > EOLC_SOLVE_FP64_JACOBI)
-        return fail(c, EOLC_ERR_INVALID, "velocity_solve: unknown mode");
+    if (!valid_mode(mode)) return fail(c, EOLC_ERR_INVALID, "velocity_solve: unknown mode");
     EOLC_HIP(hipSetDevice(c->device));
     if (st) std::memset(st, 0, sizeof(*st));
     return solve_impl(c, n_fixed, fixed_dof, fixed_val, tol, maxit, mode, st);
+}
+
+int eolc_ml_info(eolc_ctx *c, int *levels, int *counts)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_topo) return fail(c, EOLC_ERR_INVALID, "ml_info: no topology");
+    if (!levels) return fail(c, EOLC_ERR_INVALID, "ml_info: null output");
+    EOLC_HIP(hipSetDevice(c->device));
+    const int rc = ml_prepare(c);
+    if (rc) return rc;
+    *levels = c->ml.lv.L;
+    if (counts)
+        for (int l = 0; l <= c->ml.lv.L; ++l) counts[l] = c->ml.counts[l];
+    return EOLC_OK;
+}
+
+int eolc_ml_aggregates(eolc_ctx *c, int level, int *agg)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_topo) return fail(c, EOLC_ERR_INVALID, "ml_aggregates: no topology");
+    EOLC_HIP(hipSetDevice(c->device));
+    const int rc = ml_prepare(c);
+    if (rc) return rc;
+    if (!agg || level < 1 || level > c->ml.lv.L) return fail(c, EOLC_ERR_INVALID, "ml_aggregates: bad level or null output");
+    std::copy(c->ml.maps[level].begin(), c->ml.maps[level].end(), agg);
+    return EOLC_OK;
+}
+
+int eolc_precond_apply(eolc_ctx *c, int n_fixed, const int *fixed_dof, int mode, const double *r, double *z)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_forces) return fail(c, EOLC_ERR_INVALID, "precond_apply: call forces_fill first");
+    if (!r || !z || n_fixed < 0 || (n_fixed && !fixed_dof)) return fail(c, EOLC_ERR_INVALID, "precond_apply: bad arguments");
+    if (!valid_mode(mode)) return fail(c, EOLC_ERR_INVALID, "precond_apply: unknown mode");
+    EOLC_HIP(hipSetDevice(c->device));
+    const int base = mode & ~EOLC_SOLVE_MULTILEVEL;
+    const bool multilevel = (mode & EOLC_SOLVE_MULTILEVEL) != 0;
+    const bool mixed = mixed_mode(mode);
+    const size_t n3 = 3 * (size_t)c->topo.N;
+    for (size_t k = 0; k < n3; ++k)
+        if (!std::isfinite(r[k])) return fail(c, EOLC_ERR_NONFINITE, "precond_apply: non-finite r");
+    int rc = EOLC_OK;
+    if (base == EOLC_SOLVE_MIXED_LEARNED) {
+        if (!c->lp.have_weights) return fail(c, EOLC_ERR_INVALID, "precond_apply: learned mode without weights");
+        if ((rc = learned_buffers(c))) return rc;
+    }
+    if (multilevel && (rc = ml_prepare(c))) return rc;
+    FixedScope fs;
+    if ((rc = fs.set(c, n_fixed, fixed_dof, nullptr))) return rc;
+    MlScope mls(c, multilevel);
+    std::vector<double> rm(r, r + n3);
+    for (int i = 0; i < n_fixed; ++i) rm[(size_t)fixed_dof[i]] = 0.0;
+    EOLC_HIP(eolc::pcg_setup(c, mixed, base == EOLC_SOLVE_FP64_JACOBI ? 1 : 0));
+    if (base == EOLC_SOLVE_MIXED_LEARNED) EOLC_HIP(eolc::learned_precond(c));
+    EOLC_HIP(hipMemcpyAsync(c->r64, rm.data(), n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    EOLC_HIP(eolc::pcg_precondition(c, mixed));
+    if (mixed) {
+        std::vector<float> zf(n3);
+        EOLC_HIP(hipMemcpyAsync(zf.data(), c->z32, n3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(hipStreamSynchronize(c->stream));
+        for (size_t k = 0; k < n3; ++k) z[k] = zf[k];
+    } else {
+        EOLC_HIP(hipMemcpyAsync(z, c->z64, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(hipStreamSynchronize(c->stream));
+    }
+    return EOLC_OK;
 }
 
 int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, const int *fixed_nodes,
@@ -875,7 +1009,7 @@ int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, co
         return fail(c, EOLC_ERR_INVALID, "step: topology, material and state must be set first");
     if (!grav || !(h > 0.0) || n_fixed_nodes < 0 || (n_fixed_nodes && !fixed_nodes) || !(tol > 0.0) || maxit <= 0)
         return fail(c, EOLC_ERR_INVALID, "step: bad arguments");
-    if (mode < EOLC_SOLVE_FP64_BJ || mode > EOLC_SOLVE_FP64_JACOBI) return fail(c, EOLC_ERR_INVALID, "step: unknown mode");
+    if (!valid_mode(mode)) return fail(c, EOLC_ERR_INVALID, "step: unknown mode");
     EOLC_HIP(hipSetDevice(c->device));
     const int N = c->topo.N;
     for (int i = 0; i < n_fixed_nodes; ++i)
@@ -883,7 +1017,7 @@ int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, co
     hipEvent_t ev[5];
     for (auto &e : ev) EOLC_HIP(hipEventCreate(&e));
     EOLC_HIP(hipEventRecord(ev[0], c->stream));
-    const bool mixed = (mode == EOLC_SOLVE_MIXED_BJ || mode == EOLC_SOLVE_MIXED_LEARNED);
+    const bool mixed = mixed_mode(mode);
     int rc = forces_fill_async(c, grav, h, mixed);
     if (rc) return rc;
     EOLC_HIP(hipEventRecord(ev[1], c->stream));

@@ -129,6 +129,40 @@ struct BatchState {
     int graph_maxit = 0, graph_variant = -1;
 };
 
+// Aggregation hierarchy of the additive multilevel preconditioner (k_ml.hip).
+// Level l >= 1 partitions the vertices into aggregates, each level-l aggregate a
+// union of level-(l-1) aggregates.  Aggregate ids are hierarchical: the members of
+// every level-1 aggregate are a contiguous range of `hord` (vertices in hierarchy
+// order), and the children of every level-l aggregate (l >= 2) a contiguous range
+// of level-(l-1) ids.  Per-aggregate arrays of all levels are concatenated: level l
+// starts at aggregate `off[l]`.
+constexpr int kMLMaxLevels = 8;
+struct MLLevels {
+    int L;                        // coarse levels built (0: none)
+    int n[kMLMaxLevels + 1];      // aggregates per level (n[0] = vertices)
+    int off[kMLMaxLevels + 2];    // first aggregate of level l in the concatenated arrays
+    int coff[kMLMaxLevels + 1];   // first entry of level l's child ranges in cptr (l >= 2)
+};
+
+struct MLState {
+    bool built = false;           // hierarchy matches the current topology
+    MLLevels lv{};
+    int *hord = nullptr;          // N: vertices in hierarchy order
+    int *ptr1 = nullptr;          // n_1 + 1: member range of each level-1 aggregate in hord
+    int *agg1 = nullptr;          // N: level-1 aggregate of each vertex
+    int *gpos = nullptr;          // N: SELL storage position of each row
+    int *cptr = nullptr;          // child ranges of levels 2..L (n_l + 1 entries each)
+    int *par = nullptr;           // parent (level l+1 id) of each aggregate of levels 1..L-1
+    unsigned char *lam = nullptr; // per SELL entry: first level whose aggregate holds both
+                                  // vertices (0: diagonal, L+1: none, 255: padding)
+    double *V = nullptr;          // set-up sums: level l holds L x 9 x n_l
+    double *Dinv = nullptr;       // 6 per aggregate: inverse of the Galerkin diagonal block
+    double *R = nullptr;          // 3 per aggregate: restricted residual
+    double *C = nullptr;          // 3 per aggregate: accumulated coarse correction
+    std::vector<int> counts;      // host copy of n[0..L]
+    std::vector<std::vector<int>> maps;   // host: level-l aggregate of every vertex (l = 1..L)
+};
+
 // Learned block preconditioner (message-passing network over the vertex graph).
 constexpr int kLpFeat = 16, kLpHidden = 64, kLpOut = 6, kLpLayers = 3;
 // flat parameter vector: W_in[16][64], b_in[64], {W_l[128][64], b_l[64]} x 3, W_out[64][6], b_out[6]
@@ -229,6 +263,9 @@ struct eolc_ctx {
     double *beta_hinge = nullptr;
     eolc::BatchState batch;
     int bj_iters_hint = 0;      // iterations of the last converged block-Jacobi solve
+    int bj_iters_hint_ml = 0;   // the same with the multilevel preconditioner
+    eolc::MLState ml;           // aggregation hierarchy (built on first use per topology)
+    bool ml_on = false;         // the current solve applies the multilevel levels
 };
 
 namespace eolc {
@@ -245,6 +282,7 @@ hipError_t pcg_init(eolc_ctx *c, bool mixed);
 hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters);
 hipError_t pcg_reliable_update(eolc_ctx *c);
 hipError_t pcg_restart_inner(eolc_ctx *c, int k);
+hipError_t pcg_precondition(eolc_ctx *c, bool mixed);   // z = P r for r in r64 (parity 0)
 hipError_t pcg_finish(eolc_ctx *c, bool mixed);
 hipError_t spmv_true_residual(eolc_ctx *c);
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
@@ -261,6 +299,12 @@ hipError_t learned_loss_grad(eolc_ctx *c);             // lp.e set and masked; l
 hipError_t learned_adam(eolc_ctx *c, float lr);
 hipError_t learned_mask_error(eolc_ctx *c);            // lp.e <- e with fixed DOFs zeroed
 hipError_t scal_restart(eolc_ctx *c, int k);           // iteration k starts with beta = 0
+// multilevel preconditioner (ml_host.cpp builds the hierarchy, k_ml.hip applies it)
+std::string ml_build(const Topology &t, std::vector<std::vector<int>> &agg_levels, std::vector<int> &hord,
+                     std::vector<int> &ptr1, std::vector<int> &agg1, std::vector<int> &cptr, std::vector<int> &par,
+                     std::vector<unsigned char> &lam, MLLevels &lv);
+hipError_t ml_setup(eolc_ctx *c);                             // Galerkin diagonal blocks of every level
+hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done);   // z = D0 r + coarse, r.z partials
 // batched cloths (k_batch.hip)
 hipError_t batch_set_fixmask(eolc_ctx *c, const int *d_nodes, int n);
 hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant);

@@ -1,0 +1,319 @@
+// Additive multilevel preconditioner over an aggregation hierarchy (gfx950).
+//
+//   z = D0 r + sum_{l = 1..L} Z_l D_l^-1 Z_l^T r
+//
+// D0 holds the fine 3x3 blocks (block-Jacobi inverse, or the learned blocks).  Level
+// l groups the vertices into nested aggregates (ml_host.cpp); Z_l is the piecewise-
+// constant prolongation that copies one 3-vector per aggregate to its member
+// vertices (zero on fixed DOFs), and D_l is the 3x3 diagonal block of the Galerkin
+// operator Z_l^T A Z_l: the sum of all blocks A_ij, fixed DOFs masked, whose two
+// vertices lie in the same level-l aggregate.  Each term is symmetric positive
+// (semi-)definite for an SPD A, so z = P r defines an SPD preconditioner: an additive
+// multilevel method of the BPX family (Bramble, Pasciak and Xu 1990) with
+// aggregation coarse spaces.  The coarse terms carry the smooth, slowly converging
+// modes (sheet-wide translations and bending) that no per-vertex block reaches.
+//
+// Set-up per matrix (ml_setup): k_ml_diag1 sums, per level-1 aggregate, the blocks
+// of its member rows bucketed by the first level whose aggregate holds both vertices
+// (`lam`, precomputed per SELL entry); k_ml_diagup adds children upwards.  Every
+// level's blocks are inverted in FP64.
+//
+// Per PCG iteration, after the vector update (ml_apply):
+//   k_ml_restrict : one wavefront per level-2 aggregate, lanes over its children:
+//                   R_1(a) = sum of r over a's members, R_2 = sum of the children's R_1
+//   k_ml_coarse   : one workgroup: R_l for l >= 3 from the children, then from the
+//                   top level down C_l = D_l^-1 R_l + C_{l+1}(parent), l >= 2
+//   k_ml_prolong  : z_i = D0 r_i + D_1^-1 R_1(a_i) + C_2(parent(a_i)); r.z partials
+// All coarse arithmetic is FP64 in a fixed order, so the result does not depend on
+// the launch shape.
+#include "eolc_internal.h"
+#include "pcg_device.h"
+
+using eolc::MLLevels;
+using eolc::SolverScalars;
+
+namespace {
+
+constexpr int kML = eolc::kMLMaxLevels;
+
+// Inverse of a symmetric 3x3 block (upper entries out).  A component whose diagonal
+// is not positive (all of the aggregate's DOFs in it fixed) is left out: its row
+// and column of the inverse are zero.  A block that is not positive definite on
+// the remaining components gives a zero inverse (no correction from it).
+__device__ __forceinline__ void inv_spd3(const double a[9], double *__restrict__ out)
+{
+    double s00 = a[0], s11 = a[4], s22 = a[8];
+    double s01 = 0.5 * (a[1] + a[3]), s02 = 0.5 * (a[2] + a[6]), s12 = 0.5 * (a[5] + a[7]);
+    const bool k0 = s00 > 0.0, k1 = s11 > 0.0, k2 = s22 > 0.0;
+    if (!k0) { s00 = 1.0; s01 = 0.0; s02 = 0.0; }
+    if (!k1) { s11 = 1.0; s01 = 0.0; s12 = 0.0; }
+    if (!k2) { s22 = 1.0; s02 = 0.0; s12 = 0.0; }
+    const double c00 = s11 * s22 - s12 * s12, c01 = s02 * s12 - s01 * s22, c02 = s01 * s12 - s02 * s11;
+    const double m2 = s00 * s11 - s01 * s01;
+    const double det = s00 * c00 + s01 * c01 + s02 * c02;
+    double o[6] = {0, 0, 0, 0, 0, 0};
+    if (s00 > 0.0 && m2 > 0.0 && det > 0.0 && isfinite(det)) {
+        const double id = 1.0 / det;
+        o[0] = c00 * id;
+        o[1] = c01 * id;
+        o[2] = c02 * id;
+        o[3] = (s00 * s22 - s02 * s02) * id;
+        o[4] = (s01 * s02 - s00 * s12) * id;
+        o[5] = m2 * id;
+        if (!k0) { o[0] = 0.0; o[1] = 0.0; o[2] = 0.0; }
+        if (!k1) { o[1] = 0.0; o[3] = 0.0; o[4] = 0.0; }
+        if (!k2) { o[2] = 0.0; o[4] = 0.0; o[5] = 0.0; }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = o[k];
+}
+
+__device__ __forceinline__ void apply_inv(const double *__restrict__ d, double r0, double r1, double r2, double &c0,
+                                          double &c1, double &c2)
+{
+    c0 = fma(d[2], r2, fma(d[1], r1, d[0] * r0));
+    c1 = fma(d[4], r2, fma(d[3], r1, d[1] * r0));
+    c2 = fma(d[5], r2, fma(d[4], r1, d[2] * r0));
+}
+
+// One thread per level-1 aggregate: V_1[m](a) = sum over a's member rows of the
+// blocks (fixed DOFs masked) whose first common level is <= m + 1, m = 0..L-1;
+// D_1^-1 from V_1[0].
+__global__ void __launch_bounds__(256) k_ml_diag1(MLLevels lv, const int *__restrict__ hord,
+                                                  const int *__restrict__ ptr1, const int *__restrict__ gpos,
+                                                  const int *__restrict__ slice_off, const int *__restrict__ slen,
+                                                  const int *__restrict__ scol, const unsigned char *__restrict__ lam,
+                                                  const double *__restrict__ A, int64_t nbs,
+                                                  const unsigned char *__restrict__ fixmask, double *__restrict__ V,
+                                                  double *__restrict__ Dinv)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n1 = lv.n[1], L = lv.L;
+    if (a >= n1) return;
+    double acc[kML][9];
+#pragma unroll
+    for (int m = 0; m < kML; ++m)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) acc[m][c] = 0.0;
+    for (int k = ptr1[a]; k < ptr1[a + 1]; ++k) {
+        const int i = hord[k];
+        const int g = gpos[i];
+        const int len = slen[g];
+        const int base = slice_off[g / eolc::kSlice] + (g % eolc::kSlice);
+        const unsigned fi = fixmask[i];
+        for (int kk = 0; kk < len; ++kk) {
+            const int e = base + kk * eolc::kSlice;
+            const int lm = lam[e];
+            if (lm > L) continue;
+            const unsigned fj = fixmask[scol[e]];
+            double b[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                const int r = c / 3, s = c % 3;
+                b[c] = (((fi >> r) | (fj >> s)) & 1u) ? 0.0 : A[(int64_t)c * nbs + e];
+            }
+#pragma unroll
+            for (int m = 0; m < kML; ++m)
+                if (m < L && lm <= m + 1)
+#pragma unroll
+                    for (int c = 0; c < 9; ++c) acc[m][c] += b[c];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kML; ++m)
+        if (m < L)
+#pragma unroll
+            for (int c = 0; c < 9; ++c) V[(int64_t)(m * 9 + c) * n1 + a] = acc[m][c];
+    inv_spd3(acc[0], Dinv + 6 * (int64_t)a);
+}
+
+// One thread per level-l aggregate (l >= 2): V_l[m] = sum of the children's V_{l-1}[m]
+// for m >= l - 1; D_l^-1 from V_l[l - 1].
+__global__ void __launch_bounds__(256) k_ml_diagup(int l, MLLevels lv, const int *__restrict__ cptr,
+                                                   double *__restrict__ V, double *__restrict__ Dinv)
+{
+    const int A = blockIdx.x * blockDim.x + threadIdx.x;
+    const int L = lv.L, nl = lv.n[l], nc = lv.n[l - 1];
+    if (A >= nl) return;
+    const double *__restrict__ Vc = V + (int64_t)9 * L * lv.off[l - 1];
+    double *__restrict__ Vl = V + (int64_t)9 * L * lv.off[l];
+    const int c0 = cptr[lv.coff[l] + A], c1 = cptr[lv.coff[l] + A + 1];
+    double d[9];
+    for (int m = l - 1; m < L; ++m)
+        for (int c = 0; c < 9; ++c) {
+            double s = 0.0;
+            for (int a = c0; a < c1; ++a) s += Vc[(int64_t)(m * 9 + c) * nc + a];
+            Vl[(int64_t)(m * 9 + c) * nl + A] = s;
+            if (m == l - 1) d[c] = s;
+        }
+    inv_spd3(d, Dinv + 6 * (int64_t)(lv.off[l] + A));
+}
+
+// Restriction of r to levels 1 and 2 (see the file comment).  With a single level,
+// one thread per level-1 aggregate.
+template <class T>
+__global__ void __launch_bounds__(256) k_ml_restrict(MLLevels lv, const int *__restrict__ hord,
+                                                     const int *__restrict__ ptr1, const int *__restrict__ cptr,
+                                                     const T *__restrict__ r, double *__restrict__ R,
+                                                     const SolverScalars *scal, int check_done)
+{
+    if (check_done && scal->done) return;
+    if (lv.L >= 2) {
+        const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;   // uniform in the wavefront
+        const int lane = threadIdx.x % 64;
+        if (w >= lv.n[2]) return;
+        const int c0 = cptr[lv.coff[2] + w], c1 = cptr[lv.coff[2] + w + 1];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int a = c0 + lane; a < c1; a += 64) {
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+            for (int k = ptr1[a]; k < ptr1[a + 1]; ++k) {
+                const int i = hord[k];
+                t0 += (double)r[3 * i];
+                t1 += (double)r[3 * i + 1];
+                t2 += (double)r[3 * i + 2];
+            }
+            R[3 * a] = t0; R[3 * a + 1] = t1; R[3 * a + 2] = t2;
+            s0 += t0; s1 += t1; s2 += t2;
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        if (lane == 0) {
+            double *o = R + 3 * (int64_t)(lv.off[2] + w);
+            o[0] = s0; o[1] = s1; o[2] = s2;
+        }
+    } else {
+        const int a = blockIdx.x * blockDim.x + threadIdx.x;
+        if (a >= lv.n[1]) return;
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (int k = ptr1[a]; k < ptr1[a + 1]; ++k) {
+            const int i = hord[k];
+            t0 += (double)r[3 * i];
+            t1 += (double)r[3 * i + 1];
+            t2 += (double)r[3 * i + 2];
+        }
+        R[3 * a] = t0; R[3 * a + 1] = t1; R[3 * a + 2] = t2;
+    }
+}
+
+// Levels >= 2 in one workgroup (they hold a few thousand aggregates at most for a
+// million vertices).
+constexpr int kCoarseWG = 1024;
+__global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int *__restrict__ cptr,
+                                                         const int *__restrict__ par, const double *__restrict__ Dinv,
+                                                         double *__restrict__ R, double *__restrict__ C,
+                                                         const SolverScalars *scal, int check_done)
+{
+    if (check_done && scal->done) return;
+    const int L = lv.L;
+    for (int l = 3; l <= L; ++l) {
+        for (int A = threadIdx.x; A < lv.n[l]; A += kCoarseWG) {
+            const int c0 = cptr[lv.coff[l] + A], c1 = cptr[lv.coff[l] + A + 1];
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int a = c0; a < c1; ++a) {
+                const double *q = R + 3 * (int64_t)(lv.off[l - 1] + a);
+                s0 += q[0]; s1 += q[1]; s2 += q[2];
+            }
+            double *o = R + 3 * (int64_t)(lv.off[l] + A);
+            o[0] = s0; o[1] = s1; o[2] = s2;
+        }
+        __syncthreads();
+    }
+    for (int l = L; l >= 2; --l) {
+        for (int a = threadIdx.x; a < lv.n[l]; a += kCoarseWG) {
+            const int64_t g = lv.off[l] + a;
+            double c0, c1, c2;
+            apply_inv(Dinv + 6 * g, R[3 * g], R[3 * g + 1], R[3 * g + 2], c0, c1, c2);
+            if (l < L) {
+                const double *up = C + 3 * (int64_t)(lv.off[l + 1] + par[g]);
+                c0 += up[0]; c1 += up[1]; c2 += up[2];
+            }
+            C[3 * g] = c0; C[3 * g + 1] = c1; C[3 * g + 2] = c2;
+        }
+        __syncthreads();
+    }
+}
+
+// z = D0 r + coarse correction (zero on fixed DOFs); r.z partials -> PB[parity] slot 0.
+template <class T>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ par,
+             const double *__restrict__ Dinv, const double *__restrict__ R, const double *__restrict__ C,
+             const T *__restrict__ D, const T *__restrict__ r, T *__restrict__ z,
+             const unsigned char *__restrict__ fixmask, const SolverScalars *scal, double *partials, int check_done)
+{
+    __shared__ double lds[2 * (WG / 64) + 2];
+    if (check_done && scal->done) return;
+    int rb, re;
+    wg_rows(N, &rb, &re);
+    double srz = 0.0;
+    for (int i = rb + threadIdx.x; i < re; i += WG) {
+        const int a = agg1[i];
+        double c0, c1, c2;
+        apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
+        if (lv.L >= 2) {
+            const double *up = C + 3 * (int64_t)(lv.off[2] + par[a]);
+            c0 += up[0]; c1 += up[1]; c2 += up[2];
+        }
+        const T r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+        T f0, f1, f2;
+        eolc::apply_sym3<T>(D + i, N, r0, r1, r2, f0, f1, f2);
+        const unsigned fm = fixmask[i];
+        const T z0 = (fm & 1u) ? (T)0 : (T)((double)f0 + c0);
+        const T z1 = (fm & 2u) ? (T)0 : (T)((double)f1 + c1);
+        const T z2 = (fm & 4u) ? (T)0 : (T)((double)f2 + c2);
+        z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
+        srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
+    }
+    double v[1] = {srz};
+    write_partials<1>(v, PB(partials, parity), lds);
+}
+
+inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
+
+}  // namespace
+
+namespace eolc {
+
+hipError_t ml_setup(eolc_ctx *c)
+{
+    const MLState &m = c->ml;
+    const MLLevels &lv = m.lv;
+    if (lv.L <= 0) return hipSuccess;
+    const Topology &t = c->topo;
+    hipLaunchKernelGGL(k_ml_diag1, dim3(blocks_for(lv.n[1])), dim3(256), 0, c->stream, lv, m.hord, m.ptr1, m.gpos,
+                       c->slice_off, c->slen, c->scol, m.lam, c->A64, t.nbs, c->fixmask, m.V, m.Dinv);
+    for (int l = 2; l <= lv.L; ++l)
+        hipLaunchKernelGGL(k_ml_diagup, dim3(blocks_for(lv.n[l])), dim3(256), 0, c->stream, l, lv, m.cptr, m.V,
+                           m.Dinv);
+    return hipGetLastError();
+}
+
+hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
+{
+    const MLState &m = c->ml;
+    const MLLevels &lv = m.lv;
+    const int N = c->topo.N;
+    const int G = pcg_grid(N);
+    const int rblocks = lv.L >= 2 ? blocks_for((int64_t)64 * lv.n[2]) : blocks_for(lv.n[1]);
+    if (mixed) {
+        hipLaunchKernelGGL(k_ml_restrict<float>, dim3(rblocks), dim3(256), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
+                           c->r32, m.R, c->scal, check_done);
+    } else {
+        hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(256), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
+                           c->r64, m.R, c->scal, check_done);
+    }
+    if (lv.L >= 2)
+        hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
+                           c->scal, check_done);
+    if (mixed) {
+        hipLaunchKernelGGL(k_ml_prolong<float>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.par, m.Dinv,
+                           m.R, m.C, c->D32, c->r32, c->z32, c->fixmask, c->scal, c->partials, check_done);
+    } else {
+        hipLaunchKernelGGL(k_ml_prolong<double>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.par,
+                           m.Dinv, m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, check_done);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace eolc

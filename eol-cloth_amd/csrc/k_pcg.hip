@@ -31,6 +31,7 @@
 #include <type_traits>
 
 #include "eolc_internal.h"
+#include "pcg_device.h"
 
 using eolc::SolverScalars;
 
@@ -39,99 +40,6 @@ using eolc::SolverScalars;
 #pragma clang fp contract(off)
 
 namespace {
-
-constexpr int WG = eolc::kPcgWG;
-
-__device__ __forceinline__ int logical_wg()
-{
-    int G = gridDim.x;
-    int w = blockIdx.x;
-    return (w & 7) * (G >> 3) + (w >> 3);
-}
-
-__device__ __forceinline__ void wg_rows(int N, int *begin, int *end)
-{
-    int G = gridDim.x;
-    int per = (N + G - 1) / G;
-    int lw = logical_wg();
-    int b = lw * per, e = b + per;
-    *begin = b < N ? b : N;
-    *end = e < N ? e : N;
-}
-
-// slice owned by this wavefront (16 per workgroup)
-__device__ __forceinline__ int wave_slice() { return logical_wg() * (WG / 64) + (threadIdx.x >> 6); }
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-// Sum over the workgroup of K values per thread; result broadcast to all threads.
-template <int K>
-__device__ __forceinline__ void block_allsum(double (&v)[K], double *lds /* K * WG/64 + K */)
-{
-    constexpr int NW = WG / 64;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        double s = wave_sum(v[k]);
-        if (lane == 0) lds[k * NW + wid] = s;
-    }
-    __syncthreads();
-    if (wid == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            double s = lane < NW ? lds[k * NW + lane] : 0.0;
-#pragma unroll
-            for (int o = NW / 2; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-            if (lane == 0) lds[K * NW + k] = s;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = lds[K * NW + k];
-}
-
-// Fixed-order sum of K partial arrays (stride kMaxPartials) of length G, done by
-// every workgroup; the result is identical everywhere.
-template <int K>
-__device__ __forceinline__ void reduce_partials(const double *__restrict__ part, double (&out)[K], double *lds)
-{
-    const int G = gridDim.x;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < G; i += WG) s += part[k * eolc::kMaxPartials + i];
-        out[k] = s;
-    }
-    block_allsum<K>(out, lds);
-}
-
-template <int K>
-__device__ __forceinline__ void write_partials(double (&v)[K], double *__restrict__ part, double *lds)
-{
-    block_allsum<K>(v, lds);
-    if (threadIdx.x == 0) {
-        const int lw = logical_wg();
-#pragma unroll
-        for (int k = 0; k < K; ++k) part[k * eolc::kMaxPartials + lw] = v[k];
-    }
-}
-
-template <class T>
-__device__ __forceinline__ void load3(const T *__restrict__ a, int i, double &x, double &y, double &z)
-{
-    x = (double)a[3 * i]; y = (double)a[3 * i + 1]; z = (double)a[3 * i + 2];
-}
-
-// Partials layout (doubles, stride kMaxPartials): PA at slot 0, PB[parity] (r.z, r.r)
-// at slots 1 + 2 parity and 2 + 2 parity, residual-norm partials at slot 5.
-__device__ __forceinline__ double *PA(double *p) { return p; }
-__device__ __forceinline__ double *PB(double *p, int parity) { return p + (1 + 2 * parity) * eolc::kMaxPartials; }
 
 // ---------------------------------------------------------------------------
 // DIFF = false (FP64 path): q = A p with the assembled blocks of MDK.
@@ -372,8 +280,9 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
 }
 
 // ---------------------------------------------------------------------------
-// TX is the solution accumulator (double in FP64 mode, float in mixed mode).
-template <class T, class TX>
+// TX is the solution accumulator (double in FP64 mode, float in mixed mode).  ML:
+// the multilevel kernels form z and r.z afterwards (k_ml.hip); only r.r here.
+template <class T, class TX, bool ML>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, const T *__restrict__ q,
              TX *__restrict__ x, T *__restrict__ r, T *__restrict__ z, SolverScalars *scal, double *partials)
@@ -403,14 +312,21 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
         const T r1 = fma(-alphaT, q[3 * i + 1], r[3 * i + 1]);
         const T r2 = fma(-alphaT, q[3 * i + 2], r[3 * i + 2]);
         r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
-        T z0, z1, z2;
-        eolc::apply_sym3<T>(D + i, N, r0, r1, r2, z0, z1, z2);
-        z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
-        srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
+        if constexpr (!ML) {
+            T z0, z1, z2;
+            eolc::apply_sym3<T>(D + i, N, r0, r1, r2, z0, z1, z2);
+            z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
+            srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
+        }
         srr += ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2;
     }
-    double v[2] = {srz, srr};
-    write_partials<2>(v, PB(partials, (k + 1) & 1), lds);
+    if constexpr (ML) {
+        double v[1] = {srr};
+        write_partials<1>(v, PB(partials, (k + 1) & 1) + eolc::kMaxPartials, lds);
+    } else {
+        double v[2] = {srz, srr};
+        write_partials<2>(v, PB(partials, (k + 1) & 1), lds);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal->iters = k + 1;
         scal->pq = pq[0];
@@ -470,8 +386,9 @@ __global__ void __launch_bounds__(WG) k_reduce_to(int G, const double *__restric
     if (threadIdx.x == 0) *dst = s[0];
 }
 
-// z = D r (r converted from FP64 to T), partials r.z, r.r -> PB[parity].
-template <class T>
+// z = D r (r converted from FP64 to T), partials r.z, r.r -> PB[parity].  ML: r and
+// r.r only; the multilevel kernels form z and r.z.
+template <class T, bool ML>
 __global__ void __launch_bounds__(WG)
 k_precond_init(int parity, int N, const T *__restrict__ D, const double *r64, T *r, T *__restrict__ z,
                double *partials)
@@ -483,18 +400,25 @@ k_precond_init(int parity, int N, const T *__restrict__ D, const double *r64, T 
     for (int i = rb + threadIdx.x; i < re; i += WG) {
         const T r0 = (T)r64[3 * i], r1 = (T)r64[3 * i + 1], r2 = (T)r64[3 * i + 2];
         r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
-        const T *d = D + i;
         const double s0 = r0, s1 = r1, s2 = r2;
-        const double e00 = d[0], e01 = d[N], e02 = d[2 * N], e11 = d[3 * N], e12 = d[4 * N], e22 = d[5 * N];
-        const T z0 = (T)fma(e02, s2, fma(e01, s1, e00 * s0));
-        const T z1 = (T)fma(e12, s2, fma(e11, s1, e01 * s0));
-        const T z2 = (T)fma(e22, s2, fma(e12, s1, e02 * s0));
-        z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
-        rz += (s0 * (double)z0 + s1 * (double)z1) + s2 * (double)z2;
+        if constexpr (!ML) {
+            const T *d = D + i;
+            const double e00 = d[0], e01 = d[N], e02 = d[2 * N], e11 = d[3 * N], e12 = d[4 * N], e22 = d[5 * N];
+            const T z0 = (T)fma(e02, s2, fma(e01, s1, e00 * s0));
+            const T z1 = (T)fma(e12, s2, fma(e11, s1, e01 * s0));
+            const T z2 = (T)fma(e22, s2, fma(e12, s1, e02 * s0));
+            z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
+            rz += (s0 * (double)z0 + s1 * (double)z1) + s2 * (double)z2;
+        }
         rr += (s0 * s0 + s1 * s1) + s2 * s2;
     }
-    double v[2] = {rz, rr};
-    write_partials<2>(v, PB(partials, parity), lds);
+    if constexpr (ML) {
+        double v[1] = {rr};
+        write_partials<1>(v, PB(partials, parity) + eolc::kMaxPartials, lds);
+    } else {
+        double v[2] = {rz, rr};
+        write_partials<2>(v, PB(partials, parity), lds);
+    }
 }
 
 // Block-Jacobi inverse of the diagonal blocks, with fixed DOFs replaced by
@@ -682,8 +606,40 @@ void launch_iteration(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const 
     const int pin = k & 1, pout = pin ^ 1;
     constexpr bool diff = std::is_same<T, float>::value;
     launch_spmv<T>(c, c->spmv_variant[diff ? 0 : 1], k, G, A, Mm, z, p[pin], p[pout], q);
-    hipLaunchKernelGGL((k_pcg_update<T, T>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r, z, c->scal,
-                       c->partials);
+    if (c->ml_on) {
+        hipLaunchKernelGGL((k_pcg_update<T, T, true>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r,
+                           z, c->scal, c->partials);
+        (void)eolc::ml_apply(c, diff, (k + 1) & 1, 1);   // launch errors surface at the chunk's hipGetLastError
+    } else {
+        hipLaunchKernelGGL((k_pcg_update<T, T, false>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r,
+                           z, c->scal, c->partials);
+    }
+}
+
+// z = P r for the residual r64 (FP64): mixed -> r32, z32; FP64 -> z64 (r64 in place);
+// r.z, r.r -> PB[parity].
+hipError_t precondition(eolc_ctx *c, bool mixed, int parity)
+{
+    const int N = c->topo.N;
+    const int G = eolc::pcg_grid(N);
+    if (mixed) {
+        if (c->ml_on)
+            hipLaunchKernelGGL((k_precond_init<float, true>), dim3(G), dim3(WG), 0, c->stream, parity, N, c->D32,
+                               c->r64, c->r32, c->z32, c->partials);
+        else
+            hipLaunchKernelGGL((k_precond_init<float, false>), dim3(G), dim3(WG), 0, c->stream, parity, N, c->D32,
+                               c->r64, c->r32, c->z32, c->partials);
+    } else {
+        if (c->ml_on)
+            hipLaunchKernelGGL((k_precond_init<double, true>), dim3(G), dim3(WG), 0, c->stream, parity, N, c->D64,
+                               c->r64, c->r64, c->z64, c->partials);
+        else
+            hipLaunchKernelGGL((k_precond_init<double, false>), dim3(G), dim3(WG), 0, c->stream, parity, N, c->D64,
+                               c->r64, c->r64, c->z64, c->partials);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && c->ml_on) e = eolc::ml_apply(c, mixed, parity, 0);
+    return e;
 }
 
 void launch_resid64(eolc_ctx *c, int G, const double *xin, const double *b, double *out)
@@ -710,7 +666,9 @@ hipError_t pcg_setup(eolc_ctx *c, bool /*mixed*/, int precond_kind)
     int N = c->topo.N;
     hipLaunchKernelGGL(k_block_jacobi, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, c->diagpos, c->A64,
                        c->topo.nbs, c->fixmask, precond_kind, c->D64, c->D32);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && c->ml_on) e = ml_setup(c);
+    return e;
 }
 
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set)
@@ -739,13 +697,10 @@ hipError_t pcg_init(eolc_ctx *c, bool mixed)
     if (mixed) {
         e = hipMemsetAsync(c->xl32, 0, (size_t)3 * N * sizeof(float), c->stream);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_precond_init<float>, dim3(G), dim3(WG), 0, c->stream, 0, N, c->D32, c->r64, c->r32,
-                           c->z32, c->partials);
-    } else {
-        hipLaunchKernelGGL(k_precond_init<double>, dim3(G), dim3(WG), 0, c->stream, 0, N, c->D64, c->r64, c->r64,
-                           c->z64, c->partials);
     }
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return precondition(c, mixed, 0);
 }
 
 hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters)
@@ -775,14 +730,10 @@ hipError_t pcg_reliable_update(eolc_ctx *c)
 }
 
 // Continue the inner iteration at index k from the replaced residual, keeping p.
-hipError_t pcg_restart_inner(eolc_ctx *c, int k)
-{
-    const int N = c->topo.N;
-    const int G = pcg_grid(N);
-    hipLaunchKernelGGL(k_precond_init<float>, dim3(G), dim3(WG), 0, c->stream, k & 1, N, c->D32, c->r64, c->r32,
-                       c->z32, c->partials);
-    return hipGetLastError();
-}
+hipError_t pcg_restart_inner(eolc_ctx *c, int k) { return precondition(c, true, k & 1); }
+
+// z = P r of the current mode for r in r64 (the preconditioner-application entry point)
+hipError_t pcg_precondition(eolc_ctx *c, bool mixed) { return precondition(c, mixed, 0); }
 
 hipError_t pcg_finish(eolc_ctx *c, bool /*mixed*/)
 {

@@ -21,6 +21,8 @@ OPT_INCLUDE_BENDING_FORCE = 0x1
 OPT_SPD_PROJECT_BENDING = 0x2
 
 SOLVE_FP64_BJ, SOLVE_MIXED_BJ, SOLVE_MIXED_LEARNED, SOLVE_FP64_JACOBI = 0, 1, 2, 3
+SOLVE_MULTILEVEL = 0x10   # flag: add the aggregation levels of the multilevel preconditioner
+ML_MAX_LEVELS = 8
 MAT_M, MAT_MDK = 0, 1
 
 EXPORTED_SYMBOLS = (
@@ -31,6 +33,7 @@ EXPORTED_SYMBOLS = (
     "eolc_learned_loss_grad", "eolc_learned_train", "eolc_autotune", "eolc_set_spmv_variant",
     "eolc_set_batch_topology", "eolc_set_batch_material", "eolc_batch_step", "eolc_batch_autotune",
     "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta", "eolc_collision_constraints",
+    "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition",
 )
 
 BATCH_EAGER = 0x1
@@ -114,6 +117,11 @@ def lib():
         L.eolc_stream_copy.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.eolc_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_void_p]
         L.eolc_set_spmv_variant.argtypes = [vp, C.c_int, C.c_int]
+        L.eolc_ml_info.argtypes = [vp, C.POINTER(C.c_int), C.c_void_p]
+        L.eolc_ml_aggregates.argtypes = [vp, C.c_int, C.c_void_p]
+        L.eolc_precond_apply.argtypes = [vp, C.c_int, C.c_void_p, C.c_int, _dp, _dp]
+        L.eolc_ml_partition.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.c_void_p,
+                                        C.c_void_p]
         L.eolc_learned_set_weights.argtypes = [vp, C.c_void_p, C.c_int]
         L.eolc_learned_get_weights.argtypes = [vp, C.c_void_p, C.c_int]
         L.eolc_learned_infer.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -135,6 +143,20 @@ def _i(a):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def ml_partition(rowptr, col):
+    """Multilevel aggregation hierarchy of a vertex block pattern (host code, no GPU):
+    list of per-vertex aggregate maps for levels 1..L."""
+    rp, cl = _i(rowptr), _i(col)
+    N = rp.size - 1
+    L = C.c_int(0)
+    cnt = np.zeros(ML_MAX_LEVELS + 1, dtype=np.int32)
+    agg = np.zeros(ML_MAX_LEVELS * N, dtype=np.int32)
+    rc = lib().eolc_ml_partition(N, _ptr(rp), _ptr(cl), C.byref(L), _ptr(cnt), _ptr(agg))
+    if rc != OK:
+        raise EolcError(rc, "ml_partition: bad pattern")
+    return [agg[l * N:(l + 1) * N].copy() for l in range(L.value)]
 
 
 class Context:
@@ -233,6 +255,25 @@ class Context:
                              _ptr(sc), float(tol), int(maxit), int(mode), C.byref(st))
         self._check(rc, allow=allow)
         return rc, st
+
+    def ml_info(self):
+        """Aggregates per level of the multilevel hierarchy: [N, n_1, ..., n_L]."""
+        L = C.c_int(0)
+        cnt = np.zeros(ML_MAX_LEVELS + 1, dtype=np.int32)
+        self._check(lib().eolc_ml_info(self._h, C.byref(L), _ptr(cnt)))
+        return [int(v) for v in cnt[: L.value + 1]]
+
+    def ml_aggregates(self, level):
+        agg = np.zeros(self.N, dtype=np.int32)
+        self._check(lib().eolc_ml_aggregates(self._h, int(level), _ptr(agg)))
+        return agg
+
+    def precond_apply(self, r, fixed_dof=(), mode=SOLVE_FP64_BJ):
+        """z = P r with the preconditioner of `mode` for the last forces_fill."""
+        fd, r = _i(fixed_dof), _d(r)
+        z = np.zeros(3 * self.N)
+        self._check(lib().eolc_precond_apply(self._h, fd.size, _ptr(fd) if fd.size else None, int(mode), r, z))
+        return z
 
     def time_spmv(self, precision, reps=20):
         ms = C.c_double(0)

@@ -52,11 +52,39 @@ typedef enum {
                                        block-Jacobi after twice the iterations of the last
                                        converged block-Jacobi solve on the context */
 #define EOLC_SOLVE_FP64_JACOBI   3  /* FP64 scalar-Jacobi PCG (Eigen's default) */
+#define EOLC_SOLVE_MULTILEVEL 0x10  /* flag, OR-ed into any mode: the mode's fine blocks plus
+                                       the coarse levels of an additive multilevel
+                                       preconditioner over an aggregation hierarchy of the
+                                       vertex graph (built on first use per topology); the
+                                       coarse terms carry the smooth modes that limit the
+                                       block preconditioners.  MIXED_LEARNED's fallback
+                                       keeps the flag. */
 
 /* Reliable-update trigger of the mixed modes (default 0.5): the FP32 correction is
  * folded into the FP64 solution and the residual replaced by the FP64 true residual
  * whenever the recursive residual has dropped by this factor since the last update. */
 int eolc_set_mixed_delta(eolc_ctx *ctx, double delta);
+
+/* Aggregation hierarchy of EOLC_SOLVE_MULTILEVEL (no counterpart in the reference:
+ * a preconditioner of the solver-"none" solve).  eolc_ml_info builds it if needed
+ * and reports the number of coarse levels and the aggregates per level: counts[0] =
+ * vertices, counts[l] for l = 1..levels (room for EOLC_ML_MAX_LEVELS + 1 ints).
+ * eolc_ml_aggregates: the level-l aggregate of every vertex (N ints), l = 1..levels. */
+#define EOLC_ML_MAX_LEVELS 8
+int eolc_ml_info(eolc_ctx *ctx, int *levels, int *counts);
+int eolc_ml_aggregates(eolc_ctx *ctx, int level, int *agg);
+/* The same hierarchy without a context or a GPU, for a block pattern given as vertex
+ * CSR (rowptr: N + 1 ints, col: the stored blocks' columns; the diagonal may be
+ * present).  levels, counts as above; agg (may be null): room for
+ * EOLC_ML_MAX_LEVELS x N ints, level l's map at (l - 1) N. */
+int eolc_ml_partition(int N, const int *rowptr, const int *col, int *levels, int *counts, int *agg);
+
+/* z = P r with the preconditioner of `mode` (any solve mode, with or without
+ * EOLC_SOLVE_MULTILEVEL) for the matrix of the last eolc_forces_fill, fixed DOFs
+ * handled as in eolc_velocity_solve: their entries of r are ignored and their
+ * entries of z are 0.  r, z: 3N doubles (mixed modes apply P in FP32).  For tests
+ * and for callers that run their own Krylov method. */
+int eolc_precond_apply(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, int mode, const double *r, double *z);
 
 /* matrices (eolc_download_csr) */
 #define EOLC_MAT_M   0
