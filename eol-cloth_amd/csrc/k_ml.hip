@@ -19,11 +19,12 @@
 // level's blocks are inverted in FP64.
 //
 // Per PCG iteration, after the vector update (ml_apply):
-//   k_ml_restrict : one wavefront per level-2 aggregate, lanes over its children:
+//   k_ml_restrict : one wavefront per level-2 aggregate, vertices staged through LDS:
 //                   R_1(a) = sum of r over a's members, R_2 = sum of the children's R_1
-//   k_ml_coarse   : one workgroup: R_l for l >= 3 from the children, then from the
-//                   top level down C_l = D_l^-1 R_l + C_{l+1}(parent), l >= 2
-//   k_ml_prolong  : z_i = D0 r_i + D_1^-1 R_1(a_i) + C_2(parent(a_i)); r.z partials
+//   k_ml_coarse   : one workgroup (levels >= 3 only): R_l for l >= 3, then
+//                   C_l = D_l^-1 R_l + C_{l+1}(parent) from the top level down to 3
+//   k_ml_prolong  : z_i = D0 r_i + D_1^-1 R_1(a_i) + D_2^-1 R_2(b_i) + C_3(parent(b_i)),
+//                   b_i the level-2 aggregate; r.z partials
 // All coarse arithmetic is FP64 in a fixed order, so the result does not depend on
 // the launch shape.
 #include "eolc_internal.h"
@@ -149,32 +150,74 @@ __global__ void __launch_bounds__(256) k_ml_diagup(int l, MLLevels lv, const int
     inv_spd3(d, Dinv + 6 * (int64_t)(lv.off[l] + A));
 }
 
-// Restriction of r to levels 1 and 2 (see the file comment).  With a single level,
-// one thread per level-1 aggregate.
-template <class T>
-__global__ void __launch_bounds__(256) k_ml_restrict(MLLevels lv, const int *__restrict__ hord,
-                                                     const int *__restrict__ ptr1, const int *__restrict__ cptr,
-                                                     const T *__restrict__ r, double *__restrict__ R,
-                                                     const SolverScalars *scal, int check_done)
+// Ordering of LDS traffic between the lanes of one wavefront (no workgroup barrier:
+// the wavefronts of a workgroup run different trip counts).
+__device__ __forceinline__ void wave_sync()
 {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Restriction of r to levels 1 and 2: one wavefront per level-2 aggregate.  Its
+// vertices are a contiguous range of hord; the lanes stage 64 of them at a time in
+// LDS (one vertex per lane), then each lane sums the members of one level-1 child
+// that fall in the chunk, in member order.  R_2 is the wavefront sum of the
+// children's R_1 (fixed shuffle tree).  With a single level, a wavefront takes 64
+// level-1 aggregates.
+constexpr int kRestrictWG = 256;
+template <class T>
+__global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const int *__restrict__ hord,
+                                                             const int *__restrict__ ptr1, const int *__restrict__ cptr,
+                                                             const T *__restrict__ r, double *__restrict__ R,
+                                                             const SolverScalars *scal, int check_done)
+{
+    __shared__ double stage[kRestrictWG / 64][3 * 64];
     if (check_done && scal->done) return;
+    const int lane = threadIdx.x % 64;
+    double *__restrict__ buf = stage[threadIdx.x / 64];
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;   // uniform in the wavefront
+    int a0, a1;
     if (lv.L >= 2) {
-        const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;   // uniform in the wavefront
-        const int lane = threadIdx.x % 64;
         if (w >= lv.n[2]) return;
-        const int c0 = cptr[lv.coff[2] + w], c1 = cptr[lv.coff[2] + w + 1];
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int a = c0 + lane; a < c1; a += 64) {
-            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-            for (int k = ptr1[a]; k < ptr1[a + 1]; ++k) {
+        a0 = cptr[lv.coff[2] + w];
+        a1 = cptr[lv.coff[2] + w + 1];
+    } else {
+        a0 = w * 64;
+        if (a0 >= lv.n[1]) return;
+        a1 = min(a0 + 64, lv.n[1]);
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int g0 = a0; g0 < a1; g0 += 64) {   // groups of up to 64 children
+        const int g1 = min(g0 + 64, a1);
+        const int a = g0 + lane;
+        const bool mine = a < g1;
+        const int m0 = mine ? ptr1[a] : 0, m1 = mine ? ptr1[a + 1] : 0;
+        const int k0 = ptr1[g0], k1 = ptr1[g1];
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (int kb = k0; kb < k1; kb += 64) {
+            const int k = kb + lane;
+            if (k < k1) {
                 const int i = hord[k];
-                t0 += (double)r[3 * i];
-                t1 += (double)r[3 * i + 1];
-                t2 += (double)r[3 * i + 2];
+                buf[3 * lane] = (double)r[3 * i];
+                buf[3 * lane + 1] = (double)r[3 * i + 1];
+                buf[3 * lane + 2] = (double)r[3 * i + 2];
             }
+            wave_sync();
+            const int e0 = max(m0, kb), e1 = min(m1, kb + 64);
+            for (int m = e0; m < e1; ++m) {
+                t0 += buf[3 * (m - kb)];
+                t1 += buf[3 * (m - kb) + 1];
+                t2 += buf[3 * (m - kb) + 2];
+            }
+            wave_sync();
+        }
+        if (mine) {
             R[3 * a] = t0; R[3 * a + 1] = t1; R[3 * a + 2] = t2;
             s0 += t0; s1 += t1; s2 += t2;
         }
+    }
+    if (lv.L >= 2) {
         s0 = wave_sum(s0);
         s1 = wave_sum(s1);
         s2 = wave_sum(s2);
@@ -182,59 +225,145 @@ __global__ void __launch_bounds__(256) k_ml_restrict(MLLevels lv, const int *__r
             double *o = R + 3 * (int64_t)(lv.off[2] + w);
             o[0] = s0; o[1] = s1; o[2] = s2;
         }
-    } else {
-        const int a = blockIdx.x * blockDim.x + threadIdx.x;
-        if (a >= lv.n[1]) return;
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-        for (int k = ptr1[a]; k < ptr1[a + 1]; ++k) {
-            const int i = hord[k];
-            t0 += (double)r[3 * i];
-            t1 += (double)r[3 * i + 1];
-            t2 += (double)r[3 * i + 2];
-        }
-        R[3 * a] = t0; R[3 * a + 1] = t1; R[3 * a + 2] = t2;
     }
 }
 
-// Levels >= 2 in one workgroup (they hold a few thousand aggregates at most for a
-// million vertices).
+// Levels >= 3 in one workgroup: R_l from the children (l = 3..L), then from the top
+// down C_l = D_l^-1 R_l + C_{l+1}(parent) (l = L..3).  These levels hold a few hundred
+// aggregates for a million vertices: then each thread owns one, their R and C live
+// in LDS and only C_3 is written out.  Larger hierarchies loop over global memory.
 constexpr int kCoarseWG = 1024;
+constexpr int kCoarseLds = 1024;
 __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int *__restrict__ cptr,
                                                          const int *__restrict__ par, const double *__restrict__ Dinv,
                                                          double *__restrict__ R, double *__restrict__ C,
                                                          const SolverScalars *scal, int check_done)
 {
+    __shared__ double lds[6 * kCoarseLds];
     if (check_done && scal->done) return;
     const int L = lv.L;
+    const int base = lv.off[3];
+    const int M = lv.off[L + 1] - base;   // aggregates of levels >= 3
+    if (M <= kCoarseWG) {
+        // one aggregate per thread: its inverse block, parent and child range are
+        // loaded up front (with level 3's children sums), then every phase runs in LDS
+        double *Rl = lds, *Cl = lds + 3 * kCoarseLds;
+        const int j = threadIdx.x;
+        int l = 0, pa = 0, c0 = 0, c1 = 0;
+        double d[6] = {0, 0, 0, 0, 0, 0};
+        if (j < M) {
+            l = 3;
+            while (l < L && j >= lv.off[l + 1] - base) ++l;
+            const int a = j - (lv.off[l] - base);
+            const int64_t g = base + j;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) d[k] = Dinv[6 * g + k];
+            if (l < L) pa = par[g];
+            c0 = cptr[lv.coff[l] + a];
+            c1 = cptr[lv.coff[l] + a + 1];
+            if (l == 3) {
+                const double *src = R + 3 * (int64_t)lv.off[2];
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+                for (int b = c0; b < c1; b += 8) {   // children in order, 8 loads in flight
+                    double v[8][3];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const bool ok = b + u < c1;
+                        const double *q = src + 3 * (ok ? b + u : c0);
+                        v[u][0] = ok ? q[0] : 0.0;
+                        v[u][1] = ok ? q[1] : 0.0;
+                        v[u][2] = ok ? q[2] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        s0 += v[u][0]; s1 += v[u][1]; s2 += v[u][2];
+                    }
+                }
+                Rl[3 * j] = s0; Rl[3 * j + 1] = s1; Rl[3 * j + 2] = s2;
+            }
+        }
+        __syncthreads();
+        for (int lev = 4; lev <= L; ++lev) {
+            if (j < M && l == lev) {
+                const int cb = lv.off[lev - 1] - base;
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+                for (int b = c0; b < c1; ++b) {
+                    s0 += Rl[3 * (cb + b)]; s1 += Rl[3 * (cb + b) + 1]; s2 += Rl[3 * (cb + b) + 2];
+                }
+                Rl[3 * j] = s0; Rl[3 * j + 1] = s1; Rl[3 * j + 2] = s2;
+            }
+            __syncthreads();
+        }
+        for (int lev = L; lev >= 3; --lev) {
+            if (j < M && l == lev) {
+                double e0, e1, e2;
+                apply_inv(d, Rl[3 * j], Rl[3 * j + 1], Rl[3 * j + 2], e0, e1, e2);
+                if (lev < L) {
+                    const int up = lv.off[lev + 1] - base + pa;
+                    e0 += Cl[3 * up]; e1 += Cl[3 * up + 1]; e2 += Cl[3 * up + 2];
+                }
+                Cl[3 * j] = e0; Cl[3 * j + 1] = e1; Cl[3 * j + 2] = e2;
+            }
+            __syncthreads();
+        }
+        if (j < M && l == 3) {
+            double *o = C + 3 * ((int64_t)base + j);
+            o[0] = Cl[3 * j]; o[1] = Cl[3 * j + 1]; o[2] = Cl[3 * j + 2];
+        }
+        return;
+    }
+    const bool in_lds = M <= kCoarseLds;
+    double *Rb = in_lds ? lds : R + 3 * (int64_t)base;   // level l at off[l] - off[3]
+    double *Cb = in_lds ? lds + 3 * kCoarseLds : C + 3 * (int64_t)base;
     for (int l = 3; l <= L; ++l) {
         for (int A = threadIdx.x; A < lv.n[l]; A += kCoarseWG) {
             const int c0 = cptr[lv.coff[l] + A], c1 = cptr[lv.coff[l] + A + 1];
+            const double *src = (l == 3) ? R + 3 * (int64_t)lv.off[2] : Rb + 3 * (lv.off[l - 1] - base);
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-            for (int a = c0; a < c1; ++a) {
-                const double *q = R + 3 * (int64_t)(lv.off[l - 1] + a);
-                s0 += q[0]; s1 += q[1]; s2 += q[2];
+            // children in order; the loads of 8 children are issued before their sums
+            // (a missing child adds an exact 0)
+            for (int a = c0; a < c1; a += 8) {
+                double v[8][3];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool ok = a + u < c1;
+                    const double *q = src + 3 * (ok ? a + u : c0);
+                    v[u][0] = ok ? q[0] : 0.0;
+                    v[u][1] = ok ? q[1] : 0.0;
+                    v[u][2] = ok ? q[2] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    s0 += v[u][0]; s1 += v[u][1]; s2 += v[u][2];
+                }
             }
-            double *o = R + 3 * (int64_t)(lv.off[l] + A);
+            double *o = Rb + 3 * (lv.off[l] - base + A);
             o[0] = s0; o[1] = s1; o[2] = s2;
         }
         __syncthreads();
     }
-    for (int l = L; l >= 2; --l) {
+    for (int l = L; l >= 3; --l) {
         for (int a = threadIdx.x; a < lv.n[l]; a += kCoarseWG) {
-            const int64_t g = lv.off[l] + a;
+            const int g = lv.off[l] + a;
+            const double *q = Rb + 3 * (g - base);
             double c0, c1, c2;
-            apply_inv(Dinv + 6 * g, R[3 * g], R[3 * g + 1], R[3 * g + 2], c0, c1, c2);
+            apply_inv(Dinv + 6 * (int64_t)g, q[0], q[1], q[2], c0, c1, c2);
             if (l < L) {
-                const double *up = C + 3 * (int64_t)(lv.off[l + 1] + par[g]);
+                const double *up = Cb + 3 * (lv.off[l + 1] - base + par[g]);
                 c0 += up[0]; c1 += up[1]; c2 += up[2];
             }
-            C[3 * g] = c0; C[3 * g + 1] = c1; C[3 * g + 2] = c2;
+            double *o = Cb + 3 * (g - base);
+            o[0] = c0; o[1] = c1; o[2] = c2;
         }
         __syncthreads();
     }
+    if (in_lds)
+        for (int t = threadIdx.x; t < 3 * lv.n[3]; t += kCoarseWG) C[3 * (int64_t)base + t] = Cb[t];
 }
 
 // z = D0 r + coarse correction (zero on fixed DOFs); r.z partials -> PB[parity] slot 0.
+// Levels 1 and 2 are applied here, per vertex, from the restricted sums; levels >= 3
+// arrive accumulated in C_3.
 template <class T>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ par,
@@ -252,8 +381,14 @@ k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const
         double c0, c1, c2;
         apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
         if (lv.L >= 2) {
-            const double *up = C + 3 * (int64_t)(lv.off[2] + par[a]);
-            c0 += up[0]; c1 += up[1]; c2 += up[2];
+            const int64_t g = lv.off[2] + par[a];
+            double d0, d1, d2;
+            apply_inv(Dinv + 6 * g, R[3 * g], R[3 * g + 1], R[3 * g + 2], d0, d1, d2);
+            if (lv.L >= 3) {
+                const double *up = C + 3 * (int64_t)(lv.off[3] + par[g]);
+                d0 += up[0]; d1 += up[1]; d2 += up[2];
+            }
+            c0 += d0; c1 += d1; c2 += d2;
         }
         const T r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
         T f0, f1, f2;
@@ -295,15 +430,16 @@ hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
     const MLLevels &lv = m.lv;
     const int N = c->topo.N;
     const int G = pcg_grid(N);
-    const int rblocks = lv.L >= 2 ? blocks_for((int64_t)64 * lv.n[2]) : blocks_for(lv.n[1]);
+    const int waves = lv.L >= 2 ? lv.n[2] : (lv.n[1] + 63) / 64;
+    const int rblocks = blocks_for((int64_t)64 * waves, kRestrictWG);
     if (mixed) {
-        hipLaunchKernelGGL(k_ml_restrict<float>, dim3(rblocks), dim3(256), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
+        hipLaunchKernelGGL(k_ml_restrict<float>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
                            c->r32, m.R, c->scal, check_done);
     } else {
-        hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(256), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
+        hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
                            c->r64, m.R, c->scal, check_done);
     }
-    if (lv.L >= 2)
+    if (lv.L >= 3)
         hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
                            c->scal, check_done);
     if (mixed) {

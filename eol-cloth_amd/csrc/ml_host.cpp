@@ -5,11 +5,12 @@
 // adjacent when their 3x3 block is stored, so the bending stencil's second ring
 // counts.  Level l + 1 aggregates the graph of level-l aggregates, two aggregates
 // being adjacent when any of their members are.  On every level the seeds are taken
-// greedily in index order such that no two lie within kSeedDist hops of each other;
+// greedily in index order such that no two lie within 2 hops of each other (3 above
+// the first level);
 // every node then joins the seed that a breadth-first search started from all seeds
 // at once reaches first (a graph Voronoi partition).  The aggregates are connected
-// and compact: about 10 vertices on the first level of a triangulated grid, then
-// about 9 children per aggregate.  Levels stop at one aggregate per connected
+// and compact: about 12 vertices on the first level of a triangulated grid, then
+// about 16 children per aggregate.  Levels stop at one aggregate per connected
 // component, when a level merges nothing, or at kMLMaxLevels.
 //
 // Numbering is hierarchical (top-down, stable in the original order), so that every
@@ -24,7 +25,8 @@
 namespace eolc {
 namespace {
 
-constexpr int kSeedDist = 2;
+constexpr int kSeedDist = 2;        // level 1 (vertices)
+constexpr int kSeedDistUpper = 3;   // levels >= 2: fewer, larger aggregates (fewer coarse levels)
 
 struct Graph {
     int n = 0;
@@ -139,7 +141,7 @@ std::string ml_build(const Topology &t, std::vector<std::vector<int>> &maps, std
     std::vector<int> n{N};
     while ((int)P.size() < kMLMaxLevels) {
         int na = 0;
-        std::vector<int> a = voronoi(g, kSeedDist, na);
+        std::vector<int> a = voronoi(g, P.empty() ? kSeedDist : kSeedDistUpper, na);
         if (na >= g.n) break;   // nothing merged: no edges left
         P.push_back(std::move(a));
         n.push_back(na);

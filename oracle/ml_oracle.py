@@ -11,8 +11,9 @@ restatement follows the definition
 with Z_l the piecewise-constant prolongation of the level-l aggregates (zero rows on
 fixed DOFs) and the Galerkin products formed as explicit sparse products, and the
 aggregation from its rule: on every level, seeds taken greedily in index order at
-pairwise hop distance > 2, every node joined to the seed a breadth-first search from
-all seeds reaches first; the next level aggregates the graph of the aggregates.
+pairwise hop distance > 2 (> 3 above the first level), every node joined to the
+seed a breadth-first search from all seeds reaches first; the next level
+aggregates the graph of the aggregates.
 
 Pins (tests/test_ml_oracle.py): for an unconstrained cloth the single top-level block
 equals the total mass times I (translations span the stiffness null space); P is
@@ -25,7 +26,8 @@ from collections import deque
 import numpy as np
 import scipy.sparse as sp
 
-SEED_DIST = 2
+SEED_DIST = 2         # level 1
+SEED_DIST_UPPER = 3   # levels >= 2
 MAX_LEVELS = 8
 
 
@@ -79,14 +81,14 @@ def coarse_graph(adj, agg, na):
     return [sorted(s) for s in nb]
 
 
-def hierarchy(rowptr, col, d=SEED_DIST, max_levels=MAX_LEVELS):
+def hierarchy(rowptr, col, d=SEED_DIST, d_upper=SEED_DIST_UPPER, max_levels=MAX_LEVELS):
     """Per-vertex aggregate maps of levels 1..L (labels in build order), with the
     library's stopping rules: stop when a level merges nothing, at one aggregate,
     when the aggregates are the connected components, or at max_levels."""
     adj = vertex_graph(rowptr, col)
     maps, cur = [], np.arange(len(adj))
     while len(maps) < max_levels:
-        agg, seeds = voronoi(adj, d)
+        agg, seeds = voronoi(adj, d if not maps else d_upper)
         na = len(seeds)
         if na >= len(adj):
             break
