@@ -52,9 +52,10 @@ def train(ctx, n, seeds, steps_per_scene, snapshots, adam_steps, lr, theta0=None
     return ctx.learned_get_weights(), hist
 
 
-def evaluate(ctx, n, seeds, steps, theta, tol=1e-8, log=print, sigma_x=1e-3, sigma_v=0.05):
+def evaluate(ctx, n, seeds, steps, theta, tol=1e-8, log=print, sigma_x=1e-3, sigma_v=0.05, flags=0):
     """Iterations and solve time of MIXED_BJ and MIXED_LEARNED on the same systems
-    (each step advanced with the block-Jacobi solution)."""
+    (each step advanced with the block-Jacobi solution); `flags` is OR-ed into both
+    modes (eolc.SOLVE_MULTILEVEL: the fine blocks inside the multilevel preconditioner)."""
     ctx.learned_set_weights(theta)
     rows = []
     for sd in seeds:
@@ -66,7 +67,7 @@ def evaluate(ctx, n, seeds, steps, theta, tol=1e-8, log=print, sigma_x=1e-3, sig
             x0, v0 = ctx.download_state()
             vals = 0.99 * v0[fd]
             out = {}
-            for name, mode in (("bj", eolc.SOLVE_MIXED_BJ), ("learned", eolc.SOLVE_MIXED_LEARNED)):
+            for name, mode in (("bj", eolc.SOLVE_MIXED_BJ | flags), ("learned", eolc.SOLVE_MIXED_LEARNED | flags)):
                 ctx.upload_state(x0, v0)
                 ctx.forces_fill(GRAV, H)
                 t = time.perf_counter()
@@ -104,10 +105,13 @@ def main(argv=None):
         if a.out:
             np.save(a.out, theta)
             log(f"weights -> {a.out}")
-        rows = evaluate(ctx, a.n, [5000 + 17 * i for i in range(a.eval_scenes)], a.eval_steps, theta, log=log)
-        bj = sum(r[2] for r in rows)
-        le = sum(r[3] for r in rows)
-        log(f"held-out: block-Jacobi {bj} iterations, learned {le} ({bj / max(le, 1):.3f}x fewer)")
+        for flags, what in ((0, "block-Jacobi"), (eolc.SOLVE_MULTILEVEL, "multilevel")):
+            rows = evaluate(ctx, a.n, [5000 + 17 * i for i in range(a.eval_scenes)], a.eval_steps, theta, log=log,
+                            flags=flags)
+            bj = sum(r[2] for r in rows)
+            le = sum(r[3] for r in rows)
+            log(f"held-out, {what}: without the network {bj} iterations, with it {le} "
+                f"({bj / max(le, 1):.3f}x fewer)")
     return 0
 
 

@@ -185,3 +185,29 @@ def test_learned_mode_requires_weights():
         assert rc == eolc.ERR_INVALID and "weights" in c.last_error()
     finally:
         c.close()
+
+
+def test_trained_network_gain(ctx):
+    """What the trained network gains: train on simulator systems of two 32x32 scenes
+    (GPU forward/backward/Adam kernels), then count mixed-PCG iterations to 1e-8 on a
+    held-out scene with and without the network, both alone and inside the
+    multilevel preconditioner.  The gain is reported, not assumed: block-diagonal
+    corrections can at best match block-Jacobi on these systems (see the ceiling
+    study in the design notes), so the bar is convergence of every solve and no more
+    than 25% extra iterations."""
+    from eolc import train_mpn
+    print("learned: train on 2 scenes 32x32, evaluate a held-out scene", flush=True)
+    theta, hist = train_mpn.train(ctx, 32, [1000, 1017], 2, (0, 20), 20, 1e-3, log=lambda m: None)
+    first = np.mean([h[3] for h in hist])
+    last = np.mean([h[4] for h in hist])
+    print(f"  training loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < first
+    for flags, what in ((0, "alone"), (eolc.SOLVE_MULTILEVEL, "with the multilevel levels")):
+        rows = train_mpn.evaluate(ctx, 32, [5000], 2, theta, log=lambda m: None, flags=flags)
+        bj = sum(r[2] for r in rows)
+        le = sum(r[3] for r in rows)
+        print(f"  {what}: block-Jacobi blocks {bj} iterations, trained network {le} "
+              f"(gain {bj / le:.3f}x)", flush=True)
+        assert all(r[6] <= 1e-8 for r in rows)
+        assert le <= 1.25 * bj
+
