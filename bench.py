@@ -146,40 +146,53 @@ def measured_traffic(kernel, n):
     return None
 
 
-def run_batched(steps, warmup, maxit):
+def run_batched(steps, warmup, maxit, multilevel=True):
     """The many-cloth workload: 256 cloths, sides uniform in {16..128} quads, E
     log-uniform in [1e2, 1e4] Pa, beta log-uniform in [1e-6, 1e-4], the single-cloth
     noise, seed 1237; one persistent launch per solve, graph-captured step, solve
-    workgroup size from the batch autotuner.  Steps/s of the whole batch."""
+    workgroup size from the batch autotuner.  Steps/s of the whole batch, with the
+    block-Jacobi preconditioner and (from the same state) with the multilevel
+    coarse levels added."""
     bs = scene.BatchScene()
     t = time.perf_counter()
     ctx = eolc.Context(0)
     bs.setup(ctx)
     ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    x0, v0 = ctx.download_state()
     setup_s = time.perf_counter() - t
-    best, ms = ctx.batch_autotune(GRAV, H, tol=TOL, maxit=maxit)
-    log(f"batched: {bs.B} cloths, {bs.N_real} vertices, set-up {setup_s:.1f} s, "
-        f"autotune variant {best} ms {np.round(ms, 2).tolist()}")
-    for _ in range(warmup):
-        ctx.batch_step(GRAV, H, tol=TOL, maxit=maxit)
-    t0 = time.perf_counter()
-    its = []
-    for _ in range(steps):
-        rc, st = ctx.batch_step(GRAV, H, tol=TOL, maxit=maxit)
-        bad = [s.as_dict() for s in st if not (s.true_relres <= TOL)]
-        if rc != eolc.OK or bad:
-            raise RuntimeError(f"batched solve missed the tolerance: {bad[:1]}")
-        its.append([s.iters for s in st])
-    dt = (time.perf_counter() - t0) / steps
+    out = {"workload": f"{bs.B} cloths, sides {int(bs.sides.min())}..{int(bs.sides.max())} quads, "
+                       f"{bs.N_real} vertices"}
+    for name, ml in (("block_jacobi", False), ("multilevel", True)):
+        if ml and not multilevel:
+            continue
+        ctx.set_batch_multilevel(ml)
+        ctx.upload_state(x0, v0)
+        best, ms = ctx.batch_autotune(GRAV, H, tol=TOL, maxit=maxit)
+        log(f"batched {name}: {bs.B} cloths, {bs.N_real} vertices, set-up {setup_s:.1f} s, "
+            f"autotune variant {best} ms {np.round(ms, 2).tolist()}")
+        for _ in range(warmup):
+            ctx.batch_step(GRAV, H, tol=TOL, maxit=maxit)
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(steps):
+            rc, st = ctx.batch_step(GRAV, H, tol=TOL, maxit=maxit)
+            bad = [s.as_dict() for s in st if not (s.true_relres <= TOL)]
+            if rc != eolc.OK or bad:
+                raise RuntimeError(f"batched solve missed the tolerance: {bad[:1]}")
+            its.append([s.iters for s in st])
+        dt = (time.perf_counter() - t0) / steps
+        its = np.array(its)
+        log(f"batched {name}: {1 / dt:.3f} steps/s, iterations per cloth min {its.min()} median "
+            f"{int(np.median(its))} max {its.max()}")
+        out[name] = {"steps_per_s": 1 / dt, "ms_per_step": dt * 1e3, "cloth_steps_per_s": bs.B / dt,
+                     "iters_min": int(its.min()), "iters_median": int(np.median(its)),
+                     "iters_max": int(its.max()), "solve_variant": best,
+                     "variant_ms": [round(float(m), 3) for m in ms]}
     ctx.close()
-    its = np.array(its)
-    log(f"batched: {1 / dt:.3f} steps/s, iterations per cloth min {its.min()} median "
-        f"{int(np.median(its))} max {its.max()}")
-    return {"workload": f"{bs.B} cloths, sides {int(bs.sides.min())}..{int(bs.sides.max())} quads, "
-                        f"{bs.N_real} vertices", "steps_per_s": 1 / dt, "ms_per_step": dt * 1e3,
-            "cloth_steps_per_s": bs.B / dt, "iters_min": int(its.min()), "iters_median": int(np.median(its)),
-            "iters_max": int(its.max()), "solve_variant": best,
-            "variant_ms": [round(float(m), 3) for m in ms]}
+    head = out["multilevel"] if multilevel else out["block_jacobi"]
+    out.update({"steps_per_s": head["steps_per_s"], "ms_per_step": head["ms_per_step"],
+                "preconditioner": "multilevel" if multilevel else "block_jacobi"})
+    return out
 
 
 def main():
@@ -310,7 +323,7 @@ def main():
             "kernel": kname, "variant": tuned[32]["variant"], "algorithmic_bytes": alg,
             "avg_launch_ms": spmv32_ms, "copy_kernel_GBps": copy_gbps}
     ctx.close()
-    batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit)
+    batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit, not args.no_ml)
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -8,6 +8,7 @@
 //   GeneralizedSolver::velocitySolve  GeneralizedSolver.cpp:107-172
 //   Cloth::step                       Cloth.cpp:359-413
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -80,6 +81,8 @@ void free_topology_buffers(eolc_ctx *c)
     L.nbs = 0;
     eolc::BatchState &S = c->batch;
     dfree(S.d_inst); dfree(S.d_order); dfree(S.d_user_fixed); dfree(S.slice_part); dfree(S.stats); dfree(S.dstats);
+    dfree(S.d_ml_lo);
+    S.ml = S.graph_ml = false;
     if (S.graph) (void)hipGraphExecDestroy(S.graph);
     S.graph = nullptr;
     S.B = 0;
@@ -1357,7 +1360,8 @@ int eolc_batch_step(eolc_ctx *c, const double grav[3], double h, double tol, int
         rc = batch_step_enqueue(c, grav, h, tol, maxit, S.variant);
     } else {
         const bool same = S.graph && S.graph_h == h && S.graph_tol == tol && S.graph_maxit == maxit &&
-                          S.graph_variant == S.variant && S.graph_grav[0] == grav[0] && S.graph_grav[1] == grav[1] &&
+                          S.graph_variant == S.variant && S.graph_ml == S.ml && S.graph_grav[0] == grav[0] &&
+                          S.graph_grav[1] == grav[1] &&
                           S.graph_grav[2] == grav[2];
         if (!same) {
             batch_invalidate_graph(c);
@@ -1377,6 +1381,7 @@ int eolc_batch_step(eolc_ctx *c, const double grav[3], double h, double tol, int
             S.graph_tol = tol;
             S.graph_maxit = maxit;
             S.graph_variant = S.variant;
+            S.graph_ml = S.ml;
             for (int k = 0; k < 3; ++k) S.graph_grav[k] = grav[k];
         }
         EOLC_HIP(hipGraphLaunch(S.graph, c->stream));
@@ -1439,6 +1444,40 @@ int eolc_batch_autotune(eolc_ctx *c, const double grav[3], double h, double tol,
     batch_invalidate_graph(c);
     c->have_forces = false;
     if (best) *best = arg;
+    return EOLC_OK;
+}
+
+int eolc_set_batch_multilevel(eolc_ctx *c, int on)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    eolc::BatchState &S = c->batch;
+    if (S.B <= 0) return fail(c, EOLC_ERR_INVALID, "set_batch_multilevel: no batch topology");
+    EOLC_HIP(hipSetDevice(c->device));
+    if (on && !S.d_ml_lo) {
+        int rc = ml_prepare(c);
+        if (rc) return rc;
+        const eolc::MLState &m = c->ml;
+        constexpr int St = eolc::kMLMaxLevels + 1;
+        std::vector<int> lo((size_t)(S.B + 1) * St, 0);
+        for (int l = 1; l <= m.lv.L; ++l) {
+            int expect = 0;
+            for (int b = 0; b < S.B; ++b) {
+                int mn = INT_MAX, mx = -1;
+                for (int v = S.inst[b].v0; v < S.inst[b].v1; ++v) {
+                    mn = std::min(mn, m.maps[l][v]);
+                    mx = std::max(mx, m.maps[l][v]);
+                }
+                if (mn != expect) return fail(c, EOLC_ERR_INVALID, "set_batch_multilevel: aggregates span instances");
+                lo[(size_t)b * St + l] = mn;
+                expect = mx + 1;
+            }
+            if (expect != m.lv.n[l]) return fail(c, EOLC_ERR_INVALID, "set_batch_multilevel: aggregates outside instances");
+            lo[(size_t)S.B * St + l] = m.lv.n[l];
+        }
+        EOLC_HIP(upload(c, &S.d_ml_lo, lo));
+    }
+    S.ml = on != 0 && c->ml.lv.L > 0;
+    batch_invalidate_graph(c);
     return EOLC_OK;
 }
 

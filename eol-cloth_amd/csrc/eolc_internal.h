@@ -123,6 +123,9 @@ struct BatchState {
     int *stats = nullptr;            // per instance: iters, refinements, status, spare
     double *dstats = nullptr;        // per instance: bnorm2, true_rr, relres, spare
     int variant = 2;                 // workgroup size choice
+    bool ml = false;                 // multilevel coarse levels in the persistent solve
+    int *d_ml_lo = nullptr;          // (B + 1) x (kMLMaxLevels + 1) aggregate ranges per instance
+    bool graph_ml = false;
     hipGraphExec_t graph = nullptr;  // captured step
     double graph_h = 0.0, graph_tol = 0.0;
     double graph_grav[3] = {0, 0, 0};

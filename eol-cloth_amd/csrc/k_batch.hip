@@ -150,6 +150,120 @@ __device__ void inst_precond(const eolc::BatchInst &I, const int *__restrict__ s
     rr = s[1];
 }
 
+// Multilevel coarse levels inside the persistent solve: the hierarchy of the
+// concatenated topology never spans two instances, and its hierarchical numbering
+// gives every instance a contiguous aggregate range per level (lo), so a workgroup
+// applies its instance's levels alone, with workgroup barriers between levels.
+struct BatchML {
+    eolc::MLLevels lv;
+    const int *hord, *ptr1, *agg1, *par, *cptr;
+    const int *lo;   // (B + 1) x (kMLMaxLevels + 1): first aggregate of instance b at level l
+    const double *Dinv;
+    double *R, *C;
+};
+
+__device__ __forceinline__ void ml_inv_apply(const double *__restrict__ d, double r0, double r1, double r2, double &c0,
+                                             double &c1, double &c2)
+{
+    c0 = fma(d[2], r2, fma(d[1], r1, d[0] * r0));
+    c1 = fma(d[4], r2, fma(d[3], r1, d[1] * r0));
+    c2 = fma(d[5], r2, fma(d[4], r1, d[2] * r0));
+}
+
+// z = D r + sum_l Z_l D_l^-1 Z_l^T r over instance b (r32 of its rows complete);
+// returns r.z over the instance.  Sums in member / child order, FP64.
+template <int WGS>
+__device__ double inst_ml_apply(const eolc::BatchInst &I, int b, const BatchML &m, const int *__restrict__ srow,
+                                int N, const float *__restrict__ D, const float *__restrict__ r32,
+                                float *__restrict__ z32, const unsigned char *__restrict__ fixmask, double *part,
+                                int nslices, double *red)
+{
+    constexpr int NW = WGS / 64;
+    constexpr int S = eolc::kMLMaxLevels + 1;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int L = m.lv.L;
+    __syncthreads();   // every row's r32 is written
+    {
+        const int lo = m.lo[b * S + 1], hi = m.lo[(b + 1) * S + 1];
+        for (int a = lo + (int)threadIdx.x; a < hi; a += WGS) {
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+            for (int k = m.ptr1[a]; k < m.ptr1[a + 1]; ++k) {
+                const int i = m.hord[k];
+                t0 += (double)r32[3 * i];
+                t1 += (double)r32[3 * i + 1];
+                t2 += (double)r32[3 * i + 2];
+            }
+            m.R[3 * a] = t0; m.R[3 * a + 1] = t1; m.R[3 * a + 2] = t2;
+        }
+    }
+    __syncthreads();
+    for (int l = 2; l <= L; ++l) {
+        const int lo = m.lo[b * S + l], hi = m.lo[(b + 1) * S + l];
+        for (int A = lo + (int)threadIdx.x; A < hi; A += WGS) {
+            const int c0 = m.cptr[m.lv.coff[l] + A], c1 = m.cptr[m.lv.coff[l] + A + 1];
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+            for (int a = c0; a < c1; ++a) {
+                const double *q = m.R + 3 * (int64_t)(m.lv.off[l - 1] + a);
+                t0 += q[0]; t1 += q[1]; t2 += q[2];
+            }
+            double *o = m.R + 3 * (int64_t)(m.lv.off[l] + A);
+            o[0] = t0; o[1] = t1; o[2] = t2;
+        }
+        __syncthreads();
+    }
+    for (int l = L; l >= 1; --l) {
+        const int lo = m.lo[b * S + l], hi = m.lo[(b + 1) * S + l];
+        for (int a = lo + (int)threadIdx.x; a < hi; a += WGS) {
+            const int64_t g = m.lv.off[l] + a;
+            double c0, c1, c2;
+            ml_inv_apply(m.Dinv + 6 * g, m.R[3 * g], m.R[3 * g + 1], m.R[3 * g + 2], c0, c1, c2);
+            if (l < L) {
+                const double *up = m.C + 3 * (int64_t)(m.lv.off[l + 1] + m.par[g]);
+                c0 += up[0]; c1 += up[1]; c2 += up[2];
+            }
+            m.C[3 * g] = c0; m.C[3 * g + 1] = c1; m.C[3 * g + 2] = c2;
+        }
+        __syncthreads();
+    }
+    for (int sl = I.s0 + wave; sl < I.s1; sl += NW) {
+        const int i = srow[sl * SL + lane];
+        const float r0 = r32[3 * i], r1 = r32[3 * i + 1], r2 = r32[3 * i + 2];
+        float f0, f1, f2;
+        eolc::apply_sym3<float>(D + i, N, r0, r1, r2, f0, f1, f2);
+        const double *cc = m.C + 3 * (int64_t)m.agg1[i];
+        const unsigned fm = fixmask[i];
+        const float z0 = (fm & 1u) ? 0.0f : (float)((double)f0 + cc[0]);
+        const float z1 = (fm & 2u) ? 0.0f : (float)((double)f1 + cc[1]);
+        const float z2 = (fm & 4u) ? 0.0f : (float)((double)f2 + cc[2]);
+        z32[3 * i] = z0; z32[3 * i + 1] = z1; z32[3 * i + 2] = z2;
+        const double v[1] = {((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2};
+        slice_partial<1>(v, sl, part, nslices);
+    }
+    double s[1];
+    inst_sum<1>(s, I, part, nslices, red);
+    return s[0];
+}
+
+// r32 = (float) r64 and r.r over the instance (the multilevel path forms z after).
+template <int WGS>
+__device__ double inst_convert(const eolc::BatchInst &I, const int *__restrict__ srow, const double *__restrict__ r64,
+                               float *__restrict__ r32, double *part, int nslices, double *red)
+{
+    constexpr int NW = WGS / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int sl = I.s0 + wave; sl < I.s1; sl += NW) {
+        const int i = srow[sl * SL + lane];
+        const float r0 = (float)r64[3 * i], r1 = (float)r64[3 * i + 1], r2 = (float)r64[3 * i + 2];
+        r32[3 * i] = r0; r32[3 * i + 1] = r1; r32[3 * i + 2] = r2;
+        const double s0 = r0, s1 = r1, s2 = r2;
+        const double v[1] = {(s0 * s0 + s1 * s1) + s2 * s2};
+        slice_partial<1>(v, sl, part, nslices);
+    }
+    double s[1];
+    inst_sum<1>(s, I, part, nslices, red);
+    return s[0];
+}
+
 struct BatchArgs {
     const eolc::BatchInst *inst;
     const int *order;
@@ -167,10 +281,12 @@ struct BatchArgs {
     int maxit;
     int *stats;
     double *dstats;
+    BatchML ml;
 };
 
-// Persistent mixed-precision PCG, one workgroup per instance.
-template <int WGS>
+// Persistent mixed-precision PCG, one workgroup per instance.  ML: the preconditioner
+// adds the instance's multilevel coarse levels to the block-Jacobi blocks.
+template <int WGS, bool ML>
 __global__ void __launch_bounds__(WGS) k_batch_pcg(BatchArgs a)
 {
     constexpr int NW = WGS / 64;
@@ -198,7 +314,12 @@ __global__ void __launch_bounds__(WGS) k_batch_pcg(BatchArgs a)
     double rmax2 = inst_residual64<WGS>(I, a.slice_off, a.srow, a.slen, a.scol, a.A64, nbs, a.fixmask, a.xs64, a.b64,
                                         a.r64, a.part, ns, red);
     double rz, rr;
-    inst_precond<WGS>(I, a.srow, N, a.D32, a.r64, a.r32, a.z32, a.part, ns, red, rz, rr);
+    if constexpr (ML) {
+        rr = inst_convert<WGS>(I, a.srow, a.r64, a.r32, a.part, ns, red);
+        rz = inst_ml_apply<WGS>(I, b, a.ml, a.srow, N, a.D32, a.r32, a.z32, a.fixmask, a.part, ns, red);
+    } else {
+        inst_precond<WGS>(I, a.srow, N, a.D32, a.r64, a.r32, a.z32, a.part, ns, red, rz, rr);
+    }
     double rz_prev = 1.0;
     int k = 0, first = 0, refinements = 0, status = EOLC_ERR_MAXIT;
     float *pold = a.p32a, *pnew = a.p32b;
@@ -232,7 +353,12 @@ __global__ void __launch_bounds__(WGS) k_batch_pcg(BatchArgs a)
                 break;
             }
             if (k >= a.maxit || refinements > eolc::kMaxRefinements) break;
-            inst_precond<WGS>(I, a.srow, N, a.D32, a.r64, a.r32, a.z32, a.part, ns, red, rz, rr);
+            if constexpr (ML) {
+                rr = inst_convert<WGS>(I, a.srow, a.r64, a.r32, a.part, ns, red);
+                rz = inst_ml_apply<WGS>(I, b, a.ml, a.srow, N, a.D32, a.r32, a.z32, a.fixmask, a.part, ns, red);
+            } else {
+                inst_precond<WGS>(I, a.srow, N, a.D32, a.r64, a.r32, a.z32, a.part, ns, red, rz, rr);
+            }
             continue;
         }
         if (k >= a.maxit) break;
@@ -287,18 +413,30 @@ __global__ void __launch_bounds__(WGS) k_batch_pcg(BatchArgs a)
             const float r1 = fmaf(-alphaT, a.q32[3 * i + 1], a.r32[3 * i + 1]);
             const float r2 = fmaf(-alphaT, a.q32[3 * i + 2], a.r32[3 * i + 2]);
             a.r32[3 * i] = r0; a.r32[3 * i + 1] = r1; a.r32[3 * i + 2] = r2;
-            float z0, z1, z2;
-            eolc::apply_sym3<float>(a.D32 + i, N, r0, r1, r2, z0, z1, z2);
-            a.z32[3 * i] = z0; a.z32[3 * i + 1] = z1; a.z32[3 * i + 2] = z2;
-            const double v[2] = {((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2,
-                                 ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2};
-            slice_partial<2>(v, sl, a.part, ns);
+            if constexpr (ML) {
+                const double v[1] = {((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2};
+                slice_partial<1>(v, sl, a.part, ns);
+            } else {
+                float z0, z1, z2;
+                eolc::apply_sym3<float>(a.D32 + i, N, r0, r1, r2, z0, z1, z2);
+                a.z32[3 * i] = z0; a.z32[3 * i + 1] = z1; a.z32[3 * i + 2] = z2;
+                const double v[2] = {((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2,
+                                     ((double)r0 * (double)r0 + (double)r1 * (double)r1) + (double)r2 * (double)r2};
+                slice_partial<2>(v, sl, a.part, ns);
+            }
         }
-        double s2[2];
-        inst_sum<2>(s2, I, a.part, ns, red);
         rz_prev = rz;
-        rz = s2[0];
-        rr = s2[1];
+        if constexpr (ML) {
+            double s1[1];
+            inst_sum<1>(s1, I, a.part, ns, red);
+            rr = s1[0];
+            rz = inst_ml_apply<WGS>(I, b, a.ml, a.srow, N, a.D32, a.r32, a.z32, a.fixmask, a.part, ns, red);
+        } else {
+            double s2[2];
+            inst_sum<2>(s2, I, a.part, ns, red);
+            rz = s2[0];
+            rr = s2[1];
+        }
         float *t = pold;
         pold = pnew;
         pnew = t;
@@ -352,9 +490,12 @@ __global__ void k_batch_fixmask(int n, const int *__restrict__ nodes, unsigned c
 }
 
 template <int WGS>
-void launch_batch(eolc_ctx *c, const BatchArgs &args)
+void launch_batch(eolc_ctx *c, const BatchArgs &args, bool ml)
 {
-    hipLaunchKernelGGL((k_batch_pcg<WGS>), dim3(c->batch.B), dim3(WGS), 0, c->stream, args);
+    if (ml)
+        hipLaunchKernelGGL((k_batch_pcg<WGS, true>), dim3(c->batch.B), dim3(WGS), 0, c->stream, args);
+    else
+        hipLaunchKernelGGL((k_batch_pcg<WGS, false>), dim3(c->batch.B), dim3(WGS), 0, c->stream, args);
 }
 
 }  // namespace
@@ -378,7 +519,10 @@ hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant)
     if (nu > 0)
         hipLaunchKernelGGL(k_batch_vfix, dim3(blocks_for(3 * nu)), dim3(256), 0, c->stream, nu, S.d_user_fixed, c->v,
                            c->vfix);
+    const bool ml = S.ml && S.d_ml_lo && c->ml.lv.L > 0;
+    c->ml_on = ml;   // pcg_setup then also forms the coarse levels' blocks
     hipError_t e = pcg_setup(c, true, 0);
+    c->ml_on = false;
     if (e != hipSuccess) return e;
     BatchArgs a;
     a.inst = S.d_inst; a.order = S.d_order;
@@ -391,10 +535,12 @@ hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant)
     a.part = S.slice_part;
     a.tol2 = tol * tol; a.delta2 = c->mixed_delta * c->mixed_delta; a.maxit = maxit;
     a.stats = S.stats; a.dstats = S.dstats;
+    const MLState &m = c->ml;
+    a.ml = BatchML{m.lv, m.hord, m.ptr1, m.agg1, m.par, m.cptr, S.d_ml_lo, m.Dinv, m.R, m.C};
     switch (variant) {
-    case 0: launch_batch<256>(c, a); break;
-    case 1: launch_batch<512>(c, a); break;
-    default: launch_batch<1024>(c, a); break;
+    case 0: launch_batch<256>(c, a, ml); break;
+    case 1: launch_batch<512>(c, a, ml); break;
+    default: launch_batch<1024>(c, a, ml); break;
     }
     return hipGetLastError();
 }

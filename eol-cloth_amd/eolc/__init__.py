@@ -33,7 +33,7 @@ EXPORTED_SYMBOLS = (
     "eolc_learned_loss_grad", "eolc_learned_train", "eolc_autotune", "eolc_set_spmv_variant",
     "eolc_set_batch_topology", "eolc_set_batch_material", "eolc_batch_step", "eolc_batch_autotune",
     "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta", "eolc_collision_constraints",
-    "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition",
+    "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
 )
 
 BATCH_EAGER = 0x1
@@ -110,6 +110,7 @@ def lib():
         L.eolc_batch_autotune.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int),
                                           C.c_void_p]
         L.eolc_set_batch_variant.argtypes = [vp, C.c_int]
+        L.eolc_set_batch_multilevel.argtypes = [vp, C.c_int]
         L.eolc_set_mixed_delta.argtypes = [vp, C.c_double]
         L.eolc_collision_constraints.argtypes = [vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_int),
@@ -357,6 +358,10 @@ class Context:
         self._check(lib().eolc_batch_autotune(self._h, _d(grav), float(h), float(tol), int(maxit), C.byref(best),
                                               _ptr(ms)))
         return best.value, ms
+
+    def set_batch_multilevel(self, on=True):
+        """Multilevel coarse levels in the batched persistent solve."""
+        return self._check(lib().eolc_set_batch_multilevel(self._h, int(bool(on))))
 
     def set_batch_variant(self, variant):
         return self._check(lib().eolc_set_batch_variant(self._h, int(variant)))

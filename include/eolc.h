@@ -250,6 +250,11 @@ int eolc_set_batch_material(eolc_ctx *ctx, const double *E, const double *beta);
 int eolc_batch_step(eolc_ctx *ctx, const double grav[3], double h, double tol, int maxit, int flags,
                     eolc_solve_stats *stats);
 
+/* Multilevel coarse levels (EOLC_SOLVE_MULTILEVEL's hierarchy, which never spans two
+ * cloths) inside the batched persistent solve: on != 0 adds them to every instance's
+ * block-Jacobi preconditioner; applies to later eolc_batch_step / _autotune calls. */
+int eolc_set_batch_multilevel(eolc_ctx *ctx, int on);
+
 /* Autotuner of the batched solve: times the persistent solve with workgroup sizes
  * 256, 512 and 1024 (ms: 3 entries, may be NULL) on the current state's system
  * and keeps the fastest.  The state is left unchanged.  Results do not depend on

@@ -93,3 +93,52 @@ def test_batch_reports_per_instance_maxit():
         rc, st = c.batch_step(GRAV, H, tol=1e-14, maxit=3, eager=True, allow=(eolc.ERR_MAXIT,))
         assert rc == eolc.ERR_MAXIT
         assert all(s.status == eolc.ERR_MAXIT and s.iters <= 3 for s in st)
+
+
+def test_batch_multilevel():
+    """The multilevel coarse levels inside the persistent solve: every instance
+    matches the same cloth stepped alone with EOLC_SOLVE_MULTILEVEL (tol 1e-10 both
+    sides), needs fewer iterations than the block-Jacobi batch, graph replays equal
+    the eager step bitwise, and every workgroup size gives the same bits."""
+    bs = scene.BatchScene(B=4, side_lo=10, side_hi=40, seed=78)
+    print("batch multilevel, sides", bs.sides, flush=True)
+    with eolc.Context(0) as c:
+        bs.setup(c)
+        x0, v0 = c.download_state()
+        rc, st_bj = c.batch_step(GRAV, H, tol=1e-10, maxit=20000, eager=True)
+        c.set_batch_multilevel(True)
+        ref = None
+        for variant in range(eolc.BATCH_VARIANTS):
+            c.set_batch_variant(variant)
+            c.upload_state(x0, v0)
+            rc, st = c.batch_step(GRAV, H, tol=1e-10, maxit=20000, eager=True)
+            assert rc == eolc.OK
+            out = c.download_state()
+            if ref is None:
+                ref, st_ml = out, st
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])
+        print(f"  iterations block-Jacobi {[s.iters for s in st_bj]}, multilevel {[s.iters for s in st_ml]}",
+              flush=True)
+        for replay in range(2):
+            c.upload_state(x0, v0)
+            c.download_state()
+            rc, st = c.batch_step(GRAV, H, tol=1e-10, maxit=20000)
+            out = c.download_state()
+            assert rc == eolc.OK
+            assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])
+    xs, vs = bs.unpack(ref[0]), bs.unpack(ref[1])
+    assert sum(s.iters for s in st_ml) < 0.8 * sum(s.iters for s in st_bj)
+    for b, sc in enumerate(bs.inst):
+        assert st_ml[b].status == eolc.OK and st_ml[b].true_relres <= 1e-10
+        with eolc.Context(0) as s:
+            s.set_topology(sc.X, sc.tri, sc.hinge)
+            s.set_material(0.1, float(bs.E[b]), 0.3, float(bs.beta[b]), 0.0, 1.0)
+            s.upload_state(sc.x0, sc.v0)
+            rc1, st1 = s.step(GRAV, H, sc.fixed, tol=1e-10, maxit=20000,
+                              mode=eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL)
+            x1, v1 = s.download_state()
+        err = np.max(np.abs(vs[b] - v1)) / np.max(np.abs(v1))
+        print(f"  instance {b} side {sc.n}: batch iters {st_ml[b].iters} single {st1.iters} v rel err {err:.2e}",
+              flush=True)
+        assert err < 1e-6
+        assert np.max(np.abs(xs[b] - x1)) <= 1e-9 * np.max(np.abs(x1))
