@@ -938,6 +938,36 @@ int eolc_velocity_solve(eolc_ctx *c, int n_fixed, const int *fixed_dof, const do
     return solve_impl(c, n_fixed, fixed_dof, fixed_val, tol, maxit, mode, st);
 }
 
+int eolc_velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
+                           const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
+                           double *bary, double *ms)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (n_old <= 0 || f_old < 0 || n_new < 0 || !u_old || !v_old || (f_old && !tri_old) ||
+        (n_new && (!u_new || !v_new || !src || !face)))
+        return fail(c, EOLC_ERR_INVALID, "velocity_transfer: bad arguments");
+    for (size_t k = 0; k < 2 * (size_t)n_old; ++k)
+        if (!std::isfinite(u_old[k])) return fail(c, EOLC_ERR_NONFINITE, "velocity_transfer: non-finite old position");
+    for (size_t k = 0; k < 3 * (size_t)n_old; ++k)
+        if (!std::isfinite(v_old[k])) return fail(c, EOLC_ERR_NONFINITE, "velocity_transfer: non-finite old velocity");
+    for (size_t k = 0; k < 2 * (size_t)n_new; ++k)
+        if (!std::isfinite(u_new[k])) return fail(c, EOLC_ERR_NONFINITE, "velocity_transfer: non-finite new position");
+    for (size_t k = 0; k < 3 * (size_t)f_old; ++k)
+        if (tri_old[k] < 0 || tri_old[k] >= n_old) return fail(c, EOLC_ERR_INVALID, "velocity_transfer: triangle index out of range");
+    if (ms) *ms = 0.0;
+    if (n_new == 0) return EOLC_OK;
+    EOLC_HIP(hipSetDevice(c->device));
+    int missing = 0;
+    float t = 0.f;
+    EOLC_HIP(eolc::velocity_transfer(c, n_old, u_old, v_old, f_old, tri_old, n_new, u_new, v_new, src, face, bary,
+                                     &missing, &t));
+    if (ms) *ms = t;
+    if (missing)
+        return fail(c, EOLC_ERR_INVALID, "velocity_transfer: " + std::to_string(missing) +
+                                             " new vertices have neither an old vertex within 1e-6 nor an enclosing old face");
+    return EOLC_OK;
+}
+
 int eolc_ml_info(eolc_ctx *c, int *levels, int *counts)
 {
     if (!c) return EOLC_ERR_INVALID;

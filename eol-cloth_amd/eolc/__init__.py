@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "eolc_set_batch_topology", "eolc_set_batch_material", "eolc_batch_step", "eolc_batch_autotune",
     "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta", "eolc_collision_constraints",
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
+    "eolc_velocity_transfer",
 )
 
 BATCH_EAGER = 0x1
@@ -111,6 +112,8 @@ def lib():
                                           C.c_void_p]
         L.eolc_set_batch_variant.argtypes = [vp, C.c_int]
         L.eolc_set_batch_multilevel.argtypes = [vp, C.c_int]
+        L.eolc_velocity_transfer.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _ip, _ip, _dp,
+                                             C.POINTER(C.c_double)]
         L.eolc_set_mixed_delta.argtypes = [vp, C.c_double]
         L.eolc_collision_constraints.argtypes = [vp, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_int),
@@ -256,6 +259,21 @@ class Context:
                              _ptr(sc), float(tol), int(maxit), int(mode), C.byref(st))
         self._check(rc, allow=allow)
         return rc, st
+
+    def velocity_transfer(self, u_old, v_old, tri_old, u_new, allow=()):
+        """Cloth::velocityTransfer (non-EOL) on the GPU: (rc, v_new, src, face, bary, ms)."""
+        u_old, v_old, u_new = _d(u_old), _d(v_old), _d(u_new)
+        tri_old = _i(tri_old)
+        n_new = u_new.size // 2
+        v_new = np.zeros(3 * n_new)
+        src = np.zeros(max(n_new, 1), dtype=np.int32)
+        face = np.zeros(max(n_new, 1), dtype=np.int32)
+        bary = np.zeros(max(3 * n_new, 1))
+        ms = C.c_double(0)
+        rc = lib().eolc_velocity_transfer(self._h, u_old.size // 2, u_old, v_old, tri_old.size // 3, tri_old, n_new,
+                                          u_new, v_new, src, face, bary, C.byref(ms))
+        self._check(rc, allow=allow)
+        return rc, v_new, src[:n_new], face[:n_new], bary[:3 * n_new], ms.value
 
     def ml_info(self):
         """Aggregates per level of the multilevel hierarchy: [N, n_1, ..., n_L]."""

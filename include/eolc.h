@@ -65,6 +65,19 @@ typedef enum {
  * whenever the recursive residual has dropped by this factor since the last update. */
 int eolc_set_mixed_delta(eolc_ctx *ctx, double delta);
 
+/* Velocity transfer after a remesh, non-EOL vertices: Cloth::velocityTransfer
+ * (Cloth.cpp:195-230, 265-281).  Each new vertex takes the velocity of the first old
+ * vertex (index order) within material-space distance 1e-6 (src = its index), else
+ * the barycentric combination over the first old face that ArcSim's is_inside
+ * accepts (face = its index, bary its coordinates), found through uniform grids
+ * instead of the reference's full scans; results are bitwise the scans'.  u: 2
+ * material coordinates per vertex, v: 3 per vertex, tri_old: 3 per face; bary (3 per
+ * new vertex) and ms (device time) may be NULL.  A new vertex with neither gets
+ * face -1, velocity 0 and EOLC_ERR_INVALID (the reference dereferences a null face). */
+int eolc_velocity_transfer(eolc_ctx *ctx, int n_old, const double *u_old, const double *v_old, int f_old,
+                           const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
+                           double *bary, double *ms);
+
 /* Aggregation hierarchy of EOLC_SOLVE_MULTILEVEL (no counterpart in the reference:
  * a preconditioner of the solver-"none" solve).  eolc_ml_info builds it if needed
  * and reports the number of coarse levels and the aggregates per level: counts[0] =

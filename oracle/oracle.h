@@ -132,6 +132,13 @@ int  orc_box_tri(double thr, const double *whd, const double *E /* 4x4 col-major
                  int N, const double *verts, int F, const int *tri, const int *isEOL, int eol,
                  orc_collision *out, int cap);
 
+/* ---- velocity transfer after a remesh (orc_transfer.c) ------------------- */
+/* u: 2 per vertex (material space), v: 3 per vertex; per new vertex: src (first old
+ * vertex within 1e-6, or -1), face (first enclosing old face, or -1), bary (3).
+ * Returns the number of new vertices with neither. */
+int orc_velocity_transfer(int n_old, const double *u_old, const double *v_old, int f_old, const int *tri_old,
+                          int n_new, const double *u_new, double *v_new, int *src, int *face, double *bary);
+
 /* ---- input generators (orc_rng.c) --------------------------------------- */
 uint64_t orc_splitmix64(uint64_t *state);
 void orc_gaussian(uint64_t seed, int64_t count, double sigma, double *out);

@@ -62,6 +62,7 @@ def lib():
         L.orc_build_hinges.argtypes = [C.c_int, C.c_int, _ip, _ip, C.c_int]
         for nm in ("orc_inertial",):
             getattr(L, nm).argtypes = [_dp] * 7 + [C.c_double, _dp, _dp, _dp]
+        L.orc_velocity_transfer.argtypes = [C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _ip, _ip, _dp]
         L.orc_membrane.argtypes = [_dp] * 6 + [C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
         L.orc_bending.argtypes = [_dp] * 8 + [C.c_double, _dp, _dp, _dp]
         L.orc_face_PQ.argtypes = [_dp] * 8
@@ -267,6 +268,21 @@ def _collect(fn, cap, *args):
         buf = (Collision * n)()
         n = fn(*args, buf, n)
     return n, [buf[i] for i in range(max(n, 0))]
+
+
+def velocity_transfer(u_old, v_old, tri_old, u_new):
+    """Cloth::velocityTransfer for non-EOL nodes (brute force): (v_new, src, face,
+    bary, missing)."""
+    u_old, v_old, u_new = _d(u_old).reshape(-1), _d(v_old).reshape(-1), _d(u_new).reshape(-1)
+    tri_old = _i(tri_old).reshape(-1)
+    n_new = u_new.size // 2
+    v_new = np.zeros(3 * n_new)
+    src = np.zeros(n_new, dtype=np.int32)
+    face = np.zeros(n_new, dtype=np.int32)
+    bary = np.zeros(3 * n_new)
+    missing = lib().orc_velocity_transfer(u_old.size // 2, u_old, v_old, tri_old.size // 3, tri_old, n_new, u_new,
+                                          v_new, src, face, bary)
+    return v_new, src, face, bary, missing
 
 
 def point_tri(thr, pts, nrm, verts, tri, eol=False, cap=4096):
