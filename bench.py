@@ -118,17 +118,19 @@ def load_weights():
 
 # SpMV launch variants: (entries per trip, column prefetch, non-temporal loads,
 # waves per SIMD, symmetric layout), in the library's order
-SPMV_VARIANTS = [(1, False, False, 8, False), (1, True, False, 8, False), (1, True, True, 8, False),
-                 (1, False, True, 8, False), (1, True, True, 6, False), (2, True, True, 6, False),
-                 (2, True, False, 6, False), (2, True, True, 4, False), (1, True, True, 4, False),
-                 (1, False, False, 8, True), (1, False, False, 6, True), (1, False, True, 8, True)]
+SPMV_VARIANTS = [(1, False, False, 8, 0), (1, True, False, 8, 0), (1, True, True, 8, 0),
+                 (1, False, True, 8, 0), (1, True, True, 6, 0), (2, True, True, 6, 0),
+                 (2, True, False, 6, 0), (2, True, True, 4, 0), (1, True, True, 4, 0),
+                 (1, False, False, 8, 1), (1, False, False, 6, 1), (1, False, True, 8, 1),
+                 (1, False, False, 8, 2), (1, False, True, 8, 2), (1, False, False, 6, 2),
+                 (1, False, False, 8, 3), (1, False, True, 8, 3), (1, False, True, 6, 3)]
 
 
 def spmv_kernel_name(fp32, variant):
-    u, pf, nt, w, sym = SPMV_VARIANTS[variant]
+    u, pf, nt, w, lay = SPMV_VARIANTS[variant]
     b = lambda x: "true" if x else "false"
     t = "float, true" if fp32 else "double, false"
-    return f"k_pcg_spmv<{t}, {u}, {b(pf)}, {b(nt)}, {w}, {b(sym and fp32)}>"
+    return f"k_pcg_spmv<{t}, {u}, {b(pf)}, {b(nt)}, {w}, {lay if (fp32 or lay == 3) else 0}>"
 
 
 def measured_traffic(kernel, n):
@@ -233,7 +235,9 @@ def main():
     ctx.forces_fill(GRAV, H)
     copy_ms = ctx.stream_copy(1 << 30, 10)
     copy_gbps = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9
-    log(f"copy kernel: 1 GiB read + 1 GiB written in {copy_ms:.3f} ms ({copy_gbps:.0f} GB/s)")
+    read_gbps = (1 << 30) / (ctx.stream_read(1 << 30, 10) * 1e-3) / 1e9
+    log(f"copy kernel: 1 GiB read + 1 GiB written in {copy_ms:.3f} ms ({copy_gbps:.0f} GB/s); "
+        f"read kernel {read_gbps:.0f} GB/s")
     tuned = {}
     for prec in (32, 64):
         best, ms = ctx.autotune(prec, reps=10)
@@ -321,7 +325,7 @@ def main():
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname, args.n),
             "kernel": kname, "variant": tuned[32]["variant"], "algorithmic_bytes": alg,
-            "avg_launch_ms": spmv32_ms, "copy_kernel_GBps": copy_gbps}
+            "avg_launch_ms": spmv32_ms, "copy_kernel_GBps": copy_gbps, "read_kernel_GBps": read_gbps}
     ctx.close()
     batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit, not args.no_ml)
     cpu = None

@@ -63,7 +63,7 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->stageK); dfree(c->stageF); dfree(c->stageM);
     dfree(c->x); dfree(c->v); dfree(c->f); dfree(c->rhs);
     dfree(c->A64); dfree(c->S32); dfree(c->M32); dfree(c->entry_is_diag); dfree(c->Mval); dfree(c->D64); dfree(c->D32);
-    dfree(c->supper); dfree(c->uoff); dfree(c->loff); dfree(c->lrec); dfree(c->e2u); dfree(c->S32u);
+    dfree(c->supper); dfree(c->uoff); dfree(c->loff); dfree(c->lrec); dfree(c->e2u); dfree(c->S32u); dfree(c->S32a);
     dfree(c->fixmask); dfree(c->vfix);
     dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
     dfree(c->p64[0]); dfree(c->p64[1]);
@@ -628,6 +628,7 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(dalloc(c, &c->S32, 9 * (size_t)T.nbs));
     EOLC_HIP(dalloc(c, &c->M32, (size_t)N));
     EOLC_HIP(dalloc(c, &c->S32u, 10 * (size_t)std::max<int64_t>(T.nup, 1)));
+    EOLC_HIP(dalloc(c, &c->S32a, 10 * (size_t)T.nbs));
     {
         std::vector<int> isd((size_t)T.nbs, 0);
         for (int i = 0; i < N; ++i) isd[T.sdiag[i]] = 1;
@@ -1547,7 +1548,18 @@ int eolc_stream_copy(eolc_ctx *c, int64_t bytes, int reps, double *avg_ms)
     if (bytes < 16 || reps <= 0 || !avg_ms) return fail(c, EOLC_ERR_INVALID, "stream_copy: bad arguments");
     EOLC_HIP(hipSetDevice(c->device));
     float ms = 0.f;
-    EOLC_HIP(eolc::stream_copy(c, (size_t)bytes, reps, &ms));
+    EOLC_HIP(eolc::stream_copy(c, (size_t)bytes, reps, &ms, 0));
+    *avg_ms = ms / reps;
+    return EOLC_OK;
+}
+
+int eolc_stream_read(eolc_ctx *c, int64_t bytes, int reps, double *avg_ms)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (bytes < 16 || reps <= 0 || !avg_ms) return fail(c, EOLC_ERR_INVALID, "stream_read: bad arguments");
+    EOLC_HIP(hipSetDevice(c->device));
+    float ms = 0.f;
+    EOLC_HIP(eolc::stream_copy(c, (size_t)bytes, reps, &ms, 1));
     *avg_ms = ms / reps;
     return EOLC_OK;
 }

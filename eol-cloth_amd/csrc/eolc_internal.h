@@ -232,6 +232,7 @@ struct eolc_ctx {
     // symmetric FP32 layout: upper records {column bits, S'_ij (9)} and lower records
     int *supper = nullptr, *uoff = nullptr, *loff = nullptr, *lrec = nullptr, *e2u = nullptr;
     float *S32u = nullptr;      // 10 x nup
+    float *S32a = nullptr;      // 10 x nbs: {column bits, S'_ij} per SELL entry (record layout)
     int *entry_is_diag = nullptr;  // nbs
     double *Mval = nullptr;     // nbs mass scalars (M block = m I3)
     double *D64 = nullptr;      // 6 x N  block-Jacobi inverse (symmetric, SoA)
@@ -290,9 +291,9 @@ hipError_t pcg_finish(eolc_ctx *c, bool mixed);
 hipError_t spmv_true_residual(eolc_ctx *c);
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
-constexpr int kSpmvVariants = 12;
+constexpr int kSpmvVariants = 18;
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms);
-hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms);
+hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms, int read_only);
 // learned preconditioner (k_learned.hip); buffers allocated by the API layer
 void learned_wgrad_shape(int N, int *G, int *rows);   // weight-gradient workgroups, rows per workgroup
 hipError_t learned_quantize(eolc_ctx *c);

@@ -27,7 +27,8 @@ __global__ void __launch_bounds__(256)
 k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *__restrict__ clist,
            const int *__restrict__ entry_is_diag, const double *__restrict__ stageK, const double *__restrict__ stageM,
            double *__restrict__ A64, float *__restrict__ S32, const int *__restrict__ e2u,
-           const int *__restrict__ scol, float *__restrict__ S32u, double *__restrict__ Mval)
+           const int *__restrict__ scol, float *__restrict__ S32u, float *__restrict__ S32a,
+           double *__restrict__ Mval)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbs) return;
@@ -62,6 +63,10 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
             sv[c] = diag ? 0.0f : (float)(stf[c] + (c % 4 == 0 ? m : 0.0));
             S32[c * nbs + b] = sv[c];
         }
+        float *ra = S32a + (size_t)10 * b;   // record layout: every entry
+        ra[0] = __int_as_float(scol[b]);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) ra[1 + c] = sv[c];
         const int u = e2u[b];
         if (u >= 0) {   // upper block: its record in the symmetric layout
             float *rec = S32u + (size_t)10 * u;
@@ -120,7 +125,7 @@ hipError_t launch_assemble(eolc_ctx *c, bool want32)
     if (nb == 0) return hipSuccess;
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
                        c->entry_is_diag, c->stageK, c->stageM, c->A64, want32 ? c->S32 : nullptr, c->e2u, c->scol,
-                       c->S32u, c->Mval);
+                       c->S32u, c->S32a, c->Mval);
     return hipGetLastError();
 }
 

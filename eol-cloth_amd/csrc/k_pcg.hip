@@ -116,9 +116,15 @@ struct SpmvEntry {
 struct SymLayout {
     const int *supper, *uoff, *loff, *lrec;
     const float *S32u;
+    const float *S32a;   // record layout: {column bits, S'_ij (9)} for every SELL entry, in SELL order
 };
 
-template <class T, bool DIFF, int U, bool PF, bool NT, int W, bool SYM>
+// Block layouts (LAY): 0 = nine SoA value arrays plus the column array (coalesced
+// 4-byte loads); 1 = symmetric records (above); 2 = one 40-byte record {column,
+// S'_ij} per SELL entry, read as five 8-byte loads per lane (five load instructions
+// per block instead of ten); 3 = layout 0 with the next entry's loads issued before
+// the current entry's products.  1 and 2 are FP32 difference form only.
+template <class T, bool DIFF, int U, bool PF, bool NT, int W, int LAY>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const int *__restrict__ srow,
            const int *__restrict__ slen, const int *__restrict__ scol, const T *__restrict__ A,
@@ -174,7 +180,23 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
             y2 = (double)(mb * ui2);
         }
         int kk = 0;
-        if constexpr (DIFF && SYM) {
+        if constexpr (DIFF && LAY == 2) {
+            static_assert(std::is_same<T, float>::value, "record layout is FP32");
+            typedef float f2v __attribute__((ext_vector_type(2)));
+            const float *__restrict__ rb = sym.S32a + (size_t)10 * base;
+#pragma unroll 1
+            for (kk = 0; kk < len; ++kk) {
+                const f2v *rec = reinterpret_cast<const f2v *>(rb + (size_t)640 * kk);
+                const f2v r0 = ldm<f2v, NT>(rec), r1 = ldm<f2v, NT>(rec + 1), r2 = ldm<f2v, NT>(rec + 2),
+                          r3 = ldm<f2v, NT>(rec + 3), r4 = ldm<f2v, NT>(rec + 4);
+                const int j = __float_as_int(r0.x);
+                const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
+                const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
+                const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                block_apply<T, DIFF>(r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x, r4.y, u0, u1, u2, ui0, ui1, ui2,
+                                     y0, y1, y2);
+            }
+        } else if constexpr (DIFF && LAY == 1) {
             static_assert(std::is_same<T, float>::value, "symmetric layout is FP32");
             const int nu = sym.supper[g];
             const int nl = len - nu;
@@ -206,6 +228,31 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
                 const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
                 block_apply<T, DIFF>(r0.y, r2.x, r3.y, r1.x, r2.y, r4.x, r1.y, r3.x, r4.y, u0, u1, u2, ui0, ui1, ui2,
                                      y0, y1, y2);
+            }
+        } else if constexpr (LAY == 3) {
+            // SoA, software-pipelined: the next entry's column and nine values are
+            // loaded before the current entry's gathers and products (twice the loads
+            // in flight per wavefront)
+            if (len > 0) {
+                int jn = ldm<int, NT>(cl);
+                T n0 = ldm<T, NT>(a), n1 = ldm<T, NT>(a + nbs), n2 = ldm<T, NT>(a + 2 * nbs),
+                  n3 = ldm<T, NT>(a + 3 * nbs), n4 = ldm<T, NT>(a + 4 * nbs), n5 = ldm<T, NT>(a + 5 * nbs),
+                  n6 = ldm<T, NT>(a + 6 * nbs), n7 = ldm<T, NT>(a + 7 * nbs), n8 = ldm<T, NT>(a + 8 * nbs);
+#pragma unroll 1
+                for (; kk < len; ++kk) {
+                    const int j = jn;
+                    const T c0 = n0, c1 = n1, c2 = n2, c3 = n3, c4 = n4, c5 = n5, c6 = n6, c7 = n7, c8 = n8;
+                    const int en = (kk + 1 < len ? kk + 1 : kk) * eolc::kSlice;   // clamped: a valid entry
+                    jn = ldm<int, NT>(cl + en);
+                    const T *__restrict__ an = a + en;
+                    n0 = ldm<T, NT>(an); n1 = ldm<T, NT>(an + nbs); n2 = ldm<T, NT>(an + 2 * nbs);
+                    n3 = ldm<T, NT>(an + 3 * nbs); n4 = ldm<T, NT>(an + 4 * nbs); n5 = ldm<T, NT>(an + 5 * nbs);
+                    n6 = ldm<T, NT>(an + 6 * nbs); n7 = ldm<T, NT>(an + 7 * nbs); n8 = ldm<T, NT>(an + 8 * nbs);
+                    const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
+                    const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
+                    const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                    block_apply<T, DIFF>(c0, c1, c2, c3, c4, c5, c6, c7, c8, u0, u1, u2, ui0, ui1, ui2, y0, y1, y2);
+                }
             }
         } else if constexpr (U == 1) {
             int jn = ldm<int, NT>(cl);
@@ -552,6 +599,27 @@ __global__ void __launch_bounds__(256) k_stream_copy(int64_t n16, const u32x4 *_
     for (; i < n16; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// Streaming read (4-byte non-temporal loads, 16 in flight per thread, grid stride):
+// the achievable read bandwidth that read-dominated kernels such as the SpMV are
+// measured against.  The XOR of the data is stored only when it equals a sentinel,
+// which keeps the loads live.
+__global__ void __launch_bounds__(256) k_stream_read(int64_t n4, const unsigned *__restrict__ src,
+                                                     unsigned *__restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned acc = 0;
+    for (; i + 15 * stride < n4; i += 16 * stride) {
+        unsigned v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc ^= v[u];
+    }
+    for (; i < n4; i += stride) acc ^= src[i];
+    if (acc == 0x9e3779b9u) out[0] = acc;
+}
+
 __global__ void k_fill(int n, double v, double *__restrict__ out)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -559,21 +627,28 @@ __global__ void k_fill(int n, double v, double *__restrict__ out)
 }
 
 // SpMV launch variants (see k_pcg_spmv); the order is the one eolc_autotune reports.
-// The symmetric-layout variants (last column) apply to the FP32 difference form;
-// the FP64 product has no such layout and runs its full-layout kernel for them.
+// The last column is the block layout; the symmetric (1) and record (2) layouts
+// apply to the FP32 difference form, and the FP64 product runs its SoA kernel for
+// them; the pipelined SoA loop (3) serves both precisions.
 #define EOLC_SPMV_VARIANT_LIST(X)   \
-    X(0, 1, false, false, 8, false) \
-    X(1, 1, true, false, 8, false)  \
-    X(2, 1, true, true, 8, false)   \
-    X(3, 1, false, true, 8, false)  \
-    X(4, 1, true, true, 6, false)   \
-    X(5, 2, true, true, 6, false)   \
-    X(6, 2, true, false, 6, false)  \
-    X(7, 2, true, true, 4, false)   \
-    X(8, 1, true, true, 4, false)   \
-    X(9, 1, false, false, 8, true)  \
-    X(10, 1, false, false, 6, true) \
-    X(11, 1, false, true, 8, true)
+    X(0, 1, false, false, 8, 0)     \
+    X(1, 1, true, false, 8, 0)      \
+    X(2, 1, true, true, 8, 0)       \
+    X(3, 1, false, true, 8, 0)      \
+    X(4, 1, true, true, 6, 0)       \
+    X(5, 2, true, true, 6, 0)       \
+    X(6, 2, true, false, 6, 0)      \
+    X(7, 2, true, true, 4, 0)       \
+    X(8, 1, true, true, 4, 0)       \
+    X(9, 1, false, false, 8, 1)     \
+    X(10, 1, false, false, 6, 1)    \
+    X(11, 1, false, true, 8, 1)     \
+    X(12, 1, false, false, 8, 2)    \
+    X(13, 1, false, true, 8, 2)     \
+    X(14, 1, false, false, 6, 2)    \
+    X(15, 1, false, false, 8, 3)    \
+    X(16, 1, false, true, 8, 3)     \
+    X(17, 1, false, true, 6, 3)
 
 template <class T>
 void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm, const T *z, const T *pold, T *pnew,
@@ -581,17 +656,17 @@ void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm
 {
     const eolc::Topology &t = c->topo;
     constexpr bool diff = std::is_same<T, float>::value;
-    const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u};
-#define EOLC_SPMV_CASE(id, U, PF, NT, W, SYM)                                                                    \
-    case id:                                                                                                     \
-        hipLaunchKernelGGL((k_pcg_spmv<T, diff, U, PF, NT, W, SYM && diff>), dim3(G), dim3(WG), 0, c->stream, k, \
+    const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u, c->S32a};
+#define EOLC_SPMV_CASE(id, U, PF, NT, W, LAY)                                                                      \
+    case id:                                                                                                       \
+        hipLaunchKernelGGL((k_pcg_spmv<T, diff, U, PF, NT, W, (diff || LAY == 3) ? LAY : 0>), dim3(G), dim3(WG), 0, c->stream, k, \
                            t.N, t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z,  \
                            pold, pnew, q, c->scal, c->partials, sym);                                             \
         break;
     switch (variant) {
         EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_CASE)
     default:
-        hipLaunchKernelGGL((k_pcg_spmv<T, diff, 1, false, false, 8, false>), dim3(G), dim3(WG), 0, c->stream, k, t.N,
+        hipLaunchKernelGGL((k_pcg_spmv<T, diff, 1, false, false, 8, 0>), dim3(G), dim3(WG), 0, c->stream, k, t.N,
                            t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, pold, pnew,
                            q, c->scal, c->partials, sym);
     }
@@ -766,9 +841,9 @@ hipError_t spmv_true_residual(eolc_ctx *c)
     return hipGetLastError();
 }
 
-// Back-to-back launches of the SpMV kernel for timing (iteration index 1, `done`
-// cleared first).  Leaves the solver state undefined.
-hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms)
+// Timed launches of the streaming copy (read_only = 0: `bytes` read and `bytes`
+// written per launch) or of the streaming read (`bytes` read per launch).
+hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms, int read_only)
 {
     const int64_t n16 = (int64_t)(bytes / 16);
     u32x4 *a = nullptr, *b = nullptr;
@@ -782,10 +857,17 @@ hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms)
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const int grid = 256 * 8;   // 8 workgroups of 256 threads per CU on 256 CUs
-    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, n16, a, b);   // first touch
+    const int grid = read_only ? 65536 : 256 * 32;   // fewer workgroups reach 10-20% less
+    auto launch = [&] {
+        if (read_only)
+            hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(256), 0, c->stream, 4 * n16,
+                               reinterpret_cast<const unsigned *>(a), reinterpret_cast<unsigned *>(b));
+        else
+            hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, n16, a, b);
+    };
+    launch();   // first touch
     (void)hipEventRecord(e0, c->stream);
-    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, c->stream, n16, a, b);
+    for (int r = 0; r < reps; ++r) launch();
     (void)hipEventRecord(e1, c->stream);
     e = hipEventSynchronize(e1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);

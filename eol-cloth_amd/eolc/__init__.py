@@ -34,13 +34,13 @@ EXPORTED_SYMBOLS = (
     "eolc_set_batch_topology", "eolc_set_batch_material", "eolc_batch_step", "eolc_batch_autotune",
     "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta", "eolc_collision_constraints",
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
-    "eolc_velocity_transfer",
+    "eolc_velocity_transfer", "eolc_stream_read",
 )
 
 BATCH_EAGER = 0x1
 BATCH_VARIANTS = 3
 
-SPMV_VARIANTS = 12
+SPMV_VARIANTS = 18
 
 LEARNED_NPARAMS = 26246
 
@@ -119,6 +119,7 @@ def lib():
                                                  C.c_void_p, C.c_void_p, C.c_int, C.c_int64, C.POINTER(C.c_int),
                                                  C.POINTER(C.c_int64)]
         L.eolc_stream_copy.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+        L.eolc_stream_read.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
         L.eolc_autotune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_void_p]
         L.eolc_set_spmv_variant.argtypes = [vp, C.c_int, C.c_int]
         L.eolc_ml_info.argtypes = [vp, C.POINTER(C.c_int), C.c_void_p]
@@ -323,6 +324,12 @@ class Context:
 
     def set_mixed_delta(self, delta):
         return self._check(lib().eolc_set_mixed_delta(self._h, float(delta)))
+
+    def stream_read(self, nbytes, reps=10):
+        """Average ms of one streaming read of nbytes."""
+        ms = C.c_double(0)
+        self._check(lib().eolc_stream_read(self._h, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
 
     def stream_copy(self, nbytes, reps=10):
         """Average ms of one copy-kernel launch reading and writing nbytes each."""

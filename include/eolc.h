@@ -287,6 +287,9 @@ int eolc_time_spmv(eolc_ctx *ctx, int precision, int reps, double *avg_ms);
  * stores) reading and writing `bytes` each, over `reps` launches: the known-size
  * kernel for counter calibration and the achievable-bandwidth reference. */
 int eolc_stream_copy(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
+/* The same for a streaming read of `bytes` (4-byte non-temporal loads): the
+ * achievable read bandwidth, the reference for read-dominated kernels. */
+int eolc_stream_read(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
 
 /* ---- launch autotuner (no counterpart in the reference) -----------------------
  * Times each SpMV launch variant (entries per loop trip, column-index prefetch,
@@ -296,7 +299,7 @@ int eolc_stream_copy(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
  * this context.  All variants give bitwise-identical results.  ms (may be NULL)
  * receives EOLC_SPMV_VARIANTS average times; *best (may be NULL) the choice.
  * eolc_set_spmv_variant sets the choice directly. */
-#define EOLC_SPMV_VARIANTS 12
+#define EOLC_SPMV_VARIANTS 18
 int eolc_autotune(eolc_ctx *ctx, int precision, int reps, int *best, double *ms);
 int eolc_set_spmv_variant(eolc_ctx *ctx, int precision, int variant);
 
