@@ -10,6 +10,8 @@
 //                     q = A p_new; partial p_new.q -> PA.
 //   k_pcg_update<k> : pq <- sum of PA; alpha = rz_k / pq; x += alpha p_new,
 //                     r -= alpha q, z = D^-1 r; partials r.z, r.r -> PB[(k+1)&1].
+// With the multilevel preconditioner the update forms x, r and r.r only, and the
+// restriction, coarse-level and prolongation kernels of k_ml.hip form z and r.z.
 // There is no grid-wide fence inside a kernel: every workgroup re-reduces the
 // previous kernel's per-workgroup partials itself, in one fixed order, so all
 // workgroups reach bitwise-identical scalars and identical stop decisions.  The
