@@ -103,6 +103,43 @@ def cpu_baseline(sc, iters_per_step, threads):
             "fill_s": t1 - t0, "iter_ms": t_iter * 1e3}
 
 
+def run_collision(reps, with_cpu):
+    """Proximity queries (CD2) of the 512x512 cloth with noise 1e-3 against 64 seeded
+    points and 4 seeded boxes at threshold 1e-3: device-resident cloth, records
+    returned to the host each call.  The oracle's brute-force restatement of the
+    same queries is timed on the host (one thread) as the CPU baseline, and its
+    record count checked against the GPU's."""
+    sc = scene.Scene(512, sigma_x=1e-3, sigma_v=0.05)
+    pts, nrm, whd, E = scene.obstacles(seed=1236, P=64, B=4)
+    ctx = eolc.Context(0)
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.upload_state(sc.x0, np.zeros_like(sc.x0))
+    recs = ctx.collide(1e-3, pts, nrm, whd, E)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        recs = ctx.collide(1e-3, pts, nrm, whd, E)
+    t_gpu = (time.perf_counter() - t0) / reps
+    ctx.close()
+    out = {"workload": "cloth 512x512 quads, noise 1e-3 m, 64 points + 4 boxes, threshold 1e-3 (CD2)",
+           "records": len(recs), "ms_per_call": t_gpu * 1e3, "calls_per_s": 1.0 / t_gpu,
+           "faces_tested_per_s": (64 + 4 * 8) * sc.F / t_gpu}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t1 = time.perf_counter()
+        n, ro = O.point_tri(1e-3, pts, nrm, sc.x0, sc.tri)
+        nrec = n
+        for b in range(len(whd)):
+            nb, rb = O.box_tri(1e-3, whd[b], E[b], sc.x0, sc.tri)
+            nrec += nb
+        t_cpu = time.perf_counter() - t1
+        out["cpu_baseline"] = {"ms_per_call": t_cpu * 1e3, "cores": 1, "kind": "port", "records": nrec}
+        if nrec != len(recs):
+            raise RuntimeError(f"collision record count differs: gpu {len(recs)} oracle {nrec}")
+    log(f"collision: {out}")
+    return out
+
+
 WEIGHTS = os.path.join(ROOT, "eol-cloth_amd", "eolc", "weights", "mpn_cloth.npy")
 TRAFFIC = os.path.join(ROOT, "profiles", "pmc_spmv_cloth1024.json")
 
@@ -208,6 +245,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-cloth workload")
+    ap.add_argument("--no-collide", action="store_true", help="skip the proximity-query measurement")
     ap.add_argument("--no-ml", action="store_true",
                     help="headline without the multilevel coarse levels (and skip the multilevel lines)")
     ap.add_argument("--profile-solve", action="store_true",
@@ -328,6 +366,7 @@ def main():
             "avg_launch_ms": spmv32_ms, "copy_kernel_GBps": copy_gbps, "read_kernel_GBps": read_gbps}
     ctx.close()
     batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit, not args.no_ml)
+    collision = None if args.no_collide else run_collision(10, not args.no_cpu)
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -358,6 +397,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "collision": collision,
     }
     print(json.dumps(res), flush=True)
 
