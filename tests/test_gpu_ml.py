@@ -138,3 +138,67 @@ def test_multilevel_step_through_driver(ctx):
         xs.append(ctx.download_state()[0])
     for x in xs[1:]:
         assert np.max(np.abs(x - xs[0])) <= 1e-9
+
+
+def two_sheets(n):
+    """Two disconnected n x n sheets in one topology (the second shifted by 2 m)."""
+    a = scene.Scene(n, sigma_x=1e-3, sigma_v=0.05)
+    b = scene.Scene(n, sigma_x=1e-3, sigma_v=0.05, seed_x=77, seed_v=78)
+    N = a.N
+    X = np.vstack([a.X, b.X + [2.0, 0.0]])
+    tri = np.vstack([a.tri, b.tri + N])
+    hinge = np.vstack([a.hinge, b.hinge + N])
+    x0 = np.concatenate([a.x0, b.x0 + np.tile([2.0, 0.0, 0.0], N)])
+    v0 = np.concatenate([a.v0, b.v0])
+    fixed = np.concatenate([a.fixed, b.fixed + N])
+    return X, tri, hinge, x0, v0, fixed
+
+
+@pytest.mark.parametrize("case", ["partial-fixing", "two-sheets", "tiny"])
+def test_multilevel_edge_cases(ctx, case):
+    """Component-wise fixed DOFs, a topology of two disconnected sheets, and a mesh
+    small enough for a single level: P r against the oracle (1e-12) and a solve to
+    1e-10 that agrees with block-Jacobi's to 1e-6."""
+    print(f"multilevel edge case: {case}", flush=True)
+    if case == "two-sheets":
+        X, tri, hinge, x0, v0, fixed = two_sheets(14)
+    else:
+        sc = scene.Scene(18 if case == "partial-fixing" else 2, sigma_x=1e-3, sigma_v=0.05)
+        X, tri, hinge, x0, v0, fixed = sc.X, sc.tri, sc.hinge, sc.x0, sc.v0, sc.fixed
+    N = len(X)
+    ctx.set_topology(X, tri, hinge)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    ctx.upload_state(x0, v0)
+    ctx.forces_fill(GRAV, H)
+    fd = learned.fixed_dofs(fixed)
+    if case == "partial-fixing":
+        # z only of a column of vertices, x and y of another
+        fd = np.concatenate([fd, 3 * np.arange(0, N, 19) + 2, 3 * np.arange(5, N, 23), 3 * np.arange(5, N, 23) + 1])
+        fd = np.unique(fd)
+    counts = ctx.ml_info()
+    print(f"  aggregates per level {counts}", flush=True)
+    A = ctx.download_csr(eolc.MAT_MDK).tocsr()
+    maps = [ctx.ml_aggregates(l) for l in range(1, len(counts))]
+    if case == "two-sheets":
+        assert counts[-1] == 2 and not set(maps[-1][:N // 2]) & set(maps[-1][N // 2:])
+    if case == "tiny":
+        assert len(counts) == 2
+    rng = np.random.default_rng(3)
+    r = rng.normal(size=3 * N)
+    z = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
+    zref = ML.apply(A, N, maps, r, fd)
+    print(f"  P r rel err {rel_err(z, zref):.2e}", flush=True)
+    assert rel_err(z, zref) <= 1e-12 and np.all(z[fd] == 0.0)
+    vals = 0.99 * v0[fd]
+    out = []
+    for mode in (eolc.SOLVE_FP64_BJ, eolc.SOLVE_FP64_BJ | MLF, eolc.SOLVE_MIXED_BJ | MLF):
+        ctx.upload_state(x0, v0)
+        ctx.forces_fill(GRAV, H)
+        rc, st = ctx.velocity_solve(fd, vals, tol=1e-10, mode=mode)
+        assert rc == eolc.OK and st.true_relres <= 1e-10
+        out.append(ctx.download_state()[1])
+        print(f"  mode {mode}: {st.iters} iterations", flush=True)
+    for v in out[1:]:
+        assert rel_err(v, out[0]) <= 1e-6
+        assert np.array_equal(v[fd], out[0][fd])
