@@ -144,3 +144,42 @@ def test_spmv_variants_bitwise_identical_and_autotune(ctx):
         best, ms = ctx.autotune(prec, reps=5)
         print(f"autotune fp{prec}: best {best}, ms {np.round(ms * 1e3, 1)} us", flush=True)
         assert 0 <= best < eolc.SPMV_VARIANTS and np.all(ms > 0) and ms[best] == ms.min()
+
+
+def test_headline_path_50_steps_vs_oracle(ctx):
+    """The benchmark's path (mixed precision, learned fine blocks at their initial
+    weights, multilevel levels) for 50 steps of a 32x32 sheet with the benchmark's
+    noise, against 50 steps of the oracle's FP64 block-Jacobi, both solved to 1e-10.
+    Reported: position and velocity error after 50 steps, and the force error of the
+    GPU's FP64 evaluation at the final state.  Bars: positions and velocities within
+    20x the oracle's own sensitivity to its solver tolerance (1e-10 vs 1e-11), with
+    floors 1e-9 and 1e-7; forces within 1e-10."""
+    from eolc import learned
+    sc = scene.Scene(32, sigma_x=1e-3, sigma_v=0.05)
+    print("headline path: 50 steps 32x32 vs oracle", flush=True)
+    mat = O.material()
+    runs = []
+    for tol in (1e-10, 1e-11):
+        x, v = sc.x0.copy(), sc.v0.copy()
+        for k in range(50):
+            rco, sto, _ = O.step(x, v, sc.X, sc.tri, sc.hinge, mat, GRAV, H, sc.fixed, tol=tol, maxit=20000)
+            assert rco == 0
+        runs.append((x, v))
+    setup(ctx, sc)
+    ctx.learned_set_weights(learned.init_weights(7))
+    head = eolc.SOLVE_MIXED_LEARNED | eolc.SOLVE_MULTILEVEL
+    for k in range(50):
+        rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-10, maxit=20000, mode=head)
+        assert rc == 0 and st.true_relres <= 1e-10
+    xg, vg = ctx.download_state()
+    (xo, vo), (x11, v11) = runs
+    ex, ev = rel_err(xg, xo), rel_err(vg, vo)
+    sx, sv = rel_err(x11, xo), rel_err(v11, vo)
+    ctx.forces_fill(GRAV, H)
+    fg = ctx.download_forces()
+    rcf, fo, _, _ = O.forces_fill(xg, sc.X, sc.tri, sc.hinge, mat, GRAV, H)
+    ef = rel_err(fg, fo)
+    print(f"  50 steps: x err {ex:.2e} (oracle sensitivity {sx:.2e}), v err {ev:.2e} ({sv:.2e}), "
+          f"forces at the final state {ef:.2e}", flush=True)
+    assert ex <= max(20 * sx, 1e-9) and ev <= max(20 * sv, 1e-7)
+    assert ef <= 1e-10
