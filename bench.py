@@ -270,8 +270,8 @@ def main():
                          "device copy for calibration, then exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; the multi-GPU layout is described in "
-              "DESIGN.md and no figure at several GPUs is produced.", file=sys.stderr)
+        print("bench.py measures one GPU: the multi-GPU layout is a design only, and no figure "
+              "at several GPUs is produced.", file=sys.stderr)
         sys.exit(2)
 
     t = time.perf_counter()

@@ -11,7 +11,7 @@
 // (MeshArrays) and matrices come back as scalar CSR (CsrMatrix).  A reference
 // build replaces the bodies of its Forces::fill / velocitySolve / CD2 by calls to
 // these classes after copying Node::x, Vert::u, Face::v[] and Edge::n[]/adjf[] into
-// a MeshArrays (see INTEGRATION.md for the exact glue).
+// a MeshArrays (flat x, X, tri and hinge arrays in ArcSim order).
 #pragma once
 #include <cstdint>
 #include <stdexcept>

@@ -1,6 +1,6 @@
 """Forces::fill parity: GPU (FP64) f, M and MDK against the CPU oracle.
 
-Tolerance (north_star): 1e-10 relative to the largest entry of the oracle's array."""
+Tolerance: 1e-10 relative to the largest entry of the oracle's array."""
 import numpy as np
 import pytest
 
