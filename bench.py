@@ -283,6 +283,12 @@ def main():
     ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
     ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
     log(f"topology set-up {time.perf_counter() - t:.1f} s")
+    # what a remesh costs on a live context: the pattern, SELL layout and gather maps
+    # rebuilt and every per-topology device buffer replaced
+    t = time.perf_counter()
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    rebuild_s = time.perf_counter() - t
+    log(f"topology rebuild {rebuild_s:.2f} s")
 
     ctx.upload_state(sc.x0, sc.v0)
     ctx.forces_fill(GRAV, H)
@@ -408,6 +414,7 @@ def main():
         "mixed_bj": out_mbj,
         **extra,
         "multilevel_build_s": ml_build_s,
+        "topology_rebuild_s": round(rebuild_s, 3),
         "autotune": tuned,
         "roofline": roof,
         "cpu_baseline": cpu,

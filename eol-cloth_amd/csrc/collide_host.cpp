@@ -136,33 +136,38 @@ template <class T> hipError_t dmalloc(DevBuf &b, size_t n)
 namespace eolc {
 
 // createEdges order and pairing (topology only).  cedges: v0, v1, face0, face1|-1.
+// The face-edges are ordered by their (signed, wrapping) 32-bit hash with ties in
+// creation order, which is what the reference's stable sort gives; here a stable
+// two-pass radix sort over the hash as an order-preserving unsigned key.
 void build_collision_edges(eolc_ctx *c, int F, const int *tri)
 {
-    struct FE {
-        int lv[3];
-        int face;
-        int hash;
-    };
     const int n = 3 * F;
-    std::vector<FE> tmp((size_t)n);
+    // items: key in the high word, creation index in the low word
+    std::vector<uint64_t> item((size_t)n), tmp((size_t)n);
     for (int k = 0; k < F; ++k)
         for (int i = 0; i < 3; ++i) {
-            FE &fe = tmp[3 * (size_t)k + i];
-            fe.lv[0] = i % 3; fe.lv[1] = (i + 1) % 3; fe.lv[2] = (i + 2) % 3;
-            fe.face = k;
-            const int a = tri[3 * k + fe.lv[0]], b = tri[3 * k + fe.lv[1]];
+            const int a = tri[3 * k + i], b = tri[3 * k + (i + 1) % 3];
             const int kmin = std::min(a, b) + 1, kmax = std::max(a, b) + 1;
-            fe.hash = (int)((uint32_t)kmin + (uint32_t)(n + 1) * (uint32_t)kmax);   // int overflow wraps
+            const int hash = (int)((uint32_t)kmin + (uint32_t)(n + 1) * (uint32_t)kmax);   // int overflow wraps
+            const uint32_t key = (uint32_t)hash ^ 0x80000000u;   // signed order as unsigned order
+            item[3 * (size_t)k + i] = ((uint64_t)key << 32) | (uint32_t)(3 * k + i);
         }
-    std::stable_sort(tmp.begin(), tmp.end(), [](const FE &x, const FE &y) { return x.hash < y.hash; });
+    for (int shift = 32; shift < 64; shift += 16) {
+        std::vector<int> cnt(65537, 0);
+        for (int i = 0; i < n; ++i) cnt[((item[i] >> shift) & 0xffffu) + 1]++;
+        for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+        for (int i = 0; i < n; ++i) tmp[cnt[(item[i] >> shift) & 0xffffu]++] = item[i];
+        item.swap(tmp);
+    }
     std::vector<int> &E = c->h_cedges;
     E.clear();
+    E.reserve(4 * (size_t)n);
     int k = 0;
     while (k < n) {
-        const int f0 = tmp[k].face;
-        const int v0 = tri[3 * f0 + tmp[k].lv[0]], v1 = tri[3 * f0 + tmp[k].lv[1]];
-        if (k < n - 1 && tmp[k].hash == tmp[k + 1].hash) {
-            E.insert(E.end(), {v0, v1, f0, tmp[k + 1].face});
+        const int s = (int)(uint32_t)item[k], f0 = s / 3, i0 = s % 3;
+        const int v0 = tri[3 * f0 + i0], v1 = tri[3 * f0 + (i0 + 1) % 3];
+        if (k < n - 1 && (item[k + 1] >> 32) == (item[k] >> 32)) {
+            E.insert(E.end(), {v0, v1, f0, (int)(uint32_t)item[k + 1] / 3});
             k += 2;
         } else {
             E.insert(E.end(), {v0, v1, f0, -1});

@@ -8,12 +8,14 @@
 //   GeneralizedSolver::velocitySolve  GeneralizedSolver.cpp:107-172
 //   Cloth::step                       Cloth.cpp:359-413
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <thread>
 
 #include "eolc_internal.h"
 
@@ -118,9 +120,87 @@ int find_block(const Topology &t, int i, int j)
     return (p != e && *p == j) ? (int)(p - t.col.data()) : -1;
 }
 
+// Runs fn(begin, end) over contiguous chunks of [0, n) on up to 16 host threads
+// (one chunk, on the calling thread, below `grain` items per thread).
+template <class Fn> void host_parallel(int64_t n, int64_t grain, Fn &&fn)
+{
+    const int64_t hw = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+    const int T = (int)std::max<int64_t>(1, std::min(hw, n / std::max<int64_t>(grain, 1)));
+    if (T <= 1) {
+        fn((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (int k = 0; k < T; ++k) th.emplace_back([&fn, n, k, T] { fn(n * k / T, n * (k + 1) / T); });
+    for (auto &x : th) x.join();
+}
+
+// Exclusive prefix sum in place over v[0..n) into v[0..n] (v has n + 1 entries,
+// v[i + 1] holds the count of item i on entry); false on 32-bit overflow.
+bool prefix_counts(std::vector<int> &v, int64_t n)
+{
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(16, n / (1 << 16)));
+    std::vector<int64_t> part(T + 1, 0);
+    host_parallel(T, 1, [&](int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; ++k) {
+            int64_t s = 0;
+            for (int64_t i = n * k / T; i < n * (k + 1) / T; ++i) s += v[i + 1];
+            part[k + 1] = s;
+        }
+    });
+    for (int k = 0; k < T; ++k) part[k + 1] += part[k];
+    if (part[T] > (int64_t)INT32_MAX) return false;
+    host_parallel(T, 1, [&](int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; ++k) {
+            int64_t s = part[k];
+            for (int64_t i = n * k / T; i < n * (k + 1) / T; ++i) {
+                s += v[i + 1];
+                v[i + 1] = (int)s;
+            }
+        }
+    });
+    v[0] = 0;
+    return true;
+}
+
+// Element incident to a vertex through force slot s (3f + v for faces, then
+// 3F + 4h + v for hinges): its vertices, their count, the local index v and the
+// element's first block slot (6f for faces, 6F + 10h for hinges).
+struct Incidence {
+    const int *verts;
+    int nv, local;
+    int64_t slot0;
+};
+inline Incidence incidence(int s, int F, const int *tri, const int *hinge)
+{
+    if (s < 3 * F) return {tri + 3 * (s / 3), 3, s % 3, 6 * (int64_t)(s / 3)};
+    const int h = (s - 3 * F) / 4;
+    return {hinge + 4 * h, 4, (s - 3 * F) % 4, 6 * (int64_t)F + 10 * (int64_t)h};
+}
+
+// Calls fn(column, code) for every block contribution of row `row`, elements in
+// slot order: an element pair (a, b) puts slot << 1 on block (x_a, x_b) and, for
+// x_a != x_b, (slot << 1) | 1 (the transpose) on block (x_b, x_a).
+template <class Fn> void row_contributions(const Topology &t, const int *tri, const int *hinge, int row, Fn &&fn)
+{
+    for (int k = t.fptr[row]; k < t.fptr[row + 1]; ++k) {
+        const Incidence in = incidence(t.flist[k], t.F, tri, hinge);
+        const int(*pairs)[2] = in.nv == 3 ? kFacePairs : kHingePairs;
+        const int np = in.nv == 3 ? 6 : 10;
+        for (int p = 0; p < np; ++p) {
+            const int a = pairs[p][0], b = pairs[p][1];
+            if (a == in.local) fn(in.verts[b], (int)((in.slot0 + p) << 1));
+            else if (b == in.local && in.verts[a] != in.verts[b]) fn(in.verts[a], (int)(((in.slot0 + p) << 1) | 1));
+        }
+    }
+}
+
 // Builds the BSR pattern and the gather maps.  Returns an error message or "".
 // seg: optional segment starts (ascending, multiples of 64, first 0); rows are
-// never reordered across a segment boundary (batched instances).
+// never reordered across a segment boundary (batched instances).  Work is split
+// over host threads by rows or slices; every output equals the sequential
+// element-order construction (blocks' contributor lists in slot order).
 std::string build_topology(Topology &t, int N, int F, const int *tri, int H, const int *hinge,
                            const std::vector<int> *seg = nullptr)
 {
@@ -129,76 +209,54 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
         if (tri[i] < 0 || tri[i] >= N) return "triangle vertex index out of range";
     for (int i = 0; i < 4 * H; ++i)
         if (hinge[i] < 0 || hinge[i] >= N) return "hinge vertex index out of range";
-    // neighbour sets
-    std::vector<int> deg(N + 1, 1);
-    for (int f = 0; f < F; ++f)
-        for (int v = 0; v < 3; ++v) deg[tri[3 * f + v]] += 2;
-    for (int h = 0; h < H; ++h)
-        for (int v = 0; v < 4; ++v) deg[hinge[4 * h + v]] += 3;
-    std::vector<int64_t> off(N + 1, 0);
-    for (int i = 0; i < N; ++i) off[i + 1] = off[i] + deg[i];
-    std::vector<int> nbr((size_t)off[N]);
-    std::vector<int64_t> cur(off.begin(), off.end() - 1);
-    for (int i = 0; i < N; ++i) nbr[cur[i]++] = i;
-    for (int f = 0; f < F; ++f)
-        for (int v = 0; v < 3; ++v)
-            for (int w = 0; w < 3; ++w)
-                if (v != w) nbr[cur[tri[3 * f + v]]++] = tri[3 * f + w];
-    for (int h = 0; h < H; ++h)
-        for (int v = 0; v < 4; ++v)
-            for (int w = 0; w < 4; ++w)
-                if (v != w) nbr[cur[hinge[4 * h + v]]++] = hinge[4 * h + w];
-    t.rowptr.assign(N + 1, 0);
-    t.col.clear();
-    t.col.reserve((size_t)off[N]);
-    for (int i = 0; i < N; ++i) {
-        auto b = nbr.begin() + off[i], e = nbr.begin() + cur[i];
-        std::sort(b, e);
-        e = std::unique(b, e);
-        t.col.insert(t.col.end(), b, e);
-        if (t.col.size() > (size_t)INT32_MAX) return "too many blocks for 32-bit indices";
-        t.rowptr[i + 1] = (int)t.col.size();
+    if (2 * (6 * (int64_t)F + 10 * (int64_t)H) + 1 > (int64_t)INT32_MAX)
+        return "too many element slots for 32-bit indices";
+    // vertex force slots: faces (3f+v), then hinges (3F + 4h + v); also the
+    // vertex -> element incidence, in slot order
+    t.fptr.assign(N + 1, 0);
+    for (int i = 0; i < 3 * F; ++i) t.fptr[tri[i] + 1]++;
+    for (int i = 0; i < 4 * H; ++i) t.fptr[hinge[i] + 1]++;
+    for (int i = 0; i < N; ++i) t.fptr[i + 1] += t.fptr[i];
+    t.flist.assign(t.fptr[N], 0);
+    {
+        std::vector<int> ff(t.fptr.begin(), t.fptr.end() - 1);
+        for (int i = 0; i < 3 * F; ++i) t.flist[ff[tri[i]]++] = i;
+        for (int i = 0; i < 4 * H; ++i) t.flist[ff[hinge[i]]++] = 3 * F + i;
+    }
+    // row patterns: the vertex itself and every vertex of its elements, sorted
+    std::vector<int> rowlen(N + 1, 0);
+    {
+        const int T = (int)std::max<int64_t>(1, std::min<int64_t>(16, N / 4096));
+        std::vector<std::vector<int>> local(T);
+        host_parallel(T, 1, [&](int64_t k0, int64_t k1) {
+            for (int64_t k = k0; k < k1; ++k) {
+                std::vector<int> &out = local[k];
+                std::vector<int> buf;
+                for (int64_t i = (int64_t)N * k / T; i < (int64_t)N * (k + 1) / T; ++i) {
+                    buf.assign(1, (int)i);
+                    for (int s = t.fptr[i]; s < t.fptr[i + 1]; ++s) {
+                        const Incidence in = incidence(t.flist[s], F, tri, hinge);
+                        buf.insert(buf.end(), in.verts, in.verts + in.nv);
+                    }
+                    std::sort(buf.begin(), buf.end());
+                    const auto e = std::unique(buf.begin(), buf.end());
+                    out.insert(out.end(), buf.begin(), e);
+                    rowlen[i + 1] = (int)(e - buf.begin());
+                }
+            }
+        });
+        int64_t total = 0;
+        for (auto &l : local) total += (int64_t)l.size();
+        if (total > (int64_t)INT32_MAX) return "too many blocks for 32-bit indices";
+        t.rowptr.assign(N + 1, 0);
+        for (int i = 0; i < N; ++i) t.rowptr[i + 1] = t.rowptr[i] + rowlen[i + 1];
+        t.col.resize((size_t)total);
+        host_parallel(T, 1, [&](int64_t k0, int64_t k1) {
+            for (int64_t k = k0; k < k1; ++k)
+                std::copy(local[k].begin(), local[k].end(), t.col.begin() + t.rowptr[(int64_t)N * k / T]);
+        });
     }
     t.nb = (int64_t)t.col.size();
-    // block contributions in reference insertion order (slot order), per CSR block
-    const int64_t nslots = 6 * (int64_t)F + 10 * (int64_t)H;
-    if (2 * nslots + 1 > (int64_t)INT32_MAX) return "too many element slots for 32-bit indices";
-    std::vector<int> cnt(t.nb + 1, 0);
-    std::vector<int> slotblk((size_t)nslots * 2);
-    auto visit = [&](int64_t slot, int i, int j) {
-        int b1 = find_block(t, i, j);
-        slotblk[2 * slot] = b1;
-        cnt[b1 + 1]++;
-        if (i != j) {
-            int b2 = find_block(t, j, i);
-            slotblk[2 * slot + 1] = b2;
-            cnt[b2 + 1]++;
-        } else {
-            slotblk[2 * slot + 1] = -1;
-        }
-    };
-    for (int f = 0; f < F; ++f)
-        for (int p = 0; p < 6; ++p)
-            visit(6 * (int64_t)f + p, tri[3 * f + kFacePairs[p][0]], tri[3 * f + kFacePairs[p][1]]);
-    for (int h = 0; h < H; ++h)
-        for (int p = 0; p < 10; ++p)
-            visit(6 * (int64_t)F + 10 * (int64_t)h + p, hinge[4 * h + kHingePairs[p][0]],
-                  hinge[4 * h + kHingePairs[p][1]]);
-    std::vector<int> bptr(t.nb + 1, 0);
-    for (int64_t b = 0; b < t.nb; ++b) bptr[b + 1] = bptr[b] + cnt[b + 1];
-    std::vector<int> blist(bptr[t.nb], 0);
-    {
-        std::vector<int> fill(bptr.begin(), bptr.end() - 1);
-        for (int64_t slot = 0; slot < nslots; ++slot) {
-            int b1 = slotblk[2 * slot], b2 = slotblk[2 * slot + 1];
-            blist[fill[b1]++] = (int)(slot << 1);
-            if (b2 >= 0) blist[fill[b2]++] = (int)((slot << 1) | 1);
-        }
-    }
-    t.has_tri.assign(t.nb, 0);
-    for (int64_t b = 0; b < t.nb; ++b)
-        for (int e = bptr[b]; e < bptr[b + 1]; ++e)
-            if ((blist[e] >> 1) < 6 * F) { t.has_tri[b] = 1; break; }
     // SELL-C-sigma layout: rows sorted by decreasing length inside windows of
     // sigma = kSellSigma rows (never across a segment boundary), then cut into
     // slices of C = 64.
@@ -207,21 +265,28 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
     t.slice_off.assign(t.nslices + 1, 0);
     t.srow.assign(N, 0);
     t.slen.assign(N, 0);
-    std::vector<int> bounds = seg ? *seg : std::vector<int>{0};
-    bounds.push_back(N);
-    for (size_t si = 0; si + 1 < bounds.size(); ++si) {
-        for (int w0 = bounds[si]; w0 < bounds[si + 1]; w0 += eolc::kSellSigma) {
-            const int w1 = std::min(bounds[si + 1], w0 + eolc::kSellSigma);
+    {
+        std::vector<int> bounds = seg ? *seg : std::vector<int>{0};
+        bounds.push_back(N);
+        std::vector<std::pair<int, int>> win;
+        for (size_t si = 0; si + 1 < bounds.size(); ++si)
+            for (int w0 = bounds[si]; w0 < bounds[si + 1]; w0 += eolc::kSellSigma)
+                win.emplace_back(w0, std::min(bounds[si + 1], w0 + eolc::kSellSigma));
+        host_parallel((int64_t)win.size(), 256, [&](int64_t b, int64_t e) {
             std::vector<int> rows;
-            for (int r = w0; r < w1; ++r) rows.push_back(r);
-            std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) {
-                return (t.rowptr[a + 1] - t.rowptr[a]) > (t.rowptr[b + 1] - t.rowptr[b]);
-            });
-            for (int l = 0; l < (int)rows.size(); ++l) {
-                t.srow[w0 + l] = rows[l];
-                t.slen[w0 + l] = t.rowptr[rows[l] + 1] - t.rowptr[rows[l]];
+            for (int64_t w = b; w < e; ++w) {
+                const int w0 = win[w].first, w1 = win[w].second;
+                rows.clear();
+                for (int r = w0; r < w1; ++r) rows.push_back(r);
+                std::stable_sort(rows.begin(), rows.end(), [&](int x, int y) {
+                    return (t.rowptr[x + 1] - t.rowptr[x]) > (t.rowptr[y + 1] - t.rowptr[y]);
+                });
+                for (int l = 0; l < (int)rows.size(); ++l) {
+                    t.srow[w0 + l] = rows[l];
+                    t.slen[w0 + l] = t.rowptr[rows[l] + 1] - t.rowptr[rows[l]];
+                }
             }
-        }
+        });
     }
     for (int sl = 0; sl < t.nslices; ++sl) {
         const int r0 = sl * C, r1 = std::min(N, r0 + C);
@@ -241,35 +306,75 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
     // early and never reads the diagonal, and the symmetric FP32 layout below sums
     // in exactly this order.
     t.supper.assign(N, 0);
-    for (int sl = 0; sl < t.nslices; ++sl) {
-        const int r0 = sl * C, r1 = std::min(N, r0 + C);
-        for (int l = 0; l < r1 - r0; ++l) {
-            const int row = t.srow[r0 + l];
-            int k = 0, bdiag = -1;
-            for (int pass = 0; pass < 2; ++pass)
-                for (int b = t.rowptr[row]; b < t.rowptr[row + 1]; ++b) {
-                    if (t.col[b] == row) {
-                        bdiag = b;
-                        continue;
+    std::atomic<bool> bad{false};
+    host_parallel(t.nslices, 64, [&](int64_t s0, int64_t s1) {
+        for (int sl = (int)s0; sl < (int)s1; ++sl) {
+            const int r0 = sl * C, r1 = std::min(N, r0 + C);
+            for (int l = 0; l < r1 - r0; ++l) {
+                const int row = t.srow[r0 + l];
+                int k = 0, bdiag = -1;
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int b = t.rowptr[row]; b < t.rowptr[row + 1]; ++b) {
+                        if (t.col[b] == row) {
+                            bdiag = b;
+                            continue;
+                        }
+                        if ((pass == 0) != (t.col[b] > row)) continue;
+                        const int e = t.slice_off[sl] + k * C + l;
+                        t.scol[e] = t.col[b];
+                        t.sellpos[b] = e;
+                        entry_block[e] = b;
+                        ++k;
+                        if (pass == 0) t.supper[r0 + l] = k;
                     }
-                    if ((pass == 0) != (t.col[b] > row)) continue;
-                    const int e = t.slice_off[sl] + k * C + l;
-                    t.scol[e] = t.col[b];
-                    t.sellpos[b] = e;
-                    entry_block[e] = b;
-                    ++k;
-                    if (pass == 0) t.supper[r0 + l] = k;
+                if (bdiag < 0 || k != t.slen[r0 + l] - 1) {
+                    bad = true;
+                    return;
                 }
-            if (bdiag < 0 || k != t.slen[r0 + l] - 1) return "row without its diagonal block";
-            const int e = t.slice_off[sl] + k * C + l;
-            t.scol[e] = row;
-            t.sellpos[bdiag] = e;
-            entry_block[e] = bdiag;
-            t.sdiag[row] = e;
-            // padding entries point at the row itself (never read as values)
-            for (int k = t.slen[r0 + l]; k < (t.slice_off[sl + 1] - t.slice_off[sl]) / C; ++k)
-                t.scol[t.slice_off[sl] + k * C + l] = row;
+                const int e = t.slice_off[sl] + k * C + l;
+                t.scol[e] = row;
+                t.sellpos[bdiag] = e;
+                entry_block[e] = bdiag;
+                t.sdiag[row] = e;
+                // padding entries point at the row itself (never read as values)
+                for (int k = t.slen[r0 + l]; k < (t.slice_off[sl + 1] - t.slice_off[sl]) / C; ++k)
+                    t.scol[t.slice_off[sl] + k * C + l] = row;
+            }
         }
+    });
+    if (bad) return "row without its diagonal block";
+    // contributor lists in SELL entry order (padding entries: empty), each in slot
+    // order; has_tri marks the blocks of the mass pattern
+    auto entry_of = [&](int row, int j) {
+        const int *b = t.col.data() + t.rowptr[row], *e = t.col.data() + t.rowptr[row + 1];
+        return t.sellpos[std::lower_bound(b, e, j) - t.col.data()];
+    };
+    t.cptr.assign(t.nbs + 1, 0);
+    host_parallel(N, 4096, [&](int64_t r0, int64_t r1) {
+        for (int row = (int)r0; row < (int)r1; ++row)
+            row_contributions(t, tri, hinge, row, [&](int j, int) { t.cptr[entry_of(row, j) + 1]++; });
+    });
+    if (!prefix_counts(t.cptr, t.nbs)) return "too many element contributions for 32-bit indices";
+    t.clist.assign(t.cptr[t.nbs], 0);
+    t.has_tri.assign(t.nb, 0);
+    {
+        std::vector<int> fill(t.cptr.begin(), t.cptr.end() - 1);
+        host_parallel(N, 4096, [&](int64_t r0, int64_t r1) {
+            for (int row = (int)r0; row < (int)r1; ++row) {
+                row_contributions(t, tri, hinge, row, [&](int j, int code) {
+                    const int e = entry_of(row, j);
+                    t.clist[fill[e]++] = code;
+                });
+                for (int b = t.rowptr[row]; b < t.rowptr[row + 1]; ++b) {
+                    const int e = t.sellpos[b];
+                    int *L = t.clist.data() + t.cptr[e], n = t.cptr[e + 1] - t.cptr[e];
+                    for (int a = 1; a < n; ++a)   // insertion sort: lists are short and almost sorted
+                        for (int c = a; c > 0 && L[c - 1] > L[c]; --c) std::swap(L[c - 1], L[c]);
+                    for (int a = 0; a < n; ++a)
+                        if ((L[a] >> 1) < 6 * F) { t.has_tri[b] = 1; break; }
+                }
+            }
+        });
     }
     // Symmetric FP32 layout: each off-diagonal block stored once, with its upper
     // row.  Per slice, upper records (column + 9 values, 40 B) at
@@ -293,70 +398,54 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
     t.ucol.assign(t.nup, 0);
     t.lrec.assign(2 * (size_t)t.nlo, 0);
     std::vector<int> uposb(t.nb, -1);
-    for (int sl = 0; sl < t.nslices; ++sl) {
-        const int r0 = sl * C, r1 = std::min(N, r0 + C);
-        for (int l = 0; l < r1 - r0; ++l) {
-            const int row = t.srow[r0 + l];
-            const int wu = (t.uoff[sl + 1] - t.uoff[sl]) / C;
-            for (int k = 0; k < wu; ++k) {
-                const int u = t.uoff[sl] + k * C + l;
-                if (k < t.supper[r0 + l]) {
-                    const int e = t.slice_off[sl] + k * C + l;
-                    t.e2u[e] = u;
-                    t.ucol[u] = t.scol[e];
-                    uposb[entry_block[e]] = u;
-                } else {
-                    t.ucol[u] = row;   // padding
+    host_parallel(t.nslices, 64, [&](int64_t s0, int64_t s1) {
+        for (int sl = (int)s0; sl < (int)s1; ++sl) {
+            const int r0 = sl * C, r1 = std::min(N, r0 + C);
+            for (int l = 0; l < r1 - r0; ++l) {
+                const int row = t.srow[r0 + l];
+                const int wu = (t.uoff[sl + 1] - t.uoff[sl]) / C;
+                for (int k = 0; k < wu; ++k) {
+                    const int u = t.uoff[sl] + k * C + l;
+                    if (k < t.supper[r0 + l]) {
+                        const int e = t.slice_off[sl] + k * C + l;
+                        t.e2u[e] = u;
+                        t.ucol[u] = t.scol[e];
+                        uposb[entry_block[e]] = u;
+                    } else {
+                        t.ucol[u] = row;   // padding
+                    }
                 }
             }
         }
-    }
-    for (int sl = 0; sl < t.nslices; ++sl) {
-        const int r0 = sl * C, r1 = std::min(N, r0 + C);
-        for (int l = 0; l < r1 - r0; ++l) {
-            const int row = t.srow[r0 + l];
-            const int nl = t.slen[r0 + l] - 1 - t.supper[r0 + l];
-            const int wl = (t.loff[sl + 1] - t.loff[sl]) / C;
-            for (int k = 0; k < wl; ++k) {
-                const int lo = t.loff[sl] + k * C + l;
-                if (k < nl) {
-                    const int e = t.slice_off[sl] + (t.supper[r0 + l] + k) * C + l;
-                    const int j = t.scol[e];
-                    const int bt = find_block(t, j, row);
-                    if (bt < 0 || uposb[bt] < 0) return "block pattern is not symmetric";
-                    t.lrec[2 * (size_t)lo] = j;
-                    t.lrec[2 * (size_t)lo + 1] = uposb[bt];
-                } else {
-                    t.lrec[2 * (size_t)lo] = row;   // padding: never read
-                    t.lrec[2 * (size_t)lo + 1] = 0;
+    });
+    host_parallel(t.nslices, 64, [&](int64_t s0, int64_t s1) {
+        for (int sl = (int)s0; sl < (int)s1; ++sl) {
+            const int r0 = sl * C, r1 = std::min(N, r0 + C);
+            for (int l = 0; l < r1 - r0; ++l) {
+                const int row = t.srow[r0 + l];
+                const int nl = t.slen[r0 + l] - 1 - t.supper[r0 + l];
+                const int wl = (t.loff[sl + 1] - t.loff[sl]) / C;
+                for (int k = 0; k < wl; ++k) {
+                    const int lo = t.loff[sl] + k * C + l;
+                    if (k < nl) {
+                        const int e = t.slice_off[sl] + (t.supper[r0 + l] + k) * C + l;
+                        const int j = t.scol[e];
+                        const int bt = find_block(t, j, row);
+                        if (bt < 0 || uposb[bt] < 0) {
+                            bad = true;
+                            return;
+                        }
+                        t.lrec[2 * (size_t)lo] = j;
+                        t.lrec[2 * (size_t)lo + 1] = uposb[bt];
+                    } else {
+                        t.lrec[2 * (size_t)lo] = row;   // padding: never read
+                        t.lrec[2 * (size_t)lo + 1] = 0;
+                    }
                 }
             }
         }
-    }
-    // contributor lists in SELL entry order (padding entries: empty)
-    t.cptr.assign(t.nbs + 1, 0);
-    for (int64_t e = 0; e < t.nbs; ++e) {
-        const int b = entry_block[e];
-        t.cptr[e + 1] = t.cptr[e] + (b >= 0 ? bptr[b + 1] - bptr[b] : 0);
-    }
-    t.clist.assign(t.cptr[t.nbs], 0);
-    for (int64_t e = 0; e < t.nbs; ++e) {
-        const int b = entry_block[e];
-        if (b >= 0) std::copy(blist.begin() + bptr[b], blist.begin() + bptr[b + 1], t.clist.begin() + t.cptr[e]);
-    }
-    // vertex force slots: faces (3f+v), then hinges (3F + 4h + v)
-    t.fptr.assign(N + 1, 0);
-    for (int f = 0; f < F; ++f)
-        for (int v = 0; v < 3; ++v) t.fptr[tri[3 * f + v] + 1]++;
-    for (int h = 0; h < H; ++h)
-        for (int v = 0; v < 4; ++v) t.fptr[hinge[4 * h + v] + 1]++;
-    for (int i = 0; i < N; ++i) t.fptr[i + 1] += t.fptr[i];
-    t.flist.assign(t.fptr[N], 0);
-    std::vector<int> ff(t.fptr.begin(), t.fptr.end() - 1);
-    for (int f = 0; f < F; ++f)
-        for (int v = 0; v < 3; ++v) t.flist[ff[tri[3 * f + v]]++] = 3 * f + v;
-    for (int h = 0; h < H; ++h)
-        for (int v = 0; v < 4; ++v) t.flist[ff[hinge[4 * h + v]]++] = 3 * F + 4 * h + v;
+    });
+    if (bad) return "block pattern is not symmetric";
     return "";
 }
 
