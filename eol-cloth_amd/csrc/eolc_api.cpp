@@ -82,7 +82,7 @@ void free_topology_buffers(eolc_ctx *c)
     L.N = 0;
     L.nbs = 0;
     eolc::BatchState &S = c->batch;
-    dfree(S.d_inst); dfree(S.d_order); dfree(S.d_user_fixed); dfree(S.slice_part); dfree(S.stats); dfree(S.dstats);
+    dfree(S.d_inst); dfree(S.d_order); dfree(S.d_user_fixed); dfree(S.d_scripted); dfree(S.slice_part); dfree(S.stats); dfree(S.dstats);
     dfree(S.d_ml_lo);
     S.ml = S.graph_ml = false;
     if (S.graph) (void)hipGraphExecDestroy(S.graph);
@@ -872,8 +872,10 @@ int eolc_download_csr(eolc_ctx *c, int which, int64_t *rowptr, int *col, double 
 }
 
 // ---------------------------------------------------------------------------
+// scripted != 0: fixed_val holds scripted velocities and the fixed values are
+// (1 - 0.01) v_old + fixed_val, formed on the device (no host read of v_old).
 static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *fixed_val, double tol,
-                      int maxit, int mode, eolc_solve_stats *st)
+                      int maxit, int mode, eolc_solve_stats *st, int scripted = 0)
 {
     const int N = c->topo.N;
     const bool multilevel = (mode & EOLC_SOLVE_MULTILEVEL) != 0;
@@ -901,6 +903,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         EOLC_HIP(hipMalloc((void **)&d_vals, sizeof(double) * n_fixed));
         EOLC_HIP(hipMemcpyAsync(d_dofs, fixed_dof, sizeof(int) * n_fixed, hipMemcpyHostToDevice, c->stream));
         EOLC_HIP(hipMemcpyAsync(d_vals, fixed_val, sizeof(double) * n_fixed, hipMemcpyHostToDevice, c->stream));
+        if (scripted) EOLC_HIP(eolc::fixed_from_vold(c, n_fixed, d_dofs, d_vals));
         EOLC_HIP(eolc::set_fixed(c, n_fixed, d_dofs, d_vals, 1));
     }
     const int precond = (mode == EOLC_SOLVE_FP64_JACOBI) ? 1 : 0;
@@ -1124,19 +1127,19 @@ int eolc_precond_apply(eolc_ctx *c, int n_fixed, const int *fixed_dof, int mode,
     return EOLC_OK;
 }
 
-int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, const int *fixed_nodes,
-              const double *scripted_vel, double tol, int maxit, int mode, eolc_solve_stats *st)
+// One step with single-DOF fixed rows: v[dof] = (1 - 0.01) v_old[dof] + scripted.
+static int step_impl(eolc_ctx *c, const double grav[3], double h, int n_dofs, const int *dofs,
+                     const double *scripted, double tol, int maxit, int mode, eolc_solve_stats *st)
 {
-    if (!c) return EOLC_ERR_INVALID;
     if (!c->have_topo || !c->have_mat || !c->have_state)
         return fail(c, EOLC_ERR_INVALID, "step: topology, material and state must be set first");
-    if (!grav || !(h > 0.0) || n_fixed_nodes < 0 || (n_fixed_nodes && !fixed_nodes) || !(tol > 0.0) || maxit <= 0)
+    if (!grav || !(h > 0.0) || n_dofs < 0 || (n_dofs && !dofs) || !(tol > 0.0) || maxit <= 0)
         return fail(c, EOLC_ERR_INVALID, "step: bad arguments");
     if (!valid_mode(mode)) return fail(c, EOLC_ERR_INVALID, "step: unknown mode");
     EOLC_HIP(hipSetDevice(c->device));
     const int N = c->topo.N;
-    for (int i = 0; i < n_fixed_nodes; ++i)
-        if (fixed_nodes[i] < 0 || fixed_nodes[i] >= N) return fail(c, EOLC_ERR_INVALID, "step: fixed node out of range");
+    for (int i = 0; i < n_dofs; ++i)
+        if (dofs[i] < 0 || dofs[i] >= 3 * N) return fail(c, EOLC_ERR_INVALID, "step: fixed node or DOF out of range");
     hipEvent_t ev[5];
     for (auto &e : ev) EOLC_HIP(hipEventCreate(&e));
     EOLC_HIP(hipEventRecord(ev[0], c->stream));
@@ -1146,24 +1149,14 @@ int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, co
     EOLC_HIP(hipEventRecord(ev[1], c->stream));
     EOLC_HIP(eolc::launch_rhs(c, h));
     EOLC_HIP(hipEventRecord(ev[2], c->stream));
-    // v_fixed = (1 - 0.01) v_old + scripted (Constraints.cpp:114-119)
-    std::vector<int> dofs(3 * (size_t)n_fixed_nodes);
-    std::vector<double> vals(3 * (size_t)n_fixed_nodes);
-    if (n_fixed_nodes > 0) {
-        std::vector<double> vold(3 * (size_t)n_fixed_nodes);
-        for (int i = 0; i < n_fixed_nodes; ++i)
-            EOLC_HIP(hipMemcpyAsync(&vold[3 * i], c->v + 3 * (size_t)fixed_nodes[i], 3 * sizeof(double),
-                                    hipMemcpyDeviceToHost, c->stream));
-        EOLC_HIP(hipStreamSynchronize(c->stream));
-        for (int i = 0; i < n_fixed_nodes; ++i)
-            for (int k = 0; k < 3; ++k) {
-                dofs[3 * i + k] = 3 * fixed_nodes[i] + k;
-                vals[3 * i + k] = (1 - 0.01) * vold[3 * i + k] + (scripted_vel ? scripted_vel[3 * i + k] : 0.0);
-            }
-    }
+    // v_fixed = (1 - 0.01) v_old + scripted (Constraints.cpp:114-119), formed on the
+    // device from the resident v_old
+    std::vector<double> vals((size_t)n_dofs, 0.0);
+    if (scripted)
+        for (int i = 0; i < n_dofs; ++i) vals[i] = scripted[i];
     eolc_solve_stats local;
     std::memset(&local, 0, sizeof(local));
-    rc = solve_impl(c, 3 * n_fixed_nodes, dofs.data(), vals.data(), tol, maxit, mode, &local);
+    rc = solve_impl(c, n_dofs, dofs, vals.data(), tol, maxit, mode, &local, 1);
     EOLC_HIP(hipEventRecord(ev[3], c->stream));
     if (rc == EOLC_OK) EOLC_HIP(eolc::launch_update_x(c, h));
     EOLC_HIP(hipEventRecord(ev[4], c->stream));
@@ -1175,6 +1168,27 @@ int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, co
     for (auto &e : ev) (void)hipEventDestroy(e);
     if (st) *st = local;
     return rc;
+}
+
+int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, const int *fixed_nodes,
+              const double *scripted_vel, double tol, int maxit, int mode, eolc_solve_stats *st)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (n_fixed_nodes < 0 || (n_fixed_nodes && !fixed_nodes)) return fail(c, EOLC_ERR_INVALID, "step: bad arguments");
+    std::vector<int> dofs(3 * (size_t)n_fixed_nodes);
+    for (int i = 0; i < n_fixed_nodes; ++i) {
+        if (fixed_nodes[i] < 0 || fixed_nodes[i] > INT_MAX / 3 - 1)
+            return fail(c, EOLC_ERR_INVALID, "step: fixed node out of range");
+        for (int k = 0; k < 3; ++k) dofs[3 * i + k] = 3 * fixed_nodes[i] + k;
+    }
+    return step_impl(c, grav, h, 3 * n_fixed_nodes, dofs.data(), scripted_vel, tol, maxit, mode, st);
+}
+
+int eolc_step_dofs(eolc_ctx *c, const double grav[3], double h, int n_fixed_dofs, const int *fixed_dofs,
+                   const double *scripted_vel, double tol, int maxit, int mode, eolc_solve_stats *st)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    return step_impl(c, grav, h, n_fixed_dofs, fixed_dofs, scripted_vel, tol, maxit, mode, st);
 }
 
 // ---- learned block preconditioner ------------------------------------------
@@ -1371,6 +1385,8 @@ int eolc_set_batch_topology(eolc_ctx *c, int B, const int *Nb, const double *X, 
     EOLC_HIP(hipMemcpy(S.d_inst, S.inst.data(), sizeof(eolc::BatchInst) * B, hipMemcpyHostToDevice));
     EOLC_HIP(upload(c, &S.d_order, S.order));
     EOLC_HIP(upload(c, &S.d_user_fixed, S.user_fixed));
+    EOLC_HIP(dalloc(c, &S.d_scripted, 3 * S.user_fixed.size()));
+    EOLC_HIP(hipMemset(S.d_scripted, 0, sizeof(double) * 3 * std::max<size_t>(S.user_fixed.size(), 1)));
     EOLC_HIP(dalloc(c, &S.slice_part, 4 * (size_t)c->topo.nslices));
     EOLC_HIP(dalloc(c, &S.stats, 4 * (size_t)B));
     EOLC_HIP(dalloc(c, &S.dstats, 4 * (size_t)B));
@@ -1564,6 +1580,27 @@ int eolc_batch_autotune(eolc_ctx *c, const double grav[3], double h, double tol,
     batch_invalidate_graph(c);
     c->have_forces = false;
     if (best) *best = arg;
+    return EOLC_OK;
+}
+
+int eolc_set_batch_scripted(eolc_ctx *c, const double *vel)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    eolc::BatchState &S = c->batch;
+    if (S.B <= 0) return fail(c, EOLC_ERR_INVALID, "set_batch_scripted: no batch topology");
+    const size_t n = 3 * S.user_fixed.size();
+    std::vector<double> h(n, 0.0);
+    if (vel)
+        for (size_t i = 0; i < n; ++i) {
+            if (!std::isfinite(vel[i])) return fail(c, EOLC_ERR_INVALID, "set_batch_scripted: non-finite velocity");
+            h[i] = vel[i];
+        }
+    if (n == 0) return EOLC_OK;
+    EOLC_HIP(hipSetDevice(c->device));
+    // ordered after every step already enqueued (eager or replayed graph); the
+    // captured step reads the buffer at each replay, so no re-capture is needed
+    EOLC_HIP(hipMemcpyAsync(S.d_scripted, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
 }
 

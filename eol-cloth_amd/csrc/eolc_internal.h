@@ -115,10 +115,11 @@ struct BatchState {
     std::vector<BatchInst> inst;
     std::vector<int> order;          // launch order: largest instance first
     std::vector<int> fixed_nodes;    // padded vertex indices with all DOFs fixed (incl. padding)
-    std::vector<int> user_fixed;     // the caller's fixed nodes (scripted: v = 0.99 v_old)
+    std::vector<int> user_fixed;     // the caller's fixed nodes (v = 0.99 v_old + scripted)
     BatchInst *d_inst = nullptr;
     int *d_order = nullptr;
     int *d_user_fixed = nullptr;
+    double *d_scripted = nullptr;    // 3 per user-fixed node: scripted velocity (zero by default)
     double *slice_part = nullptr;    // per-slice FP64 partial sums (4 x nslices)
     int *stats = nullptr;            // per instance: iters, refinements, status, spare
     double *dstats = nullptr;        // per instance: bnorm2, true_rr, relres, spare
@@ -291,6 +292,8 @@ hipError_t pcg_finish(eolc_ctx *c, bool mixed);
 hipError_t spmv_true_residual(eolc_ctx *c);
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
+// d_vals[t] = (1 - 0.01) v[d_dofs[t]] + d_vals[t] on the context's stream.
+hipError_t fixed_from_vold(eolc_ctx *c, int n, const int *d_dofs, double *d_vals);
 constexpr int kSpmvVariants = 18;
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms);
 hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms, int read_only);

@@ -472,14 +472,15 @@ __global__ void __launch_bounds__(WGS) k_batch_pcg(BatchArgs a)
     }
 }
 
-// Fixed values of the scripted nodes: v_fixed = (1 - 0.01) v_old (all three DOFs).
-__global__ void k_batch_vfix(int n, const int *__restrict__ nodes, const double *__restrict__ v,
-                             double *__restrict__ vfix)
+// Fixed values of the caller's fixed nodes (all three DOFs), Constraints.cpp:114-119:
+// v_fixed = (1 - 0.01) v_old + scripted velocity.
+__global__ void k_batch_vfix(int n, const int *__restrict__ nodes, const double *__restrict__ scripted,
+                             const double *__restrict__ v, double *__restrict__ vfix)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 3 * n) return;
     const int i = nodes[t / 3], c = t % 3;
-    vfix[3 * i + c] = (1 - 0.01) * v[3 * i + c];
+    vfix[3 * i + c] = (1 - 0.01) * v[3 * i + c] + scripted[t];
 }
 
 // All DOFs of the listed nodes fixed (set once per batch topology).
@@ -517,8 +518,8 @@ hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant)
     const Topology &t = c->topo;
     const int nu = (int)S.user_fixed.size();
     if (nu > 0)
-        hipLaunchKernelGGL(k_batch_vfix, dim3(blocks_for(3 * nu)), dim3(256), 0, c->stream, nu, S.d_user_fixed, c->v,
-                           c->vfix);
+        hipLaunchKernelGGL(k_batch_vfix, dim3(blocks_for(3 * nu)), dim3(256), 0, c->stream, nu, S.d_user_fixed, S.d_scripted,
+                           c->v, c->vfix);
     const bool ml = S.ml && S.d_ml_lo && c->ml.lv.L > 0;
     c->ml_on = ml;   // pcg_setup then also forms the coarse levels' blocks
     hipError_t e = pcg_setup(c, true, 0);

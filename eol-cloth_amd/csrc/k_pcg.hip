@@ -515,6 +515,16 @@ k_block_jacobi(int N, const int *__restrict__ diagpos, const double *__restrict_
     }
 }
 
+// Fixed values of Constraints.cpp:114-119 from the device-resident velocity:
+// vals[t] = (1 - 0.01) v_old[dofs[t]] + vals[t], vals holding the scripted velocity.
+__global__ void k_fixed_from_vold(int n, const int *__restrict__ dofs, const double *__restrict__ v,
+                                  double *__restrict__ vals)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    vals[t] = (1 - 0.01) * v[dofs[t]] + vals[t];
+}
+
 __global__ void k_set_fixed(int n, const int *__restrict__ dofs, const double *__restrict__ vals,
                             unsigned char *__restrict__ fixmask, double *__restrict__ vfix, int set)
 {
@@ -746,6 +756,13 @@ hipError_t pcg_setup(eolc_ctx *c, bool /*mixed*/, int precond_kind)
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && c->ml_on) e = ml_setup(c);
     return e;
+}
+
+hipError_t fixed_from_vold(eolc_ctx *c, int n, const int *d_dofs, double *d_vals)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fixed_from_vold, dim3(blocks_for(n)), dim3(256), 0, c->stream, n, d_dofs, c->v, d_vals);
+    return hipGetLastError();
 }
 
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set)

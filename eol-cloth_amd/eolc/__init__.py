@@ -34,7 +34,8 @@ EXPORTED_SYMBOLS = (
     "eolc_set_batch_topology", "eolc_set_batch_material", "eolc_batch_step", "eolc_batch_autotune",
     "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta", "eolc_collision_constraints",
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
-    "eolc_velocity_transfer", "eolc_stream_read",
+    "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
+    "eolc_rigid_integrate", "eolc_box_step",
 )
 
 BATCH_EAGER = 0x1
@@ -112,6 +113,11 @@ def lib():
                                           C.c_void_p]
         L.eolc_set_batch_variant.argtypes = [vp, C.c_int]
         L.eolc_set_batch_multilevel.argtypes = [vp, C.c_int]
+        L.eolc_set_batch_scripted.argtypes = [vp, C.c_void_p]
+        L.eolc_step_dofs.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                     C.c_int, C.c_int, C.POINTER(SolveStats)]
+        L.eolc_rigid_integrate.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+        L.eolc_box_step.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
         L.eolc_velocity_transfer.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _ip, _ip, _dp,
                                              C.POINTER(C.c_double)]
         L.eolc_set_mixed_delta.argtypes = [vp, C.c_double]
@@ -162,6 +168,31 @@ def ml_partition(rowptr, col):
     if rc != OK:
         raise EolcError(rc, "ml_partition: bad pattern")
     return [agg[l * N:(l + 1) * N].copy() for l in range(L.value)]
+
+
+def _frame(E):
+    """4x4 frame (array-like, row-major as numpy indexes it) -> Eigen's column-major 16."""
+    return np.ascontiguousarray(np.asarray(E, dtype=np.float64).reshape(4, 4).T).reshape(-1)
+
+
+def rigid_integrate(E0, phi, h):
+    """Rigid::integrate on the host: E0 (4x4) times the exponential of h * phi^,
+    phi = (w, v) a body-frame twist.  Returns the 4x4 frame."""
+    out = np.zeros(16)
+    rc = lib().eolc_rigid_integrate(_ptr(_frame(E0)), _ptr(_d(phi)), C.c_double(h), _ptr(out))
+    if rc != OK:
+        raise EolcError(rc, "rigid_integrate: bad arguments")
+    return out.reshape(4, 4).T.copy()
+
+
+def box_step(E, vel, h):
+    """Box::step on the host: the frame (4x4) after one step with Box::v = vel
+    (angular, linear)."""
+    out = np.zeros(16)
+    rc = lib().eolc_box_step(_ptr(_frame(E)), _ptr(_d(vel)), C.c_double(h), _ptr(out))
+    if rc != OK:
+        raise EolcError(rc, "box_step: bad arguments")
+    return out.reshape(4, 4).T.copy()
 
 
 class Context:
@@ -258,6 +289,17 @@ class Context:
         st = SolveStats()
         rc = lib().eolc_step(self._h, _d(grav), float(h), fn.size, _ptr(fn) if fn.size else None,
                              _ptr(sc), float(tol), int(maxit), int(mode), C.byref(st))
+        self._check(rc, allow=allow)
+        return rc, st
+
+    def step_dofs(self, grav, h, fixed_dofs=(), scripted=None, tol=1e-8, maxit=100000,
+                  mode=SOLVE_FP64_BJ, allow=()):
+        """eolc_step with single-DOF fixed rows (scripted: one velocity per DOF)."""
+        fd = _i(fixed_dofs)
+        sc = None if scripted is None else _d(scripted)
+        st = SolveStats()
+        rc = lib().eolc_step_dofs(self._h, _d(grav), float(h), fd.size, _ptr(fd) if fd.size else None,
+                                  _ptr(sc), float(tol), int(maxit), int(mode), C.byref(st))
         self._check(rc, allow=allow)
         return rc, st
 
@@ -383,6 +425,12 @@ class Context:
         self._check(lib().eolc_batch_autotune(self._h, _d(grav), float(h), float(tol), int(maxit), C.byref(best),
                                               _ptr(ms)))
         return best.value, ms
+
+    def set_batch_scripted(self, vel=None):
+        """Scripted velocities of the batch's fixed nodes (3 per node, the order of
+        set_batch_topology's fixed lists); None resets them to zero."""
+        v = None if vel is None else _d(vel)
+        return self._check(lib().eolc_set_batch_scripted(self._h, _ptr(v)))
 
     def set_batch_multilevel(self, on=True):
         """Multilevel coarse levels in the batched persistent solve."""

@@ -2,12 +2,20 @@
 // reference (Cloth.cpp:359-413) for a square n x n cloth with its two y = 1
 // corners fixed, remeshing and EOL off, solver "none".
 //
-//   cloth_step <n> <steps> <out.bin>
+//   cloth_step <n> <steps> <out.bin> [scene]
 //
 // Writes the final positions (3N doubles) to out.bin and prints one summary line.
+// With "scene", the steps run through eolc::Scene (Scene.cpp:72-98) instead: the
+// two corners start fixed and, from t > 0.0475 s, move with scripted velocities
+// (one corner x = 0.1 m/s on all axes, the other only its z axis scripted at
+// 0.05 m/s, as two FixedLists); a box rises and turns under the cloth and two
+// static points sit near it, with CD2 on every step.  out.bin then also holds the
+// box frame (16 doubles, column-major) after the positions, and out.bin.cd the
+// CD2 records (eolc_collision) of the final state.
 #include <cstdio>
 #include <cstdlib>
 #include <map>
+#include <string>
 #include <utility>
 
 #include "eolc_dropin.hpp"
@@ -66,15 +74,78 @@ void hinges(eolc::MeshArrays &m)
     }
 }
 
+int run_scene(int n, int steps, const char *out)
+{
+    try {
+        eolc::MeshArrays mesh;
+        grid(n, mesh);
+        hinges(mesh);
+        eolc::Device dev(0);
+        dev.setTopology(mesh);
+        dev.check(eolc_set_material(dev.ctx(), 0.1, 500.0, 0.3, 1e-5, 0.0, 1.0));
+        dev.check(eolc_upload_state(dev.ctx(), mesh.x.data(), mesh.v.data()));
+        eolc::Scene s(dev);
+        s.tol = 1e-12;
+        s.threshold = 5e-3;
+        const int cA = n * (n + 1), cB = (n + 1) * (n + 1) - 1;   // the y = 1 corners
+        eolc::FixedList f0, f1;
+        f0.ci[0] = f1.ci[0] = cA;
+        f0.ci[1] = f1.ci[1] = cB;
+        for (int a = 0; a < 3; ++a) f0.c[0][a] = f0.c[1][a] = f1.c[0][a] = 1.0;
+        for (int a = 3; a < 6; ++a) f0.c[0][a] = f0.c[1][a] = f1.c[0][a] = 0.0;
+        f1.when = 0.0475;
+        f1.c[0][3] = 0.1;                                         // corner A: x scripted
+        const double cBspec[6] = {0.0, 0.0, 1.0, 0.0, 0.0, 0.05};  // corner B: z only
+        for (int a = 0; a < 6; ++a) f1.c[1][a] = cBspec[a];
+        s.fs = {f0, f1};
+        eolc::Box b;
+        b.dim[0] = 0.4; b.dim[1] = 0.3; b.dim[2] = 0.1;
+        b.E[12] = 0.5; b.E[13] = 0.45; b.E[14] = -0.08;           // centre
+        const double bv[6] = {0.0, 0.0, 2.0, 0.0, 0.0, 0.4};      // turns about z, rises
+        for (int k = 0; k < 6; ++k) b.v[k] = bv[k];
+        s.boxes = {b};
+        s.pts = {0.3, 0.3, -0.004, 0.7, 0.6, 0.003};
+        s.nrm = {0.0, 0.0, 1.0, 0.0, 0.0, -1.0};
+        int iters = 0, switched = -1;
+        size_t records = 0;
+        for (int k = 0; k < steps; ++k) {
+            const size_t before = s.fsindex;
+            s.step();
+            if (s.fsindex != before) switched = k;
+            iters += s.stats.iters;
+            records += s.cls.size();
+        }
+        s.detect();   // the CD2 of the final state (what the next step would hand the QP)
+        std::vector<double> x(mesh.x.size()), v(mesh.x.size());
+        dev.check(eolc_download_state(dev.ctx(), x.data(), v.data()));
+        FILE *fp = std::fopen(out, "wb");
+        if (!fp) return 1;
+        std::fwrite(x.data(), sizeof(double), x.size(), fp);
+        std::fwrite(s.boxes[0].E, sizeof(double), 16, fp);
+        std::fclose(fp);
+        fp = std::fopen((std::string(out) + ".cd").c_str(), "wb");
+        if (!fp) return 1;
+        std::fwrite(s.cls.data(), sizeof(eolc_collision), s.cls.size(), fp);
+        std::fclose(fp);
+        std::printf("cloth_step scene n=%d steps=%d switch_step=%d iterations=%d records_per_run=%zu final_records=%zu\n",
+                    n, steps, switched, iters, records, s.cls.size());
+    } catch (const eolc::Error &e) {
+        std::fprintf(stderr, "eolc error %d: %s\n", e.code, e.what());
+        return 1;
+    }
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     if (argc < 4) {
-        std::fprintf(stderr, "usage: cloth_step <n> <steps> <out.bin>\n");
+        std::fprintf(stderr, "usage: cloth_step <n> <steps> <out.bin> [scene]\n");
         return 2;
     }
     const int n = std::atoi(argv[1]), steps = std::atoi(argv[2]);
+    if (argc > 4 && std::string(argv[4]) == "scene") return run_scene(n, steps, argv[3]);
     try {
         eolc::MeshArrays mesh;
         grid(n, mesh);

@@ -5,6 +5,8 @@
 //   eolc::GeneralizedSolver  GeneralizedSolver.h:12-32 -- velocitySolve(...) for solver "none"
 //   eolc::CD / eolc::CD2     Collisions.h:9-11 -- proximity queries into btc::Collision-like records
 //   eolc::fillConstraints    Constraints.cpp:114-119, 417-514 -- A_ineq rows from the records, fixed rows
+//   eolc::FixedList, eolc::Box, eolc::Scene  FixedList.h, Box.cpp:103-114, Scene.cpp:72-98 --
+//                            the scene step on the device-resident state (scripted corners, obstacle motion)
 //
 // The reference's types (ArcSim Mesh, Eigen vectors and sparse matrices) are not
 // available here, so the mesh is passed as the flat arrays those types hold
@@ -205,5 +207,108 @@ inline QpConstraints fillConstraints(const std::vector<eolc_collision> &cls, int
         }
     return q;
 }
+
+// FixedList (FixedList.h) as parseParams fills it (parseParams.cpp:214-250, 295-304):
+// from time `when` on, corner k (node ci[k]) is unused when c[k][0] == -1 ("nothing");
+// otherwise axis a is fixed when c[k][a] == 1, with scripted velocity c[k][a + 3].
+struct FixedList {
+    double when = 0.0;
+    int ci[4] = {0, 0, 0, 0};
+    double c[4][6] = {{-1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1},
+                      {-1, -1, -1, -1, -1, -1}};
+};
+
+// The single-DOF equality rows of a FixedList in Constraints::fill's order
+// (Constraints.cpp:472-496: corners 1..4, axes x, y, z): DOF and scripted velocity.
+inline void fixedRows(const FixedList &fs, std::vector<int> &dofs, std::vector<double> &scripted)
+{
+    dofs.clear();
+    scripted.clear();
+    for (int k = 0; k < 4; ++k) {
+        if (fs.c[k][0] == -1) continue;
+        for (int a = 0; a < 3; ++a)
+            if (fs.c[k][a] == 1.0) {
+                dofs.push_back(3 * fs.ci[k] + a);
+                scripted.push_back(fs.c[k][a + 3]);
+            }
+    }
+}
+
+// Box (Box.h): frame E1 (4x4, column-major), dimensions, velocity (angular, linear)
+// as Box::v holds it.  step(h) is Box::step (Box.cpp:103-114).
+struct Box {
+    double dim[3] = {1.0, 1.0, 1.0};
+    double E[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    void step(double h)
+    {
+        const int rc = eolc_box_step(E, v, h, E);
+        if (rc != EOLC_OK) throw Error(rc, "Box::step: non-finite frame or velocity");
+    }
+};
+
+// Scene::step (Scene.cpp:72-98) with remeshing and EOL off and solver "none", on the
+// device-resident state (set it with Device + eolc_upload_state first):
+//   Cloth::updateFix(t)            Cloth.cpp:415-420 (at most one list switch per step)
+//   CD2 -> cls                     Constraints::fill's records for the host QP
+//                                  (solver "none" does not read them)
+//   Cloth::step                    forces, rhs, solve with the current FixedList's
+//                                  rows, x += h v (eolc_step_dofs)
+//   Obstacles::step(h)             every box moves; points are static
+//   t += h
+class Scene {
+public:
+    explicit Scene(Device &d) : dev_(d) {}
+    double t = 0.0, h = 5e-3;
+    double grav[3] = {0.0, 0.0, -9.8};
+    std::vector<FixedList> fs{FixedList{}};
+    size_t fsindex = 0;
+    double threshold = 5e-3;          // Obstacles::cdthreshold
+    std::vector<double> pts, nrm;     // static points (3P each)
+    std::vector<Box> boxes;
+    bool collisions = true;
+    int mode = EOLC_SOLVE_FP64_BJ;
+    double tol = 1e-8;
+    int maxit = 100000;
+    std::vector<eolc_collision> cls;  // records of the last step's CD2
+    eolc_solve_stats stats{};
+
+    void updateFix()
+    {
+        if (fsindex + 1 >= fs.size()) return;
+        if (t > fs[fsindex + 1].when) fsindex++;
+    }
+
+    void detect()
+    {
+        cls.clear();
+        if (pts.empty() && boxes.empty()) return;
+        Obstacles o;
+        o.threshold = threshold;
+        o.pts = pts;
+        o.nrm = nrm;
+        for (const Box &b : boxes) {
+            o.whd.insert(o.whd.end(), b.dim, b.dim + 3);
+            o.E.insert(o.E.end(), b.E, b.E + 16);
+        }
+        CD2(dev_, o, cls);
+    }
+
+    void step()
+    {
+        updateFix();
+        if (collisions) detect();
+        std::vector<int> dofs;
+        std::vector<double> scripted;
+        fixedRows(fs[fsindex], dofs, scripted);
+        dev_.check(eolc_step_dofs(dev_.ctx(), grav, h, (int)dofs.size(), dofs.data(), scripted.data(), tol, maxit,
+                                  mode, &stats));
+        for (Box &b : boxes) b.step(h);
+        t += h;
+    }
+
+private:
+    Device &dev_;
+};
 
 }  // namespace eolc

@@ -178,6 +178,28 @@ int eolc_step(eolc_ctx *ctx, const double grav[3], double h, int n_fixed_nodes,
               const int *fixed_nodes, const double *scripted_vel /* 3 per node or NULL */,
               double tol, int maxit, int mode, eolc_solve_stats *stats);
 
+/* The same step with single-DOF fixed rows, as Constraints::fill adds them per
+ * axis from a FixedList (Constraints.cpp:472-496: corner k, axis a fixed when
+ * c_k(a) == 1, scripted velocity c_k(a + 3)): v[fixed_dofs[i]] =
+ * 0.99 v_old[fixed_dofs[i]] + scripted_vel[i] (scripted_vel: one per DOF or NULL).
+ * The fixed values are formed on the device from the resident v_old. */
+int eolc_step_dofs(eolc_ctx *ctx, const double grav[3], double h, int n_fixed_dofs,
+                   const int *fixed_dofs, const double *scripted_vel, double tol, int maxit,
+                   int mode, eolc_solve_stats *stats);
+
+/* ---- obstacle motion (Obstacles::step) ---------------------------------------
+ * Host-only: ctx-free, no device work.  Frames are 4x4 homogeneous transforms,
+ * column-major as Eigen stores Matrix4d.
+ * eolc_rigid_integrate: Rigid::integrate (Rigid.cpp:79-112), E1 = E0 exp(h phi^)
+ *   with phi = (w, v) a body-frame twist (Rodrigues closed form; a pure
+ *   translation h v when |w| <= 1e-10).  E1 may alias E0.
+ * eolc_box_step: Box::step (Box.cpp:103-114), the box frame after one step of
+ *   length h with velocity vel = (angular, linear) as Box::v holds it: phi is
+ *   vel mapped by blockdiag(R^T, R^T), R the frame's rotation.  E_new may alias E.
+ * Both return EOLC_ERR_INVALID for null or non-finite arguments. */
+int eolc_rigid_integrate(const double E0[16], const double phi[6], double h, double E1[16]);
+int eolc_box_step(const double E[16], const double vel[6], double h, double E_new[16]);
+
 /* ---- CD2 / CD ------------------------------------------------------------------
  * Proximity queries of the device-resident cloth positions against P points
  * (pts, nrm: 3P) and B boxes (whd: 3B; E: 16B, each a column-major 4x4 frame as
@@ -238,11 +260,12 @@ int eolc_learned_train(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, const d
 /* ---- batched cloths (no counterpart in the reference) ---------------------------
  * B independent cloths in one context, solved together.  Instance b has Nb[b]
  * vertices, Fb[b] faces, Hb[b] hinges and nfixb[b] fixed nodes (all DOFs fixed,
- * v = 0.99 v_old as in eolc_step); X, tri, hinge and fixed are the instances'
- * arrays one after the other with instance-local vertex indices.  Each instance
- * is padded to a multiple of 64 vertices with isolated fixed vertices: state
- * arrays (eolc_upload_state / eolc_download_state) use the padded numbering,
- * instance b at vertices [vert_off[b], vert_off[b+1]) (vert_off: B+1 outputs).
+ * v = 0.99 v_old + scripted velocity as in eolc_step, the scripted part set by
+ * eolc_set_batch_scripted); X, tri, hinge and fixed are the instances' arrays one
+ * after the other with instance-local vertex indices.  Each instance is padded
+ * to a multiple of 64 vertices with isolated fixed vertices: state arrays
+ * (eolc_upload_state / eolc_download_state) use the padded numbering, instance b
+ * at vertices [vert_off[b], vert_off[b+1]) (vert_off: B+1 outputs).
  * Replaces eolc_set_topology; density, Poisson ratio and damping come from
  * eolc_set_material, Young's modulus and bending stiffness per instance from
  * eolc_set_batch_material. */
@@ -262,6 +285,14 @@ int eolc_set_batch_material(eolc_ctx *ctx, const double *E, const double *beta);
 #define EOLC_BATCH_EAGER 0x1
 int eolc_batch_step(eolc_ctx *ctx, const double grav[3], double h, double tol, int maxit, int flags,
                     eolc_solve_stats *stats);
+
+/* Scripted velocities of the batch's fixed nodes (Cloth::updateFix / FixedList,
+ * Constraints.cpp:114-119): vel holds 3 values per node of the `fixed` arrays of
+ * eolc_set_batch_topology, in that order (NULL: all zero, the default).  Later
+ * steps fix v = 0.99 v_old + vel on those nodes.  The values live in a device
+ * buffer that the captured step reads on every replay: changing them at a
+ * schedule switch costs one small copy and no re-capture. */
+int eolc_set_batch_scripted(eolc_ctx *ctx, const double *vel);
 
 /* Multilevel coarse levels (EOLC_SOLVE_MULTILEVEL's hierarchy, which never spans two
  * cloths) inside the batched persistent solve: on != 0 adds them to every instance's
