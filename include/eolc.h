@@ -173,7 +173,8 @@ int eolc_velocity_solve(eolc_ctx *ctx, int n_fixed, const int *fixed_dof,
 
 /* ---- Cloth::step (Cloth.cpp:359-413, no remesh, no EOL) ----------------------
  * forces_fill, rhs, velocity solve with fixed nodes
- * (v_fixed = 0.99 v_old + scripted velocity), then x += h v. */
+ * (v_fixed = 0.99 v_old + scripted velocity, formed on the device from the
+ * resident v_old), then x += h v. */
 int eolc_step(eolc_ctx *ctx, const double grav[3], double h, int n_fixed_nodes,
               const int *fixed_nodes, const double *scripted_vel /* 3 per node or NULL */,
               double tol, int maxit, int mode, eolc_solve_stats *stats);
