@@ -74,6 +74,14 @@ int orc_forces_fill(int N, const double *x, const double *X,
                     const orc_material *mat, const double *grav, double h, int flags,
                     double *f, orc_csr *M, orc_csr *MDK, int nthreads);
 
+/* Forces::fill with EOL nodes (Forces.cpp:177-329, 399-497, 580-683, 746-884):
+ * eol_nodes[k] has EoL_index k and two more DOFs at 3N + 2k; f has 3N + 2 n_eol
+ * entries and M, MDK that many rows.  Returns 1 for a bad or repeated node. */
+int orc_forces_fill_eol(int N, const double *x, const double *X,
+                        int F, const int *tri, int H, const int *hinge,
+                        const orc_material *mat, const double *grav, double h, int flags,
+                        int n_eol, const int *eol_nodes, double *f, orc_csr *M, orc_csr *MDK, int nthreads);
+
 void orc_csr_matvec(const orc_csr *A, const double *x, double *y, int nthreads);
 
 /* ---- solve (orc_solve.c) ------------------------------------------------- */

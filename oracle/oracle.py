@@ -70,6 +70,9 @@ def lib():
         L.orc_forces_fill.argtypes = [C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _ip,
                                       C.POINTER(Material), _dp, C.c_double, C.c_int, _dp,
                                       C.POINTER(Csr), C.POINTER(Csr), C.c_int]
+        L.orc_forces_fill_eol.argtypes = [C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _ip,
+                                          C.POINTER(Material), _dp, C.c_double, C.c_int, C.c_int, _ip, _dp,
+                                          C.POINTER(Csr), C.POINTER(Csr), C.c_int]
         L.orc_csr_free.argtypes = [C.POINTER(Csr)]
         L.orc_csr_matvec.argtypes = [C.POINTER(Csr), _dp, _dp, C.c_int]
         L.orc_velocity_solve.argtypes = [C.POINTER(Csr), _dp, C.c_int, _ip, _dp, C.c_void_p,
@@ -193,6 +196,25 @@ def forces_fill(x, X, tri, hinge, mat, grav, h, flags=0, nthreads=1):
     rc = lib().orc_forces_fill(N, x, X, tri.size // 3, tri, hinge.size // 4, hinge, C.byref(mat),
                                _d(grav), float(h), int(flags), f, C.byref(M), C.byref(MDK),
                                int(nthreads))
+    Ms, MDKs = csr_to_scipy(M), csr_to_scipy(MDK)
+    lib().orc_csr_free(C.byref(M))
+    lib().orc_csr_free(C.byref(MDK))
+    return rc, f, Ms, MDKs
+
+
+def forces_fill_eol(x, X, tri, hinge, mat, grav, h, eol_nodes, flags=0, nthreads=1):
+    """Forces::fill with EOL nodes: f (3N + 2E), M and MDK ((3N + 2E) square)."""
+    x, X = _d(x).reshape(-1), _d(X).reshape(-1)
+    tri, hinge = _i(tri).reshape(-1), _i(hinge).reshape(-1)
+    en = _i(eol_nodes).reshape(-1)
+    N = x.size // 3
+    f = np.zeros(3 * N + 2 * en.size)
+    M, MDK = Csr(), Csr()
+    rc = lib().orc_forces_fill_eol(N, x, X, tri.size // 3, tri, hinge.size // 4, hinge, C.byref(mat),
+                                   _d(grav), float(h), int(flags), en.size, en, f, C.byref(M), C.byref(MDK),
+                                   int(nthreads))
+    if rc == 1:
+        return rc, None, None, None
     Ms, MDKs = csr_to_scipy(M), csr_to_scipy(MDK)
     lib().orc_csr_free(C.byref(M))
     lib().orc_csr_free(C.byref(MDK))
