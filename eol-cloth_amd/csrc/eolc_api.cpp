@@ -57,6 +57,15 @@ template <class T> void dfree(T *&p)
     p = nullptr;
 }
 
+void free_eol(eolc_ctx *c)
+{
+    eolc::EolState &S = c->eol;
+    dfree(S.d_elem); dfree(S.d_eidx); dfree(S.d_soff); dfree(S.d_foff); dfree(S.d_cptr); dfree(S.d_clist);
+    dfree(S.d_fptr); dfree(S.d_flist); dfree(S.d_xv); dfree(S.d_xvptr); dfree(S.d_xvslot);
+    dfree(S.d_stK); dfree(S.d_stM); dfree(S.d_stF); dfree(S.d_A); dfree(S.d_M); dfree(S.d_fX);
+    S = eolc::EolState{};
+}
+
 void free_topology_buffers(eolc_ctx *c)
 {
     dfree(c->X); dfree(c->tri); dfree(c->hinge);
@@ -90,6 +99,7 @@ void free_topology_buffers(eolc_ctx *c)
     S.B = 0;
     S.inst.clear(); S.order.clear(); S.fixed_nodes.clear(); S.user_fixed.clear();
     dfree(c->e_face); dfree(c->beta_hinge);
+    free_eol(c);
     eolc::MLState &ml = c->ml;
     dfree(ml.hord); dfree(ml.ptr1); dfree(ml.agg1); dfree(ml.gpos); dfree(ml.cptr); dfree(ml.par); dfree(ml.lam);
     dfree(ml.V); dfree(ml.Dinv); dfree(ml.R); dfree(ml.C);
@@ -735,6 +745,7 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     for (float **p : {&c->r32, &c->z32, &c->q32, &c->xl32, &c->p32[0], &c->p32[1]}) EOLC_HIP(dalloc(c, p, n3));
     // proximity-query tables (createEdges order, jitter stream)
     c->topo_tri.assign(tri, tri + 3 * (size_t)F);
+    c->topo_hinge.assign(hinge, hinge + 4 * (size_t)H);
     eolc::build_collision_edges(c, F, tri);
     EOLC_HIP(upload(c, &c->cedges, c->h_cedges));
     std::vector<double> unit;
@@ -820,7 +831,131 @@ int eolc_forces_fill(eolc_ctx *c, const double grav[3], double h)
     int rc = forces_fill_async(c, grav, h, true);
     if (rc) return rc;
     EOLC_HIP(eolc::launch_rhs(c, h));
+    EOLC_HIP(eolc::eol_fill(c));
     EOLC_HIP(hipStreamSynchronize(c->stream));
+    return EOLC_OK;
+}
+
+int eolc_set_eol(eolc_ctx *c, int n, const int *nodes)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_topo) return fail(c, EOLC_ERR_INVALID, "set_eol: no topology");
+    if (c->batch.B > 0) return fail(c, EOLC_ERR_INVALID, "set_eol: not for a batch topology");
+    if (n < 0 || (n && !nodes)) return fail(c, EOLC_ERR_INVALID, "set_eol: bad arguments");
+    const Topology &T = c->topo;
+    const int N = T.N, F = T.F, H = T.H;
+    std::vector<int> eidx((size_t)N, -1);
+    for (int k = 0; k < n; ++k) {
+        if (nodes[k] < 0 || nodes[k] >= N || eidx[nodes[k]] >= 0)
+            return fail(c, EOLC_ERR_INVALID, "set_eol: node out of range or repeated");
+        eidx[nodes[k]] = k;
+    }
+    EOLC_HIP(hipSetDevice(c->device));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    free_eol(c);
+    c->have_forces = false;
+    if (n == 0) return EOLC_OK;
+    eolc::EolState &S = c->eol;
+    S.E = n;
+    S.nodes.assign(nodes, nodes + n);
+    const int *tri = c->topo_tri.data(), *hinge = c->topo_hinge.data();
+    // EOL elements in the reference's order (faces, then hinges) and their staged
+    // entries: (X row, column, stage index), forces (X row, stage index)
+    struct Trip { int row, col, sidx; char face; };
+    std::vector<Trip> trips;
+    std::vector<std::pair<int, int>> ftrips;
+    std::vector<int> elem;
+    std::vector<int64_t> soff{0}, foff{0};
+    std::vector<std::pair<int, int>> xslots;   // (vertex, hinge force slot) of EOL hinges
+    for (int id = 0; id < F + H; ++id) {
+        const bool face = id < F;
+        const int nv = face ? 3 : 4;
+        const int *vt = face ? tri + 3 * (size_t)id : hinge + 4 * (size_t)(id - F);
+        int loc[4], nE = 0;
+        for (int v = 0; v < nv; ++v)
+            if (eidx[vt[v]] >= 0) loc[nE++] = v;
+        if (nE == 0) continue;
+        const int64_t base = soff.back(), fbase = foff.back();
+        const int rec = 4 * nE + 6 * nv;
+        if (base + (int64_t)nE * rec > (int64_t)INT32_MAX) return fail(c, EOLC_ERR_INVALID, "set_eol: too many EOL entries");
+        for (int ev = 0; ev < nE; ++ev) {
+            const int row0 = 2 * eidx[vt[loc[ev]]];
+            const int64_t r0 = base + (int64_t)ev * rec;
+            for (int ew = 0; ew < nE; ++ew)
+                for (int j = 0; j < 2; ++j)
+                    for (int k = 0; k < 2; ++k)
+                        trips.push_back({row0 + j, 3 * N + 2 * eidx[vt[loc[ew]]] + k, (int)(r0 + 4 * ew + 2 * j + k),
+                                         (char)face});
+            for (int u = 0; u < nv; ++u)
+                for (int j = 0; j < 2; ++j)
+                    for (int k = 0; k < 3; ++k)
+                        trips.push_back({row0 + j, 3 * vt[u] + k, (int)(r0 + 4 * nE + 6 * u + 3 * j + k), (char)face});
+            for (int j = 0; j < 2; ++j) ftrips.push_back({row0 + j, (int)(fbase + 2 * ev + j)});
+        }
+        if (!face)
+            for (int v = 0; v < 4; ++v) xslots.push_back({vt[v], 3 * F + 4 * (id - F) + v});
+        elem.push_back(id);
+        soff.push_back(base + (int64_t)nE * rec);
+        foff.push_back(fbase + 2 * nE);
+    }
+    S.ne = (int)elem.size();
+    std::stable_sort(trips.begin(), trips.end(),
+                     [](const Trip &a, const Trip &b) { return a.row != b.row ? a.row < b.row : a.col < b.col; });
+    S.rowptr.assign(2 * (size_t)n + 1, 0);
+    std::vector<int64_t> cptr{0};
+    std::vector<int> clist;
+    clist.reserve(trips.size());
+    for (size_t k = 0; k < trips.size(); ++k) {
+        const bool first = k == 0 || trips[k].row != trips[k - 1].row || trips[k].col != trips[k - 1].col;
+        if (first) {
+            if (k) cptr.push_back((int64_t)clist.size());
+            S.col.push_back(trips[k].col);
+            S.has_face.push_back(0);
+            S.rowptr[trips[k].row + 1]++;
+        }
+        clist.push_back(trips[k].sidx);
+        if (trips[k].face) S.has_face.back() = 1;
+    }
+    if (!trips.empty()) cptr.push_back((int64_t)clist.size());
+    for (int r = 0; r < 2 * n; ++r) S.rowptr[r + 1] += S.rowptr[r];
+    std::stable_sort(ftrips.begin(), ftrips.end(),
+                     [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+    std::vector<int> fptr(2 * (size_t)n + 1, 0), flist;
+    for (auto &t : ftrips) {
+        fptr[t.first + 1]++;
+        flist.push_back(t.second);
+    }
+    for (int r = 0; r < 2 * n; ++r) fptr[r + 1] += fptr[r];
+    std::stable_sort(xslots.begin(), xslots.end(),
+                     [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+    std::vector<int> xv, xvptr{0}, xvslot;
+    for (size_t k = 0; k < xslots.size(); ++k) {
+        if (k == 0 || xslots[k].first != xslots[k - 1].first) {
+            if (k) xvptr.push_back((int)xvslot.size());
+            xv.push_back(xslots[k].first);
+        }
+        xvslot.push_back(xslots[k].second);
+    }
+    if (!xslots.empty()) xvptr.push_back((int)xvslot.size());
+    S.nxv = (int)xv.size();
+    const size_t nst = (size_t)soff.back(), nsf = (size_t)foff.back(), nnz = S.col.size();
+    EOLC_HIP(upload(c, &S.d_elem, elem));
+    EOLC_HIP(upload(c, &S.d_eidx, eidx));
+    EOLC_HIP(upload(c, &S.d_soff, soff));
+    EOLC_HIP(upload(c, &S.d_foff, foff));
+    EOLC_HIP(upload(c, &S.d_cptr, cptr));
+    EOLC_HIP(upload(c, &S.d_clist, clist));
+    EOLC_HIP(upload(c, &S.d_fptr, fptr));
+    EOLC_HIP(upload(c, &S.d_flist, flist));
+    EOLC_HIP(upload(c, &S.d_xv, xv));
+    EOLC_HIP(upload(c, &S.d_xvptr, xvptr));
+    EOLC_HIP(upload(c, &S.d_xvslot, xvslot));
+    EOLC_HIP(dalloc(c, &S.d_stK, nst));
+    EOLC_HIP(dalloc(c, &S.d_stM, nst));
+    EOLC_HIP(dalloc(c, &S.d_stF, nsf));
+    EOLC_HIP(dalloc(c, &S.d_A, nnz));
+    EOLC_HIP(dalloc(c, &S.d_M, nnz));
+    EOLC_HIP(dalloc(c, &S.d_fX, 2 * (size_t)n));
     return EOLC_OK;
 }
 
@@ -830,6 +965,9 @@ int eolc_download_forces(eolc_ctx *c, double *f)
     if (!c->have_forces || !f) return fail(c, EOLC_ERR_INVALID, "download_forces: no forces");
     EOLC_HIP(hipSetDevice(c->device));
     EOLC_HIP(hipMemcpyAsync(f, c->f, 3 * (size_t)c->topo.N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (c->eol.E > 0)
+        EOLC_HIP(hipMemcpyAsync(f + 3 * (size_t)c->topo.N, c->eol.d_fX, 2 * (size_t)c->eol.E * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
 }
@@ -844,30 +982,67 @@ int eolc_download_csr(eolc_ctx *c, int which, int64_t *rowptr, int *col, double 
     const Topology &T = c->topo;
     const int N = T.N;
     // blocks present: MDK = all; M = blocks with at least one triangle contribution
+    const eolc::EolState &S = c->eol;
+    const bool wantM = which == EOLC_MAT_M;
     int64_t need = 0;
-    for (int64_t b = 0; b < T.nb; ++b) need += (which == EOLC_MAT_MDK || T.has_tri[b]) ? 9 : 0;
+    for (int64_t b = 0; b < T.nb; ++b) need += (!wantM || T.has_tri[b]) ? 9 : 0;
+    // EOL rows, and their transposes in the x rows (the reference pushes both)
+    std::vector<std::vector<std::pair<int, int64_t>>> mirror;
+    if (S.E > 0) {
+        mirror.resize(3 * (size_t)N);
+        for (int r = 0; r < 2 * S.E; ++r)
+            for (int64_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k) {
+                if (wantM && !S.has_face[k]) continue;
+                ++need;
+                if (S.col[k] < 3 * N) {
+                    mirror[S.col[k]].push_back({3 * N + r, k});
+                    ++need;
+                }
+            }
+    }
     *nnz = need;
     if (!rowptr || !col || !val || cap < need) return fail(c, EOLC_ERR_OVERFLOW, "download_csr: capacity too small");
-    std::vector<double> A((size_t)9 * T.nbs), m((size_t)T.nbs);
+    std::vector<double> A((size_t)9 * T.nbs), m((size_t)T.nbs), xa;
     EOLC_HIP(hipMemcpyAsync(A.data(), c->A64, A.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipMemcpyAsync(m.data(), c->Mval, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (S.E > 0) {
+        xa.resize(S.col.size());
+        if (!xa.empty())
+            EOLC_HIP(hipMemcpyAsync(xa.data(), wantM ? S.d_M : S.d_A, xa.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, c->stream));
+    }
     EOLC_HIP(hipStreamSynchronize(c->stream));
     int64_t p = 0;
     for (int i = 0; i < N; ++i)
         for (int a = 0; a < 3; ++a) {
             rowptr[3 * i + a] = p;
             for (int k = T.rowptr[i]; k < T.rowptr[i + 1]; ++k) {
-                if (which == EOLC_MAT_M && !T.has_tri[k]) continue;
+                if (wantM && !T.has_tri[k]) continue;
                 const int j = T.col[k];
                 const size_t e = (size_t)T.sellpos[k];
                 for (int cc = 0; cc < 3; ++cc) {
                     col[p] = 3 * j + cc;
-                    val[p] = (which == EOLC_MAT_MDK) ? A[(size_t)(3 * a + cc) * T.nbs + e] : (a == cc ? m[e] : 0.0);
+                    val[p] = !wantM ? A[(size_t)(3 * a + cc) * T.nbs + e] : (a == cc ? m[e] : 0.0);
                     ++p;
                 }
             }
+            if (S.E > 0)
+                for (const auto &q : mirror[3 * i + a]) {
+                    col[p] = q.first;
+                    val[p] = xa[q.second];
+                    ++p;
+                }
         }
-    rowptr[3 * N] = p;
+    for (int r = 0; r < 2 * S.E; ++r) {
+        rowptr[3 * N + r] = p;
+        for (int64_t k = S.rowptr[r]; k < S.rowptr[r + 1]; ++k) {
+            if (wantM && !S.has_face[k]) continue;
+            col[p] = S.col[k];
+            val[p] = xa[k];
+            ++p;
+        }
+    }
+    rowptr[3 * N + 2 * S.E] = p;
     return EOLC_OK;
 }
 
@@ -878,6 +1053,9 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
                       int maxit, int mode, eolc_solve_stats *st, int scripted = 0)
 {
     const int N = c->topo.N;
+    if (c->eol.E > 0)
+        return fail(c, EOLC_ERR_INVALID,
+                    "velocity_solve: EOL nodes are set; the EOL system goes to the host QP with its EOL rows");
     const bool multilevel = (mode & EOLC_SOLVE_MULTILEVEL) != 0;
     mode &= ~EOLC_SOLVE_MULTILEVEL;
     if (multilevel) {
@@ -1136,6 +1314,8 @@ static int step_impl(eolc_ctx *c, const double grav[3], double h, int n_dofs, co
     if (!grav || !(h > 0.0) || n_dofs < 0 || (n_dofs && !dofs) || !(tol > 0.0) || maxit <= 0)
         return fail(c, EOLC_ERR_INVALID, "step: bad arguments");
     if (!valid_mode(mode)) return fail(c, EOLC_ERR_INVALID, "step: unknown mode");
+    if (c->eol.E > 0)
+        return fail(c, EOLC_ERR_INVALID, "step: EOL nodes are set; the EOL system goes to the host QP with its EOL rows");
     EOLC_HIP(hipSetDevice(c->device));
     const int N = c->topo.N;
     for (int i = 0; i < n_dofs; ++i)

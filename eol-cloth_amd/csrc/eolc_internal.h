@@ -110,6 +110,34 @@ constexpr int kBatchVariants = 3;   // persistent-solve workgroup sizes 256, 512
 // solve with EOLC_ERR_MAXIT instead of looping.
 constexpr int kMaxRefinements = 1000;
 
+// EOL nodes (Forces.cpp:177-329, 399-497, 580-683, 746-884): node nodes[k] gets two
+// material-space DOFs at 3N + 2k.  The maps below are built by eolc_set_eol on the
+// host; k_eol.hip fills the values after each forces_fill.
+//   stage:  per EOL element e (faces, then hinges, with an EOL vertex), for each EOL
+//           local vertex v (in local order) a record of 4 nE + 6 nv values: the 2x2
+//           blocks (X_v, X_w) for the EOL locals w, then the 2x3 blocks (X_v, x_u)
+//           for every local u; forces: 2 per EOL local vertex.
+//   X rows: CSR over rows 3N .. 3N + 2E - 1 (stored from 0), full columns, each
+//           entry's contributions in element order (setFromTriplets' fold order).
+struct EolState {
+    int E = 0;
+    std::vector<int> nodes;              // EoL_index -> node
+    int ne = 0;                          // EOL elements
+    std::vector<int64_t> rowptr;         // 2E + 1
+    std::vector<int> col;                // columns of the X-row entries
+    std::vector<char> has_face;          // an entry of the M pattern (a face contributes)
+    int *d_elem = nullptr;               // ne: f for a face, F + h for a hinge
+    int *d_eidx = nullptr;               // N: EoL_index or -1
+    int64_t *d_soff = nullptr, *d_foff = nullptr;   // ne + 1 stage offsets
+    int64_t *d_cptr = nullptr;           // entries + 1
+    int *d_clist = nullptr;              // stage indices per entry
+    int *d_fptr = nullptr, *d_flist = nullptr;      // 2E + 1, force stage indices per X DOF
+    int nxv = 0;                         // vertices of EOL hinges (their bending force joins f)
+    int *d_xv = nullptr, *d_xvptr = nullptr, *d_xvslot = nullptr;
+    double *d_stK = nullptr, *d_stM = nullptr, *d_stF = nullptr;
+    double *d_A = nullptr, *d_M = nullptr, *d_fX = nullptr;   // X-row MDK and M values, X forces
+};
+
 struct BatchState {
     int B = 0;
     std::vector<BatchInst> inst;
@@ -256,6 +284,8 @@ struct eolc_ctx {
     int n_cedges = 0;
     std::vector<int> h_cedges;
     std::vector<int> topo_tri;
+    std::vector<int> topo_hinge;
+    eolc::EolState eol;
     double *cd_unit = nullptr;  // 3N jitter values in (-1, 1)
     int cd_host_decisions = 0;  // angle tests re-decided on the host (diagnostic)
     // learned preconditioner
@@ -294,6 +324,8 @@ hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
 // d_vals[t] = (1 - 0.01) v[d_dofs[t]] + d_vals[t] on the context's stream.
 hipError_t fixed_from_vold(eolc_ctx *c, int n, const int *d_dofs, double *d_vals);
+// EOL rows of the last forces_fill (k_eol.hip); after launch_rhs.
+hipError_t eol_fill(eolc_ctx *c);
 constexpr int kSpmvVariants = 18;
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms);
 hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms, int read_only);

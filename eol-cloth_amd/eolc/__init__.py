@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = (
     "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta", "eolc_collision_constraints",
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
     "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
-    "eolc_rigid_integrate", "eolc_box_step",
+    "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol",
 )
 
 BATCH_EAGER = 0x1
@@ -114,6 +114,7 @@ def lib():
         L.eolc_set_batch_variant.argtypes = [vp, C.c_int]
         L.eolc_set_batch_multilevel.argtypes = [vp, C.c_int]
         L.eolc_set_batch_scripted.argtypes = [vp, C.c_void_p]
+        L.eolc_set_eol.argtypes = [vp, C.c_int, C.c_void_p]
         L.eolc_step_dofs.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                                      C.c_int, C.c_int, C.POINTER(SolveStats)]
         L.eolc_rigid_integrate.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
@@ -204,6 +205,7 @@ class Context:
         if rc != OK:
             raise EolcError(rc, "eolc_create failed (no usable GPU?)")
         self.N = 0
+        self.n_eol = 0
 
     def close(self):
         if self._h:
@@ -233,6 +235,7 @@ class Context:
     def set_topology(self, X, tri, hinge):
         X, tri, hinge = _d(X), _i(tri), _i(hinge)
         self.N = X.size // 2
+        self.n_eol = 0
         return self._check(lib().eolc_set_topology(self._h, self.N, X, tri.size // 3, tri,
                                                    hinge.size // 4, hinge))
 
@@ -253,8 +256,15 @@ class Context:
     def forces_fill(self, grav, h):
         return self._check(lib().eolc_forces_fill(self._h, _d(grav), float(h)))
 
+    def set_eol(self, nodes=()):
+        """EOL nodes (EoL_index = position in `nodes`): f, M and MDK get 2 rows per
+        node after the next forces_fill; empty clears them."""
+        nd = _i(nodes).reshape(-1)
+        self._check(lib().eolc_set_eol(self._h, nd.size, _ptr(nd) if nd.size else None))
+        self.n_eol = int(nd.size)
+
     def download_forces(self):
-        f = np.zeros(3 * self.N)
+        f = np.zeros(3 * self.N + 2 * self.n_eol)
         self._check(lib().eolc_download_forces(self._h, f))
         return f
 
@@ -264,7 +274,7 @@ class Context:
         nnz = C.c_int64(0)
         rc = lib().eolc_download_csr(self._h, int(which), None, None, None, 0, C.byref(nnz))
         self._check(rc, allow=(ERR_OVERFLOW,))
-        n = 3 * self.N
+        n = 3 * self.N + 2 * self.n_eol
         rp = np.zeros(n + 1, dtype=np.int64)
         col = np.zeros(max(nnz.value, 1), dtype=np.int32)
         val = np.zeros(max(nnz.value, 1))
@@ -405,6 +415,7 @@ class Context:
         self._check(lib().eolc_set_batch_topology(self._h, B, _ptr(Nb), _ptr(X), _ptr(Fb), _ptr(T), _ptr(Hb),
                                                   _ptr(Hh), _ptr(Pb), _ptr(P), _ptr(off)))
         self.N = int(off[-1])
+        self.n_eol = 0
         self.B = B
         return off
 

@@ -71,13 +71,23 @@ public:
     {
         check(eolc_set_topology(ctx_, m.N(), m.X.data(), (int)(m.tri.size() / 3), m.tri.data(),
                                 (int)(m.hinge.size() / 4), m.hinge.data()));
+        nEol_ = 0;
     }
+    // EOL nodes in EoL_index order (Node::EoL, Node::EoL_index); after setTopology
+    void setEol(const std::vector<int> &nodes)
+    {
+        check(eolc_set_eol(ctx_, (int)nodes.size(), nodes.data()));
+        nEol_ = (int)nodes.size();
+    }
+    int eolCount() const { return nEol_; }
 
 private:
     eolc_ctx *ctx_ = nullptr;
+    int nEol_ = 0;
 };
 
-// Forces: f, M and MDK = M + dampingB h^2 K of the current state (Forces.cpp:912-930).
+// Forces: f, M and MDK = M + dampingB h^2 K of the current state (Forces.cpp:912-930),
+// with 3N + 2 EoL_Count rows when EOL nodes are set (Device::setEol).
 class Forces {
 public:
     explicit Forces(Device &d) : dev_(d) {}
@@ -90,7 +100,7 @@ public:
         dev_.check(eolc_set_material(dev_.ctx(), mat.density, mat.e, mat.nu, mat.beta, mat.dampingA, mat.dampingB));
         dev_.check(eolc_upload_state(dev_.ctx(), mesh.x.data(), mesh.v.data()));
         dev_.check(eolc_forces_fill(dev_.ctx(), grav, h));
-        f.resize(mesh.x.size());
+        f.resize(mesh.x.size() + 2 * (size_t)dev_.eolCount());   // 3N + 2 EoL_Count
         dev_.check(eolc_download_forces(dev_.ctx(), f.data()));
         EoL_cutoff = (int)mesh.x.size();
         if (downloadMatrices) {

@@ -153,12 +153,26 @@ int eolc_upload_state(eolc_ctx *ctx, const double *x, const double *v);      /* 
 int eolc_download_state(eolc_ctx *ctx, double *x, double *v);
 
 /* ---- Forces::fill ------------------------------------------------------------
- * Evaluates every element at the device-resident x and assembles f (3N),
- * M and MDK = M + dampingB h^2 K on the device. */
+ * Evaluates every element at the device-resident x and assembles f (3N; 3N + 2E
+ * with E EOL nodes, see eolc_set_eol), M and MDK = M + dampingB h^2 K on the
+ * device. */
 int eolc_forces_fill(eolc_ctx *ctx, const double grav[3], double h);
 int eolc_download_forces(eolc_ctx *ctx, double *f);
-/* Scalar CSR (3N x 3N, sorted columns) of M or MDK.  On EOLC_ERR_OVERFLOW,
- * *nnz holds the required capacity. */
+/* ---- EOL nodes (Forces.cpp:177-329, 399-497, 580-683, 746-884) ---------------
+ * nodes[k] becomes an EOL node with EoL_index k: from the next forces_fill on,
+ * f, M and MDK have 3N + 2n rows, the two material-space DOFs of node nodes[k] at
+ * 3N + 2k.  Faces and hinges with an EOL vertex add, through the deformation
+ * gradient F of the face (deform_grad, UtilEOL.cpp:13-28; for a hinge the mean
+ * over its two faces), f_X = -F^T f_v and the blocks F^T B F, -F^T B, -B F of
+ * fillXMI / fillXXMI / fillXxMI / fillxXMI and their bending twins; an EOL hinge
+ * also adds its bending force to the x rows (edgeBasedF's EOL branch).  The x-x
+ * blocks do not change.  The EOL system goes to the host QP with its EOL
+ * constraint rows: velocity_solve and step report EOLC_ERR_INVALID while EOL
+ * nodes are set.  n = 0 clears them; set_topology clears them too. */
+int eolc_set_eol(eolc_ctx *ctx, int n, const int *nodes);
+
+/* Scalar CSR (3N x 3N, sorted columns; 3N + 2E with E EOL nodes) of M or MDK.
+ * On EOLC_ERR_OVERFLOW, *nnz holds the required capacity. */
 int eolc_download_csr(eolc_ctx *ctx, int which, int64_t *rowptr, int *col, double *val,
                       int64_t cap, int64_t *nnz);
 
