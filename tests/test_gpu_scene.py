@@ -69,6 +69,7 @@ def test_step_dofs_schedule_matches_oracle(ctx):
     whens, specs, corners = schedule(n)
     ctx.set_topology(sc.X, sc.tri, sc.hinge)
     ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(0)   # the session's context may carry another test's options
     ctx.upload_state(sc.x0, sc.v0)
     t, idx = 0.0, 0
     for k in range(steps):
