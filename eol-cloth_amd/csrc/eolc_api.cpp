@@ -1209,9 +1209,42 @@ int eolc_velocity_solve(eolc_ctx *c, int n_fixed, const int *fixed_dof, const do
     return solve_impl(c, n_fixed, fixed_dof, fixed_val, tol, maxit, mode, st);
 }
 
+static int transfer_impl(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
+                         const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
+                         double *bary, double *ms, const double *x_old, const int *eol_old, int n_eol,
+                         const double *V_old);
+
 int eolc_velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
                            const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
                            double *bary, double *ms)
+{
+    return transfer_impl(c, n_old, u_old, v_old, f_old, tri_old, n_new, u_new, v_new, src, face, bary, ms, nullptr,
+                         nullptr, 0, nullptr);
+}
+
+int eolc_velocity_transfer_eol(eolc_ctx *c, int n_old, const double *u_old, const double *x_old, const double *v_old,
+                               const int *eol_old, int n_eol, const double *V_old, int f_old, const int *tri_old,
+                               int n_new, const double *u_new, double *v_new, int *src, int *face, double *bary,
+                               double *ms)
+{
+    if (!c) return EOLC_ERR_INVALID;
+    if (n_old <= 0 || n_eol < 0 || !x_old || !eol_old || (n_eol && !V_old))
+        return fail(c, EOLC_ERR_INVALID, "velocity_transfer_eol: bad arguments");
+    for (size_t k = 0; k < 3 * (size_t)n_old; ++k)
+        if (!std::isfinite(x_old[k])) return fail(c, EOLC_ERR_NONFINITE, "velocity_transfer_eol: non-finite old x");
+    for (size_t k = 0; k < 2 * (size_t)n_eol; ++k)
+        if (!std::isfinite(V_old[k])) return fail(c, EOLC_ERR_NONFINITE, "velocity_transfer_eol: non-finite old V");
+    for (int j = 0; j < n_old; ++j)
+        if (eol_old[j] < -1 || eol_old[j] >= n_eol)
+            return fail(c, EOLC_ERR_INVALID, "velocity_transfer_eol: EoL index out of range");
+    return transfer_impl(c, n_old, u_old, v_old, f_old, tri_old, n_new, u_new, v_new, src, face, bary, ms, x_old,
+                         eol_old, n_eol, V_old);
+}
+
+static int transfer_impl(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
+                         const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
+                         double *bary, double *ms, const double *x_old, const int *eol_old, int n_eol,
+                         const double *V_old)
 {
     if (!c) return EOLC_ERR_INVALID;
     if (n_old <= 0 || f_old < 0 || n_new < 0 || !u_old || !v_old || (f_old && !tri_old) ||
@@ -1231,7 +1264,7 @@ int eolc_velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const do
     int missing = 0;
     float t = 0.f;
     EOLC_HIP(eolc::velocity_transfer(c, n_old, u_old, v_old, f_old, tri_old, n_new, u_new, v_new, src, face, bary,
-                                     &missing, &t));
+                                     &missing, &t, x_old, eol_old, n_eol, V_old));
     if (ms) *ms = t;
     if (missing)
         return fail(c, EOLC_ERR_INVALID, "velocity_transfer: " + std::to_string(missing) +

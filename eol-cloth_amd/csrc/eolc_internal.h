@@ -347,7 +347,8 @@ hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done);   // z
 // velocity transfer after a remesh (k_transfer.hip); host arrays in and out
 hipError_t velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
                              const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
-                             double *bary, int *n_missing, float *ms);
+                             double *bary, int *n_missing, float *ms, const double *x_old = nullptr,
+                             const int *eol_old = nullptr, int n_eol = 0, const double *V_old = nullptr);
 // batched cloths (k_batch.hip)
 hipError_t batch_set_fixmask(eolc_ctx *c, const int *d_nodes, int n);
 hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant);

@@ -232,6 +232,45 @@ __global__ void k_transfer(int n_new, const double *__restrict__ u_new, const do
         v_new[3 * n + cc] = b0 * v_old[3 * t[0] + cc] + b1 * v_old[3 * t[1] + cc] + b2 * v_old[3 * t[2] + cc];
 }
 
+// EOL correction of a new LAG vertex (Cloth.cpp:254-262): each vertex of its old face
+// that was an EOL node contributes v - F V instead of v, F the old face's deformation
+// gradient (deform_grad, UtilEOL.cpp:13-28, at the old world positions).  Recomputes
+// v_new only for vertices whose face has an EOL vertex, in the reference's order:
+// vw = v + (-F) V, then b0 vwA + b1 vwB + b2 vwC.
+__global__ void k_transfer_eol(int n_new, const int *__restrict__ face, const double *__restrict__ bary,
+                               const int *__restrict__ tri, const double *__restrict__ x_old,
+                               const double *__restrict__ u_old, const double *__restrict__ v_old,
+                               const int *__restrict__ eidx, const double *__restrict__ V_old,
+                               double *__restrict__ v_new)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_new || face[n] < 0) return;
+    const int *t = tri + 3 * face[n];
+    if (eidx[t[0]] < 0 && eidx[t[1]] < 0 && eidx[t[2]] < 0) return;
+    const int a = t[0], b = t[1], c = t[2];
+    const double d00 = u_old[2 * b] - u_old[2 * a], d01 = u_old[2 * c] - u_old[2 * a];
+    const double d10 = u_old[2 * b + 1] - u_old[2 * a + 1], d11 = u_old[2 * c + 1] - u_old[2 * a + 1];
+    const double det = d00 * d11 - d01 * d10;
+    const double i00 = d11 / det, i01 = -d01 / det, i10 = -d10 / det, i11 = d00 / det;
+    double F[3][2];
+    for (int r = 0; r < 3; ++r) {
+        const double e0 = x_old[3 * b + r] - x_old[3 * a + r], e1 = x_old[3 * c + r] - x_old[3 * a + r];
+        F[r][0] = e0 * i00 + e1 * i10;
+        F[r][1] = e0 * i01 + e1 * i11;
+    }
+    double vw[3][3];
+    for (int k = 0; k < 3; ++k) {
+        const int j = t[k];
+        for (int r = 0; r < 3; ++r) vw[k][r] = v_old[3 * j + r];
+        if (eidx[j] >= 0) {
+            const double V0 = V_old[2 * eidx[j]], V1 = V_old[2 * eidx[j] + 1];
+            for (int r = 0; r < 3; ++r) vw[k][r] = vw[k][r] + ((-F[r][0]) * V0 + (-F[r][1]) * V1);
+        }
+    }
+    const double b0 = bary[3 * n], b1 = bary[3 * n + 1], b2 = bary[3 * n + 2];
+    for (int r = 0; r < 3; ++r) v_new[3 * n + r] = b0 * vw[0][r] + b1 * vw[1][r] + b2 * vw[2][r];
+}
+
 inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
 // exclusive scan of cnt[0..n) into start[0..n], start[n] = total
@@ -267,7 +306,8 @@ namespace eolc {
 // enclosing old face in *n_missing.  Host arrays in and out.
 hipError_t velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
                              const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
-                             double *bary, int *n_missing, float *ms)
+                             double *bary, int *n_missing, float *ms, const double *x_old, const int *eol_old,
+                             int n_eol, const double *V_old)
 {
     hipStream_t st = c->stream;
     // grid over the old vertices' bounding box, about 4 vertices per cell
@@ -299,6 +339,14 @@ hipError_t velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const 
     TRY(hipMemcpyAsync(dv_old.p, v_old, 3 * (size_t)n_old * sizeof(double), hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(du_new.p, u_new, 2 * (size_t)n_new * sizeof(double), hipMemcpyHostToDevice, st));
     TRY(hipMemcpyAsync(dtri.p, tri_old, 3 * (size_t)f_old * sizeof(int), hipMemcpyHostToDevice, st));
+    DevBuf<double> dx_old, dV;
+    DevBuf<int> deidx;
+    if (eol_old) {
+        TRY(dx_old.alloc(3 * (size_t)n_old)); TRY(deidx.alloc(n_old)); TRY(dV.alloc(2 * (size_t)n_eol));
+        TRY(hipMemcpyAsync(dx_old.p, x_old, 3 * (size_t)n_old * sizeof(double), hipMemcpyHostToDevice, st));
+        TRY(hipMemcpyAsync(deidx.p, eol_old, (size_t)n_old * sizeof(int), hipMemcpyHostToDevice, st));
+        if (n_eol) TRY(hipMemcpyAsync(dV.p, V_old, 2 * (size_t)n_eol * sizeof(double), hipMemcpyHostToDevice, st));
+    }
     Events ev;
     TRY(hipEventCreate(&ev.a));
     TRY(hipEventCreate(&ev.b));
@@ -325,6 +373,11 @@ hipError_t velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const 
     hipLaunchKernelGGL(k_transfer, dim3(blocks_for(n_new)), dim3(256), 0, st, n_new, du_new.p, du_old.p, dv_old.p,
                        dtri.p, g, vstart.p, vlist.p, fstart.p, flist.p, dv_new.p, dsrc.p, dface.p, dbary.p, dmiss.p);
     TRY(hipGetLastError());
+    if (eol_old) {
+        hipLaunchKernelGGL(k_transfer_eol, dim3(blocks_for(n_new)), dim3(256), 0, st, n_new, dface.p, dbary.p, dtri.p,
+                           dx_old.p, du_old.p, dv_old.p, deidx.p, dV.p, dv_new.p);
+        TRY(hipGetLastError());
+    }
     TRY(hipEventRecord(ev.b, st));
     TRY(hipMemcpyAsync(v_new, dv_new.p, 3 * (size_t)n_new * sizeof(double), hipMemcpyDeviceToHost, st));
     TRY(hipMemcpyAsync(src, dsrc.p, (size_t)n_new * sizeof(int), hipMemcpyDeviceToHost, st));

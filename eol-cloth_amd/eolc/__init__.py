@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = (
     "eolc_set_batch_variant", "eolc_stream_copy", "eolc_set_mixed_delta", "eolc_collision_constraints",
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
     "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
-    "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol",
+    "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol", "eolc_velocity_transfer_eol",
 )
 
 BATCH_EAGER = 0x1
@@ -115,6 +115,9 @@ def lib():
         L.eolc_set_batch_multilevel.argtypes = [vp, C.c_int]
         L.eolc_set_batch_scripted.argtypes = [vp, C.c_void_p]
         L.eolc_set_eol.argtypes = [vp, C.c_int, C.c_void_p]
+        L.eolc_velocity_transfer_eol.argtypes = [vp, C.c_int] + [C.c_void_p] * 4 + [C.c_int, C.c_void_p, C.c_int,
+                                                                                    C.c_void_p, C.c_int] + \
+            [C.c_void_p] * 5 + [C.POINTER(C.c_double)]
         L.eolc_step_dofs.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                                      C.c_int, C.c_int, C.POINTER(SolveStats)]
         L.eolc_rigid_integrate.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
@@ -325,6 +328,28 @@ class Context:
         ms = C.c_double(0)
         rc = lib().eolc_velocity_transfer(self._h, u_old.size // 2, u_old, v_old, tri_old.size // 3, tri_old, n_new,
                                           u_new, v_new, src, face, bary, C.byref(ms))
+        self._check(rc, allow=allow)
+        return rc, v_new, src[:n_new], face[:n_new], bary[:3 * n_new], ms.value
+
+    def velocity_transfer_eol(self, u_old, x_old, v_old, eol_old, V_old, tri_old, u_new, allow=()):
+        """velocity_transfer with EOL nodes in the old mesh (eol_old: EoL index per old
+        vertex or -1; V_old: 2 per EOL node): (rc, v_new, src, face, bary, ms)."""
+        u_old, x_old, v_old, u_new = _d(u_old), _d(x_old), _d(v_old), _d(u_new)
+        eol_old, tri_old = _i(eol_old).reshape(-1), _i(tri_old)
+        V = _d(V_old)
+        n_eol = V.size // 2
+        if V.size == 0:
+            V = np.zeros(2)
+        n_new = u_new.size // 2
+        v_new = np.zeros(3 * n_new)
+        src = np.zeros(max(n_new, 1), dtype=np.int32)
+        face = np.zeros(max(n_new, 1), dtype=np.int32)
+        bary = np.zeros(max(3 * n_new, 1))
+        ms = C.c_double(0)
+        rc = lib().eolc_velocity_transfer_eol(self._h, u_old.size // 2, _ptr(u_old), _ptr(x_old), _ptr(v_old),
+                                              _ptr(eol_old), n_eol, _ptr(V), tri_old.size // 3, _ptr(tri_old), n_new,
+                                              _ptr(u_new), _ptr(v_new), _ptr(src), _ptr(face), _ptr(bary),
+                                              C.byref(ms))
         self._check(rc, allow=allow)
         return rc, v_new, src[:n_new], face[:n_new], bary[:3 * n_new], ms.value
 

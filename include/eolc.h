@@ -78,6 +78,20 @@ int eolc_velocity_transfer(eolc_ctx *ctx, int n_old, const double *u_old, const 
                            const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
                            double *bary, double *ms);
 
+/* The same with EOL nodes in the old mesh (Cloth.cpp:254-262): a new vertex placed
+ * by barycentric averaging takes, from each vertex j of its old face that was an
+ * EOL node (eol_old[j] = its EoL_index, else -1), the velocity v_j - F V_j instead
+ * of v_j, F the old face's deformation gradient (deform_grad, UtilEOL.cpp:13-28)
+ * at the old world positions x_old (3 per vertex) and V_j = V_old[2 eol_old[j] ..]
+ * the node's material-space velocity.  Vertices matched to an old vertex still
+ * take its v (the caller copies the EOL part with src, as the reference copies
+ * vert->v).  The other EOL branches (WasEOL, NewEOLFromSplit, a new EOL node's
+ * split) need the remesher's node states and stay with it. */
+int eolc_velocity_transfer_eol(eolc_ctx *ctx, int n_old, const double *u_old, const double *x_old,
+                               const double *v_old, const int *eol_old, int n_eol, const double *V_old,
+                               int f_old, const int *tri_old, int n_new, const double *u_new, double *v_new,
+                               int *src, int *face, double *bary, double *ms);
+
 /* Aggregation hierarchy of EOLC_SOLVE_MULTILEVEL (no counterpart in the reference:
  * a preconditioner of the solver-"none" solve).  eolc_ml_info builds it if needed
  * and reports the number of coarse levels and the aggregates per level: counts[0] =

@@ -146,6 +146,11 @@ int  orc_box_tri(double thr, const double *whd, const double *E /* 4x4 col-major
  * Returns the number of new vertices with neither. */
 int orc_velocity_transfer(int n_old, const double *u_old, const double *v_old, int f_old, const int *tri_old,
                           int n_new, const double *u_new, double *v_new, int *src, int *face, double *bary);
+/* The same with EOL nodes in the old mesh (Cloth.cpp:254-262): eidx[j] = EoL index
+ * of old vertex j or -1, V_old 2 per EOL node, x_old the old world positions. */
+int orc_velocity_transfer_eol(int n_old, const double *u_old, const double *x_old, const double *v_old,
+                              const int *eidx, const double *V_old, int f_old, const int *tri_old,
+                              int n_new, const double *u_new, double *v_new, int *src, int *face, double *bary);
 
 /* ---- input generators (orc_rng.c) --------------------------------------- */
 uint64_t orc_splitmix64(uint64_t *state);

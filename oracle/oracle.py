@@ -63,6 +63,8 @@ def lib():
         for nm in ("orc_inertial",):
             getattr(L, nm).argtypes = [_dp] * 7 + [C.c_double, _dp, _dp, _dp]
         L.orc_velocity_transfer.argtypes = [C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _ip, _ip, _dp]
+        L.orc_velocity_transfer_eol.argtypes = [C.c_int, _dp, _dp, _dp, _ip, _dp, C.c_int, _ip, C.c_int, _dp, _dp,
+                                                _ip, _ip, _dp]
         L.orc_membrane.argtypes = [_dp] * 6 + [C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
         L.orc_bending.argtypes = [_dp] * 8 + [C.c_double, _dp, _dp, _dp]
         L.orc_face_PQ.argtypes = [_dp] * 8
@@ -304,6 +306,24 @@ def velocity_transfer(u_old, v_old, tri_old, u_new):
     bary = np.zeros(3 * n_new)
     missing = lib().orc_velocity_transfer(u_old.size // 2, u_old, v_old, tri_old.size // 3, tri_old, n_new, u_new,
                                           v_new, src, face, bary)
+    return v_new, src, face, bary, missing
+
+
+def velocity_transfer_eol(u_old, x_old, v_old, eidx, V_old, tri_old, u_new):
+    """velocity_transfer with EOL nodes in the old mesh (eidx: EoL index per old
+    vertex or -1; V_old: 2 per EOL node): (v_new, src, face, bary, missing)."""
+    u_old, x_old, v_old = _d(u_old).reshape(-1), _d(x_old).reshape(-1), _d(v_old).reshape(-1)
+    u_new, V_old = _d(u_new).reshape(-1), _d(V_old).reshape(-1)
+    if V_old.size == 0:
+        V_old = np.zeros(2)
+    tri_old, eidx = _i(tri_old).reshape(-1), _i(eidx).reshape(-1)
+    n_new = u_new.size // 2
+    v_new = np.zeros(3 * n_new)
+    src = np.zeros(n_new, dtype=np.int32)
+    face = np.zeros(n_new, dtype=np.int32)
+    bary = np.zeros(3 * n_new)
+    missing = lib().orc_velocity_transfer_eol(u_old.size // 2, u_old, x_old, v_old, eidx, V_old, tri_old.size // 3,
+                                              tri_old, n_new, u_new, v_new, src, face, bary)
     return v_new, src, face, bary, missing
 
 

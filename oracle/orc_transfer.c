@@ -84,3 +84,43 @@ int orc_velocity_transfer(int n_old, const double *u_old, const double *v_old, i
     }
     return missing;
 }
+
+/* The EOL correction of a new LAG vertex (Cloth.cpp:254-262): every vertex j of its
+ * old face that was an EOL node (eidx[j] >= 0) contributes vw = v_j + (-F) V_j,
+ * F = deform_grad(old face) (UtilEOL.cpp:13-28: Dx DX^-1 at the old world positions,
+ * DX^-1 by its adjugate over the determinant), and v = b0 vwA + b1 vwB + b2 vwC.
+ * Vertices matched to an old vertex keep that vertex's v. */
+int orc_velocity_transfer_eol(int n_old, const double *u_old, const double *x_old, const double *v_old,
+                              const int *eidx, const double *V_old, int f_old, const int *tri_old,
+                              int n_new, const double *u_new, double *v_new, int *src, int *face, double *bary)
+{
+    const int missing = orc_velocity_transfer(n_old, u_old, v_old, f_old, tri_old, n_new, u_new, v_new, src, face, bary);
+    for (int n = 0; n < n_new; ++n) {
+        if (face[n] < 0) continue;
+        const int *t = tri_old + 3 * face[n];
+        if (eidx[t[0]] < 0 && eidx[t[1]] < 0 && eidx[t[2]] < 0) continue;
+        const int a = t[0], b = t[1], c = t[2];
+        const double d00 = u_old[2 * b] - u_old[2 * a], d01 = u_old[2 * c] - u_old[2 * a];
+        const double d10 = u_old[2 * b + 1] - u_old[2 * a + 1], d11 = u_old[2 * c + 1] - u_old[2 * a + 1];
+        const double det = d00 * d11 - d01 * d10;
+        const double i00 = d11 / det, i01 = -d01 / det, i10 = -d10 / det, i11 = d00 / det;
+        double F[3][2], vw[3][3];
+        for (int r = 0; r < 3; ++r) {
+            const double e0 = x_old[3 * b + r] - x_old[3 * a + r], e1 = x_old[3 * c + r] - x_old[3 * a + r];
+            F[r][0] = e0 * i00 + e1 * i10;
+            F[r][1] = e0 * i01 + e1 * i11;
+        }
+        for (int k = 0; k < 3; ++k) {
+            const int j = t[k];
+            for (int r = 0; r < 3; ++r) vw[k][r] = v_old[3 * j + r];
+            if (eidx[j] >= 0) {
+                const double V0 = V_old[2 * eidx[j]], V1 = V_old[2 * eidx[j] + 1];
+                for (int r = 0; r < 3; ++r) vw[k][r] = vw[k][r] + ((-F[r][0]) * V0 + (-F[r][1]) * V1);
+            }
+        }
+        const double b0 = bary[3 * n], b1 = bary[3 * n + 1], b2 = bary[3 * n + 2];
+        for (int r = 0; r < 3; ++r) v_new[3 * n + r] = b0 * vw[0][r] + b1 * vw[1][r] + b2 * vw[2][r];
+    }
+    return missing;
+}
+

@@ -74,3 +74,31 @@ def test_transfer_at_scale():
     assert np.max(np.abs(vg.reshape(-1, 3)[inf] - (P[inf] @ A.T + 1.0))) < 1e-12
     onv = n_s + np.nonzero(r_s >= 0)[0]
     assert np.array_equal(vg.reshape(-1, 3)[onv], V[sg[onv]])
+
+
+def test_transfer_with_eol_nodes_matches_bruteforce(ctx):
+    """EOL nodes in the old mesh (the boundary and a few interior nodes): the GPU's
+    EOL correction of new LAG vertices equals the oracle's, bit for bit, and changes
+    the velocities of the points whose old face has an EOL vertex."""
+    n = 16
+    rng = np.random.default_rng(5)
+    sc = scene.Scene(n, sigma_x=1e-3, sigma_v=0.05)
+    ij = np.array([(k % (n + 1), k // (n + 1)) for k in range(sc.N)])
+    eol = np.where((ij[:, 0] == 0) | (ij[:, 0] == n) | (ij[:, 1] == 0) | (ij[:, 1] == n))[0]
+    eol = np.concatenate([eol, [5 * (n + 1) + 5, 9 * (n + 1) + 3]])
+    eidx = -np.ones(sc.N, dtype=np.int32)
+    eidx[eol] = np.arange(len(eol))
+    V = rng.normal(scale=0.05, size=(len(eol), 2))
+    P = remesh_points(sc, rng, 1000)
+    print(f"transfer with {len(eol)} EOL nodes: {len(P)} new points", flush=True)
+    vo, so, fo, bo, missing = O.velocity_transfer_eol(sc.X, sc.x0, sc.v0, eidx, V, sc.tri, P)
+    rc, vg, sg, fg, bg, ms = ctx.velocity_transfer_eol(sc.X, sc.x0, sc.v0, eidx, V, sc.tri, P)
+    assert missing == 0 and rc == eolc.OK
+    assert np.array_equal(sg, so) and np.array_equal(fg, fo) and np.array_equal(bg, bo)
+    assert np.array_equal(vg, vo)
+    vp, *_ = O.velocity_transfer(sc.X, sc.v0, sc.tri, P)
+    touched = np.array([f >= 0 and np.any(eidx[sc.tri[f]] >= 0) for f in fo])
+    print(f"  {touched.sum()} points in faces with an EOL vertex", flush=True)
+    assert touched.sum() > 0
+    assert not np.array_equal(vg.reshape(-1, 3)[touched], vp.reshape(-1, 3)[touched])
+    assert np.array_equal(vg.reshape(-1, 3)[~touched], vp.reshape(-1, 3)[~touched])

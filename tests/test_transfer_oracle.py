@@ -62,3 +62,31 @@ def test_tie_and_tolerance_rules():
     vn, src, _, _, _ = O.velocity_transfer(u2, v2, tri2, np.array([[4e-7, 0.0], [0.0, 1.5e-6]]))
     assert src[0] == 0 and np.array_equal(vn[:3], v2[0])
     assert src[1] == -1
+
+
+def test_eol_correction_closed_form():
+    """An old face whose vertices were EOL nodes: the new vertex's velocity is
+    sum_i b_i (v_i - F V_i) with F = Dx DX^-1 of the old face (numpy's inverse here);
+    V = 0 gives the plain transfer bit for bit; a vertex matched to an old vertex
+    keeps its v."""
+    rng = np.random.default_rng(11)
+    u, v, tri = one_tri()
+    u = u * np.array([0.7, 0.4]) + np.array([0.1, 0.2])
+    x = np.concatenate([u, np.zeros((3, 1))], 1) + rng.normal(scale=0.05, size=(3, 3))
+    P = np.array([[0.3, 0.3], [0.1, 0.2]])
+    for eidx in ([0, 1, 2], [-1, 0, -1]):
+        nE = sum(e >= 0 for e in eidx)
+        V = rng.normal(size=(max(nE, 1), 2))
+        vn, src, face, bary, missing = O.velocity_transfer_eol(u, x, v, eidx, V, tri, P)
+        assert missing == 0 and face[0] == 0 and src[1] == 0
+        F = np.stack([x[1] - x[0], x[2] - x[0]], 1) @ np.linalg.inv(np.stack([u[1] - u[0], u[2] - u[0]], 1))
+        vw = v.copy()
+        for i, e in enumerate(eidx):
+            if e >= 0:
+                vw[i] = v[i] - F @ V[e]
+        ref = bary[:3] @ vw
+        assert np.max(np.abs(vn[:3] - ref)) <= 1e-14 * np.max(np.abs(ref))
+        assert np.array_equal(vn[3:], v[0])
+        vz, *_ = O.velocity_transfer_eol(u, x, v, eidx, np.zeros_like(V), tri, P)
+        vp, *_ = O.velocity_transfer(u, v, tri, P)
+        assert np.array_equal(vz, vp)
