@@ -200,6 +200,33 @@ def measured_traffic(kernel, n):
     return None
 
 
+def run_eol_fill(ctx, sc, n, reps=5):
+    """Forces::fill with the sheet's boundary as EOL nodes (the EOL rows of
+    Forces.cpp:177-329, 399-497, 580-683, 746-884) against the plain fill: wall time
+    per call (each call ends with a stream synchronisation)."""
+    ij = np.stack([np.arange(sc.N) % (n + 1), np.arange(sc.N) // (n + 1)], 1)
+    eol = np.where((ij[:, 0] == 0) | (ij[:, 0] == n) | (ij[:, 1] == 0) | (ij[:, 1] == n))[0]
+
+    def timed():
+        ctx.forces_fill(GRAV, H)
+        t = time.perf_counter()
+        for _ in range(reps):
+            ctx.forces_fill(GRAV, H)
+        return (time.perf_counter() - t) / reps * 1e3
+
+    ctx.upload_state(sc.x0, sc.v0)
+    plain = timed()
+    ctx.set_eol(eol)
+    with_eol = timed()
+    f = ctx.download_forces()
+    ctx.set_eol([])
+    out = {"workload": f"cloth {n}x{n}, the {len(eol)} boundary nodes as EOL nodes", "eol_nodes": int(len(eol)),
+           "ms_fill_plain": plain, "ms_fill_eol": with_eol, "rows": int(f.size),
+           "max_abs_f_X": float(np.max(np.abs(f[3 * sc.N:])))}
+    log(f"EOL fill: {out}")
+    return out
+
+
 def run_batched(steps, warmup, maxit, multilevel=True):
     """The many-cloth workload: 256 cloths, sides uniform in {16..128} quads, E
     log-uniform in [1e2, 1e4] Pa, beta log-uniform in [1e-6, 1e-4], the single-cloth
@@ -261,6 +288,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-cloth workload")
     ap.add_argument("--no-collide", action="store_true", help="skip the proximity-query measurement")
+    ap.add_argument("--no-eol", action="store_true", help="skip the EOL-rows fill measurement")
     ap.add_argument("--no-ml", action="store_true",
                     help="headline without the multilevel coarse levels (and skip the multilevel lines)")
     ap.add_argument("--profile-solve", action="store_true",
@@ -385,6 +413,7 @@ def main():
             "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname, args.n),
             "kernel": kname, "variant": tuned[32]["variant"], "algorithmic_bytes": alg,
             "avg_launch_ms": spmv32_ms, "copy_kernel_GBps": copy_gbps, "read_kernel_GBps": read_gbps}
+    eol_out = None if args.no_eol else run_eol_fill(ctx, sc, args.n)
     ctx.close()
     batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit, not args.no_ml)
     collision = None if args.no_collide else run_collision(10, not args.no_cpu)
@@ -415,6 +444,7 @@ def main():
         **extra,
         "multilevel_build_s": ml_build_s,
         "topology_rebuild_s": round(rebuild_s, 3),
+        "eol_fill": eol_out,
         "autotune": tuned,
         "roofline": roof,
         "cpu_baseline": cpu,
