@@ -1,0 +1,76 @@
+"""The path at the benchmark's sizes, through properties that do not need the oracle
+to finish a 1024^2 solve.
+
+- 512^2 with the benchmark's noise (the proximity workload's cloth): f, M and MDK
+  against the oracle's full fill, 1e-10 of the largest entry, same sparsity.
+- 1024^2, the benchmark's headline path (mixed precision, learned blocks at their
+  initial weights, multilevel levels, SPD-projected bending): the solved velocity
+  satisfies the FP64 system downloaded from the device, checked on the host with
+  scipy: ||b_F - A_FF v_F - A_FD v_D|| <= 1e-8 ||b_F - A_FD v_D||, with
+  b = M v_old + h f and the fixed DOFs at 0.99 v_old; and the mass of the sheet (sum of M) equals
+  rho * area in each coordinate."""
+import numpy as np
+import pytest
+
+import eolc
+import oracle as O
+from conftest import GRAV, H, rel_err
+from eolc import learned, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forces_parity_512(ctx):
+    sc = scene.Scene(512, sigma_x=1e-3, sigma_v=0.05)
+    print(f"forces parity 512x512: N={sc.N}", flush=True)
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(0)
+    ctx.upload_state(sc.x0, sc.v0)
+    ctx.forces_fill(GRAV, H)
+    fg, Mg, Kg = ctx.download_forces(), ctx.download_csr(eolc.MAT_M), ctx.download_csr(eolc.MAT_MDK)
+    rc, fo, Mo, Ko = O.forces_fill(sc.x0, sc.X, sc.tri, sc.hinge, O.material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0),
+                                   GRAV, H, nthreads=16)
+    assert rc == 0
+    for A, B in ((Mg, Mo), (Kg, Ko)):
+        A, B = A.tocsr(), B.tocsr()
+        A.sort_indices(); B.sort_indices()
+        assert np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
+        e = np.max(np.abs(A.data - B.data)) / np.max(np.abs(B.data))
+        print(f"  rel err {e:.2e} over {A.nnz} entries", flush=True)
+        assert e <= 1e-10
+    assert rel_err(fg, fo) <= 1e-10
+
+
+def test_headline_solve_1024_satisfies_the_downloaded_system(ctx):
+    n = 1024
+    sc = scene.Scene(n, sigma_x=1e-3, sigma_v=0.05)
+    print(f"headline solve {n}x{n}: N={sc.N}", flush=True)
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    ctx.learned_set_weights(learned.init_weights(3))
+    ctx.upload_state(sc.x0, sc.v0)
+    ctx.forces_fill(GRAV, H)
+    A = ctx.download_csr(eolc.MAT_MDK).tocsr()
+    M = ctx.download_csr(eolc.MAT_M).tocsr()
+    f = ctx.download_forces()
+    assert abs(M.sum() - 3 * 0.1 * 1.0) <= 1e-12
+    fd = (3 * np.asarray(sc.fixed)[:, None] + np.arange(3)).reshape(-1)
+    vals = 0.99 * sc.v0[fd]
+    mode = eolc.SOLVE_MIXED_LEARNED | eolc.SOLVE_MULTILEVEL
+    rc, st = ctx.velocity_solve(fd, vals, tol=1e-8, mode=mode)
+    assert rc == eolc.OK
+    _, v = ctx.download_state()
+    assert np.array_equal(v[fd], vals)
+    b = M @ sc.v0 + H * f
+    free = np.ones(3 * sc.N, bool)
+    free[fd] = False
+    vD = np.zeros(3 * sc.N)
+    vD[fd] = vals
+    b_red = (b - A @ vD)[free]
+    r = (b - A @ v)[free]
+    rel = np.linalg.norm(r) / np.linalg.norm(b_red)
+    print(f"  {st.iters} iterations; host FP64 residual {rel:.3e} (device reports {st.true_relres:.3e})", flush=True)
+    assert rel <= 1e-8
+    assert abs(rel - st.true_relres) <= 1e-3 * rel
