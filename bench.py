@@ -107,7 +107,14 @@ def cpu_baseline(sc, iters_per_step, threads):
                      nthreads=1)
     iter1 = (time.perf_counter() - t5) / 2
     step1 = fill1 + iters_per_step * iter1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": 1.0 / t_step, "unit": "steps/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "nproc": os.cpu_count(),
             "sample": f"oracle Forces::fill once + {k} FP64 block-Jacobi PCG iterations at "
                       f"{sc.n}x{sc.n} ({(t1 - t0):.2f} s fill, {t_iter * 1e3:.1f} ms/iter), "
                       f"scaled with {iters_per_step:.0f} iterations/step",
