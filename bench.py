@@ -142,9 +142,17 @@ def run_collision(reps, with_cpu):
         recs = ctx.collide(1e-3, pts, nrm, whd, E)
     t_gpu = (time.perf_counter() - t0) / reps
     ctx.close()
+    # the QP's collision rows from the records (Constraints::fill, host code): the
+    # library call on a contiguous record array (size query, then the rows)
+    arr = eolc.records_array(recs)
+    eolc.collision_constraints(arr, sc.N)   # warm-up (imports scipy.sparse)
+    t0 = time.perf_counter()
+    Aineq = eolc.collision_constraints(arr, sc.N)
+    t_rows = time.perf_counter() - t0
     out = {"workload": "cloth 512x512 quads, noise 1e-3 m, 64 points + 4 boxes, threshold 1e-3 (CD2)",
            "records": len(recs), "ms_per_call": t_gpu * 1e3, "calls_per_s": 1.0 / t_gpu,
-           "faces_tested_per_s": (64 + 4 * 8) * sc.F / t_gpu}
+           "faces_tested_per_s": (64 + 4 * 8) * sc.F / t_gpu,
+           "constraint_rows": {"rows": int(Aineq.shape[0]), "nnz": int(Aineq.nnz), "ms": t_rows * 1e3}}
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
@@ -205,6 +213,25 @@ def measured_traffic(kernel, n):
         if k.get("kernel_template") == kernel:
             return k.get("hbm_bytes")
     return None
+
+
+def run_transfer(sc, reps=3):
+    """Cloth::velocityTransfer after a remesh-like change at the benchmark's size: the
+    old 1024^2 mesh, every second old vertex kept (moved by 1e-9) and 1 M new points
+    at seeded random positions.  Device time per call (binning and search; the
+    uploads excluded), the old O(N_new N_old) scans being what it replaces."""
+    rng = np.random.default_rng(5)
+    P = np.concatenate([sc.X[::2] + 1e-9, rng.uniform(0.0, 1.0, size=(1_000_000, 2))])
+    with eolc.Context(0) as c:
+        ms = []
+        for _ in range(reps + 1):
+            rc, vg, sg, fg, bg, t = c.velocity_transfer(sc.X, sc.v0, sc.tri, P)
+            ms.append(t)
+    out = {"workload": f"old mesh {sc.N} vertices, {len(P)} new points", "ms_device": float(np.median(ms[1:])),
+           "points_per_s": len(P) / (np.median(ms[1:]) * 1e-3), "matched": int(np.sum(sg >= 0)),
+           "in_faces": int(np.sum(fg >= 0))}
+    log(f"velocity transfer: {out}")
+    return out
 
 
 def run_eol_fill(ctx, sc, n, reps=5):
@@ -296,6 +323,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-cloth workload")
     ap.add_argument("--no-collide", action="store_true", help="skip the proximity-query measurement")
     ap.add_argument("--no-eol", action="store_true", help="skip the EOL-rows fill measurement")
+    ap.add_argument("--no-transfer", action="store_true", help="skip the velocity-transfer measurement")
     ap.add_argument("--no-ml", action="store_true",
                     help="headline without the multilevel coarse levels (and skip the multilevel lines)")
     ap.add_argument("--profile-solve", action="store_true",
@@ -424,6 +452,7 @@ def main():
     ctx.close()
     batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit, not args.no_ml)
     collision = None if args.no_collide else run_collision(10, not args.no_cpu)
+    transfer = None if args.no_transfer else run_transfer(sc)
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -452,6 +481,7 @@ def main():
         "multilevel_build_s": ml_build_s,
         "topology_rebuild_s": round(rebuild_s, 3),
         "eol_fill": eol_out,
+        "velocity_transfer": transfer,
         "autotune": tuned,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -510,12 +510,20 @@ class Context:
         return losses
 
 
+def records_array(records):
+    """A contiguous ctypes array of CollisionRec from a list of records."""
+    return (CollisionRec * max(len(records), 1))(*records)
+
+
 def collision_constraints(records, N, eol=None):
     """Constraints::fill collision rows from CD2 records: scipy CSR A_ineq (rows x 3N).
     Host-only (no GPU context needed)."""
     import scipy.sparse as sp
     n = len(records)
-    buf = (CollisionRec * max(n, 1))(*records) if n else (CollisionRec * 1)()
+    if isinstance(records, C.Array):   # already contiguous (e.g. records_array)
+        buf = records if n else (CollisionRec * 1)()
+    else:
+        buf = (CollisionRec * max(n, 1))(*records) if n else (CollisionRec * 1)()
     e = None if eol is None else np.ascontiguousarray(eol, dtype=np.uint8)
     rows, nnz = C.c_int(0), C.c_int64(0)
     rc = lib().eolc_collision_constraints(None, buf, n, int(N), _ptr(e), None, None, None, 0, 0, C.byref(rows),
