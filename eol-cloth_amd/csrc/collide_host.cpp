@@ -456,7 +456,7 @@ int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double
 extern "C" int eolc_collide(eolc_ctx *c, double threshold, int P, const double *pts, const double *nrm, int B,
                             const double *whd, const double *E, int eol_points, int box_npts, int box_nedges,
                             eolc_collision *out, int cap, int *count)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!count) { c->err = "collide: null count"; return EOLC_ERR_INVALID; }
     *count = 0;
@@ -476,4 +476,6 @@ extern "C" int eolc_collide(eolc_ctx *c, double threshold, int P, const double *
     }
     if (!recs.empty()) std::memcpy(out, recs.data(), recs.size() * sizeof(eolc_collision));
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "collide");
 }

@@ -21,7 +21,7 @@
 extern "C" int eolc_collision_constraints(eolc_ctx *c, const eolc_collision *recs, int n, int N,
                                           const unsigned char *eol, int64_t *rowptr, int *col, double *val,
                                           int cap_rows, int64_t cap_nnz, int *rows, int64_t *nnz)
-{
+try {
     auto fail = [&](int code, const char *msg) {
         if (c) c->err = msg;
         return code;
@@ -85,4 +85,6 @@ extern "C" int eolc_collision_constraints(eolc_ctx *c, const eolc_collision *rec
         }
     }
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "collision_constraints");
 }

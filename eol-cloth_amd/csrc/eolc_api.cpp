@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <new>
 #include <thread>
 
@@ -131,7 +132,8 @@ int find_block(const Topology &t, int i, int j)
 }
 
 // Runs fn(begin, end) over contiguous chunks of [0, n) on up to 16 host threads
-// (one chunk, on the calling thread, below `grain` items per thread).
+// (one chunk, on the calling thread, below `grain` items per thread).  An exception
+// in a worker is rethrown on the calling thread after all workers joined.
 template <class Fn> void host_parallel(int64_t n, int64_t grain, Fn &&fn)
 {
     const int64_t hw = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
@@ -140,10 +142,20 @@ template <class Fn> void host_parallel(int64_t n, int64_t grain, Fn &&fn)
         fn((int64_t)0, n);
         return;
     }
+    std::vector<std::exception_ptr> err(T);
     std::vector<std::thread> th;
     th.reserve(T);
-    for (int k = 0; k < T; ++k) th.emplace_back([&fn, n, k, T] { fn(n * k / T, n * (k + 1) / T); });
+    for (int k = 0; k < T; ++k)
+        th.emplace_back([&fn, &err, n, k, T] {
+            try {
+                fn(n * k / T, n * (k + 1) / T);
+            } catch (...) {
+                err[k] = std::current_exception();
+            }
+        });
     for (auto &x : th) x.join();
+    for (auto &e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 // Exclusive prefix sum in place over v[0..n) into v[0..n] (v has n + 1 entries,
@@ -618,7 +630,7 @@ void drop_batch_graph(eolc_ctx *c)
 extern "C" {
 
 int eolc_create(int device, eolc_ctx **out)
-{
+try {
     if (!out) return EOLC_ERR_INVALID;
     *out = nullptr;
     int ndev = 0;
@@ -643,6 +655,8 @@ int eolc_create(int device, eolc_ctx **out)
     }
     *out = c;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(nullptr, "create");
 }
 
 void eolc_destroy(eolc_ctx *c)
@@ -664,8 +678,10 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
                              const std::vector<int> *seg);
 
 int eolc_set_topology(eolc_ctx *c, int N, const double *X, int F, const int *tri, int H, const int *hinge)
-{
+try {
     return set_topology_impl(c, N, X, F, tri, H, hinge, nullptr);
+} catch (...) {
+    return eolc::report_exception(c, "set_topology");
 }
 
 static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const int *tri, int H, const int *hinge,
@@ -757,7 +773,7 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
 
 int eolc_set_material(eolc_ctx *c, double density, double e, double nu, double beta, double dampingA,
                       double dampingB)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!(std::isfinite(density) && std::isfinite(e) && std::isfinite(nu) && std::isfinite(beta) &&
           std::isfinite(dampingB)) || nu >= 0.5 || nu <= -1.0)
@@ -767,10 +783,12 @@ int eolc_set_material(eolc_ctx *c, double density, double e, double nu, double b
     c->have_forces = false;
     drop_batch_graph(c);   // the captured step holds the old scalars
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_material");
 }
 
 int eolc_set_options(eolc_ctx *c, int flags)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (flags & ~(EOLC_OPT_INCLUDE_BENDING_FORCE | EOLC_OPT_SPD_PROJECT_BENDING))
         return fail(c, EOLC_ERR_INVALID, "set_options: unknown flag");
@@ -778,10 +796,12 @@ int eolc_set_options(eolc_ctx *c, int flags)
     c->have_forces = false;
     drop_batch_graph(c);   // the captured step chose its kernels by the old flags
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_options");
 }
 
 int eolc_upload_state(eolc_ctx *c, const double *x, const double *v)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_topo) return fail(c, EOLC_ERR_INVALID, "upload_state: no topology");
     if (!x || !v) return fail(c, EOLC_ERR_INVALID, "upload_state: null array");
@@ -796,10 +816,12 @@ int eolc_upload_state(eolc_ctx *c, const double *x, const double *v)
     c->have_state = true;
     c->have_forces = false;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "upload_state");
 }
 
 int eolc_download_state(eolc_ctx *c, double *x, double *v)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_state) return fail(c, EOLC_ERR_INVALID, "download_state: no state");
     EOLC_HIP(hipSetDevice(c->device));
@@ -808,6 +830,8 @@ int eolc_download_state(eolc_ctx *c, double *x, double *v)
     if (v) EOLC_HIP(hipMemcpyAsync(v, c->v, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "download_state");
 }
 
 static int forces_fill_async(eolc_ctx *c, const double grav[3], double h, bool want32)
@@ -822,7 +846,7 @@ static int forces_fill_async(eolc_ctx *c, const double grav[3], double h, bool w
 }
 
 int eolc_forces_fill(eolc_ctx *c, const double grav[3], double h)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_topo || !c->have_mat || !c->have_state)
         return fail(c, EOLC_ERR_INVALID, "forces_fill: topology, material and state must be set first");
@@ -834,10 +858,12 @@ int eolc_forces_fill(eolc_ctx *c, const double grav[3], double h)
     EOLC_HIP(eolc::eol_fill(c));
     EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "forces_fill");
 }
 
 int eolc_set_eol(eolc_ctx *c, int n, const int *nodes)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_topo) return fail(c, EOLC_ERR_INVALID, "set_eol: no topology");
     if (c->batch.B > 0) return fail(c, EOLC_ERR_INVALID, "set_eol: not for a batch topology");
@@ -957,10 +983,12 @@ int eolc_set_eol(eolc_ctx *c, int n, const int *nodes)
     EOLC_HIP(dalloc(c, &S.d_M, nnz));
     EOLC_HIP(dalloc(c, &S.d_fX, 2 * (size_t)n));
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_eol");
 }
 
 int eolc_download_forces(eolc_ctx *c, double *f)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_forces || !f) return fail(c, EOLC_ERR_INVALID, "download_forces: no forces");
     EOLC_HIP(hipSetDevice(c->device));
@@ -970,10 +998,12 @@ int eolc_download_forces(eolc_ctx *c, double *f)
                                 hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "download_forces");
 }
 
 int eolc_download_csr(eolc_ctx *c, int which, int64_t *rowptr, int *col, double *val, int64_t cap, int64_t *nnz)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_forces) return fail(c, EOLC_ERR_INVALID, "download_csr: call forces_fill first");
     if (which != EOLC_MAT_M && which != EOLC_MAT_MDK) return fail(c, EOLC_ERR_INVALID, "download_csr: bad matrix id");
@@ -1044,6 +1074,8 @@ int eolc_download_csr(eolc_ctx *c, int which, int64_t *rowptr, int *col, double 
     }
     rowptr[3 * N + 2 * S.E] = p;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "download_csr");
 }
 
 // ---------------------------------------------------------------------------
@@ -1198,7 +1230,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
 
 int eolc_velocity_solve(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *fixed_val, double tol,
                         int maxit, int mode, eolc_solve_stats *st)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_forces) return fail(c, EOLC_ERR_INVALID, "velocity_solve: call forces_fill first");
     if (n_fixed < 0 || (n_fixed && (!fixed_dof || !fixed_val)) || !(tol > 0.0) || maxit <= 0)
@@ -1207,6 +1239,8 @@ int eolc_velocity_solve(eolc_ctx *c, int n_fixed, const int *fixed_dof, const do
     EOLC_HIP(hipSetDevice(c->device));
     if (st) std::memset(st, 0, sizeof(*st));
     return solve_impl(c, n_fixed, fixed_dof, fixed_val, tol, maxit, mode, st);
+} catch (...) {
+    return eolc::report_exception(c, "velocity_solve");
 }
 
 static int transfer_impl(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
@@ -1217,16 +1251,18 @@ static int transfer_impl(eolc_ctx *c, int n_old, const double *u_old, const doub
 int eolc_velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
                            const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,
                            double *bary, double *ms)
-{
+try {
     return transfer_impl(c, n_old, u_old, v_old, f_old, tri_old, n_new, u_new, v_new, src, face, bary, ms, nullptr,
                          nullptr, 0, nullptr);
+} catch (...) {
+    return eolc::report_exception(c, "velocity_transfer");
 }
 
 int eolc_velocity_transfer_eol(eolc_ctx *c, int n_old, const double *u_old, const double *x_old, const double *v_old,
                                const int *eol_old, int n_eol, const double *V_old, int f_old, const int *tri_old,
                                int n_new, const double *u_new, double *v_new, int *src, int *face, double *bary,
                                double *ms)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (n_old <= 0 || n_eol < 0 || !x_old || !eol_old || (n_eol && !V_old))
         return fail(c, EOLC_ERR_INVALID, "velocity_transfer_eol: bad arguments");
@@ -1239,6 +1275,8 @@ int eolc_velocity_transfer_eol(eolc_ctx *c, int n_old, const double *u_old, cons
             return fail(c, EOLC_ERR_INVALID, "velocity_transfer_eol: EoL index out of range");
     return transfer_impl(c, n_old, u_old, v_old, f_old, tri_old, n_new, u_new, v_new, src, face, bary, ms, x_old,
                          eol_old, n_eol, V_old);
+} catch (...) {
+    return eolc::report_exception(c, "velocity_transfer_eol");
 }
 
 static int transfer_impl(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
@@ -1273,7 +1311,7 @@ static int transfer_impl(eolc_ctx *c, int n_old, const double *u_old, const doub
 }
 
 int eolc_ml_info(eolc_ctx *c, int *levels, int *counts)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_topo) return fail(c, EOLC_ERR_INVALID, "ml_info: no topology");
     if (!levels) return fail(c, EOLC_ERR_INVALID, "ml_info: null output");
@@ -1284,10 +1322,12 @@ int eolc_ml_info(eolc_ctx *c, int *levels, int *counts)
     if (counts)
         for (int l = 0; l <= c->ml.lv.L; ++l) counts[l] = c->ml.counts[l];
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "ml_info");
 }
 
 int eolc_ml_aggregates(eolc_ctx *c, int level, int *agg)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_topo) return fail(c, EOLC_ERR_INVALID, "ml_aggregates: no topology");
     EOLC_HIP(hipSetDevice(c->device));
@@ -1296,10 +1336,12 @@ int eolc_ml_aggregates(eolc_ctx *c, int level, int *agg)
     if (!agg || level < 1 || level > c->ml.lv.L) return fail(c, EOLC_ERR_INVALID, "ml_aggregates: bad level or null output");
     std::copy(c->ml.maps[level].begin(), c->ml.maps[level].end(), agg);
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "ml_aggregates");
 }
 
 int eolc_precond_apply(eolc_ctx *c, int n_fixed, const int *fixed_dof, int mode, const double *r, double *z)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_forces) return fail(c, EOLC_ERR_INVALID, "precond_apply: call forces_fill first");
     if (!r || !z || n_fixed < 0 || (n_fixed && !fixed_dof)) return fail(c, EOLC_ERR_INVALID, "precond_apply: bad arguments");
@@ -1336,6 +1378,8 @@ int eolc_precond_apply(eolc_ctx *c, int n_fixed, const int *fixed_dof, int mode,
         EOLC_HIP(hipStreamSynchronize(c->stream));
     }
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "precond_apply");
 }
 
 // One step with single-DOF fixed rows: v[dof] = (1 - 0.01) v_old[dof] + scripted.
@@ -1385,7 +1429,7 @@ static int step_impl(eolc_ctx *c, const double grav[3], double h, int n_dofs, co
 
 int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, const int *fixed_nodes,
               const double *scripted_vel, double tol, int maxit, int mode, eolc_solve_stats *st)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (n_fixed_nodes < 0 || (n_fixed_nodes && !fixed_nodes)) return fail(c, EOLC_ERR_INVALID, "step: bad arguments");
     std::vector<int> dofs(3 * (size_t)n_fixed_nodes);
@@ -1395,18 +1439,22 @@ int eolc_step(eolc_ctx *c, const double grav[3], double h, int n_fixed_nodes, co
         for (int k = 0; k < 3; ++k) dofs[3 * i + k] = 3 * fixed_nodes[i] + k;
     }
     return step_impl(c, grav, h, 3 * n_fixed_nodes, dofs.data(), scripted_vel, tol, maxit, mode, st);
+} catch (...) {
+    return eolc::report_exception(c, "step");
 }
 
 int eolc_step_dofs(eolc_ctx *c, const double grav[3], double h, int n_fixed_dofs, const int *fixed_dofs,
                    const double *scripted_vel, double tol, int maxit, int mode, eolc_solve_stats *st)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     return step_impl(c, grav, h, n_fixed_dofs, fixed_dofs, scripted_vel, tol, maxit, mode, st);
+} catch (...) {
+    return eolc::report_exception(c, "step_dofs");
 }
 
 // ---- learned block preconditioner ------------------------------------------
 int eolc_learned_set_weights(eolc_ctx *c, const float *theta, int n)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!theta || n != eolc::kLpParams) return fail(c, EOLC_ERR_INVALID, "learned_set_weights: need EOLC_LEARNED_NPARAMS values");
     for (int k = 0; k < n; ++k)
@@ -1429,10 +1477,12 @@ int eolc_learned_set_weights(eolc_ctx *c, const float *theta, int n)
     L.adam_t = 0;
     L.have_weights = true;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "learned_set_weights");
 }
 
 int eolc_learned_get_weights(eolc_ctx *c, float *theta, int n)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!theta || n != eolc::kLpParams) return fail(c, EOLC_ERR_INVALID, "learned_get_weights: need EOLC_LEARNED_NPARAMS values");
     if (!c->lp.have_weights) return fail(c, EOLC_ERR_INVALID, "learned_get_weights: no weights set");
@@ -1440,6 +1490,8 @@ int eolc_learned_get_weights(eolc_ctx *c, float *theta, int n)
     EOLC_HIP(hipMemcpyAsync(theta, c->lp.theta, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "learned_get_weights");
 }
 
 static int learned_ready(eolc_ctx *c, const char *who, int n_fixed, const int *fixed_dof)
@@ -1452,7 +1504,7 @@ static int learned_ready(eolc_ctx *c, const char *who, int n_fixed, const int *f
 }
 
 int eolc_learned_infer(eolc_ctx *c, int n_fixed, const int *fixed_dof, float *out, float *P)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     int rc = learned_ready(c, "learned_infer", n_fixed, fixed_dof);
     if (rc) return rc;
@@ -1469,6 +1521,8 @@ int eolc_learned_infer(eolc_ctx *c, int n_fixed, const int *fixed_dof, float *ou
         for (size_t i = 0; i < N; ++i)
             for (int k = 0; k < 9; ++k) P[9 * i + k] = soa[(size_t)eolc::kSymIdx[k] * N + i];
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "learned_infer");
 }
 
 static int learned_upload_error(eolc_ctx *c, const double *e)
@@ -1482,7 +1536,7 @@ static int learned_upload_error(eolc_ctx *c, const double *e)
 }
 
 int eolc_learned_loss_grad(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *e, double *loss, float *grad)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!e) return fail(c, EOLC_ERR_INVALID, "learned_loss_grad: null error vector");
     int rc = learned_ready(c, "learned_loss_grad", n_fixed, fixed_dof);
@@ -1498,11 +1552,13 @@ int eolc_learned_loss_grad(eolc_ctx *c, int n_fixed, const int *fixed_dof, const
     if (loss) *loss = sc[4];
     if (!std::isfinite(sc[4]) || !(sc[1] > 0.0)) return fail(c, EOLC_ERR_NONFINITE, "learned_loss_grad: r.e <= 0 or non-finite loss");
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "learned_loss_grad");
 }
 
 int eolc_learned_train(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *e, int steps, double lr,
                        double *losses)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!e || steps <= 0 || !(lr > 0.0)) return fail(c, EOLC_ERR_INVALID, "learned_train: bad arguments");
     int rc = learned_ready(c, "learned_train", n_fixed, fixed_dof);
@@ -1517,6 +1573,8 @@ int eolc_learned_train(eolc_ctx *c, int n_fixed, const int *fixed_dof, const dou
     }
     EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "learned_train");
 }
 
 // ---- batched cloths -------------------------------------------------------------
@@ -1524,7 +1582,7 @@ static void batch_invalidate_graph(eolc_ctx *c) { drop_batch_graph(c); }
 
 int eolc_set_batch_topology(eolc_ctx *c, int B, const int *Nb, const double *X, const int *Fb, const int *tri,
                             const int *Hb, const int *hinge, const int *nfixb, const int *fixed, int *vert_off)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (B <= 0 || !Nb || !X || !Fb || !Hb || !nfixb)
         return fail(c, EOLC_ERR_INVALID, "set_batch_topology: bad sizes or null arrays");
@@ -1612,10 +1670,12 @@ int eolc_set_batch_topology(eolc_ctx *c, int B, const int *Nb, const double *X, 
     if (vert_off)
         for (int b = 0; b <= B; ++b) vert_off[b] = off[b];
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_batch_topology");
 }
 
 int eolc_set_batch_material(eolc_ctx *c, const double *E, const double *beta)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     eolc::BatchState &S = c->batch;
     if (S.B <= 0) return fail(c, EOLC_ERR_INVALID, "set_batch_material: no batch topology");
@@ -1646,6 +1706,8 @@ int eolc_set_batch_material(eolc_ctx *c, const double *E, const double *beta)
     c->have_forces = false;
     batch_invalidate_graph(c);
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_batch_material");
 }
 
 // One batched step: forces, rhs, fixed values, persistent solve, x += h v.
@@ -1695,7 +1757,7 @@ static int batch_collect(eolc_ctx *c, eolc_solve_stats *stats, double ms)
 
 int eolc_batch_step(eolc_ctx *c, const double grav[3], double h, double tol, int maxit, int flags,
                     eolc_solve_stats *stats)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     int rc = batch_ready(c, grav, h, tol, maxit);
     if (rc) return rc;
@@ -1744,10 +1806,12 @@ int eolc_batch_step(eolc_ctx *c, const double grav[3], double h, double tol, int
     c->have_forces = true;
     c->h_last = h;
     return batch_collect(c, stats, ms);
+} catch (...) {
+    return eolc::report_exception(c, "batch_step");
 }
 
 int eolc_batch_autotune(eolc_ctx *c, const double grav[3], double h, double tol, int maxit, int *best, double *ms)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     int rc = batch_ready(c, grav, h, tol, maxit);
     if (rc) return rc;
@@ -1794,10 +1858,12 @@ int eolc_batch_autotune(eolc_ctx *c, const double grav[3], double h, double tol,
     c->have_forces = false;
     if (best) *best = arg;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "batch_autotune");
 }
 
 int eolc_set_batch_scripted(eolc_ctx *c, const double *vel)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     eolc::BatchState &S = c->batch;
     if (S.B <= 0) return fail(c, EOLC_ERR_INVALID, "set_batch_scripted: no batch topology");
@@ -1815,10 +1881,12 @@ int eolc_set_batch_scripted(eolc_ctx *c, const double *vel)
     EOLC_HIP(hipMemcpyAsync(S.d_scripted, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_batch_scripted");
 }
 
 int eolc_set_batch_multilevel(eolc_ctx *c, int on)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     eolc::BatchState &S = c->batch;
     if (S.B <= 0) return fail(c, EOLC_ERR_INVALID, "set_batch_multilevel: no batch topology");
@@ -1849,19 +1917,23 @@ int eolc_set_batch_multilevel(eolc_ctx *c, int on)
     S.ml = on != 0 && c->ml.lv.L > 0;
     batch_invalidate_graph(c);
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_batch_multilevel");
 }
 
 int eolc_set_batch_variant(eolc_ctx *c, int variant)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (variant < 0 || variant >= eolc::kBatchVariants) return fail(c, EOLC_ERR_INVALID, "set_batch_variant: bad variant");
     c->batch.variant = variant;
     batch_invalidate_graph(c);
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_batch_variant");
 }
 
 int eolc_time_spmv(eolc_ctx *c, int precision, int reps, double *avg_ms)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_forces || (precision != 32 && precision != 64) || reps <= 0 || !avg_ms)
         return fail(c, EOLC_ERR_INVALID, "time_spmv: bad arguments or no assembled matrix");
@@ -1870,19 +1942,23 @@ int eolc_time_spmv(eolc_ctx *c, int precision, int reps, double *avg_ms)
     EOLC_HIP(eolc::time_spmv(c, precision == 32, -1, reps, &ms));
     *avg_ms = ms / reps;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "time_spmv");
 }
 
 int eolc_set_mixed_delta(eolc_ctx *c, double delta)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!(delta > 0.0 && delta < 1.0)) return fail(c, EOLC_ERR_INVALID, "set_mixed_delta: need 0 < delta < 1");
     c->mixed_delta = delta;
     drop_batch_graph(c);
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_mixed_delta");
 }
 
 int eolc_stream_copy(eolc_ctx *c, int64_t bytes, int reps, double *avg_ms)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (bytes < 16 || reps <= 0 || !avg_ms) return fail(c, EOLC_ERR_INVALID, "stream_copy: bad arguments");
     EOLC_HIP(hipSetDevice(c->device));
@@ -1890,10 +1966,12 @@ int eolc_stream_copy(eolc_ctx *c, int64_t bytes, int reps, double *avg_ms)
     EOLC_HIP(eolc::stream_copy(c, (size_t)bytes, reps, &ms, 0));
     *avg_ms = ms / reps;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "stream_copy");
 }
 
 int eolc_stream_read(eolc_ctx *c, int64_t bytes, int reps, double *avg_ms)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (bytes < 16 || reps <= 0 || !avg_ms) return fail(c, EOLC_ERR_INVALID, "stream_read: bad arguments");
     EOLC_HIP(hipSetDevice(c->device));
@@ -1901,10 +1979,12 @@ int eolc_stream_read(eolc_ctx *c, int64_t bytes, int reps, double *avg_ms)
     EOLC_HIP(eolc::stream_copy(c, (size_t)bytes, reps, &ms, 1));
     *avg_ms = ms / reps;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "stream_read");
 }
 
 int eolc_autotune(eolc_ctx *c, int precision, int reps, int *best, double *ms)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if (!c->have_forces || (precision != 32 && precision != 64) || reps <= 0)
         return fail(c, EOLC_ERR_INVALID, "autotune: bad arguments or no assembled matrix");
@@ -1924,15 +2004,19 @@ int eolc_autotune(eolc_ctx *c, int precision, int reps, int *best, double *ms)
     c->spmv_variant[precision == 32 ? 0 : 1] = arg;
     if (best) *best = arg;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "autotune");
 }
 
 int eolc_set_spmv_variant(eolc_ctx *c, int precision, int variant)
-{
+try {
     if (!c) return EOLC_ERR_INVALID;
     if ((precision != 32 && precision != 64) || variant < 0 || variant >= eolc::kSpmvVariants)
         return fail(c, EOLC_ERR_INVALID, "set_spmv_variant: bad precision or variant");
     c->spmv_variant[precision == 32 ? 0 : 1] = variant;
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_spmv_variant");
 }
 
 }  // extern "C"

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -304,6 +306,32 @@ struct eolc_ctx {
 };
 
 namespace eolc {
+// Every ABI function body is a function-try-block whose handler calls this: no
+// exception crosses the C ABI.  A failed host allocation reports
+// EOLC_ERR_NO_MEMORY, anything else EOLC_ERR_INVALID; the message goes to c->err
+// when there is a context.
+inline int report_exception(eolc_ctx *c, const char *who) noexcept
+{
+    int code = EOLC_ERR_INVALID;
+    const char *what = "unexpected exception";
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        code = EOLC_ERR_NO_MEMORY;
+        what = "out of host memory";
+    } catch (const std::exception &e) {
+        what = e.what();
+    } catch (...) {
+    }
+    if (c) {
+        try {
+            c->err = std::string(who) + ": " + what;
+        } catch (...) {
+        }
+    }
+    return code;
+}
+
 // kernels (k_elements.hip, k_assemble.hip, k_pcg.hip)
 hipError_t launch_face_eval(eolc_ctx *c, double h, const double *grav);
 hipError_t launch_hinge_eval(eolc_ctx *c, double h);

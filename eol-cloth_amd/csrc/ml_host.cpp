@@ -229,7 +229,7 @@ std::string ml_build(const Topology &t, std::vector<std::vector<int>> &maps, std
 // Host-only form for callers and tests without a context: the hierarchy of a block
 // pattern given as vertex CSR (rowptr: N + 1, col: columns of the stored blocks).
 extern "C" int eolc_ml_partition(int N, const int *rowptr, const int *col, int *levels, int *counts, int *agg)
-{
+try {
     if (N <= 0 || !rowptr || !col || !levels) return EOLC_ERR_INVALID;
     eolc::Topology t;
     t.N = N;
@@ -251,4 +251,6 @@ extern "C" int eolc_ml_partition(int N, const int *rowptr, const int *col, int *
     if (agg)
         for (int l = 1; l <= lv.L; ++l) std::copy(maps[l].begin(), maps[l].end(), agg + (size_t)(l - 1) * N);
     return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(nullptr, "ml_partition");
 }

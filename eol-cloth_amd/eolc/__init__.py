@@ -13,9 +13,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libeolc.so")
 
-OK, ERR_INVALID, ERR_HIP, ERR_NOT_SPD, ERR_MAXIT, ERR_OVERFLOW, ERR_NONFINITE, ERR_DEGENERATE, ERR_NO_DEVICE = range(9)
+(OK, ERR_INVALID, ERR_HIP, ERR_NOT_SPD, ERR_MAXIT, ERR_OVERFLOW, ERR_NONFINITE, ERR_DEGENERATE, ERR_NO_DEVICE,
+ ERR_NO_MEMORY) = range(10)
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "NOT_SPD", 4: "MAXIT", 5: "OVERFLOW",
-                6: "NONFINITE", 7: "DEGENERATE", 8: "NO_DEVICE"}
+                6: "NONFINITE", 7: "DEGENERATE", 8: "NO_DEVICE", 9: "NO_MEMORY"}
 
 OPT_INCLUDE_BENDING_FORCE = 0x1
 OPT_SPD_PROJECT_BENDING = 0x2

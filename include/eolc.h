@@ -34,7 +34,8 @@ typedef enum {
     EOLC_ERR_OVERFLOW = 5,    /* caller buffer too small; required size returned */
     EOLC_ERR_NONFINITE = 6,   /* NaN/Inf met in the state or the solve */
     EOLC_ERR_DEGENERATE = 7,  /* non-positive material-space area (CW face) */
-    EOLC_ERR_NO_DEVICE = 8
+    EOLC_ERR_NO_DEVICE = 8,
+    EOLC_ERR_NO_MEMORY = 9    /* a host allocation failed (nothing is thrown across the ABI) */
 } eolc_status;
 
 /* option flags (eolc_set_options) */
