@@ -876,12 +876,15 @@ try {
             return fail(c, EOLC_ERR_INVALID, "set_eol: node out of range or repeated");
         eidx[nodes[k]] = k;
     }
-    EOLC_HIP(hipSetDevice(c->device));
-    EOLC_HIP(hipStreamSynchronize(c->stream));
-    free_eol(c);
-    c->have_forces = false;
-    if (n == 0) return EOLC_OK;
-    eolc::EolState &S = c->eol;
+    if (n == 0) {
+        EOLC_HIP(hipSetDevice(c->device));
+        EOLC_HIP(hipStreamSynchronize(c->stream));
+        free_eol(c);
+        c->have_forces = false;
+        return EOLC_OK;
+    }
+    // host maps first (the current EOL state stays as it is if this fails)
+    eolc::EolState S;
     S.E = n;
     S.nodes.assign(nodes, nodes + n);
     const int *tri = c->topo_tri.data(), *hinge = c->topo_hinge.data();
@@ -965,23 +968,37 @@ try {
     if (!xslots.empty()) xvptr.push_back((int)xvslot.size());
     S.nxv = (int)xv.size();
     const size_t nst = (size_t)soff.back(), nsf = (size_t)foff.back(), nnz = S.col.size();
-    EOLC_HIP(upload(c, &S.d_elem, elem));
-    EOLC_HIP(upload(c, &S.d_eidx, eidx));
-    EOLC_HIP(upload(c, &S.d_soff, soff));
-    EOLC_HIP(upload(c, &S.d_foff, foff));
-    EOLC_HIP(upload(c, &S.d_cptr, cptr));
-    EOLC_HIP(upload(c, &S.d_clist, clist));
-    EOLC_HIP(upload(c, &S.d_fptr, fptr));
-    EOLC_HIP(upload(c, &S.d_flist, flist));
-    EOLC_HIP(upload(c, &S.d_xv, xv));
-    EOLC_HIP(upload(c, &S.d_xvptr, xvptr));
-    EOLC_HIP(upload(c, &S.d_xvslot, xvslot));
-    EOLC_HIP(dalloc(c, &S.d_stK, nst));
-    EOLC_HIP(dalloc(c, &S.d_stM, nst));
-    EOLC_HIP(dalloc(c, &S.d_stF, nsf));
-    EOLC_HIP(dalloc(c, &S.d_A, nnz));
-    EOLC_HIP(dalloc(c, &S.d_M, nnz));
-    EOLC_HIP(dalloc(c, &S.d_fX, 2 * (size_t)n));
+    // then the device: the old state is released, the new one uploaded; a failure
+    // leaves no EOL nodes set
+    EOLC_HIP(hipSetDevice(c->device));
+    EOLC_HIP(hipStreamSynchronize(c->stream));
+    free_eol(c);
+    c->have_forces = false;
+    c->eol = std::move(S);
+    eolc::EolState &D = c->eol;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t r) { if (e == hipSuccess) e = r; };
+    step(upload(c, &D.d_elem, elem));
+    step(upload(c, &D.d_eidx, eidx));
+    step(upload(c, &D.d_soff, soff));
+    step(upload(c, &D.d_foff, foff));
+    step(upload(c, &D.d_cptr, cptr));
+    step(upload(c, &D.d_clist, clist));
+    step(upload(c, &D.d_fptr, fptr));
+    step(upload(c, &D.d_flist, flist));
+    step(upload(c, &D.d_xv, xv));
+    step(upload(c, &D.d_xvptr, xvptr));
+    step(upload(c, &D.d_xvslot, xvslot));
+    step(dalloc(c, &D.d_stK, nst));
+    step(dalloc(c, &D.d_stM, nst));
+    step(dalloc(c, &D.d_stF, nsf));
+    step(dalloc(c, &D.d_A, nnz));
+    step(dalloc(c, &D.d_M, nnz));
+    step(dalloc(c, &D.d_fX, 2 * (size_t)n));
+    if (e != hipSuccess) {
+        free_eol(c);
+        return fail(c, EOLC_ERR_HIP, std::string("set_eol: HIP error ") + hipGetErrorString(e));
+    }
     return EOLC_OK;
 } catch (...) {
     return eolc::report_exception(c, "set_eol");
