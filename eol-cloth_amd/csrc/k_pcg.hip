@@ -135,14 +135,14 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
            SymLayout sym)
 {
     static_assert(U == 1 || U == 2, "entries per trip");
-    __shared__ double lds[2 * (WG / 64) + 2];
+    __shared__ double lds[3 * (WG / 64) + 3];
     if (scal->done) return;
     double cur[2];
-    reduce_partials<2>(PB(partials, k & 1), cur, lds);
-    const double rzk = cur[0], rrk = cur[1];
     const bool restart_beta0 = (k == scal->first_iter);
     double prev[1] = {1.0};
-    if (!restart_beta0) reduce_partials<1>(PB(partials, (k + 1) & 1), prev, lds);
+    if (restart_beta0) reduce_partials<2>(PB(partials, k & 1), cur, lds);
+    else reduce_partials2<2, 1>(PB(partials, k & 1), PB(partials, (k + 1) & 1), cur, prev, lds);
+    const double rzk = cur[0], rrk = cur[1];
     const double thr = fmax(scal->tol2 * scal->bnorm2, scal->delta2 * scal->rmax2);
     if (!isfinite(rrk) || !isfinite(rzk) || rrk <= thr) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -339,8 +339,7 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
     __shared__ double lds[2 * (WG / 64) + 2];
     if (scal->done) return;
     double pq[1], rz[1];
-    reduce_partials<1>(PA(partials), pq, lds);
-    reduce_partials<1>(PB(partials, k & 1), rz, lds);
+    reduce_partials2<1, 1>(PA(partials), PB(partials, k & 1), pq, rz, lds);
     if (!(pq[0] > 0.0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             scal->done = 1;

@@ -80,6 +80,30 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
     block_allsum<K>(out, lds);
 }
 
+// Two partial arrays in one pass (K1 sums of p1, then K2 of p2): the same per-array
+// order as two reduce_partials calls, so the same bits, with one load round and one
+// workgroup sum instead of two.  lds: (K1 + K2) * WG/64 + K1 + K2 doubles.
+template <int K1, int K2>
+__device__ __forceinline__ void reduce_partials2(const double *__restrict__ p1, const double *__restrict__ p2,
+                                                 double (&o1)[K1], double (&o2)[K2], double *lds)
+{
+    const int G = gridDim.x;
+    double v[K1 + K2];
+#pragma unroll
+    for (int k = 0; k < K1 + K2; ++k) v[k] = 0.0;
+    for (int i = threadIdx.x; i < G; i += WG) {
+#pragma unroll
+        for (int k = 0; k < K1; ++k) v[k] += p1[k * eolc::kMaxPartials + i];
+#pragma unroll
+        for (int k = 0; k < K2; ++k) v[K1 + k] += p2[k * eolc::kMaxPartials + i];
+    }
+    block_allsum<K1 + K2>(v, lds);
+#pragma unroll
+    for (int k = 0; k < K1; ++k) o1[k] = v[k];
+#pragma unroll
+    for (int k = 0; k < K2; ++k) o2[k] = v[K1 + k];
+}
+
 template <int K>
 __device__ __forceinline__ void write_partials(double (&v)[K], double *__restrict__ part, double *lds)
 {
