@@ -137,6 +137,24 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
     static_assert(U == 1 || U == 2, "entries per trip");
     __shared__ double lds[3 * (WG / 64) + 3];
     if (scal->done) return;
+    // the own row's indices and vectors do not depend on beta: their loads are
+    // issued before the partial sums are re-reduced, so the two latencies overlap
+    const int sl = wave_slice();
+    const int lane = threadIdx.x & 63;
+    const int g = sl * eolc::kSlice + lane;
+    const bool active = sl < nslices && g < N;
+    int row = 0, len = 0, base = 0;
+    T zi0 = 0, zi1 = 0, zi2 = 0, po0 = 0, po1 = 0, po2 = 0, mb = 0;
+    if (active) {
+        row = srow[g];
+        // every row holds its diagonal block as the last entry; the FP32 difference
+        // form skips it (its S' block is 0 and the row mass is applied below)
+        len = slen[g] - (DIFF ? 1 : 0);
+        base = slice_off[sl] + lane;
+        zi0 = z[3 * row]; zi1 = z[3 * row + 1]; zi2 = z[3 * row + 2];
+        po0 = pold[3 * row]; po1 = pold[3 * row + 1]; po2 = pold[3 * row + 2];
+        if constexpr (DIFF) mb = Mm[row];
+    }
     double cur[2];
     const bool restart_beta0 = (k == scal->first_iter);
     double prev[1] = {1.0};
@@ -155,28 +173,19 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
     }
     const double beta = restart_beta0 ? 0.0 : rzk / prev[0];
     const T betaT = (T)beta;   // p_new = z + beta p is formed in T, identically everywhere
-    const int sl = wave_slice();
-    const int lane = threadIdx.x & 63;
-    const int g = sl * eolc::kSlice + lane;
     double dot = 0.0;
-    if (sl < nslices && g < N) {
-        const int row = srow[g];
-        // every row holds its diagonal block as the last entry; the FP32 difference
-        // form skips it (its S' block is 0 and the row mass is applied below)
-        const int len = slen[g] - (DIFF ? 1 : 0);
-        const int base = slice_off[sl] + lane;
+    if (active) {
         const T *__restrict__ a = A + base;
         const int *__restrict__ cl = scol + base;
-        T ui0 = fma(betaT, pold[3 * row], z[3 * row]);
-        T ui1 = fma(betaT, pold[3 * row + 1], z[3 * row + 1]);
-        T ui2 = fma(betaT, pold[3 * row + 2], z[3 * row + 2]);
+        T ui0 = fma(betaT, po0, zi0);
+        T ui1 = fma(betaT, po1, zi1);
+        T ui2 = fma(betaT, po2, zi2);
         if constexpr (!DIFF) {
             // the own row's p is not needed inside the loop: store it now, reload after
             pnew[3 * row] = ui0; pnew[3 * row + 1] = ui1; pnew[3 * row + 2] = ui2;
         }
         double y0 = 0, y1 = 0, y2 = 0;
         if constexpr (DIFF) {
-            const T mb = Mm[row];
             y0 = (double)(mb * ui0);
             y1 = (double)(mb * ui1);
             y2 = (double)(mb * ui2);
@@ -338,6 +347,19 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
 {
     __shared__ double lds[2 * (WG / 64) + 2];
     if (scal->done) return;
+    // the thread's first vertex is loaded before the partial sums are re-reduced
+    // (the loads do not depend on alpha), so the two latencies overlap
+    int rb, re;
+    wg_rows(N, &rb, &re);
+    const int i0 = rb + (int)threadIdx.x;
+    T pv[3] = {0, 0, 0}, qv[3] = {0, 0, 0}, rv[3] = {0, 0, 0};
+    TX xv[3] = {0, 0, 0};
+    if (i0 < re) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            pv[c] = pnew[3 * i0 + c]; qv[c] = q[3 * i0 + c]; rv[c] = r[3 * i0 + c]; xv[c] = x[3 * i0 + c];
+        }
+    }
     double pq[1], rz[1];
     reduce_partials2<1, 1>(PA(partials), PB(partials, k & 1), pq, rz, lds);
     if (!(pq[0] > 0.0)) {
@@ -350,15 +372,19 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
     }
     const double alpha = rz[0] / pq[0];
     const T alphaT = (T)alpha;
-    int rb, re;
-    wg_rows(N, &rb, &re);
     double srz = 0.0, srr = 0.0;
-    for (int i = rb + threadIdx.x; i < re; i += WG) {
+    for (int i = i0; i < re; i += WG) {
+        if (i != i0) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) x[3 * i + c] = (TX)fma(alphaT, pnew[3 * i + c], (T)x[3 * i + c]);
-        const T r0 = fma(-alphaT, q[3 * i], r[3 * i]);
-        const T r1 = fma(-alphaT, q[3 * i + 1], r[3 * i + 1]);
-        const T r2 = fma(-alphaT, q[3 * i + 2], r[3 * i + 2]);
+            for (int c = 0; c < 3; ++c) {
+                pv[c] = pnew[3 * i + c]; qv[c] = q[3 * i + c]; rv[c] = r[3 * i + c]; xv[c] = x[3 * i + c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[3 * i + c] = (TX)fma(alphaT, pv[c], (T)xv[c]);
+        const T r0 = fma(-alphaT, qv[0], rv[0]);
+        const T r1 = fma(-alphaT, qv[1], rv[1]);
+        const T r2 = fma(-alphaT, qv[2], rv[2]);
         r[3 * i] = r0; r[3 * i + 1] = r1; r[3 * i + 2] = r2;
         if constexpr (!ML) {
             T z0, z1, z2;
