@@ -173,11 +173,12 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
                                                              const SolverScalars *scal, int check_done)
 {
     __shared__ double stage[kRestrictWG / 64][3 * 64];
-    if (check_done && scal->done) return;
     const int lane = threadIdx.x % 64;
     double *__restrict__ buf = stage[threadIdx.x / 64];
     const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;   // uniform in the wavefront
     int a0, a1;
+    // the aggregate's child range is read together with the stop flag (one
+    // latency round instead of two; a stopped launch only reads these two words)
     if (lv.L >= 2) {
         if (w >= lv.n[2]) return;
         a0 = cptr[lv.coff[2] + w];
@@ -187,6 +188,7 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
         if (a0 >= lv.n[1]) return;
         a1 = min(a0 + 64, lv.n[1]);
     }
+    if (check_done && scal->done) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int g0 = a0; g0 < a1; g0 += 64) {   // groups of up to 64 children
         const int g1 = min(g0 + 64, a1);
@@ -240,7 +242,6 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
                                                          const SolverScalars *scal, int check_done)
 {
     __shared__ double lds[6 * kCoarseLds];
-    if (check_done && scal->done) return;
     const int L = lv.L;
     const int base = lv.off[3];
     const int M = lv.off[L + 1] - base;   // aggregates of levels >= 3
@@ -261,6 +262,11 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
             if (l < L) pa = par[g];
             c0 = cptr[lv.coff[l] + a];
             c1 = cptr[lv.coff[l] + a + 1];
+        }
+        // the stop flag is read with the first loads (uniform: every thread leaves
+        // before the first barrier, or none does)
+        if (check_done && scal->done) return;
+        if (j < M) {
             if (l == 3) {
                 const double *src = R + 3 * (int64_t)lv.off[2];
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -312,6 +318,7 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
         }
         return;
     }
+    if (check_done && scal->done) return;
     const bool in_lds = M <= kCoarseLds;
     double *Rb = in_lds ? lds : R + 3 * (int64_t)base;   // level l at off[l] - off[3]
     double *Cb = in_lds ? lds + 3 * kCoarseLds : C + 3 * (int64_t)base;
