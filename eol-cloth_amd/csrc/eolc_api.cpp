@@ -103,7 +103,7 @@ void free_topology_buffers(eolc_ctx *c)
     free_eol(c);
     eolc::MLState &ml = c->ml;
     dfree(ml.hord); dfree(ml.ptr1); dfree(ml.agg1); dfree(ml.gpos); dfree(ml.cptr); dfree(ml.par); dfree(ml.lam);
-    dfree(ml.V); dfree(ml.Dinv); dfree(ml.R); dfree(ml.C);
+    dfree(ml.V); dfree(ml.Dinv); dfree(ml.R); dfree(ml.C); dfree(ml.anc3);
     ml.built = false;
     ml.lv = eolc::MLLevels{};
     ml.counts.clear();
@@ -556,6 +556,11 @@ int ml_prepare(eolc_ctx *c)
         EOLC_HIP(dalloc(c, &m.Dinv, 6 * na));
         EOLC_HIP(dalloc(c, &m.R, 3 * na));
         EOLC_HIP(dalloc(c, &m.C, 3 * na));
+        if (lv.L >= 3) {
+            std::vector<int> anc3((size_t)lv.n[1]);
+            for (int a = 0; a < lv.n[1]; ++a) anc3[a] = lv.off[3] + par[(size_t)lv.off[2] + par[a]];
+            EOLC_HIP(upload(c, &m.anc3, anc3));
+        }
     }
     m.built = true;
     return EOLC_OK;

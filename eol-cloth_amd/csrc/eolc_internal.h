@@ -192,7 +192,9 @@ struct MLState {
     double *V = nullptr;          // set-up sums: level l holds L x 9 x n_l
     double *Dinv = nullptr;       // 6 per aggregate: inverse of the Galerkin diagonal block
     double *R = nullptr;          // 3 per aggregate: restricted residual
-    double *C = nullptr;          // 3 per aggregate: accumulated coarse correction
+    double *C = nullptr;          // 3 per aggregate: accumulated coarse correction; for a
+                                  // level-1 aggregate a (L >= 2): D_1^-1 R_1(a) + D_2^-1 R_2(parent)
+    int *anc3 = nullptr;          // n_1 (L >= 3): global index of each level-1 aggregate's level-3 ancestor
     std::vector<int> counts;      // host copy of n[0..L]
     std::vector<std::vector<int>> maps;   // host: level-l aggregate of every vertex (l = 1..L)
 };

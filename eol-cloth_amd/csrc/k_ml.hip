@@ -20,11 +20,11 @@
 //
 // Per PCG iteration, after the vector update (ml_apply):
 //   k_ml_restrict : one wavefront per level-2 aggregate, vertices staged through LDS:
-//                   R_1(a) = sum of r over a's members, R_2 = sum of the children's R_1
+//                   R_1(a) = sum of r over a's members, R_2 = sum of the children's R_1,
+//                   and per level-1 aggregate a, C(a) = D_1^-1 R_1(a) + D_2^-1 R_2(parent)
 //   k_ml_coarse   : one workgroup (levels >= 3 only): R_l for l >= 3, then
 //                   C_l = D_l^-1 R_l + C_{l+1}(parent) from the top level down to 3
-//   k_ml_prolong  : z_i = D0 r_i + D_1^-1 R_1(a_i) + D_2^-1 R_2(b_i) + C_3(parent(b_i)),
-//                   b_i the level-2 aggregate; r.z partials
+//   k_ml_prolong  : z_i = D0 r_i + C(a_i) + C_3(level-3 ancestor of a_i); r.z partials
 // All coarse arithmetic is FP64 in a fixed order, so the result does not depend on
 // the launch shape.
 #include "eolc_internal.h"
@@ -170,6 +170,7 @@ template <class T>
 __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const int *__restrict__ hord,
                                                              const int *__restrict__ ptr1, const int *__restrict__ cptr,
                                                              const T *__restrict__ r, double *__restrict__ R,
+                                                             const double *__restrict__ Dinv, double *__restrict__ C,
                                                              const SolverScalars *scal, int check_done)
 {
     __shared__ double stage[kRestrictWG / 64][3 * 64];
@@ -220,12 +221,26 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
         }
     }
     if (lv.L >= 2) {
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        s2 = wave_sum(s2);
+        // wave_sum leaves the total in lane 0; every lane needs it below
+        s0 = __shfl(wave_sum(s0), 0, 64);
+        s1 = __shfl(wave_sum(s1), 0, 64);
+        s2 = __shfl(wave_sum(s2), 0, 64);
+        const int64_t gw = lv.off[2] + w;
         if (lane == 0) {
-            double *o = R + 3 * (int64_t)(lv.off[2] + w);
+            double *o = R + 3 * gw;
             o[0] = s0; o[1] = s1; o[2] = s2;
+        }
+        // levels 1 and 2 of each child, for the prolongation: C(a) = D_1^-1 R_1(a) +
+        // D_2^-1 R_2(w) (each lane re-reads the R_1 it wrote above)
+        double e0, e1, e2;
+        apply_inv(Dinv + 6 * gw, s0, s1, s2, e0, e1, e2);
+        for (int g0 = a0; g0 < a1; g0 += 64) {
+            const int a = g0 + lane;
+            if (a < a1) {
+                double c0, c1, c2;
+                apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
+                C[3 * a] = c0 + e0; C[3 * a + 1] = c1 + e1; C[3 * a + 2] = c2 + e2;
+            }
         }
     }
 }
@@ -373,7 +388,7 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
 // arrive accumulated in C_3.
 template <class T>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
-k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ par,
+k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ anc3,
              const double *__restrict__ Dinv, const double *__restrict__ R, const double *__restrict__ C,
              const T *__restrict__ D, const T *__restrict__ r, T *__restrict__ z,
              const unsigned char *__restrict__ fixmask, const SolverScalars *scal, double *partials, int check_done)
@@ -386,16 +401,15 @@ k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const
     for (int i = rb + threadIdx.x; i < re; i += WG) {
         const int a = agg1[i];
         double c0, c1, c2;
-        apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
         if (lv.L >= 2) {
-            const int64_t g = lv.off[2] + par[a];
-            double d0, d1, d2;
-            apply_inv(Dinv + 6 * g, R[3 * g], R[3 * g + 1], R[3 * g + 2], d0, d1, d2);
+            // levels 1 and 2, combined per level-1 aggregate by the restriction
+            c0 = C[3 * a]; c1 = C[3 * a + 1]; c2 = C[3 * a + 2];
             if (lv.L >= 3) {
-                const double *up = C + 3 * (int64_t)(lv.off[3] + par[g]);
-                d0 += up[0]; d1 += up[1]; d2 += up[2];
+                const double *up = C + 3 * (int64_t)anc3[a];
+                c0 += up[0]; c1 += up[1]; c2 += up[2];
             }
-            c0 += d0; c1 += d1; c2 += d2;
+        } else {
+            apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
         }
         const T r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
         T f0, f1, f2;
@@ -441,19 +455,19 @@ hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
     const int rblocks = blocks_for((int64_t)64 * waves, kRestrictWG);
     if (mixed) {
         hipLaunchKernelGGL(k_ml_restrict<float>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
-                           c->r32, m.R, c->scal, check_done);
+                           c->r32, m.R, m.Dinv, m.C, c->scal, check_done);
     } else {
         hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
-                           c->r64, m.R, c->scal, check_done);
+                           c->r64, m.R, m.Dinv, m.C, c->scal, check_done);
     }
     if (lv.L >= 3)
         hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
                            c->scal, check_done);
     if (mixed) {
-        hipLaunchKernelGGL(k_ml_prolong<float>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.par, m.Dinv,
+        hipLaunchKernelGGL(k_ml_prolong<float>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.anc3, m.Dinv,
                            m.R, m.C, c->D32, c->r32, c->z32, c->fixmask, c->scal, c->partials, check_done);
     } else {
-        hipLaunchKernelGGL(k_ml_prolong<double>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.par,
+        hipLaunchKernelGGL(k_ml_prolong<double>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.anc3,
                            m.Dinv, m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, check_done);
     }
     return hipGetLastError();
