@@ -103,7 +103,7 @@ void free_topology_buffers(eolc_ctx *c)
     free_eol(c);
     eolc::MLState &ml = c->ml;
     dfree(ml.hord); dfree(ml.ptr1); dfree(ml.agg1); dfree(ml.gpos); dfree(ml.cptr); dfree(ml.par); dfree(ml.lam);
-    dfree(ml.V); dfree(ml.Dinv); dfree(ml.R); dfree(ml.C); dfree(ml.anc3);
+    dfree(ml.V); dfree(ml.Dinv); dfree(ml.R); dfree(ml.C); dfree(ml.anc3); dfree(ml.kptr2);
     ml.built = false;
     ml.lv = eolc::MLLevels{};
     ml.counts.clear();
@@ -556,9 +556,14 @@ int ml_prepare(eolc_ctx *c)
         EOLC_HIP(dalloc(c, &m.Dinv, 6 * na));
         EOLC_HIP(dalloc(c, &m.R, 3 * na));
         EOLC_HIP(dalloc(c, &m.C, 3 * na));
+        if (lv.L >= 2) {
+            std::vector<int> kptr2((size_t)lv.n[2] + 1);
+            for (int w = 0; w <= lv.n[2]; ++w) kptr2[w] = ptr1[cptr[(size_t)lv.coff[2] + w]];
+            EOLC_HIP(upload(c, &m.kptr2, kptr2));
+        }
         if (lv.L >= 3) {
-            std::vector<int> anc3((size_t)lv.n[1]);
-            for (int a = 0; a < lv.n[1]; ++a) anc3[a] = lv.off[3] + par[(size_t)lv.off[2] + par[a]];
+            std::vector<int> anc3((size_t)t.N);
+            for (int i = 0; i < t.N; ++i) anc3[i] = lv.off[3] + par[(size_t)lv.off[2] + par[agg1[i]]];
             EOLC_HIP(upload(c, &m.anc3, anc3));
         }
     }

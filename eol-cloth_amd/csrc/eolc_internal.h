@@ -194,7 +194,8 @@ struct MLState {
     double *R = nullptr;          // 3 per aggregate: restricted residual
     double *C = nullptr;          // 3 per aggregate: accumulated coarse correction; for a
                                   // level-1 aggregate a (L >= 2): D_1^-1 R_1(a) + D_2^-1 R_2(parent)
-    int *anc3 = nullptr;          // n_1 (L >= 3): global index of each level-1 aggregate's level-3 ancestor
+    int *anc3 = nullptr;          // N (L >= 3): global index of each vertex's level-3 aggregate
+    int *kptr2 = nullptr;         // n_2 + 1 (L >= 2): member range in hord of each level-2 aggregate
     std::vector<int> counts;      // host copy of n[0..L]
     std::vector<std::vector<int>> maps;   // host: level-l aggregate of every vertex (l = 1..L)
 };

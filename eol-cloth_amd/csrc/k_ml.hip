@@ -24,7 +24,7 @@
 //                   and per level-1 aggregate a, C(a) = D_1^-1 R_1(a) + D_2^-1 R_2(parent)
 //   k_ml_coarse   : one workgroup (levels >= 3 only): R_l for l >= 3, then
 //                   C_l = D_l^-1 R_l + C_{l+1}(parent) from the top level down to 3
-//   k_ml_prolong  : z_i = D0 r_i + C(a_i) + C_3(level-3 ancestor of a_i); r.z partials
+//   k_ml_prolong  : z_i = D0 r_i + C(a_i) + C_3(level-3 aggregate of i); r.z partials
 // All coarse arithmetic is FP64 in a fixed order, so the result does not depend on
 // the launch shape.
 #include "eolc_internal.h"
@@ -169,6 +169,7 @@ constexpr int kRestrictWG = 256;
 template <class T>
 __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const int *__restrict__ hord,
                                                              const int *__restrict__ ptr1, const int *__restrict__ cptr,
+                                                             const int *__restrict__ kptr2,
                                                              const T *__restrict__ r, double *__restrict__ R,
                                                              const double *__restrict__ Dinv, double *__restrict__ C,
                                                              const SolverScalars *scal, int check_done)
@@ -177,26 +178,42 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
     const int lane = threadIdx.x % 64;
     double *__restrict__ buf = stage[threadIdx.x / 64];
     const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;   // uniform in the wavefront
-    int a0, a1;
-    // the aggregate's child range is read together with the stop flag (one
-    // latency round instead of two; a stopped launch only reads these two words)
-    if (lv.L >= 2) {
+    const bool two = lv.L >= 2;
+    int a0, a1, ka = 0, kz = 0;
+    // the aggregate's child and member ranges are read together with the stop flag
+    // (one latency round; a stopped launch only reads these words)
+    if (two) {
         if (w >= lv.n[2]) return;
         a0 = cptr[lv.coff[2] + w];
         a1 = cptr[lv.coff[2] + w + 1];
+        ka = kptr2[w];
+        kz = kptr2[w + 1];
     } else {
         a0 = w * 64;
         if (a0 >= lv.n[1]) return;
         a1 = min(a0 + 64, lv.n[1]);
     }
     if (check_done && scal->done) return;
+    // inverse blocks of the level-2 aggregate and of this lane's first child: they do
+    // not depend on r, so they load with the member indices
+    double dw[6] = {0, 0, 0, 0, 0, 0}, da[6] = {0, 0, 0, 0, 0, 0};
+    if (two) {
+        const int64_t gw = lv.off[2] + w;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dw[k] = Dinv[6 * gw + k];
+        if (a0 + lane < a1)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) da[k] = Dinv[6 * (int64_t)(a0 + lane) + k];
+    }
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;   // R_1 of this lane's first child
+    int k0 = two ? ka : ptr1[a0];
     for (int g0 = a0; g0 < a1; g0 += 64) {   // groups of up to 64 children
         const int g1 = min(g0 + 64, a1);
         const int a = g0 + lane;
         const bool mine = a < g1;
         const int m0 = mine ? ptr1[a] : 0, m1 = mine ? ptr1[a + 1] : 0;
-        const int k0 = ptr1[g0], k1 = ptr1[g1];
+        const int k1 = (two && g1 == a1) ? kz : ptr1[g1];
         double t0 = 0.0, t1 = 0.0, t2 = 0.0;
         for (int kb = k0; kb < k1; kb += 64) {
             const int k = kb + lane;
@@ -218,26 +235,32 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
         if (mine) {
             R[3 * a] = t0; R[3 * a + 1] = t1; R[3 * a + 2] = t2;
             s0 += t0; s1 += t1; s2 += t2;
+            if (g0 == a0) { f0 = t0; f1 = t1; f2 = t2; }
         }
+        k0 = k1;   // the next group starts where this one ends
     }
-    if (lv.L >= 2) {
+    if (two) {
         // wave_sum leaves the total in lane 0; every lane needs it below
         s0 = __shfl(wave_sum(s0), 0, 64);
         s1 = __shfl(wave_sum(s1), 0, 64);
         s2 = __shfl(wave_sum(s2), 0, 64);
-        const int64_t gw = lv.off[2] + w;
         if (lane == 0) {
-            double *o = R + 3 * gw;
+            double *o = R + 3 * (lv.off[2] + (int64_t)w);
             o[0] = s0; o[1] = s1; o[2] = s2;
         }
         // levels 1 and 2 of each child, for the prolongation: C(a) = D_1^-1 R_1(a) +
-        // D_2^-1 R_2(w) (each lane re-reads the R_1 it wrote above)
-        double e0, e1, e2;
-        apply_inv(Dinv + 6 * gw, s0, s1, s2, e0, e1, e2);
-        for (int g0 = a0; g0 < a1; g0 += 64) {
+        // D_2^-1 R_2(w)
+        double e0, e1, e2, c0, c1, c2;
+        apply_inv(dw, s0, s1, s2, e0, e1, e2);
+        if (a0 + lane < a1) {
+            const int64_t a = a0 + lane;
+            apply_inv(da, f0, f1, f2, c0, c1, c2);
+            C[3 * a] = c0 + e0; C[3 * a + 1] = c1 + e1; C[3 * a + 2] = c2 + e2;
+        }
+        // more than 64 children: each lane re-reads the R_1 it wrote above
+        for (int g0 = a0 + 64; g0 < a1; g0 += 64) {
             const int a = g0 + lane;
             if (a < a1) {
-                double c0, c1, c2;
                 apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
                 C[3 * a] = c0 + e0; C[3 * a + 1] = c1 + e1; C[3 * a + 2] = c2 + e2;
             }
@@ -394,18 +417,29 @@ k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const
              const unsigned char *__restrict__ fixmask, const SolverScalars *scal, double *partials, int check_done)
 {
     __shared__ double lds[2 * (WG / 64) + 2];
-    if (check_done && scal->done) return;
     int rb, re;
     wg_rows(N, &rb, &re);
+    const int i0 = rb + (int)threadIdx.x;
+    // the first vertex's aggregate indices are read together with the stop flag, so
+    // both corrections below load in one round
+    int a = 0, u = 0;
+    if (i0 < re) {
+        a = agg1[i0];
+        if (lv.L >= 3) u = anc3[i0];
+    }
+    if (check_done && scal->done) return;
     double srz = 0.0;
-    for (int i = rb + threadIdx.x; i < re; i += WG) {
-        const int a = agg1[i];
+    for (int i = i0; i < re; i += WG) {
+        if (i != i0) {
+            a = agg1[i];
+            if (lv.L >= 3) u = anc3[i];
+        }
         double c0, c1, c2;
         if (lv.L >= 2) {
             // levels 1 and 2, combined per level-1 aggregate by the restriction
             c0 = C[3 * a]; c1 = C[3 * a + 1]; c2 = C[3 * a + 2];
             if (lv.L >= 3) {
-                const double *up = C + 3 * (int64_t)anc3[a];
+                const double *up = C + 3 * (int64_t)u;
                 c0 += up[0]; c1 += up[1]; c2 += up[2];
             }
         } else {
@@ -455,10 +489,10 @@ hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
     const int rblocks = blocks_for((int64_t)64 * waves, kRestrictWG);
     if (mixed) {
         hipLaunchKernelGGL(k_ml_restrict<float>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
-                           c->r32, m.R, m.Dinv, m.C, c->scal, check_done);
+                           m.kptr2, c->r32, m.R, m.Dinv, m.C, c->scal, check_done);
     } else {
         hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
-                           c->r64, m.R, m.Dinv, m.C, c->scal, check_done);
+                           m.kptr2, c->r64, m.R, m.Dinv, m.C, c->scal, check_done);
     }
     if (lv.L >= 3)
         hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
