@@ -419,6 +419,7 @@ def main():
             tt, st = run_path(ctx, sc, mode, args.steps, args.warmup, args.maxit)
             log(f"{name}: {1 / tt:.4f} steps/s, iters {[s['iters'] for s in st]}")
             extra[name] = {"steps_per_s": 1 / tt, "iters": [s["iters"] for s in st], "ms_per_step": tt * 1e3,
+                           "launched": [s["launched"] for s in st],
                            "phase_ms": {k: float(np.mean([s[k] for s in st])) for k in
                                         ("ms_assembly", "ms_solve")}}
     if args.no_ml:
@@ -427,7 +428,8 @@ def main():
         head_mode, head_name = eolc.SOLVE_MIXED_LEARNED | eolc.SOLVE_MULTILEVEL, "mixed-learned-multilevel"
     t_head, st_head = run_path(ctx, sc, head_mode, args.steps, args.warmup, args.maxit)
     log(f"{head_name}: {1 / t_head:.4f} steps/s, iters {[s['iters'] for s in st_head]}, "
-        f"fallback {[s['used_fallback'] for s in st_head]}")
+        f"fallback {[s['used_fallback'] for s in st_head]}, launched {[s['launched'] for s in st_head]}, "
+        f"host syncs {[s['host_syncs'] for s in st_head]}")
     spmv32_ms = ctx.time_spmv(32, 20)
     if out_fp64:
         out_fp64["speedup_of_headline"] = t64 / t_head
@@ -471,6 +473,8 @@ def main():
                    "bending_hessian": "spd-projected", "weights": wsrc,
                    "multilevel_aggregates": None if args.no_ml else ml_counts},
         "iters": [s["iters"] for s in st_head],
+        "launched": [s["launched"] for s in st_head],
+        "host_syncs": [s["host_syncs"] for s in st_head],
         "used_fallback": [s["used_fallback"] for s in st_head],
         "true_relres": [s["true_relres"] for s in st_head],
         "phase_ms": {k: float(np.mean([s[k] for s in st_head])) for k in

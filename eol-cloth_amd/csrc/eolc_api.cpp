@@ -1159,7 +1159,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     const double delta = mixed ? c->mixed_delta : 0.0;
     EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
     EOLC_HIP(eolc::pcg_init(c, mixed));
-    int k = 0, refinements = 0, status = EOLC_ERR_MAXIT;
+    int k = 0, refinements = 0, status = EOLC_ERR_MAXIT, launched = 0, syncs = 0;
     // Iterations are enqueued in chunks; the host synchronises once per chunk.  A
     // chunk that overruns the stop point costs one no-op launch pair per extra
     // iteration, so the next chunk is sized at 60% of the iterations that the mean
@@ -1185,8 +1185,10 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         const int todo = std::min(chunk, (learned ? cap : maxit) - k);
         if (todo <= 0) { status = EOLC_ERR_MAXIT; break; }
         EOLC_HIP(eolc::pcg_chunk(c, mixed, k, todo));
+        launched += todo;
         EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
         EOLC_HIP(hipStreamSynchronize(c->stream));
+        ++syncs;
         const eolc::SolverScalars s = *c->hscal;
         k = s.iters;
         relres = s.bnorm2 > 0 ? std::sqrt(s.rr / s.bnorm2) : 0.0;
@@ -1217,6 +1219,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         EOLC_HIP(eolc::pcg_reliable_update(c));
         EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
         EOLC_HIP(hipStreamSynchronize(c->stream));
+        ++syncs;
         ++refinements;
         const eolc::SolverScalars s2 = *c->hscal;
         relres = s2.bnorm2 > 0 ? std::sqrt(s2.rmax2 / s2.bnorm2) : 0.0;
@@ -1243,6 +1246,8 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     if (d_vals) EOLC_HIP(hipFree(d_vals));
     if (st) {
         st->iters = c->hscal->iters;
+        st->launched = launched;
+        st->host_syncs = syncs;
         st->refinements = refinements;
         st->status = status;
         st->used_fallback = used_fallback;

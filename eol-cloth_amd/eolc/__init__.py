@@ -51,7 +51,7 @@ class SolveStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("refinements", C.c_int), ("status", C.c_int),
                 ("used_fallback", C.c_int), ("relres", C.c_double), ("true_relres", C.c_double),
                 ("ms_assembly", C.c_double), ("ms_rhs", C.c_double), ("ms_solve", C.c_double),
-                ("ms_update", C.c_double)]
+                ("ms_update", C.c_double), ("launched", C.c_int), ("host_syncs", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -129,6 +129,8 @@ typedef struct {
     double ms_rhs;
     double ms_solve;
     double ms_update;
+    int launched;           /* PCG iterations enqueued; those past a stop point exit at once */
+    int host_syncs;         /* host waits on the solver's stream during the solve */
 } eolc_solve_stats;
 
 /* Collision record: btc::Collision (boxTriCollision.h:49-110) flattened. */
