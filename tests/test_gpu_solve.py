@@ -53,6 +53,9 @@ def test_mixed_solve_reaches_tolerance(ctx):
     _, vm = ctx.download_state()
     print("mixed", st.as_dict(), flush=True)
     assert st.true_relres <= 1e-8 and st.refinements >= 1
+    # every working iteration was enqueued; the host waits once per chunk and once
+    # per reliable update
+    assert st.launched >= st.iters and st.host_syncs >= st.refinements + 1
     ctx.upload_state(sc.x0, sc.v0)
     ctx.forces_fill(GRAV, H)
     rc, st64 = ctx.velocity_solve(d, val, tol=1e-13, maxit=50000, mode=eolc.SOLVE_FP64_BJ)
@@ -121,6 +124,7 @@ def test_iteration_cap_reports_maxit(ctx):
     rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-14, maxit=5, mode=eolc.SOLVE_FP64_BJ,
                       allow=(eolc.ERR_MAXIT,))
     assert rc == eolc.ERR_MAXIT and st.iters == 5
+    assert st.launched == 5 and st.host_syncs >= 1
 
 
 def test_spmv_variants_bitwise_identical_and_autotune(ctx):
