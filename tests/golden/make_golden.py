@@ -10,6 +10,7 @@ small C++ program make_jitter.cpp next to this script.
 """
 import os
 import subprocess
+import tempfile
 import sys
 
 import numpy as np
@@ -73,10 +74,10 @@ def main():
         rec["hin_W"].append(W); rec["hin_f"].append(f); rec["hin_K"].append(K)
     np.savez_compressed(os.path.join(HERE, "elements.npz"), **{k: np.array(v) for k, v in rec.items()})
     # jitter stream
-    exe = os.path.join(ROOT, "_work", "make_jitter")
-    os.makedirs(os.path.dirname(exe), exist_ok=True)
-    subprocess.check_call(["g++", "-O2", "-o", exe, os.path.join(HERE, "make_jitter.cpp")])
-    raw = subprocess.check_output([exe, "3000"])
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "make_jitter")
+        subprocess.check_call(["g++", "-O2", "-o", exe, os.path.join(HERE, "make_jitter.cpp")])
+        raw = subprocess.check_output([exe, "3000"])
     np.save(os.path.join(HERE, "jitter_seed1.npy"), np.frombuffer(raw, dtype=np.float64).copy())
     print("wrote elements.npz and jitter_seed1.npy")
 
