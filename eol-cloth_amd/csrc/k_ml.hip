@@ -165,7 +165,9 @@ __device__ __forceinline__ void wave_sync()
 // that fall in the chunk, in member order.  R_2 is the wavefront sum of the
 // children's R_1 (fixed shuffle tree).  With a single level, a wavefront takes 64
 // level-1 aggregates.
-constexpr int kRestrictWG = 256;
+// two wavefronts per workgroup: 13.9 us per restriction at 1024^2, against 14.8 (1),
+// 14.2 (4) and 14.4 (8) in the same kernel trace
+constexpr int kRestrictWG = 128;
 template <class T>
 __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const int *__restrict__ hord,
                                                              const int *__restrict__ ptr1, const int *__restrict__ cptr,
