@@ -1156,7 +1156,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         EOLC_HIP(eolc::learned_precond(c));
         if (bj_hint > 0) cap = std::min(maxit, 2 * bj_hint);
     }
-    const double delta = mixed ? c->mixed_delta : 0.0;
+    const double delta = mixed ? (c->ml_on ? c->mixed_delta_ml : c->mixed_delta) : 0.0;
     EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
     EOLC_HIP(eolc::pcg_init(c, mixed));
     int k = 0, refinements = 0, status = EOLC_ERR_MAXIT, launched = 0, syncs = 0;
@@ -1983,6 +1983,7 @@ try {
     if (!c) return EOLC_ERR_INVALID;
     if (!(delta > 0.0 && delta < 1.0)) return fail(c, EOLC_ERR_INVALID, "set_mixed_delta: need 0 < delta < 1");
     c->mixed_delta = delta;
+    c->mixed_delta_ml = delta;
     drop_batch_graph(c);
     return EOLC_OK;
 } catch (...) {
