@@ -1984,7 +1984,7 @@ try {
     if (!(delta > 0.0 && delta < 1.0)) return fail(c, EOLC_ERR_INVALID, "set_mixed_delta: need 0 < delta < 1");
     c->mixed_delta = delta;
     c->mixed_delta_ml = delta;
-    c->batch_delta_ml = delta;
+    c->batch_delta = delta;
     drop_batch_graph(c);
     return EOLC_OK;
 } catch (...) {

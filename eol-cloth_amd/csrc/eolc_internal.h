@@ -298,7 +298,7 @@ struct eolc_ctx {
     int spmv_variant[2] = {0, 0};   // SpMV launch variant for FP32 [0] and FP64 [1] solves
     double mixed_delta = 0.5;       // reliable-update trigger of the mixed solves
     double mixed_delta_ml = 0.2;    // the same with the multilevel levels (shorter cycles)
-    double batch_delta_ml = 0.1;    // the batched multilevel solve (its FP64 residuals cost more per iteration)
+    double batch_delta = 0.1;       // the batched solve (its FP64 residuals cost more per iteration)
     // per-element material (batched cloths): Young's modulus per face, bending
     // stiffness per hinge; null = the context's scalar material
     double *e_face = nullptr;

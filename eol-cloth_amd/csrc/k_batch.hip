@@ -534,8 +534,7 @@ hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant)
     a.v = c->v; a.xs64 = c->xs64; a.r64 = c->r64; a.b64 = c->b64;
     a.r32 = c->r32; a.z32 = c->z32; a.q32 = c->q32; a.xl32 = c->xl32; a.p32a = c->p32[0]; a.p32b = c->p32[1];
     a.part = S.slice_part;
-    const double delta = ml ? c->batch_delta_ml : c->mixed_delta;
-    a.tol2 = tol * tol; a.delta2 = delta * delta; a.maxit = maxit;
+    a.tol2 = tol * tol; a.delta2 = c->batch_delta * c->batch_delta; a.maxit = maxit;
     a.stats = S.stats; a.dstats = S.dstats;
     const MLState &m = c->ml;
     a.ml = BatchML{m.lv, m.hord, m.ptr1, m.agg1, m.par, m.cptr, S.d_ml_lo, m.Dinv, m.R, m.C};

@@ -62,8 +62,8 @@ typedef enum {
                                        keeps the flag. */
 
 /* Reliable-update trigger of the mixed modes (default 0.5; 0.2 for solves with
- * EOLC_SOLVE_MULTILEVEL, whose cycles are shorter, and 0.1 for the batched solve
- * with the multilevel levels): the FP32 correction is folded
+ * EOLC_SOLVE_MULTILEVEL, whose cycles are shorter, and 0.1 for the batched solve,
+ * whose FP64 residuals run inside the persistent launch): the FP32 correction is folded
  * into the FP64 solution and the residual replaced by the FP64 true residual
  * whenever the recursive residual has dropped by this factor since the last update.
  * A value set here applies to every mode. */
