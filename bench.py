@@ -60,69 +60,149 @@ def spmv_bytes(N, nb, s, diff):
     return (9 * s + 4) * blocks + (4 * N if diff else 0) + 8 * N + 4 * s * 3 * N + N
 
 
+class MissedTolerance(RuntimeError):
+    pass
+
+
 def run_path(ctx, sc, mode, steps, warmup, maxit):
+    """W untimed steps, then K timed steps of one solve mode from the scene's initial
+    state.  Every step must end EOLC_OK with its FP64 true residual <= 1e-8."""
     ctx.upload_state(sc.x0, sc.v0)
     for _ in range(warmup):
-        ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=maxit, mode=mode)
+        rc, st = ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=maxit, mode=mode, allow=(eolc.ERR_MAXIT,))
+        if rc != eolc.OK:
+            raise MissedTolerance(f"warm-up solve failed: {st.as_dict()}")
     t0 = time.perf_counter()
     stats = []
     for _ in range(steps):
-        rc, st = ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=maxit, mode=mode)
+        rc, st = ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=maxit, mode=mode, allow=(eolc.ERR_MAXIT,))
         stats.append(st.as_dict())
     t1 = time.perf_counter()
-    bad = [s for s in stats if not (s["true_relres"] <= TOL)]
+    bad = [s for s in stats if not (s["status"] == eolc.OK and s["true_relres"] <= TOL)]
     if bad:
-        raise RuntimeError(f"solve missed the tolerance: {bad[0]}")
+        raise MissedTolerance(f"solve missed the tolerance: {bad[0]}")
     return (t1 - t0) / steps, stats
 
 
+def leg(results, name, fn):
+    """Runs one measurement; a failure is recorded under its name and never hides
+    the other measurements."""
+    t = time.perf_counter()
+    try:
+        out = fn()
+        if out is None:
+            out = {}
+        out["status"] = "ok"
+    except Exception as e:  # noqa: BLE001 - recorded in the JSON line
+        out = {"status": "failed", "error": f"{type(e).__name__}: {e}"}
+        log(f"{name}: FAILED {out['error']}")
+    out["wall_s"] = round(time.perf_counter() - t, 2)
+    results[name] = out
+    return out
+
+
+def bsr_bytes(N, nb, s):
+    """A(s) = (9s + 4) nb + 4 (N + 1): the 3x3 block matrix with its column indices
+    and row pointers."""
+    return (9 * s + 4) * nb + 4 * (N + 1)
+
+
+def step_roofline(sc, nb, stats, ms_step, mixed):
+    """Step-level roofline: B_step = B_asm + k B_it + r B_res + 2 v(8) over the time
+    of a step, against the 8 TB/s HBM peak.  B_it = A(s) + D(s) + 9 v(s) (s = 4 in
+    the mixed modes), B_res = A(8) + 3 v(8), B_asm = A(8) [+ A(4) in the mixed
+    modes, which write both matrices] + 8 n + 16 N + 12 F + 16 H; k = iterations and
+    r = true-residual replacements per step (means over the timed steps)."""
+    N, n = sc.N, 3 * sc.N
+    s = 4 if mixed else 8
+    A8, As = bsr_bytes(N, nb, 8), bsr_bytes(N, nb, s)
+    b_it = As + 9 * s * N + 9 * s * n
+    b_res = A8 + 3 * 8 * n
+    b_asm = A8 + (As if mixed else 0) + 8 * n + 16 * N + 12 * sc.F + 16 * sc.H
+    k = float(np.mean([st["iters"] for st in stats]))
+    r = float(np.mean([st["refinements"] for st in stats]))
+    b_step = b_asm + k * b_it + r * b_res + 2 * 8 * n
+    ach = b_step / (ms_step * 1e-3)
+    return {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": ach / PEAK_HBM, "bytes_per_step": b_step, "B_it": b_it, "B_asm": b_asm, "B_res": b_res,
+            "iters_per_step": k, "replacements_per_step": r}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            return next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+    except OSError:
+        return None
+
+
 def cpu_baseline(sc, iters_per_step, threads):
-    """The oracle (FP64 CPU restatement) on a bounded sample of the same workload:
-    one full Forces::fill (elements + triplets + setFromTriplets-style CSR) and 10
-    iterations of its block-Jacobi PCG on the same matrix, scaled to steps/s with the
-    FP64 iteration count the GPU needed per step."""
+    """The oracle (FP64 CPU restatement of Forces::fill + the block-Jacobi PCG the dead
+    ConjugateGradient branch intended) on the host cores, on a bounded sample of the
+    same 1024^2 workload: one full Forces::fill (elements, triplets, sort-and-sum CSR)
+    measured whole, and a few PCG iterations on its matrix, at all cores and at one
+    thread; a step is the fill plus the FP64 iteration count the GPU needed per step.
+    A whole 1024^2 step (about 25,000 iterations) takes the oracle tens of minutes,
+    which is why the solve is sampled."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     mat = O.material()
-    t0 = time.perf_counter()
-    rc, f, M, K = O.forces_fill(sc.x0, sc.X, sc.tri, sc.hinge, mat, GRAV, H, nthreads=threads)
-    t1 = time.perf_counter()
-    rhs = M @ sc.v0 + H * f
     d = (3 * sc.fixed[:, None] + np.arange(3)).reshape(-1).astype(np.int32)
-    k = 10
-    t2 = time.perf_counter()
-    O.velocity_solve(K, rhs, d, 0.99 * sc.v0[d], v0=sc.v0, tol=1e-30, maxit=k,
-                     precond=O.PC_BLOCK_JACOBI, nthreads=threads)
-    t3 = time.perf_counter()
-    t_iter = (t3 - t2) / k
-    t_step = (t1 - t0) + iters_per_step * t_iter
-    # one thread, like the single-threaded reference: Forces::fill on a 256x256 sheet
-    # scaled by the element count, and 2 PCG iterations on the full matrix
-    small = scene.Scene(256, sigma_x=1e-3, sigma_v=0.05)
-    t4 = time.perf_counter()
-    O.forces_fill(small.x0, small.X, small.tri, small.hinge, mat, GRAV, H, nthreads=1)
-    fill1 = (time.perf_counter() - t4) * (sc.F + sc.H) / (small.F + small.H)
-    t5 = time.perf_counter()
-    O.velocity_solve(K, rhs, d, 0.99 * sc.v0[d], v0=sc.v0, tol=1e-30, maxit=2, precond=O.PC_BLOCK_JACOBI,
-                     nthreads=1)
-    iter1 = (time.perf_counter() - t5) / 2
-    step1 = fill1 + iters_per_step * iter1
-    model = None
-    try:
-        with open("/proc/cpuinfo") as fh:
-            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
-    except OSError:
-        pass
-    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": threads, "kind": "port",
-            "cpu_model": model, "nproc": os.cpu_count(),
-            "sample": f"oracle Forces::fill once + {k} FP64 block-Jacobi PCG iterations at "
-                      f"{sc.n}x{sc.n} ({(t1 - t0):.2f} s fill, {t_iter * 1e3:.1f} ms/iter), "
-                      f"scaled with {iters_per_step:.0f} iterations/step",
-            "fill_s": t1 - t0, "iter_ms": t_iter * 1e3,
-            "one_thread": {"value": 1.0 / step1, "unit": "steps/s", "cores": 1,
-                           "sample": "Forces::fill of a 256x256 sheet scaled by element count "
-                                     f"({fill1:.1f} s at 1024^2) + 2 PCG iterations at 1024^2 "
-                                     f"({iter1 * 1e3:.0f} ms/iter)"}}
+    out = {}
+    for th, k in ((threads, 10), (1, 3)):
+        t0 = time.perf_counter()
+        rc, f, M, K = O.forces_fill(sc.x0, sc.X, sc.tri, sc.hinge, mat, GRAV, H, nthreads=th)
+        fill = time.perf_counter() - t0
+        rhs = M @ sc.v0 + H * f
+        t0 = time.perf_counter()
+        O.velocity_solve(K, rhs, d, 0.99 * sc.v0[d], v0=sc.v0, tol=1e-30, maxit=k,
+                         precond=O.PC_BLOCK_JACOBI, nthreads=th)
+        t_iter = (time.perf_counter() - t0) / k
+        t_step = fill + iters_per_step * t_iter
+        out[th] = {"value": 1.0 / t_step, "unit": "steps/s", "cores": th, "fill_s": fill,
+                   "iter_ms": t_iter * 1e3,
+                   "sample": f"oracle Forces::fill at {sc.n}x{sc.n} measured whole ({fill:.2f} s) + {k} FP64 "
+                             f"block-Jacobi PCG iterations ({t_iter * 1e3:.1f} ms each), a step = the fill + "
+                             f"{iters_per_step:.0f} iterations (the GPU's FP64 count per step)"}
+        del f, M, K, rhs
+    # the plain method in full (20x20 quads, 200 steps, FP64, tolerance 1e-8): whole
+    # steps measured on one thread
+    s1 = scene.Scene(20)
+    x, v = s1.x0.copy(), s1.v0.copy()
+    t0 = time.perf_counter()
+    its = 0
+    for _ in range(200):
+        rc, st, _ = O.step(x, v, s1.X, s1.tri, s1.hinge, O.material(), GRAV, H, s1.fixed, tol=TOL, maxit=20000)
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed ({rc})")
+        its += st.iters
+    t1 = time.perf_counter() - t0
+    res = dict(out[threads])
+    res.update({"kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                "one_thread": out[1],
+                "config1_full": {"value": 200 / t1, "unit": "steps/s", "cores": 1, "steps": 200,
+                                 "iters_per_step": its / 200,
+                                 "sample": "20x20 quads, 200 whole oracle steps (fill + Jacobi PCG to 1e-8)"}})
+    return res
+
+
+def run_config1_gpu(steps=200):
+    """The plain method (20x20 quads, 200 steps, FP64 block-Jacobi to 1e-8) on the
+    GPU: steps/s of whole steps, and the positions after 200 steps against the
+    oracle's (the sheet buckles, so the difference reflects the solver tolerance)."""
+    s1 = scene.Scene(20)
+    with eolc.Context(0) as c:
+        c.set_topology(s1.X, s1.tri, s1.hinge)
+        c.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+        c.upload_state(s1.x0, s1.v0)
+        t0 = time.perf_counter()
+        its = 0
+        for _ in range(steps):
+            rc, st = c.step(GRAV, H, s1.fixed, tol=TOL, maxit=20000, mode=eolc.SOLVE_FP64_BJ)
+            its += st.iters
+        dt = (time.perf_counter() - t0) / steps
+    return {"workload": "cloth 20x20 quads, 200 steps, FP64 block-Jacobi PCG to 1e-8",
+            "steps_per_s": 1.0 / dt, "iters_per_step": its / steps}
 
 
 def run_collision(reps, with_cpu):
@@ -317,6 +397,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1024, help="quads per side")
     ap.add_argument("--maxit", type=int, default=200000)
+    ap.add_argument("--budget-s", type=float, default=420.0,
+                    help="after this many seconds of measurement, the remaining optional legs are skipped")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 block-Jacobi comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -396,75 +478,92 @@ def main():
 
     theta, wsrc = load_weights()
     ctx.learned_set_weights(theta)
-    # the block-Jacobi runs go first: the learned solve falls back to block-Jacobi
-    # after twice the iterations of the last block-Jacobi solve on the context
-    out_fp64 = None
-    if not args.no_fp64:
-        t64, st64 = run_path(ctx, sc, eolc.SOLVE_FP64_BJ, args.steps, args.warmup, args.maxit)
-        log(f"fp64-bj: {1 / t64:.4f} steps/s, iters {[s['iters'] for s in st64]}")
-        spmv64_ms = ctx.time_spmv(64, 20)
-        out_fp64 = {"steps_per_s": 1 / t64, "iters": [s["iters"] for s in st64],
-                    "ms_per_step": t64 * 1e3, "spmv_ms": spmv64_ms}
-    tbj, stbj = run_path(ctx, sc, eolc.SOLVE_MIXED_BJ, args.steps, args.warmup, args.maxit)
-    log(f"mixed-bj: {1 / tbj:.4f} steps/s, iters {[s['iters'] for s in stbj]}")
-    out_mbj = {"steps_per_s": 1 / tbj, "iters": [s["iters"] for s in stbj], "ms_per_step": tbj * 1e3}
-    extra = {}
-    if not args.no_ml:
-        # the coarse levels alone, in both precisions (the mixed one also sets the
-        # learned mode's fallback cap for the multilevel case)
-        for name, mode in (("fp64_bj_ml", eolc.SOLVE_FP64_BJ | eolc.SOLVE_MULTILEVEL),
-                           ("mixed_bj_ml", eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL)):
-            if name == "fp64_bj_ml" and args.no_fp64:
-                continue
+    nb = ctx.csr_nnz(eolc.MAT_MDK) // 9
+    t_start = time.perf_counter()
+
+    def over_budget(name):
+        spent = time.perf_counter() - t_start
+        if spent > args.budget_s:
+            log(f"{name}: skipped, {spent:.0f} s spent of the {args.budget_s:.0f} s budget")
+            return True
+        return False
+
+    def path_leg(mode, mixed):
+        def fn():
             tt, st = run_path(ctx, sc, mode, args.steps, args.warmup, args.maxit)
-            log(f"{name}: {1 / tt:.4f} steps/s, iters {[s['iters'] for s in st]}")
-            extra[name] = {"steps_per_s": 1 / tt, "iters": [s["iters"] for s in st], "ms_per_step": tt * 1e3,
-                           "launched": [s["launched"] for s in st],
-                           "phase_ms": {k: float(np.mean([s[k] for s in st])) for k in
-                                        ("ms_assembly", "ms_solve")}}
+            return {"steps_per_s": 1 / tt, "ms_per_step": tt * 1e3, "iters": [s["iters"] for s in st],
+                    "refinements": [s["refinements"] for s in st],
+                    "launched": [s["launched"] for s in st], "host_syncs": [s["host_syncs"] for s in st],
+                    "used_fallback": [s["used_fallback"] for s in st],
+                    "true_relres_max": max(s["true_relres"] for s in st),
+                    "phase_ms": {k: float(np.mean([s[k] for s in st])) for k in
+                                 ("ms_assembly", "ms_rhs", "ms_solve", "ms_update")},
+                    "roofline_step": step_roofline(sc, nb, st, tt * 1e3, mixed)}
+        return fn
+
+    legs = {}
     if args.no_ml:
         head_mode, head_name = eolc.SOLVE_MIXED_LEARNED, "mixed-learned"
+        bj_name, bj_mode = "mixed_bj", eolc.SOLVE_MIXED_BJ
     else:
         head_mode, head_name = eolc.SOLVE_MIXED_LEARNED | eolc.SOLVE_MULTILEVEL, "mixed-learned-multilevel"
-    t_head, st_head = run_path(ctx, sc, head_mode, args.steps, args.warmup, args.maxit)
-    log(f"{head_name}: {1 / t_head:.4f} steps/s, iters {[s['iters'] for s in st_head]}, "
-        f"fallback {[s['used_fallback'] for s in st_head]}, launched {[s['launched'] for s in st_head]}, "
-        f"host syncs {[s['host_syncs'] for s in st_head]}")
+        bj_name, bj_mode = "mixed_bj_ml", eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL
+    # The block-Jacobi run of the headline's preconditioner family goes first: the
+    # learned solve falls back to block-Jacobi after twice its iteration count.
+    leg(legs, bj_name, path_leg(bj_mode, True))
+    head = leg(legs, "headline", path_leg(head_mode, True))
+    if head["status"] == "ok":
+        log(f"{head_name}: {head['steps_per_s']:.4f} steps/s, iters {head['iters']}, "
+            f"fallback {head['used_fallback']}, host syncs {head['host_syncs']}")
     spmv32_ms = ctx.time_spmv(32, 20)
-    if out_fp64:
-        out_fp64["speedup_of_headline"] = t64 / t_head
-    out_mbj["speedup_of_headline"] = tbj / t_head
-    for v in extra.values():
-        v["speedup_of_headline"] = (1 / v["steps_per_s"]) / t_head
-    nb = None
-    # nb from the matrix pattern size: 9 scalars per block
-    import ctypes as C
-    nnz = C.c_int64(0)
-    eolc.lib().eolc_download_csr(ctx._h, eolc.MAT_MDK, None, None, None, 0, C.byref(nnz))
-    nb = nnz.value // 9
-    # algorithmic bytes over the SELL storage actually streamed (padding excluded)
+    # the baseline of the metric, then the other ingredients
+    others = [("fp64_bj", eolc.SOLVE_FP64_BJ, False, args.no_fp64), ("mixed_bj", eolc.SOLVE_MIXED_BJ, True, args.no_ml),
+              ("fp64_bj_ml", eolc.SOLVE_FP64_BJ | eolc.SOLVE_MULTILEVEL, False, args.no_ml or args.no_fp64)]
+    for name, mode, mixed, skip in others:
+        if skip or over_budget(name):
+            continue
+        r = leg(legs, name, path_leg(mode, mixed))
+        if r["status"] == "ok":
+            log(f"{name}: {r['steps_per_s']:.4f} steps/s, iters {r['iters']}, replacements {r['refinements']}")
+    spmv64_ms = ctx.time_spmv(64, 20)
+    if head["status"] == "ok":
+        for name, r in legs.items():
+            if name != "headline" and r["status"] == "ok":
+                r["speedup_of_headline"] = r["ms_per_step"] / head["ms_per_step"]
+    # the dominant kernel: algorithmic bytes over the SELL storage actually streamed
     alg = spmv_bytes(sc.N, nb, 4, True)
     achieved = alg / (spmv32_ms * 1e-3)
     kname = spmv_kernel_name(True, tuned[32]["variant"])
     roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": achieved / PEAK_HBM, "traffic": measured_traffic(kname, args.n),
             "kernel": kname, "variant": tuned[32]["variant"], "algorithmic_bytes": alg,
-            "avg_launch_ms": spmv32_ms, "copy_kernel_GBps": copy_gbps, "read_kernel_GBps": read_gbps}
-    eol_out = None if args.no_eol else run_eol_fill(ctx, sc, args.n)
+            "avg_launch_ms": spmv32_ms, "copy_kernel_GBps": copy_gbps, "read_kernel_GBps": read_gbps,
+            "fp64_spmv_ms": spmv64_ms}
+    extra = {}
+    if not args.no_eol and not over_budget("eol_fill"):
+        leg(extra, "eol_fill", lambda: run_eol_fill(ctx, sc, args.n))
     ctx.close()
-    batched = None if args.no_batch else run_batched(args.steps, args.warmup, args.maxit, not args.no_ml)
-    collision = None if args.no_collide else run_collision(10, not args.no_cpu)
-    transfer = None if args.no_transfer else run_transfer(sc)
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not over_budget("cpu_baseline"):
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        iters = np.mean(out_fp64["iters"]) if out_fp64 else np.mean([s["iters"] for s in st_head])
-        cpu = cpu_baseline(sc, iters, threads)
+        fp = legs.get("fp64_bj", {})
+        iters = np.mean(fp["iters"]) if fp.get("status") == "ok" else 25000.0
+        cpu = leg(extra, "cpu_baseline_leg", lambda: cpu_baseline(sc, iters, threads))
+        extra.pop("cpu_baseline_leg")
         log(f"cpu baseline: {cpu}")
+    if not over_budget("config1"):
+        leg(extra, "config1", run_config1_gpu)
+    if not args.no_collide and not over_budget("collision"):
+        leg(extra, "collision", lambda: run_collision(10, not args.no_cpu))
+    if not args.no_batch and not over_budget("batched"):
+        leg(extra, "batched", lambda: run_batched(args.steps, args.warmup, args.maxit, not args.no_ml))
+    if not args.no_transfer and not over_budget("velocity_transfer"):
+        leg(extra, "velocity_transfer", lambda: run_transfer(sc))
+    ok = head["status"] == "ok"
     res = {
         "metric": "steps/s, 1024^2 cloth at 1e-8 rel. residual: learned-PCG vs block-Jacobi PCG",
-        "value": 1.0 / t_head, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": t_head * 1e3, "higher_is_better": True,
+        "value": head.get("steps_per_s"), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head.get("ms_per_step"), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32-matrix/f64-residual",
         "data": "synthetic",
         "config": {"workload": f"cloth {args.n}x{args.n} quads, noise 1e-3 m, one step = "
@@ -472,27 +571,19 @@ def main():
                    "vertices": sc.N, "triangles": sc.F, "hinges": sc.H, "blocks": nb,
                    "bending_hessian": "spd-projected", "weights": wsrc,
                    "multilevel_aggregates": None if args.no_ml else ml_counts},
-        "iters": [s["iters"] for s in st_head],
-        "launched": [s["launched"] for s in st_head],
-        "host_syncs": [s["host_syncs"] for s in st_head],
-        "used_fallback": [s["used_fallback"] for s in st_head],
-        "true_relres": [s["true_relres"] for s in st_head],
-        "phase_ms": {k: float(np.mean([s[k] for s in st_head])) for k in
-                     ("ms_assembly", "ms_rhs", "ms_solve", "ms_update")},
-        "fp64_bj": out_fp64,
-        "mixed_bj": out_mbj,
-        **extra,
+        "speedup_vs_fp64_bj": legs.get("fp64_bj", {}).get("speedup_of_headline"),
+        "roofline": roof,
+        "roofline_step": head.get("roofline_step"),
+        "cpu_baseline": cpu,
+        "legs": legs,
         "multilevel_build_s": ml_build_s,
         "topology_rebuild_s": round(rebuild_s, 3),
-        "eol_fill": eol_out,
-        "velocity_transfer": transfer,
         "autotune": tuned,
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "batched": batched,
-        "collision": collision,
+        **extra,
     }
     print(json.dumps(res), flush=True)
+    if not ok:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

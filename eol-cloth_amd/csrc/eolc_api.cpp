@@ -76,7 +76,7 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->x); dfree(c->v); dfree(c->f); dfree(c->rhs);
     dfree(c->A64); dfree(c->S32); dfree(c->M32); dfree(c->entry_is_diag); dfree(c->Mval); dfree(c->D64); dfree(c->D32);
     dfree(c->supper); dfree(c->uoff); dfree(c->loff); dfree(c->lrec); dfree(c->e2u); dfree(c->S32u); dfree(c->S32a);
-    dfree(c->fixmask); dfree(c->vfix);
+    dfree(c->fixmask); dfree(c->vfix); dfree(c->fix_dofs); dfree(c->fix_vals);
     dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
     dfree(c->p64[0]); dfree(c->p64[1]);
     dfree(c->r32); dfree(c->z32); dfree(c->q32); dfree(c->xl32);
@@ -93,6 +93,8 @@ void free_topology_buffers(eolc_ctx *c)
     L.nbs = 0;
     eolc::BatchState &S = c->batch;
     dfree(S.d_inst); dfree(S.d_order); dfree(S.d_user_fixed); dfree(S.d_scripted); dfree(S.slice_part); dfree(S.stats); dfree(S.dstats);
+    dfree(S.d_all_fixed);
+    S.n_all_fixed = 0;
     dfree(S.d_ml_lo);
     S.ml = S.graph_ml = false;
     if (S.graph) (void)hipGraphExecDestroy(S.graph);
@@ -591,41 +593,53 @@ bool mixed_mode(int mode)
 }
 
 // Fixed DOFs -> device mask (and values) for the duration of one call.
+// Marks one call's fixed DOFs in fixmask / vfix from the context's own device lists
+// (capacity 3N, allocated with the topology: no allocation per call) and clears
+// them again on every return path.  vals = nullptr: zero values; scripted: vals are
+// scripted velocities and the fixed values become (1 - 0.01) v_old + vals, formed
+// on the device.  sync = false leaves the host-to-device copies in flight: the
+// caller keeps dofs and vals alive until its next stream synchronisation.
 struct FixedScope {
     eolc_ctx *c = nullptr;
     int n = 0;
-    int *d_dofs = nullptr;
-    double *d_vals = nullptr;
-    int set(eolc_ctx *ctx, int nf, const int *dofs, const double *vals)
+    int set(eolc_ctx *ctx, int nf, const int *dofs, const double *vals, bool scripted = false, bool sync = true)
     {
-        eolc_ctx *c = ctx;
-        this->c = ctx;
-        const int N = c->topo.N;
+        const int N = ctx->topo.N;
+        if (nf > 3 * N) return fail(ctx, EOLC_ERR_INVALID, "more fixed DOFs than the mesh has DOFs");
         for (int i = 0; i < nf; ++i)
             if (dofs[i] < 0 || dofs[i] >= 3 * N || (vals && !std::isfinite(vals[i])))
-                return fail(c, EOLC_ERR_INVALID, "fixed DOF index out of range or value non-finite");
+                return fail(ctx, EOLC_ERR_INVALID, "fixed DOF index out of range or value non-finite");
         if (nf <= 0) return EOLC_OK;
-        std::vector<double> zeros;
-        if (!vals) {
-            zeros.assign(nf, 0.0);
-            vals = zeros.data();
-        }
-        EOLC_HIP(hipMalloc((void **)&d_dofs, sizeof(int) * nf));
-        EOLC_HIP(hipMalloc((void **)&d_vals, sizeof(double) * nf));
-        EOLC_HIP(hipMemcpyAsync(d_dofs, dofs, sizeof(int) * nf, hipMemcpyHostToDevice, c->stream));
-        EOLC_HIP(hipMemcpyAsync(d_vals, vals, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
-        n = nf;
-        EOLC_HIP(eolc::set_fixed(c, nf, d_dofs, d_vals, 1));
-        EOLC_HIP(hipStreamSynchronize(c->stream));   // host copy of vals may be a temporary
+        c = ctx;
+        n = nf;   // from here on the destructor clears the mask
+        EOLC_HIP(hipMemcpyAsync(c->fix_dofs, dofs, sizeof(int) * nf, hipMemcpyHostToDevice, c->stream));
+        if (vals)
+            EOLC_HIP(hipMemcpyAsync(c->fix_vals, vals, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+        else
+            EOLC_HIP(hipMemsetAsync(c->fix_vals, 0, sizeof(double) * nf, c->stream));
+        if (scripted) EOLC_HIP(eolc::fixed_from_vold(c, nf, c->fix_dofs, c->fix_vals));
+        EOLC_HIP(eolc::set_fixed(c, nf, c->fix_dofs, c->fix_vals, 1));
+        if (sync) EOLC_HIP(hipStreamSynchronize(c->stream));
         return EOLC_OK;
     }
+    // stream-ordered: later work on the context's stream sees the cleared mask; a
+    // batched context gets its own fixed nodes (corners and padding) back
     ~FixedScope()
     {
-        if (!c) return;
-        if (n > 0) (void)eolc::set_fixed(c, n, d_dofs, d_vals, 0);
-        (void)hipStreamSynchronize(c->stream);
-        if (d_dofs) (void)hipFree(d_dofs);
-        if (d_vals) (void)hipFree(d_vals);
+        if (c && n > 0) {
+            (void)eolc::set_fixed(c, n, c->fix_dofs, c->fix_vals, 0);
+            if (c->batch.B > 0) (void)eolc::batch_set_fixmask(c, c->batch.d_all_fixed, c->batch.n_all_fixed);
+        }
+    }
+};
+
+// hipEvents destroyed on every return path
+struct EventSet {
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    ~EventSet()
+    {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -764,6 +778,8 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(dalloc(c, &c->D32, eolc::kDsym * (size_t)N));
     EOLC_HIP(dalloc(c, &c->fixmask, ((size_t)N + 3) & ~(size_t)3));
     EOLC_HIP(dalloc(c, &c->vfix, n3));
+    EOLC_HIP(dalloc(c, &c->fix_dofs, n3));
+    EOLC_HIP(dalloc(c, &c->fix_vals, n3));
     EOLC_HIP(hipMemset(c->fixmask, 0, ((size_t)N + 3) & ~(size_t)3));
     EOLC_HIP(hipMemset(c->vfix, 0, n3 * sizeof(double)));
     for (double **p : {&c->b64, &c->xs64, &c->r64, &c->z64, &c->q64, &c->p64[0], &c->p64[1]})
@@ -849,7 +865,7 @@ static int forces_fill_async(eolc_ctx *c, const double grav[3], double h, bool w
     EOLC_HIP(hipMemsetAsync(c->counter + 1, 0, 2 * sizeof(unsigned), c->stream));
     EOLC_HIP(eolc::launch_face_eval(c, h, grav));
     EOLC_HIP(eolc::launch_hinge_eval(c, h));
-    EOLC_HIP(eolc::launch_assemble(c, want32));
+    EOLC_HIP(eolc::launch_assemble(c, want32 ? eolc::fp32_layout(c->spmv_variant[0]) : 0));
     c->h_last = h;
     c->have_forces = true;
     return EOLC_OK;
@@ -1108,10 +1124,14 @@ try {
 // ---------------------------------------------------------------------------
 // scripted != 0: fixed_val holds scripted velocities and the fixed values are
 // (1 - 0.01) v_old + fixed_val, formed on the device (no host read of v_old).
+// A solve ends with EOLC_OK only when the FP64 true residual of its answer meets
+// the tolerance: when the recursive residual of either precision meets it, the
+// true residual replaces it and the iteration goes on from there if it does not.
+// keep_v_on_fail: a failed solve leaves the resident velocity as it was (the step
+// driver); otherwise v receives the last iterate whatever the status.
 static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *fixed_val, double tol,
-                      int maxit, int mode, eolc_solve_stats *st, int scripted = 0)
+                      int maxit, int mode, eolc_solve_stats *st, int scripted = 0, bool keep_v_on_fail = false)
 {
-    const int N = c->topo.N;
     if (c->eol.E > 0)
         return fail(c, EOLC_ERR_INVALID,
                     "velocity_solve: EOL nodes are set; the EOL system goes to the host QP with its EOL rows");
@@ -1129,21 +1149,14 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         const int rc = learned_buffers(c);
         if (rc) return rc;
     }
-    // fixed DOFs -> device mask and values
-    for (int i = 0; i < n_fixed; ++i)
-        if (fixed_dof[i] < 0 || fixed_dof[i] >= 3 * N || !std::isfinite(fixed_val[i]))
-            return fail(c, EOLC_ERR_INVALID, "velocity_solve: fixed DOF index out of range or value non-finite");
-    int *d_dofs = nullptr;
-    double *d_vals = nullptr;
-    if (n_fixed > 0) {
-        EOLC_HIP(hipMalloc((void **)&d_dofs, sizeof(int) * n_fixed));
-        EOLC_HIP(hipMalloc((void **)&d_vals, sizeof(double) * n_fixed));
-        EOLC_HIP(hipMemcpyAsync(d_dofs, fixed_dof, sizeof(int) * n_fixed, hipMemcpyHostToDevice, c->stream));
-        EOLC_HIP(hipMemcpyAsync(d_vals, fixed_val, sizeof(double) * n_fixed, hipMemcpyHostToDevice, c->stream));
-        if (scripted) EOLC_HIP(eolc::fixed_from_vold(c, n_fixed, d_dofs, d_vals));
-        EOLC_HIP(eolc::set_fixed(c, n_fixed, d_dofs, d_vals, 1));
+    // fixed DOFs -> device mask and values (the host arrays outlive the first sync below)
+    FixedScope fx;
+    {
+        const int rc = fx.set(c, n_fixed, fixed_dof, fixed_val, scripted != 0, false);
+        if (rc) return rc;
     }
     const int precond = (mode == EOLC_SOLVE_FP64_JACOBI) ? 1 : 0;
+    if (mixed) EOLC_HIP(eolc::ensure_fp32_layout(c, c->spmv_variant[0]));
     EOLC_HIP(eolc::pcg_setup(c, mixed, precond));
     // Learned mode: P replaces the FP32 block-Jacobi blocks; after twice the
     // iterations of the last converged block-Jacobi solve on this context (maxit
@@ -1177,7 +1190,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
             used_fallback = 1;
             EOLC_HIP(eolc::pcg_setup(c, mixed, 0));
             EOLC_HIP(eolc::pcg_reliable_update(c));
-            EOLC_HIP(eolc::pcg_restart_inner(c, k));
+            EOLC_HIP(eolc::pcg_restart_inner(c, mixed, k));
             EOLC_HIP(eolc::scal_restart(c, k));
             rr_cycle = -1.0;
             k_cycle = k;
@@ -1214,9 +1227,11 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         }
         if (s.reason == 2) { status = EOLC_ERR_NOT_SPD; break; }
         if (s.reason == 4) { status = EOLC_ERR_NONFINITE; break; }
-        if (!mixed) { status = EOLC_OK; break; }
-        // mixed: reliable update with the true FP64 residual
-        EOLC_HIP(eolc::pcg_reliable_update(c));
+        // Replace the recursive residual by the true FP64 residual of the current
+        // iterate: mixed, after folding the FP32 correction in (reliable update);
+        // FP64, whose recursion drifts from the true residual over long solves.
+        if (mixed) EOLC_HIP(eolc::pcg_reliable_update(c));
+        else EOLC_HIP(eolc::pcg_replace_residual64(c));
         EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
         EOLC_HIP(hipStreamSynchronize(c->stream));
         ++syncs;
@@ -1227,7 +1242,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         if (s2.rmax2 <= tol * tol * s2.bnorm2) { status = EOLC_OK; break; }
         if (k >= maxit || refinements > eolc::kMaxRefinements) { status = EOLC_ERR_MAXIT; break; }
         if (learned && k >= cap) continue;   // the fallback restarts from here
-        EOLC_HIP(eolc::pcg_restart_inner(c, k));
+        EOLC_HIP(eolc::pcg_restart_inner(c, mixed, k));
         EOLC_HIP(eolc::scal_set(c, 0, 0, 0));
     }
     if (status == EOLC_OK && !learned && used_fallback == 0 &&
@@ -1237,13 +1252,11 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         // fold the last FP32 correction in before reporting
         EOLC_HIP(eolc::pcg_reliable_update(c));
     }
-    EOLC_HIP(eolc::pcg_finish(c, mixed));
-    EOLC_HIP(eolc::spmv_true_residual(c));
+    double *out = (status == EOLC_OK || !keep_v_on_fail) ? c->v : c->xs64;
+    EOLC_HIP(eolc::pcg_finish(c, out));
+    EOLC_HIP(eolc::spmv_true_residual(c, out));
     EOLC_HIP(hipMemcpyAsync(c->hscal, c->scal, sizeof(eolc::SolverScalars), hipMemcpyDeviceToHost, c->stream));
-    if (n_fixed > 0) EOLC_HIP(eolc::set_fixed(c, n_fixed, d_dofs, d_vals, 0));
     EOLC_HIP(hipStreamSynchronize(c->stream));
-    if (d_dofs) EOLC_HIP(hipFree(d_dofs));
-    if (d_vals) EOLC_HIP(hipFree(d_vals));
     if (st) {
         st->iters = c->hscal->iters;
         st->launched = launched;
@@ -1429,8 +1442,9 @@ static int step_impl(eolc_ctx *c, const double grav[3], double h, int n_dofs, co
     const int N = c->topo.N;
     for (int i = 0; i < n_dofs; ++i)
         if (dofs[i] < 0 || dofs[i] >= 3 * N) return fail(c, EOLC_ERR_INVALID, "step: fixed node or DOF out of range");
-    hipEvent_t ev[5];
-    for (auto &e : ev) EOLC_HIP(hipEventCreate(&e));
+    EventSet es;
+    hipEvent_t *ev = es.ev;
+    for (int i = 0; i < 5; ++i) EOLC_HIP(hipEventCreate(&ev[i]));
     EOLC_HIP(hipEventRecord(ev[0], c->stream));
     const bool mixed = mixed_mode(mode);
     int rc = forces_fill_async(c, grav, h, mixed);
@@ -1445,7 +1459,7 @@ static int step_impl(eolc_ctx *c, const double grav[3], double h, int n_dofs, co
         for (int i = 0; i < n_dofs; ++i) vals[i] = scripted[i];
     eolc_solve_stats local;
     std::memset(&local, 0, sizeof(local));
-    rc = solve_impl(c, n_dofs, dofs, vals.data(), tol, maxit, mode, &local, 1);
+    rc = solve_impl(c, n_dofs, dofs, vals.data(), tol, maxit, mode, &local, 1, true);
     EOLC_HIP(hipEventRecord(ev[3], c->stream));
     if (rc == EOLC_OK) EOLC_HIP(eolc::launch_update_x(c, h));
     EOLC_HIP(hipEventRecord(ev[4], c->stream));
@@ -1454,7 +1468,6 @@ static int step_impl(eolc_ctx *c, const double grav[3], double h, int n_dofs, co
     local.ms_rhs = elapsed_ms(ev[1], ev[2]);
     local.ms_solve = elapsed_ms(ev[2], ev[3]);
     local.ms_update = elapsed_ms(ev[3], ev[4]);
-    for (auto &e : ev) (void)hipEventDestroy(e);
     if (st) *st = local;
     return rc;
 }
@@ -1567,14 +1580,35 @@ static int learned_upload_error(eolc_ctx *c, const double *e)
     return EOLC_OK;
 }
 
+int eolc_learned_set_multilevel(eolc_ctx *c, int on)
+try {
+    if (!c) return EOLC_ERR_INVALID;
+    c->lp.with_ml = on != 0;
+    return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "learned_set_multilevel");
+}
+
+// The training loss inside the multilevel preconditioner needs the hierarchy, the
+// FP64 fine blocks (the restriction divides by them nowhere, but ml_setup reads the
+// current fixed set) and the Galerkin blocks of every level for the current MDK.
+static int learned_ml_setup(eolc_ctx *c)
+{
+    EOLC_HIP(eolc::pcg_setup(c, false, 0));
+    return EOLC_OK;
+}
+
 int eolc_learned_loss_grad(eolc_ctx *c, int n_fixed, const int *fixed_dof, const double *e, double *loss, float *grad)
 try {
     if (!c) return EOLC_ERR_INVALID;
     if (!e) return fail(c, EOLC_ERR_INVALID, "learned_loss_grad: null error vector");
     int rc = learned_ready(c, "learned_loss_grad", n_fixed, fixed_dof);
     if (rc) return rc;
+    if (c->lp.with_ml && (rc = ml_prepare(c))) return rc;
     FixedScope fs;
     if ((rc = fs.set(c, n_fixed, fixed_dof, nullptr))) return rc;
+    MlScope mls(c, c->lp.with_ml);
+    if (c->lp.with_ml && (rc = learned_ml_setup(c))) return rc;
     if ((rc = learned_upload_error(c, e))) return rc;
     EOLC_HIP(eolc::learned_loss_grad(c));
     double sc[8];
@@ -1595,8 +1629,11 @@ try {
     if (!e || steps <= 0 || !(lr > 0.0)) return fail(c, EOLC_ERR_INVALID, "learned_train: bad arguments");
     int rc = learned_ready(c, "learned_train", n_fixed, fixed_dof);
     if (rc) return rc;
+    if (c->lp.with_ml && (rc = ml_prepare(c))) return rc;
     FixedScope fs;
     if ((rc = fs.set(c, n_fixed, fixed_dof, nullptr))) return rc;
+    MlScope mls(c, c->lp.with_ml);
+    if (c->lp.with_ml && (rc = learned_ml_setup(c))) return rc;
     if ((rc = learned_upload_error(c, e))) return rc;
     for (int s = 0; s < steps; ++s) {
         EOLC_HIP(eolc::learned_loss_grad(c));
@@ -1693,11 +1730,10 @@ try {
     EOLC_HIP(dalloc(c, &S.slice_part, 4 * (size_t)c->topo.nslices));
     EOLC_HIP(dalloc(c, &S.stats, 4 * (size_t)B));
     EOLC_HIP(dalloc(c, &S.dstats, 4 * (size_t)B));
-    int *d_all = nullptr;
-    EOLC_HIP(upload(c, &d_all, all_fixed));
-    EOLC_HIP(eolc::batch_set_fixmask(c, d_all, (int)all_fixed.size()));
+    EOLC_HIP(upload(c, &S.d_all_fixed, all_fixed));
+    S.n_all_fixed = (int)all_fixed.size();
+    EOLC_HIP(eolc::batch_set_fixmask(c, S.d_all_fixed, S.n_all_fixed));
     EOLC_HIP(hipStreamSynchronize(c->stream));
-    dfree(d_all);
     batch_invalidate_graph(c);
     if (vert_off)
         for (int b = 0; b <= B; ++b) vert_off[b] = off[b];
@@ -1747,7 +1783,7 @@ static int batch_step_enqueue(eolc_ctx *c, const double grav[3], double h, doubl
 {
     EOLC_HIP(eolc::launch_face_eval(c, h, grav));
     EOLC_HIP(eolc::launch_hinge_eval(c, h));
-    EOLC_HIP(eolc::launch_assemble(c, true));
+    EOLC_HIP(eolc::launch_assemble(c, eolc::kLaySoA));
     EOLC_HIP(eolc::launch_rhs(c, h));
     EOLC_HIP(eolc::batch_solve_async(c, tol, maxit, variant));
     EOLC_HIP(eolc::launch_update_x(c, h));
@@ -1858,7 +1894,7 @@ try {
     // forces once, then the solve of the same system with each workgroup size
     EOLC_HIP(eolc::launch_face_eval(c, h, grav));
     EOLC_HIP(eolc::launch_hinge_eval(c, h));
-    EOLC_HIP(eolc::launch_assemble(c, true));
+    EOLC_HIP(eolc::launch_assemble(c, eolc::kLaySoA));
     EOLC_HIP(eolc::launch_rhs(c, h));
     hipEvent_t e0, e1;
     EOLC_HIP(hipEventCreate(&e0));

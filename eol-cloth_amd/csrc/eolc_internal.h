@@ -149,6 +149,8 @@ struct BatchState {
     BatchInst *d_inst = nullptr;
     int *d_order = nullptr;
     int *d_user_fixed = nullptr;
+    int *d_all_fixed = nullptr;      // every fixed node (corners and padding): the batch's fixmask
+    int n_all_fixed = 0;
     double *d_scripted = nullptr;    // 3 per user-fixed node: scripted velocity (zero by default)
     double *slice_part = nullptr;    // per-slice FP64 partial sums (4 x nslices)
     int *stats = nullptr;            // per instance: iters, refinements, status, spare
@@ -219,6 +221,7 @@ constexpr int kLpGradWG = 256;   // maximum workgroups of the weight-gradient ke
 
 struct LearnedState {
     bool have_weights = false;
+    bool with_ml = false;        // the training loss adds the multilevel coarse corrections to z
     int N = 0;                   // vertex count the buffers were sized for
     int64_t nbs = 0;
     int adam_t = 0;
@@ -267,12 +270,15 @@ struct eolc_ctx {
     int *supper = nullptr, *uoff = nullptr, *loff = nullptr, *lrec = nullptr, *e2u = nullptr;
     float *S32u = nullptr;      // 10 x nup
     float *S32a = nullptr;      // 10 x nbs: {column bits, S'_ij} per SELL entry (record layout)
+    int s32_valid = 0;          // FP32 layouts (kLay* bits) holding the last assembly's values
     int *entry_is_diag = nullptr;  // nbs
     double *Mval = nullptr;     // nbs mass scalars (M block = m I3)
     double *D64 = nullptr;      // 6 x N  block-Jacobi inverse (symmetric, SoA)
     float *D32 = nullptr;
     unsigned char *fixmask = nullptr;  // N, bit k = DOF k fixed
     double *vfix = nullptr;     // 3N fixed values (0 where free)
+    int *fix_dofs = nullptr;    // 3N: the current call's fixed DOF list (no per-call allocation)
+    double *fix_vals = nullptr; // 3N: its values
     // PCG work vectors
     double *b64 = nullptr, *xs64 = nullptr, *r64 = nullptr, *z64 = nullptr, *q64 = nullptr;
     double *p64[2] = {nullptr, nullptr};
@@ -340,7 +346,13 @@ inline int report_exception(eolc_ctx *c, const char *who) noexcept
 // kernels (k_elements.hip, k_assemble.hip, k_pcg.hip)
 hipError_t launch_face_eval(eolc_ctx *c, double h, const double *grav);
 hipError_t launch_hinge_eval(eolc_ctx *c, double h);
-hipError_t launch_assemble(eolc_ctx *c, bool want32);
+// FP32 matrix layouts (bits of launch_assemble's `lay`): SoA S32 (SpMV layouts 0 and 3,
+// the batched solve), symmetric upper records S32u (layout 1), per-entry records
+// S32a (layout 2)
+constexpr int kLaySoA = 1, kLaySym = 2, kLayRecord = 4;
+int fp32_layout(int variant);                          // the layout bit a FP32 SpMV variant reads
+hipError_t launch_assemble(eolc_ctx *c, int lay);      // FP64 values + the FP32 layouts in lay
+hipError_t ensure_fp32_layout(eolc_ctx *c, int variant);
 hipError_t launch_rhs(eolc_ctx *c, double h);
 hipError_t launch_update_x(eolc_ctx *c, double h);
 
@@ -349,10 +361,11 @@ hipError_t pcg_setup(eolc_ctx *c, bool mixed, int precond_kind);
 hipError_t pcg_init(eolc_ctx *c, bool mixed);
 hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters);
 hipError_t pcg_reliable_update(eolc_ctx *c);
-hipError_t pcg_restart_inner(eolc_ctx *c, int k);
+hipError_t pcg_restart_inner(eolc_ctx *c, bool mixed, int k);
 hipError_t pcg_precondition(eolc_ctx *c, bool mixed);   // z = P r for r in r64 (parity 0)
-hipError_t pcg_finish(eolc_ctx *c, bool mixed);
-hipError_t spmv_true_residual(eolc_ctx *c);
+hipError_t pcg_replace_residual64(eolc_ctx *c);         // FP64 mode: r = b - A x, ||r||^2 -> rmax2
+hipError_t pcg_finish(eolc_ctx *c, double *out);        // solution with fixed values -> out
+hipError_t spmv_true_residual(eolc_ctx *c, const double *v);   // ||rhs - A v||^2 -> true_rr
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
 // d_vals[t] = (1 - 0.01) v[d_dofs[t]] + d_vals[t] on the context's stream.
@@ -377,6 +390,7 @@ std::string ml_build(const Topology &t, std::vector<std::vector<int>> &agg_level
                      std::vector<unsigned char> &lam, MLLevels &lv);
 hipError_t ml_setup(eolc_ctx *c);                             // Galerkin diagonal blocks of every level
 hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done);   // z = D0 r + coarse, r.z partials
+hipError_t ml_coarse64(eolc_ctx *c);                          // z64 = coarse corrections of r64 (no fine term)
 // velocity transfer after a remesh (k_transfer.hip); host arrays in and out
 hipError_t velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const double *v_old, int f_old,
                              const int *tri_old, int n_new, const double *u_new, double *v_new, int *src, int *face,

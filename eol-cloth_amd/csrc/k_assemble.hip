@@ -14,7 +14,9 @@
 //                S'_ij = dampingB h^2 K_ij + m_ij I off the diagonal (0 on it), with
 //                which (A p)_i = Mbar_i p_i + sum_{j != i} S'_ij (p_j - p_i),
 //                Mbar_i = sum_j m_ij (since sum_j m_ij p_j = Mbar_i p_i +
-//                sum_{j != i} m_ij (p_j - p_i)).
+//                sum_{j != i} m_ij (p_j - p_i)).  The FP32 values go only to the
+//                layouts in `lay` (SoA, symmetric upper records, per-entry records):
+//                the one the chosen SpMV variant reads, the others on demand.
 //   k_rhs      : one thread per vertex (SELL storage order); f_i = sum of its element force slots
 //                (faces, then hinges when the bending force is included),
 //                rhs_i = sum_j m_ij v_j + h f_i, the right-hand side M v + h f of
@@ -26,7 +28,7 @@ namespace {
 __global__ void __launch_bounds__(256)
 k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *__restrict__ clist,
            const int *__restrict__ entry_is_diag, const double *__restrict__ stageK, const double *__restrict__ stageM,
-           double *__restrict__ A64, float *__restrict__ S32, const int *__restrict__ e2u,
+           double *__restrict__ A64, int lay, float *__restrict__ S32, const int *__restrict__ e2u,
            const int *__restrict__ scol, float *__restrict__ S32u, float *__restrict__ S32a,
            double *__restrict__ Mval)
 {
@@ -55,19 +57,22 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
     }
 #pragma unroll
     for (int c = 0; c < 9; ++c) A64[c * nbs + b] = acc[c];
-    if (S32) {
+    if (lay) {
         const bool diag = entry_is_diag[b] != 0;
         float sv[9];
 #pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            sv[c] = diag ? 0.0f : (float)(stf[c] + (c % 4 == 0 ? m : 0.0));
-            S32[c * nbs + b] = sv[c];
-        }
-        float *ra = S32a + (size_t)10 * b;   // record layout: every entry
-        ra[0] = __int_as_float(scol[b]);
+        for (int c = 0; c < 9; ++c) sv[c] = diag ? 0.0f : (float)(stf[c] + (c % 4 == 0 ? m : 0.0));
+        if (lay & eolc::kLaySoA) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) ra[1 + c] = sv[c];
-        const int u = e2u[b];
+            for (int c = 0; c < 9; ++c) S32[c * nbs + b] = sv[c];
+        }
+        if (lay & eolc::kLayRecord) {   // record layout: every entry
+            float *ra = S32a + (size_t)10 * b;
+            ra[0] = __int_as_float(scol[b]);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) ra[1 + c] = sv[c];
+        }
+        const int u = (lay & eolc::kLaySym) ? e2u[b] : -1;
         if (u >= 0) {   // upper block: its record in the symmetric layout
             float *rec = S32u + (size_t)10 * u;
             rec[0] = __int_as_float(scol[b]);
@@ -119,14 +124,30 @@ inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
 namespace eolc {
 
-hipError_t launch_assemble(eolc_ctx *c, bool want32)
+hipError_t launch_assemble(eolc_ctx *c, int lay)
 {
     const int64_t nb = c->topo.nbs;
+    c->s32_valid = 0;
     if (nb == 0) return hipSuccess;
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
-                       c->entry_is_diag, c->stageK, c->stageM, c->A64, want32 ? c->S32 : nullptr, c->e2u, c->scol,
-                       c->S32u, c->S32a, c->Mval);
-    return hipGetLastError();
+                       c->entry_is_diag, c->stageK, c->stageM, c->A64, lay, c->S32, c->e2u, c->scol, c->S32u,
+                       c->S32a, c->Mval);
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) c->s32_valid = lay;
+    return e;
+}
+
+// The FP32 layout that SpMV variant `variant` reads, written from the staged element
+// blocks of the last assembly if that assembly did not write it (same values, same
+// bits: the fold is the same; the FP64 values are rewritten unchanged).
+hipError_t ensure_fp32_layout(eolc_ctx *c, int variant)
+{
+    const int need = fp32_layout(variant);
+    if ((c->s32_valid & need) == need) return hipSuccess;
+    const int have = c->s32_valid;
+    const hipError_t e = launch_assemble(c, need);
+    if (e == hipSuccess) c->s32_valid = have | need;
+    return e;
 }
 
 hipError_t launch_rhs(eolc_ctx *c, double h)

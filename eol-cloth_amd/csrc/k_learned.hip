@@ -19,6 +19,8 @@
 //
 // Training loss on the current system for an error vector e (r = A e):
 //     z = P r,   loss = 1 - (z.r)^2 / ((z.A z)(r.e))
+// (with eolc_learned_set_multilevel: z = P r + the multilevel coarse corrections of r,
+// the preconditioner the blocks are used in; the gradient flows through P r only)
 // -- the A-norm error left after one optimally scaled step along z relative to
 // the initial one (0 <= loss <= 1 by Cauchy-Schwarz).  Every reduction over
 // vertices goes through per-workgroup partials summed in a fixed order, so the
@@ -426,6 +428,27 @@ k_lp_precond(int N, const int *__restrict__ diagpos, const double *__restrict__ 
     }
 }
 
+// z += zc (the multilevel coarse corrections of r) and partial sums of the new z.r:
+// the training loss of the blocks used inside the multilevel preconditioner.
+__global__ void __launch_bounds__(RT)
+k_lp_add_coarse(int N, const double *__restrict__ zc, const double *__restrict__ r, double *__restrict__ z,
+                double *__restrict__ part)
+{
+    __shared__ double s[RT];
+    const int i = blockIdx.x * RT + threadIdx.x;
+    double zr = 0.0;
+    if (i < N) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double zz = z[3 * i + a] + zc[3 * i + a];
+            z[3 * i + a] = zz;
+            zr += zz * r[3 * i + a];
+        }
+    }
+    const double t = block_sum(zr, s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // y = A_red x (FP64; fixed rows give 0) in SELL order, partial sums of y.u.
 __global__ void __launch_bounds__(RT)
 k_lp_spmv_dot(int N, const int *__restrict__ srow, const int *__restrict__ slice_off, const int *__restrict__ slen,
@@ -827,6 +850,15 @@ hipError_t learned_loss_grad(eolc_ctx *c)
     hipLaunchKernelGGL(k_lp_reduce, dim3(1), dim3(RT), 0, c->stream, nblk, L.dpart, L.scal + 1);
     hipLaunchKernelGGL(k_lp_precond, dim3(nblk), dim3(RT), 0, c->stream, N, c->diagpos, c->A64, t.nbs, c->fixmask, L.o,
                        (float *)nullptr, L.r, L.z, L.dpart);
+    if (c->ml_on) {
+        // z = P r + coarse(r): the coarse part does not depend on the parameters, so
+        // the backward pass below is unchanged (it reads r, Az and the scalars)
+        err = hipMemcpyAsync(c->r64, L.r, sizeof(double) * 3 * (size_t)N, hipMemcpyDeviceToDevice, c->stream);
+        if (err != hipSuccess) return err;
+        err = ml_coarse64(c);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(k_lp_add_coarse, dim3(nblk), dim3(RT), 0, c->stream, N, c->z64, L.r, L.z, L.dpart);
+    }
     hipLaunchKernelGGL(k_lp_reduce, dim3(1), dim3(RT), 0, c->stream, nblk, L.dpart, L.scal + 2);
     hipLaunchKernelGGL(k_lp_spmv_dot, dim3(nblk), dim3(RT), 0, c->stream, N, c->srow, c->slice_off, c->slen, c->scol,
                        c->A64, t.nbs, c->fixmask, L.z, L.Az, L.z, L.dpart);

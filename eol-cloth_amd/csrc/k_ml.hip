@@ -411,7 +411,9 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
 // z = D0 r + coarse correction (zero on fixed DOFs); r.z partials -> PB[parity] slot 0.
 // Levels 1 and 2 are applied here, per vertex, from the restricted sums; levels >= 3
 // arrive accumulated in C_3.
-template <class T>
+// FINE = false: the coarse corrections alone (z_i = C(a_i) + C_3), for the learned
+// blocks' training loss, whose fine term is the network's.
+template <class T, bool FINE = true>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ anc3,
              const double *__restrict__ Dinv, const double *__restrict__ R, const double *__restrict__ C,
@@ -448,8 +450,8 @@ k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const
             apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
         }
         const T r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
-        T f0, f1, f2;
-        eolc::apply_sym3<T>(D + i, N, r0, r1, r2, f0, f1, f2);
+        T f0 = (T)0, f1 = (T)0, f2 = (T)0;
+        if constexpr (FINE) eolc::apply_sym3<T>(D + i, N, r0, r1, r2, f0, f1, f2);
         const unsigned fm = fixmask[i];
         const T z0 = (fm & 1u) ? (T)0 : (T)((double)f0 + c0);
         const T z1 = (fm & 2u) ? (T)0 : (T)((double)f1 + c1);
@@ -506,6 +508,26 @@ hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
         hipLaunchKernelGGL(k_ml_prolong<double>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.anc3,
                            m.Dinv, m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, check_done);
     }
+    return hipGetLastError();
+}
+
+// z64 = the coarse corrections of r64 alone (no fine blocks), FP64, fixed DOFs zero.
+hipError_t ml_coarse64(eolc_ctx *c)
+{
+    const MLState &m = c->ml;
+    const MLLevels &lv = m.lv;
+    const int N = c->topo.N;
+    const int G = pcg_grid(N);
+    if (lv.L <= 0) return hipMemsetAsync(c->z64, 0, sizeof(double) * 3 * (size_t)N, c->stream);
+    const int waves = lv.L >= 2 ? lv.n[2] : (lv.n[1] + 63) / 64;
+    const int rblocks = blocks_for((int64_t)64 * waves, kRestrictWG);
+    hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
+                       m.kptr2, c->r64, m.R, m.Dinv, m.C, c->scal, 0);
+    if (lv.L >= 3)
+        hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
+                           c->scal, 0);
+    hipLaunchKernelGGL((k_ml_prolong<double, false>), dim3(G), dim3(WG), 0, c->stream, 0, N, lv, m.agg1, m.anc3, m.Dinv,
+                       m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, 0);
     return hipGetLastError();
 }
 

@@ -687,6 +687,12 @@ __global__ void k_fill(int n, double v, double *__restrict__ out)
     X(16, 1, false, true, 8, 3)     \
     X(17, 1, false, true, 6, 3)
 
+// layout column of a variant -> the FP32 array it reads
+constexpr int lay_bit(int lay) { return lay == 1 ? eolc::kLaySym : lay == 2 ? eolc::kLayRecord : eolc::kLaySoA; }
+#define EOLC_SPMV_LAYOUT_CASE(id, U, PF, NT, W, LAY) \
+    case id:                                         \
+        return lay_bit(LAY);
+
 template <class T>
 void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm, const T *z, const T *pold, T *pnew,
                  T *q)
@@ -764,6 +770,15 @@ void launch_resid64(eolc_ctx *c, int G, const double *xin, const double *b, doub
 }  // namespace
 
 namespace eolc {
+
+int fp32_layout(int variant)
+{
+    switch (variant) {
+        EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_LAYOUT_CASE)
+    default:
+        return kLaySoA;
+    }
+}
 
 int pcg_grid(int N)
 {
@@ -848,17 +863,31 @@ hipError_t pcg_reliable_update(eolc_ctx *c)
     return hipGetLastError();
 }
 
-// Continue the inner iteration at index k from the replaced residual, keeping p.
-hipError_t pcg_restart_inner(eolc_ctx *c, int k) { return precondition(c, true, k & 1); }
+// Continue the iteration at index k from the replaced residual, keeping p.
+hipError_t pcg_restart_inner(eolc_ctx *c, bool mixed, int k) { return precondition(c, mixed, k & 1); }
 
 // z = P r of the current mode for r in r64 (the preconditioner-application entry point)
 hipError_t pcg_precondition(eolc_ctx *c, bool mixed) { return precondition(c, mixed, 0); }
 
-hipError_t pcg_finish(eolc_ctx *c, bool /*mixed*/)
+// FP64 mode: the recursive residual met the tolerance; replace it with the true
+// FP64 residual b - A x of the current iterate (the search direction is kept, as in
+// the mixed mode's reliable updates); ||r||^2 -> scal->rmax2.
+hipError_t pcg_replace_residual64(eolc_ctx *c)
+{
+    const int G = pcg_grid(c->topo.N);
+    launch_resid64(c, G, c->xs64, c->b64, c->r64);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, c->partials + 5 * kMaxPartials,
+                       &c->scal->rmax2);
+    return hipGetLastError();
+}
+
+// The solution with its fixed values -> out (c->v on success; xs64 itself, in
+// place, when the solve failed and the resident velocity is kept).
+hipError_t pcg_finish(eolc_ctx *c, double *out)
 {
     const int N = c->topo.N;
     hipLaunchKernelGGL(k_write_solution, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, N, c->xs64, c->fixmask,
-                       c->vfix, c->v);
+                       c->vfix, out);
     return hipGetLastError();
 }
 
@@ -874,12 +903,12 @@ hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset)
     return hipGetLastError();
 }
 
-// ||rhs - A v||^2 over free DOFs (v = final solution incl. fixed values) -> scal->true_rr
-hipError_t spmv_true_residual(eolc_ctx *c)
+// ||rhs - A v||^2 over free DOFs (v = a solution incl. fixed values) -> scal->true_rr
+hipError_t spmv_true_residual(eolc_ctx *c, const double *v)
 {
     const int N = c->topo.N;
     const int G = pcg_grid(N);
-    launch_resid64(c, G, c->v, c->rhs, c->r64);
+    launch_resid64(c, G, v, c->rhs, c->r64);
     hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, c->partials + 5 * kMaxPartials,
                        &c->scal->true_rr);
     return hipGetLastError();
@@ -929,6 +958,10 @@ hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms)
     const int N = c->topo.N;
     const int G = pcg_grid(N);
     if (variant < 0) variant = c->spmv_variant[mixed ? 0 : 1];
+    if (mixed) {
+        const hipError_t el = ensure_fp32_layout(c, variant);
+        if (el != hipSuccess) return el;
+    }
     hipEvent_t a, b;
     hipError_t e = hipEventCreate(&a);
     if (e != hipSuccess) return e;

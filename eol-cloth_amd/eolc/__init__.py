@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = (
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
     "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
     "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol", "eolc_velocity_transfer_eol",
+    "eolc_learned_set_multilevel",
 )
 
 BATCH_EAGER = 0x1
@@ -139,6 +140,7 @@ def lib():
         L.eolc_ml_partition.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.c_void_p,
                                         C.c_void_p]
         L.eolc_learned_set_weights.argtypes = [vp, C.c_void_p, C.c_int]
+        L.eolc_learned_set_multilevel.argtypes = [vp, C.c_int]
         L.eolc_learned_get_weights.argtypes = [vp, C.c_void_p, C.c_int]
         L.eolc_learned_infer.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.eolc_learned_loss_grad.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p,
@@ -272,6 +274,13 @@ class Context:
         self._check(lib().eolc_download_forces(self._h, f))
         return f
 
+    def csr_nnz(self, which):
+        """Scalar nonzeros of M or MDK (9 per stored 3x3 block)."""
+        nnz = C.c_int64(0)
+        rc = lib().eolc_download_csr(self._h, int(which), None, None, None, 0, C.byref(nnz))
+        self._check(rc, allow=(ERR_OVERFLOW,))
+        return nnz.value
+
     def download_csr(self, which):
         """Scalar CSR (scipy.sparse.csr_matrix) of M or MDK."""
         import scipy.sparse as sp
@@ -381,6 +390,7 @@ class Context:
     def collide(self, threshold, pts=None, nrm=None, whd=None, E=None, eol_points=False,
                 box_npts=8, box_nedges=12, cap=1 << 16):
         """CD2 (eol_points=False) or CD (True).  E: list of 4x4 row-major frames."""
+        fn = lib().eolc_collide
         pts = _d(pts if pts is not None else np.zeros((0, 3)))
         nrm = _d(nrm if nrm is not None else np.zeros((0, 3)))
         whd = _d(whd if whd is not None else np.zeros((0, 3)))
@@ -391,9 +401,9 @@ class Context:
         while True:
             buf = (CollisionRec * cap)()
             cnt = C.c_int(0)
-            rc = lib().eolc_collide(self._h, float(threshold), P, _ptr(pts), _ptr(nrm), B, _ptr(whd),
-                                    _ptr(Es), int(bool(eol_points)), int(box_npts), int(box_nedges),
-                                    buf, cap, C.byref(cnt))
+            rc = fn(self._h, float(threshold), P, _ptr(pts), _ptr(nrm), B, _ptr(whd),
+                    _ptr(Es), int(bool(eol_points)), int(box_npts), int(box_nedges),
+                    buf, cap, C.byref(cnt))
             if rc == ERR_OVERFLOW and cnt.value > cap:
                 cap = cnt.value
                 continue
@@ -502,6 +512,10 @@ class Context:
         self._check(lib().eolc_learned_loss_grad(self._h, fd.size, _ptr(fd) if fd.size else None,
                                                  _ptr(e), C.byref(loss), _ptr(g)))
         return loss.value, g
+
+    def learned_set_multilevel(self, on=True):
+        """Train the blocks with the multilevel coarse corrections in the loss."""
+        return self._check(lib().eolc_learned_set_multilevel(self._h, int(bool(on))))
 
     def learned_train(self, e, steps, lr, fixed_dof=()):
         fd, e = _i(fixed_dof), _d(e)

@@ -151,17 +151,22 @@ class BatchScene:
     uniform in {side_lo..side_hi} quads, Young's modulus log-uniform in [E_lo, E_hi],
     bending stiffness log-uniform in [beta_lo, beta_hi] (splitmix64 stream `seed`),
     each with the single-cloth initial state (noise sigma_x, sigma_v; noise seeds
-    seed * 1000 + 2b and seed * 1000 + 2b + 1) and its two top corners fixed."""
+    seed * 1000 + 2b and seed * 1000 + 2b + 1) and its two top corners fixed.  With
+    noise_per_spacing the position noise of each cloth is that fraction of its grid
+    spacing instead of sigma_x."""
 
     def __init__(self, B=256, side_lo=16, side_hi=128, E_lo=1e2, E_hi=1e4, beta_lo=1e-6, beta_hi=1e-4,
-                 sigma_x=1e-3, sigma_v=0.05, seed=1237):
+                 sigma_x=1e-3, sigma_v=0.05, seed=1237, noise_per_spacing=None):
         u = uniform(seed, 3 * B).reshape(B, 3)
         self.B = int(B)
         self.sides = (side_lo + np.floor(u[:, 0] * (side_hi - side_lo + 1))).astype(int)
         self.sides = np.minimum(self.sides, side_hi)
         self.E = np.exp(np.log(E_lo) + u[:, 1] * (np.log(E_hi) - np.log(E_lo)))
         self.beta = np.exp(np.log(beta_lo) + u[:, 2] * (np.log(beta_hi) - np.log(beta_lo)))
-        self.inst = [Scene(int(s), sigma_x, sigma_v, seed_x=seed * 1000 + 2 * b, seed_v=seed * 1000 + 2 * b + 1)
+        # noise_per_spacing: position noise sigma_x = noise_per_spacing / side per cloth,
+        # the same noise relative to the grid spacing on every cloth
+        self.sigma_x = [sigma_x if noise_per_spacing is None else noise_per_spacing / int(s) for s in self.sides]
+        self.inst = [Scene(int(s), self.sigma_x[b], sigma_v, seed_x=seed * 1000 + 2 * b, seed_v=seed * 1000 + 2 * b + 1)
                      for b, s in enumerate(self.sides)]
         self.N_real = int(sum(sc.N for sc in self.inst))
 

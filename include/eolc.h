@@ -199,8 +199,14 @@ int eolc_download_csr(eolc_ctx *ctx, int which, int64_t *rowptr, int *col, doubl
 /* ---- GeneralizedSolver::velocitySolve for solver "none" ----------------------
  * Solves MDK v = M v_old + h f on the DOFs not listed in fixed_dof, with
  * v[fixed_dof[i]] = fixed_val[i] (the QP's single-DOF equality rows,
- * Constraints.cpp:114-119).  v_old is the device-resident velocity; the
- * solution replaces it.  Warm start from v_old. */
+ * Constraints.cpp:114-119; at most 3N entries).  v_old is the device-resident
+ * velocity; the solution replaces it.  Warm start from v_old.
+ * EOLC_OK means the FP64 true residual of the returned v meets the tolerance:
+ * ||b - MDK v|| <= tol ||b|| on the free DOFs, stats->true_relres.  In every mode
+ * the recursive residual only triggers that check; when the check fails the true
+ * residual replaces the recursive one and the iteration continues (each such
+ * replacement counts in stats->refinements).  On any other status v holds the last
+ * iterate (with its fixed values), and stats->true_relres is its residual. */
 int eolc_velocity_solve(eolc_ctx *ctx, int n_fixed, const int *fixed_dof,
                         const double *fixed_val, double tol, int maxit, int mode,
                         eolc_solve_stats *stats);
@@ -208,7 +214,9 @@ int eolc_velocity_solve(eolc_ctx *ctx, int n_fixed, const int *fixed_dof,
 /* ---- Cloth::step (Cloth.cpp:359-413, no remesh, no EOL) ----------------------
  * forces_fill, rhs, velocity solve with fixed nodes
  * (v_fixed = 0.99 v_old + scripted velocity, formed on the device from the
- * resident v_old), then x += h v. */
+ * resident v_old), then x += h v.  When the solve fails (any status but EOLC_OK)
+ * the resident x and v are left as they were before the call; the stats describe
+ * the failed solve. */
 int eolc_step(eolc_ctx *ctx, const double grav[3], double h, int n_fixed_nodes,
               const int *fixed_nodes, const double *scripted_vel /* 3 per node or NULL */,
               double tol, int maxit, int mode, eolc_solve_stats *stats);
@@ -285,6 +293,12 @@ int eolc_learned_infer(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, float *
  * floats, may be NULL) with respect to the parameters. */
 int eolc_learned_loss_grad(eolc_ctx *ctx, int n_fixed, const int *fixed_dof, const double *e,
                            double *loss, float *grad);
+
+/* on != 0: the training loss above uses z = P r + the coarse corrections of
+ * EOLC_SOLVE_MULTILEVEL for r, the preconditioner the blocks are used in by
+ * EOLC_SOLVE_MIXED_LEARNED | EOLC_SOLVE_MULTILEVEL (the gradient flows through P r;
+ * the coarse part does not depend on the parameters).  Default off. */
+int eolc_learned_set_multilevel(eolc_ctx *ctx, int on);
 
 /* `steps` Adam steps (lr, beta1 0.9, beta2 0.999) on the current system and e;
  * losses[k] (may be NULL) is the loss before step k.  The Adam state persists

@@ -132,6 +132,9 @@ typedef struct {
 } orc_edge;
 
 void orc_mt19937_uniform(uint32_t seed, int count, double lo, double hi, double *out);
+/* intersect_triangle3_inc (raytri.cpp:259-316): 1 and *t on a hit */
+int  orc_tri3_inc(const double *orig, const double *dir, const double *v0, const double *v1, const double *v2,
+                  double *t);
 int  orc_create_edges(int N, int F, const int *tri, const double *verts, orc_edge *edges /* cap 3F */);
 int  orc_point_tri(double thr, int P, const double *pts, const double *nrm,
                    int N, const double *verts, int F, const int *tri, int eol,

@@ -86,6 +86,8 @@ def lib():
                                C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(SolveStats),
                                C.c_int, _dp]
         L.orc_mt19937_uniform.argtypes = [C.c_uint32, C.c_int, C.c_double, C.c_double, _dp]
+        L.orc_tri3_inc.argtypes = [_dp] * 6
+        L.orc_tri3_inc.restype = C.c_int
         L.orc_create_edges.argtypes = [C.c_int, C.c_int, _ip, _dp, C.POINTER(Edge)]
         L.orc_point_tri.argtypes = [C.c_double, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, _ip,
                                     C.c_int, C.POINTER(Collision), C.c_int]
@@ -271,6 +273,13 @@ def step(x, v, X, tri, hinge, mat, grav, h, fixed_nodes, flags=0, tol=1e-8, maxi
 
 
 # ---------------------------------------------------------------- collisions
+def tri3_inc(orig, dir, v0, v1, v2):
+    """The oracle's intersect_triangle3_inc: (hit, t)."""
+    t = np.zeros(1)
+    hit = lib().orc_tri3_inc(_d(orig), _d(dir), _d(v0), _d(v1), _d(v2), t)
+    return bool(hit), float(t[0])
+
+
 def mt19937_uniform(seed, count, lo=-1.0, hi=1.0):
     out = np.zeros(count)
     lib().orc_mt19937_uniform(seed, count, lo, hi, out)

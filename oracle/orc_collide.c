@@ -238,6 +238,12 @@ static int tri3_inc(const double *orig, const double *dir, const double *v0, con
     return 1;
 }
 
+/* the oracle's ray/triangle test exported for its pin against the compiled raytri.cpp */
+int orc_tri3_inc(const double *orig, const double *dir, const double *v0, const double *v1, const double *v2, double *t)
+{
+    return tri3_inc(orig, dir, v0, v1, v2, t);
+}
+
 static int intersect_square(const double *x0, const double *dx, const double *xa, const double *xb, const double *xc, double *t)
 {
     double xd[3], xe[3], d[3];

@@ -142,3 +142,48 @@ def test_batch_multilevel():
               flush=True)
         assert err < 1e-6
         assert np.max(np.abs(xs[b] - x1)) <= 1e-9 * np.max(np.abs(x1))
+
+
+def test_batch_instances_match_the_oracle():
+    """Each instance of a batch, with its own Young's modulus and bending stiffness
+    (per-face E and per-hinge beta in the device kernels), against the oracle's
+    Forces::fill with that instance's material (Forces.cpp:912-930; Material per cloth,
+    Cloth.h:28-35): f, M and MDK to 1e-10 of the largest entry with the same
+    sparsity; and the batched solve (tol 1e-10) against the oracle's dense solve of
+    the oracle's own system, with the corners at 0.99 v_old, to 1e-6 relative (the
+    bound that a 1e-10 residual gives at these systems' condition numbers)."""
+    import oracle as O
+    bs = small_batch()
+    with eolc.Context(0) as c:
+        off = bs.setup(c)
+        rc, st = c.batch_step(GRAV, H, tol=1e-10, maxit=20000, eager=True)
+        assert rc == eolc.OK
+        x, v = c.download_state()
+        f = c.download_forces()
+        M = c.download_csr(eolc.MAT_M).tocsr()
+        A = c.download_csr(eolc.MAT_MDK).tocsr()
+    vs = bs.unpack(v)
+    for b, sc in enumerate(bs.inst):
+        lo, n3 = 3 * int(off[b]), 3 * sc.N
+        mat = O.material(0.1, float(bs.E[b]), 0.3, float(bs.beta[b]), 0.0, 1.0)
+        rco, fo, Mo, Ao = O.forces_fill(sc.x0, sc.X, sc.tri, sc.hinge, mat, GRAV, H)
+        assert rco == 0
+        errs = []
+        for G, R in ((M, Mo), (A, Ao)):
+            Gb = G[lo:lo + n3, lo:lo + n3].tocsr()
+            R = R.tocsr()
+            Gb.sort_indices(); R.sort_indices()
+            assert np.array_equal(Gb.indptr, R.indptr) and np.array_equal(Gb.indices, R.indices)
+            errs.append(np.max(np.abs(Gb.data - R.data)) / np.max(np.abs(R.data)))
+        errs.append(np.max(np.abs(f[lo:lo + n3] - fo)) / np.max(np.abs(fo)))
+        # outside its block the instance's rows are empty (no coupling between cloths)
+        assert A[lo:lo + n3].nnz == A[lo:lo + n3, lo:lo + n3].nnz
+        fd = (3 * np.asarray(sc.fixed)[:, None] + np.arange(3)).reshape(-1).astype(np.int32)
+        rhs = Mo @ sc.v0 + H * fo
+        rcd, vd = O.dense_velocity_solve(Ao, rhs, fd, 0.99 * sc.v0[fd])
+        assert rcd == 0
+        ev = np.max(np.abs(vs[b] - vd)) / np.max(np.abs(vd))
+        print(f"  instance {b} side {sc.n} E {bs.E[b]:.3g} beta {bs.beta[b]:.3g}: M {errs[0]:.1e} "
+              f"MDK {errs[1]:.1e} f {errs[2]:.1e}; v vs dense {ev:.1e} ({st[b].iters} iterations)", flush=True)
+        assert max(errs) <= 1e-10
+        assert ev <= 1e-6

@@ -145,3 +145,47 @@ def test_cd2_parity_config2(ctx):
     sc = scene.Scene(512, sigma_x=1e-3)
     pts, nrm, whd, E = scene.obstacles(seed=1236, P=64, B=4)
     compare(ctx, sc, sc.x0, 1e-3, pts, nrm, whd, E)
+
+
+def _material_layout(verts, tri):
+    """A material-space layout with every face CCW (the context requires positive
+    areas): the first axis-pair projection of the positions that has one."""
+    v = np.asarray(verts, dtype=np.float64)
+    for i, j in ((0, 1), (1, 0), (0, 2), (2, 0), (1, 2), (2, 1)):
+        X = v[:, [i, j]]
+        a, b, c = X[tri[:, 0]], X[tri[:, 1]], X[tri[:, 2]]
+        if np.all((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]) > 0):
+            return X
+    raise AssertionError("no CCW projection")
+
+
+@pytest.mark.parametrize("name", ["point_over_triangle", "points_nearest_and_tie", "eol_points",
+                                  "corner_under_triangle", "vertex_inside_box", "edge_through_box_edge"])
+def test_closed_form_scenes(ctx, name):
+    """The hand-worked scenes of the oracle's pins through the GPU path: records bit
+    for bit equal to the oracle's, and equal to the closed-form values."""
+    import collide_scenes as S
+    print(f"closed-form collision scene {name}", flush=True)
+    built = getattr(S, name)()
+    if len(built) == 4:
+        verts, tri, pts, nrm = built
+        whd, E = np.zeros((0, 3)), []
+    else:
+        verts, tri = built
+        pts, nrm = np.zeros((0, 3)), np.zeros((0, 3))
+        whd, E = S.BOX_WHD[None, :], [S.box_frame(S.BOX_CENTER)]
+    eol = name == "eol_points"
+    X = _material_layout(verts, tri)
+    ctx.set_topology(X, tri, np.zeros((0, 4), dtype=np.int32))
+    x = np.asarray(verts, dtype=np.float64).reshape(-1)
+    ctx.upload_state(x, np.zeros_like(x))
+    g = ctx.collide(S.THR, pts, nrm, whd, E, eol_points=eol)
+    o = oracle_cd(S.THR, pts, nrm, whd, E, x, tri, eol=eol)
+    assert len(g) == len(o) and len(g) > 0
+    for k, (a, b) in enumerate(zip(g, o)):
+        assert rec_tuple(a) == rec_tuple(b), f"record {k} differs"
+    exp_fn = getattr(S, "expected_" + name, None)
+    if exp_fn is not None:
+        v2 = S.jittered(verts, O.mt19937_uniform)
+        exp = exp_fn(v2)
+        S.compare(g[: len(exp)] if eol else g, exp)

@@ -118,3 +118,42 @@ def test_batched_256_cloths_satisfy_the_downloaded_system():
         assert rel <= 1e-8 * (1 + 1e-6), (b, rel, st[b].as_dict())
     print(f"  worst instance host FP64 residual {worst:.3e}; iterations {min(s.iters for s in st)}.."
           f"{max(s.iters for s in st)}", flush=True)
+
+
+def test_fp64_block_jacobi_512_keeps_the_tolerance_every_step(ctx):
+    """The plain method at scale (512^2, the benchmark's noise, SPD-projected bending):
+    25 consecutive steps of FP64 block-Jacobi PCG, each ending EOLC_OK only with its
+    FP64 true residual <= 1e-8 (the status contract of GeneralizedSolver.cpp:120-126's
+    intended CG solve).  The last step is re-checked on the host against the FP64
+    system downloaded from the device."""
+    n, steps = 512, 25
+    sc = scene.Scene(n, sigma_x=1e-3, sigma_v=0.05)
+    print(f"fp64 block-Jacobi {n}x{n}, {steps} steps: N={sc.N}", flush=True)
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    ctx.upload_state(sc.x0, sc.v0)
+    fd = (3 * np.asarray(sc.fixed)[:, None] + np.arange(3)).reshape(-1)
+    its, repl = [], []
+    for s in range(steps):
+        x_old, v_old = ctx.download_state() if s == steps - 1 else (None, None)
+        rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-8, maxit=200000, mode=eolc.SOLVE_FP64_BJ)
+        assert rc == eolc.OK and st.true_relres <= 1e-8, (s, st.as_dict())
+        its.append(st.iters)
+        repl.append(st.refinements)
+    print(f"  iterations {its}; residual replacements {repl}", flush=True)
+    _, v = ctx.download_state()
+    A = ctx.download_csr(eolc.MAT_MDK).tocsr()
+    M = ctx.download_csr(eolc.MAT_M).tocsr()
+    f = ctx.download_forces()
+    vals = 0.99 * v_old[fd]
+    assert np.array_equal(v[fd], vals)
+    b = M @ v_old + H * f
+    free = np.ones(3 * sc.N, bool)
+    free[fd] = False
+    vD = np.zeros(3 * sc.N)
+    vD[fd] = vals
+    rel = np.linalg.norm((b - A @ v)[free]) / np.linalg.norm((b - A @ vD)[free])
+    print(f"  last step host FP64 residual {rel:.3e} (device {st.true_relres:.3e})", flush=True)
+    assert rel <= 1e-8 * (1 + 1e-6)
+    ctx.set_options(0)
