@@ -121,15 +121,59 @@ void make_box(BoxGeom &B, const double *whd, const double *E1 /* column-major 4x
     }
 }
 
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
+// Device memory of one collide call.  Heap mode (base == nullptr): every buffer is
+// a hipMalloc, freed when the call ends or at release().  Arena mode: buffers are
+// carved in order, 256-byte aligned, from a fixed region (the scratch array of
+// eolc_collide_noalloc); release() rewinds to a mark, and no allocation is made.
+// Reuse after a rewind is safe because every kernel runs on the context's one
+// stream and every host read of a released buffer has been synchronised.
+struct Workspace {
+    char *base = nullptr;
+    size_t cap = 0, used = 0, peak = 0;
+    bool exhausted = false;
+    std::vector<void *> owned;
+    Workspace() = default;
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+    ~Workspace()
+    {
+        for (void *p : owned) (void)hipFree(p);
+    }
+    template <class T> hipError_t get(T **p, size_t n)
+    {
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        if (!base) {
+            void *q = nullptr;
+            const hipError_t e = hipMalloc(&q, bytes);
+            if (e == hipSuccess) {
+                owned.push_back(q);
+                *p = static_cast<T *>(q);
+            }
+            return e;
+        }
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off > cap || bytes > cap - off) {
+            exhausted = true;
+            return hipErrorOutOfMemory;
+        }
+        *p = reinterpret_cast<T *>(base + off);
+        used = off + bytes;
+        peak = std::max(peak, used);
+        return hipSuccess;
+    }
+    size_t mark() const { return base ? used : owned.size(); }
+    void release(size_t m)
+    {
+        if (base) {
+            used = m;
+            return;
+        }
+        while (owned.size() > m) {
+            (void)hipFree(owned.back());
+            owned.pop_back();
+        }
+    }
 };
-
-template <class T> hipError_t dmalloc(DevBuf &b, size_t n)
-{
-    return hipMalloc(&b.p, std::max<size_t>(n, 1) * sizeof(T));
-}
 
 }  // namespace
 
@@ -191,18 +235,39 @@ void build_jitter(std::vector<double> &u, int N)
 
 namespace {
 
-int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double *nrm, int B, const double *whd,
-                 const double *Eb, int eol_points, int box_npts, int box_nedges, std::vector<eolc_collision> &out)
+// Grid-hash table size for F faces: the smallest power of two >= max(2F, 64).
+unsigned grid_table_size(int F)
+{
+    unsigned tbl = 1;
+    while (tbl < (unsigned)std::max(2 * F, 64)) tbl <<= 1;
+    return tbl;
+}
+
+// Device bytes a collide call needs before the data-dependent buffers (the grid's
+// entry list and the record outputs), alignment included.
+size_t collide_fixed_bytes(const eolc_ctx *c, int P, int B)
+{
+    const size_t N = (size_t)c->topo.N, F = (size_t)c->topo.F, E = (size_t)c->n_cedges;
+    const size_t Q = std::max<size_t>((size_t)P + 8 * (size_t)B, 1), tbl = grid_table_size(c->topo.F);
+    const size_t bytes = 8 * (3 * N + 3 * F + 6 * 256 + 6) + 4 * (3 * tbl + 2 + F) + 8 * 6 * Q +
+                         (4 + sizeof(eolc_collision)) * Q + (4 + sizeof(eolc_collision) + 8) * N + 4 +
+                         8 * 6 * (size_t)P + sizeof(BoxGeom) + 12 * E;
+    return bytes + 256 * 32;
+}
+
+int collide_impl(eolc_ctx *c, Workspace &ws, double thr, int P, const double *pts, const double *nrm, int B,
+                 const double *whd, const double *Eb, int eol_points, int box_npts, int box_nedges,
+                 std::vector<eolc_collision> &out)
 {
     const int N = c->topo.N, F = c->topo.F;
-    DevBuf xj, fn, aabb_part, aabb, qpts, qnrm, qface, qrec;
-    EOLC_HIP(dmalloc<double>(xj, 3 * (size_t)N));
-    EOLC_HIP(dmalloc<double>(fn, 3 * (size_t)F));
-    EOLC_HIP(dmalloc<double>(aabb_part, 6 * 256));
-    EOLC_HIP(dmalloc<double>(aabb, 6));
-    EOLC_HIP(eolc::cd_prep(c, thr, (double *)xj.p, (double *)fn.p, (double *)aabb_part.p, (double *)aabb.p));
+    double *xj = nullptr, *fn = nullptr, *aabb_part = nullptr, *aabb = nullptr;
+    EOLC_HIP(ws.get(&xj, 3 * (size_t)N));
+    EOLC_HIP(ws.get(&fn, 3 * (size_t)F));
+    EOLC_HIP(ws.get(&aabb_part, 6 * 256));
+    EOLC_HIP(ws.get(&aabb, 6));
+    EOLC_HIP(eolc::cd_prep(c, thr, xj, fn, aabb_part, aabb));
     double hab[6];
-    EOLC_HIP(hipMemcpyAsync(hab, aabb.p, sizeof(hab), hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipMemcpyAsync(hab, aabb, sizeof(hab), hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
     // boxes
     std::vector<BoxGeom> boxes((size_t)B);
@@ -225,30 +290,26 @@ int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double
     }
     for (int k = 0; k < 3; ++k) { g.org[k] = hab[k] - g.margin; g.dim[k] = dims[k]; }
     g.inv_cell = 1.0 / cell;
-    unsigned tbl = 1;
-    while (tbl < (unsigned)std::max(2 * F, 64)) tbl <<= 1;
+    const unsigned tbl = grid_table_size(F);
     g.mask = tbl - 1;
     g.cond_max = 1e6;
-    DevBuf bcnt, bstart, cursor, total, nalways, always, entries;
-    EOLC_HIP(dmalloc<unsigned>(bcnt, tbl));
-    EOLC_HIP(dmalloc<unsigned>(bstart, tbl));
-    EOLC_HIP(dmalloc<unsigned>(cursor, tbl));
-    EOLC_HIP(dmalloc<unsigned>(total, 1));
-    EOLC_HIP(dmalloc<unsigned>(nalways, 1));
-    EOLC_HIP(dmalloc<int>(always, (size_t)F));
-    EOLC_HIP(hipMemsetAsync(bcnt.p, 0, tbl * sizeof(unsigned), c->stream));
-    EOLC_HIP(hipMemsetAsync(cursor.p, 0, tbl * sizeof(unsigned), c->stream));
-    EOLC_HIP(hipMemsetAsync(nalways.p, 0, sizeof(unsigned), c->stream));
-    EOLC_HIP(eolc::cd_grid_build(c, g, (const double *)xj.p, (unsigned *)bcnt.p, (unsigned *)bstart.p,
-                                 (unsigned *)cursor.p, nullptr, (unsigned *)nalways.p, (int *)always.p,
-                                 (unsigned *)total.p, false));
+    unsigned *bcnt = nullptr, *bstart = nullptr, *cursor = nullptr, *total = nullptr, *nalways = nullptr;
+    int *always = nullptr, *entries = nullptr;
+    EOLC_HIP(ws.get(&bcnt, tbl));
+    EOLC_HIP(ws.get(&bstart, tbl));
+    EOLC_HIP(ws.get(&cursor, tbl));
+    EOLC_HIP(ws.get(&total, 1));
+    EOLC_HIP(ws.get(&nalways, 1));
+    EOLC_HIP(ws.get(&always, (size_t)F));
+    EOLC_HIP(hipMemsetAsync(bcnt, 0, tbl * sizeof(unsigned), c->stream));
+    EOLC_HIP(hipMemsetAsync(cursor, 0, tbl * sizeof(unsigned), c->stream));
+    EOLC_HIP(hipMemsetAsync(nalways, 0, sizeof(unsigned), c->stream));
+    EOLC_HIP(eolc::cd_grid_build(c, g, xj, bcnt, bstart, cursor, nullptr, nalways, always, total, false));
     unsigned nentries = 0;
-    EOLC_HIP(hipMemcpyAsync(&nentries, total.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    EOLC_HIP(hipMemcpyAsync(&nentries, total, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     EOLC_HIP(hipStreamSynchronize(c->stream));
-    EOLC_HIP(dmalloc<int>(entries, nentries));
-    EOLC_HIP(eolc::cd_grid_build(c, g, (const double *)xj.p, (unsigned *)bcnt.p, (unsigned *)bstart.p,
-                                 (unsigned *)cursor.p, (int *)entries.p, (unsigned *)nalways.p, (int *)always.p,
-                                 (unsigned *)total.p, true));
+    EOLC_HIP(ws.get(&entries, nentries));
+    EOLC_HIP(eolc::cd_grid_build(c, g, xj, bcnt, bstart, cursor, entries, nalways, always, total, true));
     // ---- point / corner queries against the faces
     const int Q = P + 8 * B;
     std::vector<double> hq(3 * (size_t)std::max(Q, 1)), hn(3 * (size_t)std::max(Q, 1));
@@ -259,42 +320,45 @@ int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double
                 hq[3 * (P + 8 * b + i1) + k] = boxes[b].v[i1][k];
                 hn[3 * (P + 8 * b + i1) + k] = boxes[b].vn[i1][k];
             }
-    EOLC_HIP(dmalloc<double>(qpts, hq.size()));
-    EOLC_HIP(dmalloc<double>(qnrm, hn.size()));
-    EOLC_HIP(dmalloc<int>(qface, std::max(Q, 1)));
-    EOLC_HIP(dmalloc<eolc_collision>(qrec, std::max(Q, 1)));
+    double *qpts = nullptr, *qnrm = nullptr;
+    int *qface = nullptr;
+    eolc_collision *qrec = nullptr;
+    EOLC_HIP(ws.get(&qpts, hq.size()));
+    EOLC_HIP(ws.get(&qnrm, hn.size()));
+    EOLC_HIP(ws.get(&qface, std::max(Q, 1)));
+    EOLC_HIP(ws.get(&qrec, std::max(Q, 1)));
     std::vector<eolc_collision> qres((size_t)std::max(Q, 1));
     if (Q > 0) {
-        EOLC_HIP(hipMemcpyAsync(qpts.p, hq.data(), hq.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        EOLC_HIP(hipMemcpyAsync(qnrm.p, hn.data(), hn.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        EOLC_HIP(eolc::cd_query(c, g, Q, (const double *)qpts.p, (const double *)qnrm.p, thr, (const double *)aabb.p,
-                                (const double *)xj.p, (const double *)fn.p, (const unsigned *)bstart.p,
-                                (const unsigned *)bcnt.p, (const int *)entries.p, (const unsigned *)nalways.p,
-                                (const int *)always.p, (int *)qface.p, (eolc_collision *)qrec.p));
-        EOLC_HIP(hipMemcpyAsync(qres.data(), qrec.p, sizeof(eolc_collision) * Q, hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(hipMemcpyAsync(qpts, hq.data(), hq.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(hipMemcpyAsync(qnrm, hn.data(), hn.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(eolc::cd_query(c, g, Q, qpts, qnrm, thr, aabb, xj, fn, bstart, bcnt, entries, nalways, always, qface,
+                                qrec));
+        EOLC_HIP(hipMemcpyAsync(qres.data(), qrec, sizeof(eolc_collision) * Q, hipMemcpyDeviceToHost, c->stream));
     }
     // ---- per-vertex passes (EOL points and V2-T1) share flag/record buffers
-    DevBuf flag, vrec, u32, offs, vout, err;
-    EOLC_HIP(dmalloc<int>(flag, (size_t)N));
-    EOLC_HIP(dmalloc<eolc_collision>(vrec, (size_t)N));
-    EOLC_HIP(dmalloc<unsigned>(u32, (size_t)N));
-    EOLC_HIP(dmalloc<unsigned>(offs, (size_t)N));
-    EOLC_HIP(dmalloc<int>(err, 1));
-    EOLC_HIP(hipMemsetAsync(err.p, 0, sizeof(int), c->stream));
+    int *flag = nullptr, *err = nullptr;
+    eolc_collision *vrec = nullptr;
+    unsigned *u32 = nullptr, *offs = nullptr;
+    EOLC_HIP(ws.get(&flag, (size_t)N));
+    EOLC_HIP(ws.get(&vrec, (size_t)N));
+    EOLC_HIP(ws.get(&u32, (size_t)N));
+    EOLC_HIP(ws.get(&offs, (size_t)N));
+    EOLC_HIP(ws.get(&err, 1));
+    EOLC_HIP(hipMemsetAsync(err, 0, sizeof(int), c->stream));
     auto per_vertex_records = [&](std::vector<eolc_collision> &dst) -> int {
-        EOLC_HIP(eolc::cd_compact(c, N, (const int *)flag.p, (unsigned *)u32.p, (unsigned *)offs.p,
-                                  (unsigned *)total.p, nullptr, nullptr, false));
+        EOLC_HIP(eolc::cd_compact(c, N, flag, u32, offs, total, nullptr, nullptr, false));
         unsigned cnt = 0;
-        EOLC_HIP(hipMemcpyAsync(&cnt, total.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+        EOLC_HIP(hipMemcpyAsync(&cnt, total, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
         EOLC_HIP(hipStreamSynchronize(c->stream));
         dst.resize(cnt);
         if (cnt) {
-            DevBuf o;
-            EOLC_HIP(dmalloc<eolc_collision>(o, cnt));
-            EOLC_HIP(eolc::cd_compact(c, N, (const int *)flag.p, (unsigned *)u32.p, (unsigned *)offs.p,
-                                      (unsigned *)total.p, (const eolc_collision *)vrec.p, (eolc_collision *)o.p, true));
-            EOLC_HIP(hipMemcpyAsync(dst.data(), o.p, sizeof(eolc_collision) * cnt, hipMemcpyDeviceToHost, c->stream));
+            const size_t m = ws.mark();
+            eolc_collision *o = nullptr;
+            EOLC_HIP(ws.get(&o, cnt));
+            EOLC_HIP(eolc::cd_compact(c, N, flag, u32, offs, total, vrec, o, true));
+            EOLC_HIP(hipMemcpyAsync(dst.data(), o, sizeof(eolc_collision) * cnt, hipMemcpyDeviceToHost, c->stream));
             EOLC_HIP(hipStreamSynchronize(c->stream));
+            ws.release(m);
         }
         return EOLC_OK;
     };
@@ -304,14 +368,13 @@ int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double
             for (int i = 0; i < 3; ++i) v[k].pos1_[i] = v[k].pos1[i] - snap * v[k].nor1[i];
     };
     // ---- pointTriCollision
-    DevBuf dpts, dnrm;
     if (eol_points && P > 0) {
-        EOLC_HIP(dmalloc<double>(dpts, 3 * (size_t)P));
-        EOLC_HIP(dmalloc<double>(dnrm, 3 * (size_t)P));
-        EOLC_HIP(hipMemcpyAsync(dpts.p, pts, 3 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        EOLC_HIP(hipMemcpyAsync(dnrm.p, nrm, 3 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        EOLC_HIP(eolc::cd_eolpts(c, P, thr, (const double *)dpts.p, (const double *)dnrm.p, (const double *)xj.p,
-                                 (int *)flag.p, (eolc_collision *)vrec.p));
+        double *dpts = nullptr, *dnrm = nullptr;
+        EOLC_HIP(ws.get(&dpts, 3 * (size_t)P));
+        EOLC_HIP(ws.get(&dnrm, 3 * (size_t)P));
+        EOLC_HIP(hipMemcpyAsync(dpts, pts, 3 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(hipMemcpyAsync(dnrm, nrm, 3 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(eolc::cd_eolpts(c, P, thr, dpts, dnrm, xj, flag, vrec));
         std::vector<eolc_collision> ev;
         int rc = per_vertex_records(ev);
         if (rc) return rc;
@@ -327,20 +390,21 @@ int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double
     }
     snap_all(out, 0);
     // ---- boxTriCollision per box
-    DevBuf dbox, ecnt, eoff, eamb;
-    EOLC_HIP(dmalloc<BoxGeom>(dbox, 1));
+    BoxGeom *dbox = nullptr;
+    unsigned *ecnt = nullptr, *eoff = nullptr;
+    int *eamb = nullptr;
+    EOLC_HIP(ws.get(&dbox, 1));
     const int E = c->n_cedges;
-    EOLC_HIP(dmalloc<unsigned>(ecnt, (size_t)E));
-    EOLC_HIP(dmalloc<unsigned>(eoff, (size_t)E));
-    EOLC_HIP(dmalloc<int>(eamb, (size_t)E));
+    EOLC_HIP(ws.get(&ecnt, (size_t)E));
+    EOLC_HIP(ws.get(&eoff, (size_t)E));
+    EOLC_HIP(ws.get(&eamb, (size_t)E));
     std::vector<double> hx, hxj;   // host copies, only if an acos decision must be re-made
     const size_t base_after_points = out.size();
     for (int b = 0; b < B; ++b) {
         std::vector<eolc_collision> clst;
-        EOLC_HIP(hipMemcpyAsync(dbox.p, &boxes[b], sizeof(BoxGeom), hipMemcpyHostToDevice, c->stream));
+        EOLC_HIP(hipMemcpyAsync(dbox, &boxes[b], sizeof(BoxGeom), hipMemcpyHostToDevice, c->stream));
         // V2-T1
-        EOLC_HIP(eolc::cd_v2t1(c, thr, (const BoxGeom *)dbox.p, (const double *)xj.p, (const double *)fn.p,
-                               (int *)flag.p, (eolc_collision *)vrec.p, (int *)err.p));
+        EOLC_HIP(eolc::cd_v2t1(c, thr, dbox, xj, fn, flag, vrec, err));
         {
             std::vector<eolc_collision> vv;
             int rc = per_vertex_records(vv);
@@ -358,22 +422,22 @@ int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double
         }
         // E2-E1
         if (E > 0) {
-            EOLC_HIP(eolc::cd_ee(c, 0, thr, (const BoxGeom *)dbox.p, (const double *)xj.p, (unsigned *)ecnt.p, nullptr,
-                                 (int *)eamb.p, nullptr));
-            EOLC_HIP(eolc::cd_scan(c, (unsigned)E, (const unsigned *)ecnt.p, (unsigned *)eoff.p, (unsigned *)total.p));
+            EOLC_HIP(eolc::cd_ee(c, 0, thr, dbox, xj, ecnt, nullptr, eamb, nullptr));
+            EOLC_HIP(eolc::cd_scan(c, (unsigned)E, ecnt, eoff, total));
             unsigned ne = 0;
-            EOLC_HIP(hipMemcpyAsync(&ne, total.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+            EOLC_HIP(hipMemcpyAsync(&ne, total, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
             std::vector<int> amb((size_t)E);
-            EOLC_HIP(hipMemcpyAsync(amb.data(), eamb.p, sizeof(int) * E, hipMemcpyDeviceToHost, c->stream));
+            EOLC_HIP(hipMemcpyAsync(amb.data(), eamb, sizeof(int) * E, hipMemcpyDeviceToHost, c->stream));
             EOLC_HIP(hipStreamSynchronize(c->stream));
             std::vector<eolc_collision> ee(ne);
             if (ne) {
-                DevBuf o;
-                EOLC_HIP(dmalloc<eolc_collision>(o, ne));
-                EOLC_HIP(eolc::cd_ee(c, 1, thr, (const BoxGeom *)dbox.p, (const double *)xj.p, (unsigned *)ecnt.p,
-                                     (const unsigned *)eoff.p, (int *)eamb.p, (eolc_collision *)o.p));
-                EOLC_HIP(hipMemcpyAsync(ee.data(), o.p, sizeof(eolc_collision) * ne, hipMemcpyDeviceToHost, c->stream));
+                const size_t m = ws.mark();
+                eolc_collision *o = nullptr;
+                EOLC_HIP(ws.get(&o, ne));
+                EOLC_HIP(eolc::cd_ee(c, 1, thr, dbox, xj, ecnt, eoff, eamb, o));
+                EOLC_HIP(hipMemcpyAsync(ee.data(), o, sizeof(eolc_collision) * ne, hipMemcpyDeviceToHost, c->stream));
                 EOLC_HIP(hipStreamSynchronize(c->stream));
+                ws.release(m);
             }
             // re-decide edges whose angle tests were within the band, with the C library acos
             bool any = std::any_of(amb.begin(), amb.end(), [](int m) { return m != 0; });
@@ -382,7 +446,7 @@ int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double
                     hx.resize(3 * (size_t)N);
                     hxj.resize(3 * (size_t)N);
                     EOLC_HIP(hipMemcpy(hx.data(), c->x, hx.size() * sizeof(double), hipMemcpyDeviceToHost));
-                    EOLC_HIP(hipMemcpy(hxj.data(), xj.p, hxj.size() * sizeof(double), hipMemcpyDeviceToHost));
+                    EOLC_HIP(hipMemcpy(hxj.data(), xj, hxj.size() * sizeof(double), hipMemcpyDeviceToHost));
                 }
                 std::vector<eolc_collision> fixed;
                 size_t pos = 0;
@@ -442,7 +506,7 @@ int collide_impl(eolc_ctx *c, double thr, int P, const double *pts, const double
     }
     (void)base_after_points;
     int herr = 0;
-    EOLC_HIP(hipMemcpy(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost));
+    EOLC_HIP(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
     if (herr) {
         c->err = "collide: a cloth vertex with index >= face count reached the face-normal lookup of V2-T1 "
                  "(boxTriCollision.cpp:731 indexes face normals by vertex index)";
@@ -467,7 +531,8 @@ try {
     }
     EOLC_HIP(hipSetDevice(c->device));
     std::vector<eolc_collision> recs;
-    int rc = collide_impl(c, threshold, P, pts, nrm, B, whd, E, eol_points, box_npts, box_nedges, recs);
+    Workspace ws;   // heap mode: device buffers allocated for this call
+    int rc = collide_impl(c, ws, threshold, P, pts, nrm, B, whd, E, eol_points, box_npts, box_nedges, recs);
     if (rc) return rc;
     *count = (int)recs.size();
     if ((int)recs.size() > cap || (!out && !recs.empty())) {
@@ -478,4 +543,58 @@ try {
     return EOLC_OK;
 } catch (...) {
     return eolc::report_exception(c, "collide");
+}
+
+// The same query with no device allocation: every buffer of the call is carved from
+// this GPU's scratch array (eolc::kCollideScratchBytes), whose address was looked up
+// once per device.  The call holds a per-device lock for its whole duration, so calls
+// from several host threads on one GPU run one after the other; the buffers that
+// must start at zero are cleared by each call (the array keeps the last call's data).
+extern "C" int eolc_collide_noalloc(eolc_ctx *c, double threshold, int P, const double *pts, const double *nrm,
+                                    int B, const double *whd, const double *E, int eol_points, int box_npts,
+                                    int box_nedges, eolc_collision *out, int cap, int *count)
+try {
+    if (!c) return EOLC_ERR_INVALID;
+    if (!count) { c->err = "collide_noalloc: null count"; return EOLC_ERR_INVALID; }
+    *count = 0;
+    if (!c->have_topo || !c->have_state) {
+        c->err = "collide_noalloc: topology and state must be set first";
+        return EOLC_ERR_INVALID;
+    }
+    if (!(threshold > 0.0) || P < 0 || B < 0 || (P && (!pts || !nrm)) || (B && (!whd || !E)) || cap < 0) {
+        c->err = "collide_noalloc: bad arguments";
+        return EOLC_ERR_INVALID;
+    }
+    const size_t need = collide_fixed_bytes(c, P, B);
+    if (need > eolc::kCollideScratchBytes) {
+        c->err = "collide_noalloc: the query needs at least " + std::to_string(need >> 20) + " MiB of device memory (" +
+                 std::to_string(c->topo.N) + " vertices, " + std::to_string(c->topo.F) + " faces), more than the " +
+                 std::to_string(eolc::kCollideScratchBytes >> 20) + " MiB scratch; use eolc_collide";
+        return EOLC_ERR_INVALID;
+    }
+    EOLC_HIP(hipSetDevice(c->device));
+    void *base = nullptr;
+    EOLC_HIP(eolc::collide_scratch(c->device, &base));
+    std::lock_guard<std::mutex> lk(*eolc::collide_scratch_lock(c->device));
+    Workspace ws;
+    ws.base = static_cast<char *>(base);
+    ws.cap = eolc::kCollideScratchBytes;
+    std::vector<eolc_collision> recs;
+    int rc = collide_impl(c, ws, threshold, P, pts, nrm, B, whd, E, eol_points, box_npts, box_nedges, recs);
+    if (ws.exhausted) {
+        (void)hipStreamSynchronize(c->stream);
+        c->err = "collide_noalloc: the query's grid or record buffers need more device memory than the " +
+                 std::to_string(eolc::kCollideScratchBytes >> 20) + " MiB scratch; use eolc_collide";
+        return EOLC_ERR_INVALID;
+    }
+    if (rc) return rc;
+    *count = (int)recs.size();
+    if ((int)recs.size() > cap || (!out && !recs.empty())) {
+        c->err = "collide_noalloc: output capacity too small";
+        return EOLC_ERR_OVERFLOW;
+    }
+    if (!recs.empty()) std::memcpy(out, recs.data(), recs.size() * sizeof(eolc_collision));
+    return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "collide_noalloc");
 }

@@ -677,6 +677,13 @@ try {
         eolc_destroy(c);
         return EOLC_ERR_HIP;
     }
+    // the no-allocation query's scratch array: its address on this device, looked up
+    // once per device with the device current
+    void *scratch_ptr = nullptr;
+    if (eolc::collide_scratch(device, &scratch_ptr) != hipSuccess) {
+        eolc_destroy(c);
+        return EOLC_ERR_HIP;
+    }
     *out = c;
     return EOLC_OK;
 } catch (...) {

@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <exception>
 #include <new>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -400,5 +401,11 @@ hipError_t velocity_transfer(eolc_ctx *c, int n_old, const double *u_old, const 
 hipError_t batch_set_fixmask(eolc_ctx *c, const int *d_nodes, int n);
 hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant);
 void build_collision_edges(eolc_ctx *c, int F, const int *tri);
+// scratch of eolc_collide_noalloc (k_collide.hip): the device address of this GPU's
+// array (hipGetSymbolAddress once per device, with that device current), and the
+// lock that serialises the calls that use it within the process
+constexpr size_t kCollideScratchBytes = (size_t)256 << 20;
+hipError_t collide_scratch(int device, void **ptr);
+std::mutex *collide_scratch_lock(int device);
 void build_jitter(std::vector<double> &u, int N);
 }  // namespace eolc

@@ -27,6 +27,8 @@
 //   k_cd_eolpts     one thread per cloth vertex (pointTri EOL branch)
 // Variable-length outputs are compacted in reference order by count -> scan ->
 // write passes (no atomics on record positions).
+#include <mutex>
+
 #include "eolc_internal.h"
 #include "collide_common.h"
 
@@ -456,9 +458,46 @@ __global__ void k_flag_to_u32(int n, const int *__restrict__ flag, unsigned *__r
 
 inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
+// Device memory of eolc_collide_noalloc: one array per GPU, held from the loading of
+// the code object until the process ends.  Host code never passes `scratch` itself
+// to a kernel or a copy (in host code the name is a stand-in, not a device address):
+// it uses the address that hipGetSymbolAddress returns for the current device.
+__device__ __align__(256) unsigned char scratch[eolc::kCollideScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+std::mutex g_scratch_call_mu[kMaxScratchDevices];
+
 }  // namespace
 
 namespace eolc {
+
+hipError_t collide_scratch(int device, void **ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+std::mutex *collide_scratch_lock(int device)
+{
+    return (device >= 0 && device < kMaxScratchDevices) ? &g_scratch_call_mu[device] : nullptr;
+}
 
 // Launch wrappers used by collide_host.cpp --------------------------------------
 hipError_t cd_prep(eolc_ctx *c, double thr, double *xj, double *fn, double *aabb_part, double *aabb)

@@ -37,7 +37,7 @@ EXPORTED_SYMBOLS = (
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
     "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
     "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol", "eolc_velocity_transfer_eol",
-    "eolc_learned_set_multilevel",
+    "eolc_collide_noalloc", "eolc_learned_set_multilevel",
 )
 
 BATCH_EAGER = 0x1
@@ -107,6 +107,7 @@ def lib():
         L.eolc_collide.argtypes = [vp, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                    C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(CollisionRec), C.c_int, C.POINTER(C.c_int)]
+        L.eolc_collide_noalloc.argtypes = L.eolc_collide.argtypes
         L.eolc_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.eolc_set_batch_topology.argtypes = [vp, C.c_int] + [C.c_void_p] * 9
         L.eolc_set_batch_material.argtypes = [vp, C.c_void_p, C.c_void_p]
@@ -388,9 +389,10 @@ class Context:
         return ms.value
 
     def collide(self, threshold, pts=None, nrm=None, whd=None, E=None, eol_points=False,
-                box_npts=8, box_nedges=12, cap=1 << 16):
-        """CD2 (eol_points=False) or CD (True).  E: list of 4x4 row-major frames."""
-        fn = lib().eolc_collide
+                box_npts=8, box_nedges=12, cap=1 << 16, noalloc=False):
+        """CD2 (eol_points=False) or CD (True).  E: list of 4x4 row-major frames.
+        noalloc: eolc_collide_noalloc (no device allocation, the library's scratch)."""
+        fn = lib().eolc_collide_noalloc if noalloc else lib().eolc_collide
         pts = _d(pts if pts is not None else np.zeros((0, 3)))
         nrm = _d(nrm if nrm is not None else np.zeros((0, 3)))
         whd = _d(whd if whd is not None else np.zeros((0, 3)))

@@ -254,6 +254,22 @@ int eolc_collide(eolc_ctx *ctx, double threshold, int P, const double *pts, cons
                  int B, const double *whd, const double *E, int eol_points, int box_npts,
                  int box_nedges, eolc_collision *out, int cap, int *count);
 
+/* The same query with no device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (tight per-step loops, harnesses
+ * that forbid hipMalloc).  Every device buffer of the call is carved from a
+ * file-scope device array of EOLC_COLLIDE_SCRATCH_MIB MiB per GPU, held from the
+ * loading of the library until the process ends; eolc_collide allocates and frees
+ * its buffers on every call instead.  A query that needs more (about 0.6 KiB per
+ * cloth vertex; meshes above roughly 350,000 vertices) returns EOLC_ERR_INVALID
+ * with the reason, before any device work when the size alone decides it.  The
+ * array is shared by every context on a GPU: calls on one GPU from several host
+ * threads of a process are serialised by a lock; other processes have their own
+ * copy.  Same arguments, records and errors as eolc_collide. */
+#define EOLC_COLLIDE_SCRATCH_MIB 256
+int eolc_collide_noalloc(eolc_ctx *ctx, double threshold, int P, const double *pts, const double *nrm,
+                         int B, const double *whd, const double *E, int eol_points, int box_npts,
+                         int box_nedges, eolc_collision *out, int cap, int *count);
+
 /* ---- Constraints::fill, collision rows (Constraints.cpp:417-470) --------------
  * Inequality rows A_ineq v >= 0 of the QP from CD2 records, in record order:
  * (count1, count2) = (3, 1): -nor1 on the DOFs of verts2[0]; (2, 2): -nor2 *
