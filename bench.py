@@ -13,19 +13,25 @@ The headline path is the learned-preconditioned mixed-precision PCG (FP32 matrix
 and vectors, FP64 dots and true residual with reliable updates, fine blocks P_i from
 the message-passing network with weights eol-cloth_amd/eolc/weights/mpn_cloth.npy,
 plus the coarse levels of the additive multilevel preconditioner
-(EOLC_SOLVE_MULTILEVEL), block-Jacobi fallback).  FP64 3x3 block-Jacobi PCG (the
-baseline), mixed block-Jacobi PCG, and both block-Jacobi precisions with the coarse
-levels run from the same initial state, so the gain of each ingredient is stated.  Before timing, the SpMV launch
-variant of each precision is chosen by the library's autotuner on the first
-matrix.  The noisy initial state makes the exact bending Hessian indefinite, so
-every path assembles the SPD-projected hinge Hessians
+(EOLC_SOLVE_MULTILEVEL), block-Jacobi fallback).  Every other path runs from the
+same initial state as its own leg, so the gain of each ingredient is stated: FP64
+3x3 block-Jacobi PCG (the baseline of the metric), mixed block-Jacobi, the learned
+blocks without the coarse levels, and both block-Jacobi precisions with them.  A
+leg that fails is recorded with its error and the others still run; optional legs
+are skipped once --budget-s seconds of measurement have passed.  Before timing, the
+SpMV launch variant of each precision is chosen by the library's autotuner on the
+first matrix.  The noisy initial state makes the exact bending Hessian indefinite,
+so every path assembles the SPD-projected hinge Hessians
 (EOLC_OPT_SPD_PROJECT_BENDING).
 
-Prints one JSON line (rank 0) with roofline and cpu_baseline objects.  The
-roofline's `traffic` (HBM bytes per SpMV launch) is read from
-profiles/pmc_spmv_cloth1024.json, the calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE
-summary of `bench.py --profile-spmv`, when it was measured for the same kernel
-variant and size.
+Prints one JSON line (rank 0) with `roofline` (the dominant kernel, the FP32 SpMV:
+algorithmic bytes over its average launch time, HIP events on the library's
+stream), `roofline_step` (B_step = B_asm + k B_it + r B_res + 2 v(8) over the time
+of a headline step) and `cpu_baseline` (the oracle on this host's cores, a bounded
+sample of the same workload).  The roofline's `traffic` (HBM bytes per SpMV
+launch) is read from profiles/pmc_spmv_cloth1024.json, the calibrated rocprofv3
+FETCH_SIZE/WRITE_SIZE summary of `bench.py --profile-spmv`, when it was measured
+for the same kernel variant and size.
 """
 import argparse
 import json
@@ -64,24 +70,47 @@ class MissedTolerance(RuntimeError):
     pass
 
 
-def run_path(ctx, sc, mode, steps, warmup, maxit):
-    """W untimed steps, then K timed steps of one solve mode from the scene's initial
-    state.  Every step must end EOLC_OK with its FP64 true residual <= 1e-8."""
-    ctx.upload_state(sc.x0, sc.v0)
-    for _ in range(warmup):
+def step_line(name, k, st):
+    return (f"{name} step {k}: {st.iters} iterations, {st.refinements} replacements, solve {st.ms_solve:.1f} ms, "
+            f"true residual {st.true_relres:.2e}, status {st.status}")
+
+
+def run_path(ctx, sc, mode, steps, warmup, maxit, name="", window=12):
+    """W untimed steps, then K timed steps of one solve mode along the benchmark
+    trajectory: steps 0, 1, ..., window - 1 from the scene's initial state, then the
+    state is reset to the initial one and the sequence repeats.  (The sheet degenerates
+    as it relaxes from its noise: by step 17 triangles of 1e-11 m^2 make the system
+    unsolvable to 1e-8 for every solver here, FP64 block-Jacobi included, so the
+    measured window stops short of that.)  Every step must end EOLC_OK with its FP64
+    true residual <= 1e-8.  Each step's line goes to stderr after the step; the state
+    resets (one host-to-device copy) are inside the timed region."""
+    s = 0
+
+    def one(label, k):
+        nonlocal s
+        if s % window == 0:
+            ctx.upload_state(sc.x0, sc.v0)
         rc, st = ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=maxit, mode=mode, allow=(eolc.ERR_MAXIT,))
+        log(step_line(label, k, st) + f" (trajectory step {s % window})")
+        s += 1
+        return rc, st
+
+    for w in range(warmup):
+        rc, st = one(name + " warm-up", w)
         if rc != eolc.OK:
             raise MissedTolerance(f"warm-up solve failed: {st.as_dict()}")
     t0 = time.perf_counter()
     stats = []
-    for _ in range(steps):
-        rc, st = ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=maxit, mode=mode, allow=(eolc.ERR_MAXIT,))
+    for k in range(steps):
+        rc, st = one(name, k)
         stats.append(st.as_dict())
+        if rc != eolc.OK:   # a failed step leaves the state as it was: stop here
+            break
     t1 = time.perf_counter()
     bad = [s for s in stats if not (s["status"] == eolc.OK and s["true_relres"] <= TOL)]
     if bad:
         raise MissedTolerance(f"solve missed the tolerance: {bad[0]}")
-    return (t1 - t0) / steps, stats
+    return (t1 - t0) / len(stats), stats
 
 
 def leg(results, name, fn):
@@ -251,33 +280,39 @@ def run_collision(reps, with_cpu):
 
 
 WEIGHTS = os.path.join(ROOT, "eol-cloth_amd", "eolc", "weights", "mpn_cloth.npy")
+TRAINED = os.path.join(ROOT, "eol-cloth_amd", "eolc", "weights", "mpn_cloth_trained.npy")
 TRAFFIC = os.path.join(ROOT, "profiles", "pmc_spmv_cloth1024.json")
 
 
-def load_weights():
+def load_weights(path=WEIGHTS):
+    """Network weights from `path` (a float32 .npy, loaded without pickle), else the
+    initial weights, whose zero output layer gives exactly the block-Jacobi blocks."""
     from eolc import learned
-    if os.path.exists(WEIGHTS):
-        th = np.load(WEIGHTS, allow_pickle=False).astype(np.float32)
+    if os.path.exists(path):
+        th = np.load(path, allow_pickle=False).astype(np.float32)
         if th.size == eolc.LEARNED_NPARAMS:
-            return th, os.path.relpath(WEIGHTS, ROOT)
-    return learned.init_weights(7), "initial weights (zero output = block-Jacobi)"
+            return th, os.path.relpath(path, ROOT)
+    return learned.init_weights(7), "initial weights (zero output: the block-Jacobi blocks)"
 
 
 # SpMV launch variants: (entries per trip, column prefetch, non-temporal loads,
-# waves per SIMD, symmetric layout), in the library's order
-SPMV_VARIANTS = [(1, False, False, 8, 0), (1, True, False, 8, 0), (1, True, True, 8, 0),
-                 (1, False, True, 8, 0), (1, True, True, 6, 0), (2, True, True, 6, 0),
-                 (2, True, False, 6, 0), (2, True, True, 4, 0), (1, True, True, 4, 0),
-                 (1, False, False, 8, 1), (1, False, False, 6, 1), (1, False, True, 8, 1),
-                 (1, False, False, 8, 2), (1, False, True, 8, 2), (1, False, False, 6, 2),
-                 (1, False, False, 8, 3), (1, False, True, 8, 3), (1, False, True, 6, 3)]
+# waves per SIMD, block layout, LDS-staged gathers), in the library's order
+SPMV_VARIANTS = [(1, False, False, 8, 0, False), (1, True, False, 8, 0, False), (1, True, True, 8, 0, False),
+                 (1, False, True, 8, 0, False), (1, True, True, 6, 0, False), (2, True, True, 6, 0, False),
+                 (2, True, False, 6, 0, False), (2, True, True, 4, 0, False), (1, True, True, 4, 0, False),
+                 (1, False, False, 8, 1, False), (1, False, False, 6, 1, False), (1, False, True, 8, 1, False),
+                 (1, False, False, 8, 2, False), (1, False, True, 8, 2, False), (1, False, False, 6, 2, False),
+                 (1, False, False, 8, 3, False), (1, False, True, 8, 3, False), (1, False, True, 6, 3, False),
+                 (1, False, False, 8, 4, False), (1, False, True, 8, 0, True), (1, False, True, 4, 0, True),
+                 (1, False, False, 8, 1, True), (1, False, False, 4, 4, True), (1, False, True, 8, 3, True),
+                 (1, False, True, 4, 3, True)]
 
 
 def spmv_kernel_name(fp32, variant):
-    u, pf, nt, w, lay = SPMV_VARIANTS[variant]
+    u, pf, nt, w, lay, stg = SPMV_VARIANTS[variant]
     b = lambda x: "true" if x else "false"
-    t = "float, true" if fp32 else "double, false"
-    return f"k_pcg_spmv<{t}, {u}, {b(pf)}, {b(nt)}, {w}, {lay if (fp32 or lay == 3) else 0}>"
+    t = "float, true" if fp32 else ("double, true" if lay == 4 else "double, false")
+    return f"k_pcg_spmv<{t}, {u}, {b(pf)}, {b(nt)}, {w}, {lay if (fp32 or lay >= 3) else 0}, {b(stg)}>"
 
 
 def measured_traffic(kernel, n):
@@ -289,8 +324,11 @@ def measured_traffic(kernel, n):
             t = json.load(f)
     except (OSError, ValueError):
         return None
+    # summaries made before the staged variants name the kernels without the last
+    # template argument (false)
+    short = kernel[: -len(", false>")] + ">" if kernel.endswith(", false>") else None
     for k in t.get("kernels", []):
-        if k.get("kernel_template") == kernel:
+        if k.get("kernel_template") in (kernel, short):
             return k.get("hbm_bytes")
     return None
 
@@ -396,7 +434,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1024, help="quads per side")
-    ap.add_argument("--maxit", type=int, default=200000)
+    ap.add_argument("--maxit", type=int, default=60000)
+    ap.add_argument("--window", type=int, default=12,
+                    help="trajectory steps before the state is reset to the initial one")
+    ap.add_argument("--aux-steps", type=int, default=3,
+                    help="steps of the paths without the coarse levels (from the initial state, no warm-up)")
     ap.add_argument("--budget-s", type=float, default=420.0,
                     help="after this many seconds of measurement, the remaining optional legs are skipped")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 block-Jacobi comparison")
@@ -410,6 +452,9 @@ def main():
                     help="headline without the multilevel coarse levels (and skip the multilevel lines)")
     ap.add_argument("--profile-solve", action="store_true",
                     help="kernel-trace runs: one step of each solve mode capped at 400 PCG iterations, then exit")
+    ap.add_argument("--profile-fill", action="store_true",
+                    help="counter runs: 10 mixed-mode element evaluations + assemblies (the step's fill) and "
+                         "a 1 GiB device copy for calibration, then exit")
     ap.add_argument("--profile-spmv", action="store_true",
                     help="counter runs: assemble once, 10 launches of each SpMV kernel and a 1 GiB "
                          "device copy for calibration, then exit")
@@ -454,11 +499,23 @@ def main():
     log(f"multilevel hierarchy: aggregates per level {ml_counts} ({ml_build_s:.2f} s)")
 
     if args.profile_solve:
+        # one step of each mode, PCG capped at 400 iterations
+        ML = eolc.SOLVE_MULTILEVEL
         for mode, name in ((eolc.SOLVE_MIXED_BJ, "mixed-bj"), (eolc.SOLVE_FP64_BJ, "fp64-bj"),
-                           (eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL, "mixed-bj-ml")):
+                           (eolc.SOLVE_MIXED_BJ | ML, "mixed-bj-ml"), (eolc.SOLVE_FP64_BJ | ML, "fp64-bj-ml")):
             ctx.upload_state(sc.x0, sc.v0)
             rc, st = ctx.step(GRAV, H, sc.fixed, tol=TOL, maxit=400, mode=mode, allow=(eolc.ERR_MAXIT,))
             log(f"{name}: {st.iters} iterations, solve {st.ms_solve:.1f} ms, assembly {st.ms_assembly:.1f} ms")
+        ctx.close()
+        return
+
+    if args.profile_fill:
+        # the step's fill as eolc_step runs it (face and hinge evaluation, assembly of
+        # the FP64 matrix and the SpMV's FP32 layout, rhs), 10 times, then the copy
+        for _ in range(10):
+            ctx.forces_fill(GRAV, H)
+        ms = ctx.stream_copy(1 << 30, 3)
+        log(f"calibration copy: 1 GiB read + 1 GiB written, {ms:.3f} ms per launch")
         ctx.close()
         return
 
@@ -488,10 +545,13 @@ def main():
             return True
         return False
 
-    def path_leg(mode, mixed):
+    def path_leg(mode, mixed, name, steps=None, warmup=None):
         def fn():
-            tt, st = run_path(ctx, sc, mode, args.steps, args.warmup, args.maxit)
-            return {"steps_per_s": 1 / tt, "ms_per_step": tt * 1e3, "iters": [s["iters"] for s in st],
+            k_steps = args.steps if steps is None else steps
+            w_steps = args.warmup if warmup is None else warmup
+            tt, st = run_path(ctx, sc, mode, k_steps, w_steps, args.maxit, name, args.window)
+            return {"steps_per_s": 1 / tt, "ms_per_step": tt * 1e3, "steps": k_steps, "warmup": w_steps,
+                    "iters": [s["iters"] for s in st],
                     "refinements": [s["refinements"] for s in st],
                     "launched": [s["launched"] for s in st], "host_syncs": [s["host_syncs"] for s in st],
                     "used_fallback": [s["used_fallback"] for s in st],
@@ -510,19 +570,28 @@ def main():
         bj_name, bj_mode = "mixed_bj_ml", eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL
     # The block-Jacobi run of the headline's preconditioner family goes first: the
     # learned solve falls back to block-Jacobi after twice its iteration count.
-    leg(legs, bj_name, path_leg(bj_mode, True))
-    head = leg(legs, "headline", path_leg(head_mode, True))
+    leg(legs, bj_name, path_leg(bj_mode, True, bj_name))
+    head = leg(legs, "headline", path_leg(head_mode, True, head_name))
     if head["status"] == "ok":
         log(f"{head_name}: {head['steps_per_s']:.4f} steps/s, iters {head['iters']}, "
             f"fallback {head['used_fallback']}, host syncs {head['host_syncs']}")
     spmv32_ms = ctx.time_spmv(32, 20)
-    # the baseline of the metric, then the other ingredients
-    others = [("fp64_bj", eolc.SOLVE_FP64_BJ, False, args.no_fp64), ("mixed_bj", eolc.SOLVE_MIXED_BJ, True, args.no_ml),
+    # the network trained on batched-path systems (its own leg: what training gains)
+    if os.path.exists(TRAINED) and not over_budget("trained_learned"):
+        th_t, src_t = load_weights(TRAINED)
+        ctx.learned_set_weights(th_t)
+        r = leg(legs, "trained_learned", path_leg(head_mode, True, "trained_learned"))
+        r["weights"] = src_t
+        ctx.learned_set_weights(theta)
+        if r["status"] == "ok":
+            log(f"trained_learned: {r['steps_per_s']:.4f} steps/s, iters {r['iters']}")
+    # the baseline of the metric, then the coarse levels in FP64
+    others = [("fp64_bj", eolc.SOLVE_FP64_BJ, False, args.no_fp64),
               ("fp64_bj_ml", eolc.SOLVE_FP64_BJ | eolc.SOLVE_MULTILEVEL, False, args.no_ml or args.no_fp64)]
     for name, mode, mixed, skip in others:
         if skip or over_budget(name):
             continue
-        r = leg(legs, name, path_leg(mode, mixed))
+        r = leg(legs, name, path_leg(mode, mixed, name))
         if r["status"] == "ok":
             log(f"{name}: {r['steps_per_s']:.4f} steps/s, iters {r['iters']}, replacements {r['refinements']}")
     spmv64_ms = ctx.time_spmv(64, 20)
@@ -559,6 +628,35 @@ def main():
         leg(extra, "batched", lambda: run_batched(args.steps, args.warmup, args.maxit, not args.no_ml))
     if not args.no_transfer and not over_budget("velocity_transfer"):
         leg(extra, "velocity_transfer", lambda: run_transfer(sc))
+    # the ingredients without the coarse levels, over the first --aux-steps steps of
+    # the trajectory (no warm-up; mixed_bj first: it sets the learned solve's fallback cap)
+    if not args.no_ml and args.aux_steps > 0:
+        ctx = eolc.Context(0)
+        ctx.set_topology(sc.X, sc.tri, sc.hinge)
+        ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+        ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+        ctx.upload_state(sc.x0, sc.v0)
+        ctx.forces_fill(GRAV, H)
+        ctx.set_spmv_variant(32, tuned[32]["variant"])
+        ctx.set_spmv_variant(64, tuned[64]["variant"])
+        ctx.learned_set_weights(theta)
+        aux = {}
+        for name, mode in (("mixed_bj", eolc.SOLVE_MIXED_BJ), ("mixed_learned", eolc.SOLVE_MIXED_LEARNED),
+                           ("fp64_bj_first_steps", eolc.SOLVE_FP64_BJ),
+                           ("mixed_learned_ml_first_steps", head_mode)):
+            if over_budget(name):
+                continue
+            r = leg(aux, name, path_leg(mode, name != "fp64_bj_first_steps", name, args.aux_steps, 0))
+            if r["status"] == "ok":
+                log(f"{name}: {r['steps_per_s']:.4f} steps/s over the first {args.aux_steps} steps, "
+                    f"iters {r['iters']}")
+        if aux.get("fp64_bj_first_steps", {}).get("status") == "ok":
+            base = aux["fp64_bj_first_steps"]["ms_per_step"]
+            for r in aux.values():
+                if r.get("status") == "ok":
+                    r["speedup_vs_fp64_bj"] = base / r["ms_per_step"]
+        extra["first_steps"] = aux
+        ctx.close()
     ok = head["status"] == "ok"
     res = {
         "metric": "steps/s, 1024^2 cloth at 1e-8 rel. residual: learned-PCG vs block-Jacobi PCG",
@@ -567,7 +665,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32-matrix/f64-residual",
         "data": "synthetic",
         "config": {"workload": f"cloth {args.n}x{args.n} quads, noise 1e-3 m, one step = "
-                               f"assemble + solve to 1e-8 + update", "path": head_name,
+                               f"assemble + solve to 1e-8 + update; trajectory steps 0..{args.window - 1} "
+                               f"from the initial state, repeated", "path": head_name,
                    "vertices": sc.N, "triangles": sc.F, "hinges": sc.H, "blocks": nb,
                    "bending_hessian": "spd-projected", "weights": wsrc,
                    "multilevel_aggregates": None if args.no_ml else ml_counts},

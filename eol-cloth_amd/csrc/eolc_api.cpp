@@ -76,6 +76,8 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->x); dfree(c->v); dfree(c->f); dfree(c->rhs);
     dfree(c->A64); dfree(c->S32); dfree(c->M32); dfree(c->entry_is_diag); dfree(c->Mval); dfree(c->D64); dfree(c->D32);
     dfree(c->supper); dfree(c->uoff); dfree(c->loff); dfree(c->lrec); dfree(c->e2u); dfree(c->S32u); dfree(c->S32a);
+    dfree(c->wwin);
+    dfree(c->S64u); dfree(c->M64);
     dfree(c->fixmask); dfree(c->vfix); dfree(c->fix_dofs); dfree(c->fix_vals);
     dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
     dfree(c->p64[0]); dfree(c->p64[1]);
@@ -111,6 +113,7 @@ void free_topology_buffers(eolc_ctx *c)
     ml.counts.clear();
     ml.maps.clear();
     c->bj_iters_hint = c->bj_iters_hint_ml = 0;
+    c->fp64_sticky = 0;
     c->alloc_bytes = 0;
 }
 
@@ -418,6 +421,30 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
     }
     t.nup = t.uoff[t.nslices];
     t.nlo = t.loff[t.nslices];
+    // vertex window of every PCG workgroup (16 slices each): its rows and the columns
+    // of their entries (a row's padding entries point at the row itself; the lanes of
+    // the last slice beyond N hold no row)
+    {
+        const int G = eolc::pcg_grid(N), spw = eolc::kPcgWG / C;
+        t.wwin.assign(2 * (size_t)G, 0);
+        for (int lw = 0; lw < G; ++lw) {
+            int lo = INT32_MAX, hi = -1;
+            for (int sl = lw * spw; sl < std::min(t.nslices, (lw + 1) * spw); ++sl) {
+                const int rows = std::min(N - sl * C, C), width = (t.slice_off[sl + 1] - t.slice_off[sl]) / C;
+                for (int k = 0; k < width; ++k)
+                    for (int l = 0; l < rows; ++l) {
+                        const int j = t.scol[t.slice_off[sl] + k * C + l];
+                        lo = std::min(lo, j);
+                        hi = std::max(hi, j);
+                    }
+            }
+            if (hi < 0) lo = hi = 0;
+            else ++hi;
+            t.wwin[2 * (size_t)lw] = lo;
+            t.wwin[2 * (size_t)lw + 1] = hi;
+            t.wmax = std::max(t.wmax, hi - lo);
+        }
+    }
     t.e2u.assign(t.nbs, -1);
     t.ucol.assign(t.nup, 0);
     t.lrec.assign(2 * (size_t)t.nlo, 0);
@@ -741,6 +768,10 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     free_topology_buffers(c);
     c->have_topo = c->have_state = c->have_forces = false;
     c->topo = std::move(t);
+    // an LDS-staged SpMV variant chosen for an earlier mesh: its unstaged twin when
+    // this mesh's windows are too wide
+    for (int &v : c->spmv_variant)
+        if (eolc::spmv_staged(v) && c->topo.wmax > eolc::kStageVerts) v = eolc::spmv_unstaged(v);
     const Topology &T = c->topo;
     std::vector<double> Xv(X, X + 2 * (size_t)N);
     std::vector<int> triv(tri, tri + 3 * (size_t)F), hv(hinge, hinge + 4 * (size_t)H);
@@ -757,6 +788,7 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(upload(c, &c->loff, T.loff));
     EOLC_HIP(upload(c, &c->lrec, T.lrec));
     EOLC_HIP(upload(c, &c->e2u, T.e2u));
+    EOLC_HIP(upload(c, &c->wwin, T.wwin));
     EOLC_HIP(upload(c, &c->cptr, T.cptr));
     EOLC_HIP(upload(c, &c->clist, T.clist));
     EOLC_HIP(upload(c, &c->fptr, T.fptr));
@@ -775,6 +807,8 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(dalloc(c, &c->M32, (size_t)N));
     EOLC_HIP(dalloc(c, &c->S32u, 10 * (size_t)std::max<int64_t>(T.nup, 1)));
     EOLC_HIP(dalloc(c, &c->S32a, 10 * (size_t)T.nbs));
+    EOLC_HIP(dalloc(c, &c->S64u, 10 * (size_t)std::max<int64_t>(T.nup, 1)));
+    EOLC_HIP(dalloc(c, &c->M64, (size_t)N));
     {
         std::vector<int> isd((size_t)T.nbs, 0);
         for (int i = 0; i < N; ++i) isd[T.sdiag[i]] = 1;
@@ -848,6 +882,7 @@ try {
     EOLC_HIP(hipStreamSynchronize(c->stream));
     c->have_state = true;
     c->have_forces = false;
+    c->fp64_sticky = 0;   // a new state: mixed solves try FP32 again
     return EOLC_OK;
 } catch (...) {
     return eolc::report_exception(c, "upload_state");
@@ -872,7 +907,8 @@ static int forces_fill_async(eolc_ctx *c, const double grav[3], double h, bool w
     EOLC_HIP(hipMemsetAsync(c->counter + 1, 0, 2 * sizeof(unsigned), c->stream));
     EOLC_HIP(eolc::launch_face_eval(c, h, grav));
     EOLC_HIP(eolc::launch_hinge_eval(c, h));
-    EOLC_HIP(eolc::launch_assemble(c, want32 ? eolc::fp32_layout(c->spmv_variant[0]) : 0));
+    EOLC_HIP(eolc::launch_assemble(c, (want32 ? eolc::fp32_layout(c->spmv_variant[0]) : 0) |
+                                          eolc::fp64_layout(c->spmv_variant[1])));
     c->h_last = h;
     c->have_forces = true;
     return EOLC_OK;
@@ -1150,7 +1186,16 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     }
     MlScope mls(c, multilevel);
     int &bj_hint = multilevel ? c->bj_iters_hint_ml : c->bj_iters_hint;
-    const bool mixed = (mode == EOLC_SOLVE_MIXED_BJ || mode == EOLC_SOLVE_MIXED_LEARNED);
+    bool mixed = (mode == EOLC_SOLVE_MIXED_BJ || mode == EOLC_SOLVE_MIXED_LEARNED);
+    int used_fallback = 0;
+    // After an FP32 stall (below) the next kStickyFp64 mixed solves on this context
+    // run in FP64 from the start, then FP32 is tried again.
+    if (mixed && c->fp64_sticky > 0) {
+        --c->fp64_sticky;
+        mixed = false;
+        mode = EOLC_SOLVE_FP64_BJ;
+        used_fallback = 2;
+    }
     if (mode == EOLC_SOLVE_MIXED_LEARNED) {
         if (!c->lp.have_weights) return fail(c, EOLC_ERR_INVALID, "velocity_solve: learned mode without weights");
         const int rc = learned_buffers(c);
@@ -1163,6 +1208,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         if (rc) return rc;
     }
     const int precond = (mode == EOLC_SOLVE_FP64_JACOBI) ? 1 : 0;
+    EOLC_HIP(eolc::ensure_fp64_layout(c, c->spmv_variant[1]));   // FP64 iterations, or a stall's continuation
     if (mixed) EOLC_HIP(eolc::ensure_fp32_layout(c, c->spmv_variant[0]));
     EOLC_HIP(eolc::pcg_setup(c, mixed, precond));
     // Learned mode: P replaces the FP32 block-Jacobi blocks; after twice the
@@ -1170,7 +1216,6 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     // when there was none) the solve falls back to block-Jacobi from its current
     // iterate.
     bool learned = (mode == EOLC_SOLVE_MIXED_LEARNED);
-    int used_fallback = 0;
     int cap = maxit;
     if (learned) {
         EOLC_HIP(eolc::learned_precond(c));
@@ -1190,6 +1235,30 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     int k_cycle = 0;
     double rr_cycle = -1.0;
     double relres = NAN;
+    // FP32 stall detection: an inner cycle far longer than the completed ones, or the
+    // true residual hardly falling over consecutive reliable updates, means the FP32
+    // operator no longer resolves the system (extreme stiffness ratios after the sheet
+    // deforms); the solve then continues in FP64 from its current iterate.
+    double last_true2 = -1.0;
+    int slow_updates = 0;
+    auto fp64_continue = [&]() -> int {
+        EOLC_HIP(eolc::pcg_reliable_update(c));   // x64 += the FP32 correction; r = b - A x64
+        mixed = false;
+        learned = false;
+        used_fallback |= 2;
+        c->fp64_sticky = c->stall_sticky;
+        const size_t n3 = 3 * (size_t)c->topo.N;
+        EOLC_HIP(hipMemsetAsync(c->p64[0], 0, n3 * sizeof(double), c->stream));
+        EOLC_HIP(hipMemsetAsync(c->p64[1], 0, n3 * sizeof(double), c->stream));
+        EOLC_HIP(eolc::pcg_setup(c, false, 0));
+        EOLC_HIP(eolc::scal_delta(c, 0.0));
+        EOLC_HIP(eolc::pcg_restart_inner(c, false, k));
+        EOLC_HIP(eolc::scal_restart(c, k));
+        rr_cycle = -1.0;
+        k_cycle = k;
+        chunk = 16;
+        return EOLC_OK;
+    };
     for (;;) {
         if (learned && k >= cap) {
             // fallback: block-Jacobi blocks, true residual of the current iterate, beta = 0
@@ -1222,7 +1291,14 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
             }
             chunk = next;
         }
-        if (!s.done) continue;
+        if (!s.done) {
+            const int stall = refinements > 0 ? std::max(c->stall_min, 10 * (k_cycle / refinements)) : c->stall_first;
+            if (mixed && k - k_cycle > stall) {
+                const int rc = fp64_continue();
+                if (rc) return rc;
+            }
+            continue;
+        }
         rr_cycle = -1.0;   // the residual is replaced below: a new cycle starts at k
         k_cycle = k;
         chunk = 16;
@@ -1248,6 +1324,17 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         if (!std::isfinite(s2.rmax2)) { status = EOLC_ERR_NONFINITE; break; }
         if (s2.rmax2 <= tol * tol * s2.bnorm2) { status = EOLC_OK; break; }
         if (k >= maxit || refinements > eolc::kMaxRefinements) { status = EOLC_ERR_MAXIT; break; }
+        if (mixed) {
+            // a cycle aims at a true-residual drop of delta; less than half that, thrice
+            const double slow = std::min(0.64, std::max(0.16, 4.0 * delta * delta));
+            slow_updates = (last_true2 > 0.0 && s2.rmax2 > slow * last_true2) ? slow_updates + 1 : 0;
+            last_true2 = s2.rmax2;
+            if (slow_updates >= 3) {
+                const int rc = fp64_continue();
+                if (rc) return rc;
+                continue;
+            }
+        }
         if (learned && k >= cap) continue;   // the fallback restarts from here
         EOLC_HIP(eolc::pcg_restart_inner(c, mixed, k));
         EOLC_HIP(eolc::scal_set(c, 0, 0, 0));
@@ -2021,6 +2108,20 @@ try {
     return eolc::report_exception(c, "time_spmv");
 }
 
+int eolc_set_mixed_stall(eolc_ctx *c, int first_cycle, int min_cycle, int sticky)
+try {
+    if (!c) return EOLC_ERR_INVALID;
+    if (first_cycle <= 0 || min_cycle <= 0 || sticky < 0)
+        return fail(c, EOLC_ERR_INVALID, "set_mixed_stall: need positive cycle limits and sticky >= 0");
+    c->stall_first = first_cycle;
+    c->stall_min = min_cycle;
+    c->stall_sticky = sticky;
+    c->fp64_sticky = std::min(c->fp64_sticky, sticky);
+    return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_mixed_stall");
+}
+
 int eolc_set_mixed_delta(eolc_ctx *c, double delta)
 try {
     if (!c) return EOLC_ERR_INVALID;
@@ -2069,6 +2170,10 @@ try {
     int arg = 0;
     double tbest = 0.0;
     for (int v = 0; v < eolc::kSpmvVariants; ++v) {
+        if (eolc::spmv_staged(v) && c->topo.wmax > eolc::kStageVerts) {
+            if (ms) ms[v] = INFINITY;   // not applicable to this mesh's vertex order
+            continue;
+        }
         float t = 0.f;
         EOLC_HIP(eolc::time_spmv(c, precision == 32, v, reps, &t));
         const double avg = (double)t / reps;
@@ -2090,6 +2195,9 @@ try {
     if (!c) return EOLC_ERR_INVALID;
     if ((precision != 32 && precision != 64) || variant < 0 || variant >= eolc::kSpmvVariants)
         return fail(c, EOLC_ERR_INVALID, "set_spmv_variant: bad precision or variant");
+    if (eolc::spmv_staged(variant) && (!c->have_topo || c->topo.wmax > eolc::kStageVerts))
+        return fail(c, EOLC_ERR_INVALID, "set_spmv_variant: the LDS-staged variants need every workgroup's vertex "
+                                         "window within " + std::to_string(eolc::kStageVerts) + " vertices");
     c->spmv_variant[precision == 32 ? 0 : 1] = variant;
     return EOLC_OK;
 } catch (...) {

@@ -74,6 +74,11 @@ struct Topology {
     std::vector<char> has_tri;  // per CSR block: at least one triangle contribution (M pattern)
     std::vector<int> fptr;   // N + 1
     std::vector<int> flist;  // force-vector slot (3F tri slots, then 4H hinge slots)
+    // per PCG workgroup (logical index, pcg_grid(N) of them): the vertex range
+    // [wwin[2 lw], wwin[2 lw + 1]) that holds its rows and every column they reference;
+    // wmax = the widest range (the LDS-staged SpMV variants need wmax <= kStageVerts)
+    std::vector<int> wwin;
+    int wmax = 0;
 };
 
 // Preconditioner blocks are symmetric by construction (block-Jacobi inverse by the
@@ -112,6 +117,12 @@ constexpr int kBatchVariants = 3;   // persistent-solve workgroup sizes 256, 512
 // keeps reporting convergence of the inner iteration without progress) ends the
 // solve with EOLC_ERR_MAXIT instead of looping.
 constexpr int kMaxRefinements = 1000;
+// Mixed-precision stall: a reliable-update cycle longer than max(stall_min, 10x the mean
+// completed cycle) (stall_first for the first cycle), or three reliable updates in a row
+// that each reduce the true residual by less than half the cycle's target (a factor
+// delta), end the FP32 iteration; the solve continues in FP64, and the next
+// stall_sticky mixed solves on the context start in FP64 (defaults 10000, 1000, 8;
+// eolc_set_mixed_stall).
 
 // EOL nodes (Forces.cpp:177-329, 399-497, 580-683, 746-884): node nodes[k] gets two
 // material-space DOFs at 3N + 2k.  The maps below are built by eolc_set_eol on the
@@ -269,9 +280,12 @@ struct eolc_ctx {
     float *M32 = nullptr;       // N lumped row masses sum_j m_ij (FP32, written by k_rhs)
     // symmetric FP32 layout: upper records {column bits, S'_ij (9)} and lower records
     int *supper = nullptr, *uoff = nullptr, *loff = nullptr, *lrec = nullptr, *e2u = nullptr;
+    int *wwin = nullptr;        // 2 per PCG workgroup: its vertex window (Topology::wwin)
     float *S32u = nullptr;      // 10 x nup
     float *S32a = nullptr;      // 10 x nbs: {column bits, S'_ij} per SELL entry (record layout)
-    int s32_valid = 0;          // FP32 layouts (kLay* bits) holding the last assembly's values
+    int s32_valid = 0;          // layouts (kLay* bits) holding the last assembly's values
+    double *S64u = nullptr;     // 10 x nup: the symmetric records in FP64 (FP64 SpMV layout 4)
+    double *M64 = nullptr;      // N row masses sum_j m_ij in FP64 (written by k_rhs)
     int *entry_is_diag = nullptr;  // nbs
     double *Mval = nullptr;     // nbs mass scalars (M block = m I3)
     double *D64 = nullptr;      // 6 x N  block-Jacobi inverse (symmetric, SoA)
@@ -313,6 +327,10 @@ struct eolc_ctx {
     eolc::BatchState batch;
     int bj_iters_hint = 0;      // iterations of the last converged block-Jacobi solve
     int bj_iters_hint_ml = 0;   // the same with the multilevel preconditioner
+    int fp64_sticky = 0;        // mixed solves left that start in FP64 after an FP32 stall
+    int stall_first = 10000;    // FP32 stall limits (see kStallMinCycle) and the sticky count
+    int stall_min = 1000;
+    int stall_sticky = 8;
     eolc::MLState ml;           // aggregation hierarchy (built on first use per topology)
     bool ml_on = false;         // the current solve applies the multilevel levels
 };
@@ -351,9 +369,12 @@ hipError_t launch_hinge_eval(eolc_ctx *c, double h);
 // the batched solve), symmetric upper records S32u (layout 1), per-entry records
 // S32a (layout 2)
 constexpr int kLaySoA = 1, kLaySym = 2, kLayRecord = 4;
+constexpr int kLaySym64 = 8;                            // symmetric records in FP64 (S64u, FP64 layout 4)
 int fp32_layout(int variant);                          // the layout bit a FP32 SpMV variant reads
-hipError_t launch_assemble(eolc_ctx *c, int lay);      // FP64 values + the FP32 layouts in lay
+int fp64_layout(int variant);                          // the extra layout bit a FP64 variant reads (or 0)
+hipError_t launch_assemble(eolc_ctx *c, int lay);      // FP64 values + the layouts in lay
 hipError_t ensure_fp32_layout(eolc_ctx *c, int variant);
+hipError_t ensure_fp64_layout(eolc_ctx *c, int variant);
 hipError_t launch_rhs(eolc_ctx *c, double h);
 hipError_t launch_update_x(eolc_ctx *c, double h);
 
@@ -368,12 +389,18 @@ hipError_t pcg_replace_residual64(eolc_ctx *c);         // FP64 mode: r = b - A 
 hipError_t pcg_finish(eolc_ctx *c, double *out);        // solution with fixed values -> out
 hipError_t spmv_true_residual(eolc_ctx *c, const double *v);   // ||rhs - A v||^2 -> true_rr
 hipError_t scal_set(eolc_ctx *c, double tol2, double delta2, int reset);
+hipError_t scal_delta(eolc_ctx *c, double delta2);      // reliable-update trigger only
 hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals, int set);
 // d_vals[t] = (1 - 0.01) v[d_dofs[t]] + d_vals[t] on the context's stream.
 hipError_t fixed_from_vold(eolc_ctx *c, int n, const int *d_dofs, double *d_vals);
 // EOL rows of the last forces_fill (k_eol.hip); after launch_rhs.
 hipError_t eol_fill(eolc_ctx *c);
-constexpr int kSpmvVariants = 18;
+constexpr int kSpmvVariants = 25;
+// LDS-staged SpMV variants: a workgroup's vertex window holds at most this many
+// vertices (3 x 5440 values of p: 65,280 B in FP32, 130,560 B in FP64)
+constexpr int kStageVerts = 5440;
+bool spmv_staged(int variant);   // the variant gathers p from LDS
+int spmv_unstaged(int variant);  // a variant without staging that reads the same layout
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms);
 hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms, int read_only);
 // learned preconditioner (k_learned.hip); buffers allocated by the API layer

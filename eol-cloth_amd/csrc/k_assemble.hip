@@ -30,7 +30,7 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
            const int *__restrict__ entry_is_diag, const double *__restrict__ stageK, const double *__restrict__ stageM,
            double *__restrict__ A64, int lay, float *__restrict__ S32, const int *__restrict__ e2u,
            const int *__restrict__ scol, float *__restrict__ S32u, float *__restrict__ S32a,
-           double *__restrict__ Mval)
+           double *__restrict__ S64u, double *__restrict__ Mval)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbs) return;
@@ -79,6 +79,13 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
 #pragma unroll
             for (int c = 0; c < 9; ++c) rec[1 + c] = sv[c];
         }
+        const int u64 = (lay & eolc::kLaySym64) ? e2u[b] : -1;
+        if (u64 >= 0) {   // the same record in FP64 (the column's bits in the first slot)
+            double *rec = S64u + (size_t)10 * u64;
+            rec[0] = __longlong_as_double((long long)scol[b]);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) rec[1 + c] = diag ? 0.0 : stf[c] + (c % 4 == 0 ? m : 0.0);
+        }
     }
     Mval[b] = m;
 }
@@ -89,7 +96,7 @@ k_rhs(int N, int ntriforce, int include_bending, double h, const int *__restrict
       const int *__restrict__ flist, const double *__restrict__ stageF, const int *__restrict__ slice_off,
       const int *__restrict__ srow, const int *__restrict__ slen, const int *__restrict__ scol,
       const double *__restrict__ Mval, const double *__restrict__ v, double *__restrict__ f, double *__restrict__ rhs,
-      float *__restrict__ Mrow)
+      float *__restrict__ Mrow, double *__restrict__ Mrow64)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= N) return;
@@ -113,6 +120,7 @@ k_rhs(int N, int ntriforce, int include_bending, double h, const int *__restrict
         msum += m;
     }
     if (Mrow) Mrow[i] = (float)msum;
+    if (Mrow64) Mrow64[i] = msum;
     rhs[3 * i] = y0 + h * f0;
     rhs[3 * i + 1] = y1 + h * f1;
     rhs[3 * i + 2] = y2 + h * f2;
@@ -131,18 +139,17 @@ hipError_t launch_assemble(eolc_ctx *c, int lay)
     if (nb == 0) return hipSuccess;
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
                        c->entry_is_diag, c->stageK, c->stageM, c->A64, lay, c->S32, c->e2u, c->scol, c->S32u,
-                       c->S32a, c->Mval);
+                       c->S32a, c->S64u, c->Mval);
     const hipError_t e = hipGetLastError();
     if (e == hipSuccess) c->s32_valid = lay;
     return e;
 }
 
-// The FP32 layout that SpMV variant `variant` reads, written from the staged element
+// The layout that SpMV variant `variant` reads, written from the staged element
 // blocks of the last assembly if that assembly did not write it (same values, same
 // bits: the fold is the same; the FP64 values are rewritten unchanged).
-hipError_t ensure_fp32_layout(eolc_ctx *c, int variant)
+static hipError_t ensure_layout(eolc_ctx *c, int need)
 {
-    const int need = fp32_layout(variant);
     if ((c->s32_valid & need) == need) return hipSuccess;
     const int have = c->s32_valid;
     const hipError_t e = launch_assemble(c, need);
@@ -150,12 +157,15 @@ hipError_t ensure_fp32_layout(eolc_ctx *c, int variant)
     return e;
 }
 
+hipError_t ensure_fp32_layout(eolc_ctx *c, int variant) { return ensure_layout(c, fp32_layout(variant)); }
+hipError_t ensure_fp64_layout(eolc_ctx *c, int variant) { return ensure_layout(c, fp64_layout(variant)); }
+
 hipError_t launch_rhs(eolc_ctx *c, double h)
 {
     const int N = c->topo.N;
     hipLaunchKernelGGL(k_rhs, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, 3 * c->topo.F,
                        (c->flags & EOLC_OPT_INCLUDE_BENDING_FORCE) ? 1 : 0, h, c->fptr, c->flist, c->stageF,
-                       c->slice_off, c->srow, c->slen, c->scol, c->Mval, c->v, c->f, c->rhs, c->M32);
+                       c->slice_off, c->srow, c->slen, c->scol, c->Mval, c->v, c->f, c->rhs, c->M32, c->M64);
     return hipGetLastError();
 }
 

@@ -119,23 +119,51 @@ struct SymLayout {
     const int *supper, *uoff, *loff, *lrec;
     const float *S32u;
     const float *S32a;   // record layout: {column bits, S'_ij (9)} for every SELL entry, in SELL order
+    const double *S64u;  // the symmetric records in FP64 (layout 4 of the FP64 product)
+};
+
+// Upper records of the symmetric layout: 10 values {column bits, S'_ij (9)}, read as
+// five 2-vectors (40 B records in FP32, 80 B in FP64).
+template <class T> struct SymRec;
+template <> struct SymRec<float> {
+    typedef float v2 __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ const float *base(const SymLayout &s) { return s.S32u; }
+    static __device__ __forceinline__ int col(v2 r) { return __float_as_int(r.x); }
+};
+template <> struct SymRec<double> {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    static __device__ __forceinline__ const double *base(const SymLayout &s) { return s.S64u; }
+    static __device__ __forceinline__ int col(v2 r) { return (int)__double_as_longlong(r.x); }
 };
 
 // Block layouts (LAY): 0 = nine SoA value arrays plus the column array (coalesced
 // 4-byte loads); 1 = symmetric records (above); 2 = one 40-byte record {column,
 // S'_ij} per SELL entry, read as five 8-byte loads per lane (five load instructions
 // per block instead of ten); 3 = layout 0 with the next entry's loads issued before
-// the current entry's products.  1 and 2 are FP32 difference form only.
-template <class T, bool DIFF, int U, bool PF, bool NT, int W, int LAY>
+// the current entry's products; 4 = symmetric records in either precision (the
+// FP64 product then runs in the difference form too, on 80-byte records: 41% fewer
+// algorithmic bytes than its assembled SoA form, different rounding).  1 and 2 are
+// FP32 difference form only; 4 is layout 1 for FP32.
+//
+// STG (LDS-staged gathers): the workgroup's vertex window [wwin[2 lw], wwin[2 lw + 1])
+// holds its rows and every column they reference (host-computed; at most
+// kStageVerts vertices).  Its p_old and z are loaded (coalesced) before the partial
+// sums are re-reduced, p_new = z + beta p_old is formed once per window vertex into
+// LDS, and every neighbour gather reads LDS instead of global memory: the chain
+// column index -> global gather -> product becomes column index -> LDS read.  Same
+// values (the same fma per component), so bitwise-identical results.
+template <class T, bool DIFF, int U, bool PF, bool NT, int W, int LAY, bool STG = false>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const int *__restrict__ srow,
            const int *__restrict__ slen, const int *__restrict__ scol, const T *__restrict__ A,
            const T *__restrict__ Mm, int64_t nbs, const unsigned char *__restrict__ fixmask, const T *__restrict__ z,
            const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, SolverScalars *scal, double *partials,
-           SymLayout sym)
+           SymLayout sym, const int *__restrict__ wwin)
 {
     static_assert(U == 1 || U == 2, "entries per trip");
+    static_assert(!(STG && U == 2), "the staged variants take one entry per trip");
     __shared__ double lds[3 * (WG / 64) + 3];
+    __shared__ T su[STG ? 3 * eolc::kStageVerts : 1];
     if (scal->done) return;
     // the own row's indices and vectors do not depend on beta: their loads are
     // issued before the partial sums are re-reduced, so the two latencies overlap
@@ -155,6 +183,23 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
         po0 = pold[3 * row]; po1 = pold[3 * row + 1]; po2 = pold[3 * row + 2];
         if constexpr (DIFF) mb = Mm[row];
     }
+    // the window's p_old and z: loaded now, combined with beta below
+    constexpr int kPer = STG ? (3 * eolc::kStageVerts + WG - 1) / WG : 1;
+    int wlo = 0, nf = 0;
+    T spo[kPer], sz[kPer];
+    if constexpr (STG) {
+        const int lw = logical_wg();
+        wlo = wwin[2 * lw];
+        nf = 3 * (wwin[2 * lw + 1] - wlo);
+        const T *__restrict__ pw = pold + 3 * (int64_t)wlo;
+        const T *__restrict__ zw = z + 3 * (int64_t)wlo;
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int f = t * WG + (int)threadIdx.x;
+            spo[t] = f < nf ? pw[f] : (T)0;
+            sz[t] = f < nf ? zw[f] : (T)0;
+        }
+    }
     double cur[2];
     const bool restart_beta0 = (k == scal->first_iter);
     double prev[1] = {1.0};
@@ -173,6 +218,25 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
     }
     const double beta = restart_beta0 ? 0.0 : rzk / prev[0];
     const T betaT = (T)beta;   // p_new = z + beta p is formed in T, identically everywhere
+    if constexpr (STG) {
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            const int f = t * WG + (int)threadIdx.x;
+            if (f < nf) su[f] = fma(betaT, spo[t], sz[t]);
+        }
+        __syncthreads();
+    }
+    // p_new of column j: from LDS (staged) or formed from global p_old and z
+    auto gat = [&](int j, T &u0, T &u1, T &u2) {
+        if constexpr (STG) {
+            const T *sj = su + 3 * (j - wlo);
+            u0 = sj[0]; u1 = sj[1]; u2 = sj[2];
+        } else {
+            u0 = fma(betaT, pold[3 * j], z[3 * j]);
+            u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
+            u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+        }
+    };
     double dot = 0.0;
     if (active) {
         const T *__restrict__ a = A + base;
@@ -201,27 +265,26 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
                 const f2v r0 = ldm<f2v, NT>(rec), r1 = ldm<f2v, NT>(rec + 1), r2 = ldm<f2v, NT>(rec + 2),
                           r3 = ldm<f2v, NT>(rec + 3), r4 = ldm<f2v, NT>(rec + 4);
                 const int j = __float_as_int(r0.x);
-                const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
-                const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
-                const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                T u0, u1, u2;
+                gat(j, u0, u1, u2);
                 block_apply<T, DIFF>(r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x, r4.y, u0, u1, u2, ui0, ui1, ui2,
                                      y0, y1, y2);
             }
-        } else if constexpr (DIFF && LAY == 1) {
-            static_assert(std::is_same<T, float>::value, "symmetric layout is FP32");
+        } else if constexpr (DIFF && (LAY == 1 || LAY == 4)) {
+            using R = SymRec<T>;
+            typedef typename R::v2 v2;
+            const T *__restrict__ ubase = R::base(sym);
             const int nu = sym.supper[g];
             const int nl = len - nu;
-            const float *__restrict__ ur = sym.S32u + (size_t)10 * (sym.uoff[sl] + lane);
-            typedef float f2v __attribute__((ext_vector_type(2)));
+            const T *__restrict__ ur = ubase + (size_t)10 * (sym.uoff[sl] + lane);
 #pragma unroll 1
             for (kk = 0; kk < nu; ++kk) {
-                const f2v *rec = reinterpret_cast<const f2v *>(ur + (size_t)640 * kk);
-                const f2v r0 = ldm<f2v, NT>(rec), r1 = ldm<f2v, NT>(rec + 1), r2 = ldm<f2v, NT>(rec + 2),
-                          r3 = ldm<f2v, NT>(rec + 3), r4 = ldm<f2v, NT>(rec + 4);
-                const int j = __float_as_int(r0.x);
-                const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
-                const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
-                const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                const v2 *rec = reinterpret_cast<const v2 *>(ur + (size_t)640 * kk);
+                const v2 r0 = ldm<v2, NT>(rec), r1 = ldm<v2, NT>(rec + 1), r2 = ldm<v2, NT>(rec + 2),
+                         r3 = ldm<v2, NT>(rec + 3), r4 = ldm<v2, NT>(rec + 4);
+                const int j = R::col(r0);
+                T u0, u1, u2;
+                gat(j, u0, u1, u2);
                 block_apply<T, DIFF>(r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x, r4.y, u0, u1, u2, ui0, ui1, ui2,
                                      y0, y1, y2);
             }
@@ -232,11 +295,10 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
                 const i2v jt = ldm<i2v, NT>(lr + (size_t)64 * kk);
                 const int j = jt.x;
                 // block (j, i), record {column, 9 values}: apply its transpose
-                const f2v *rec = reinterpret_cast<const f2v *>(sym.S32u + (size_t)10 * jt.y);
-                const f2v r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3], r4 = rec[4];
-                const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
-                const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
-                const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                const v2 *rec = reinterpret_cast<const v2 *>(ubase + (size_t)10 * jt.y);
+                const v2 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3], r4 = rec[4];
+                T u0, u1, u2;
+                gat(j, u0, u1, u2);
                 block_apply<T, DIFF>(r0.y, r2.x, r3.y, r1.x, r2.y, r4.x, r1.y, r3.x, r4.y, u0, u1, u2, ui0, ui1, ui2,
                                      y0, y1, y2);
             }
@@ -259,9 +321,8 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
                     n0 = ldm<T, NT>(an); n1 = ldm<T, NT>(an + nbs); n2 = ldm<T, NT>(an + 2 * nbs);
                     n3 = ldm<T, NT>(an + 3 * nbs); n4 = ldm<T, NT>(an + 4 * nbs); n5 = ldm<T, NT>(an + 5 * nbs);
                     n6 = ldm<T, NT>(an + 6 * nbs); n7 = ldm<T, NT>(an + 7 * nbs); n8 = ldm<T, NT>(an + 8 * nbs);
-                    const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
-                    const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
-                    const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                    T u0, u1, u2;
+                    gat(j, u0, u1, u2);
                     block_apply<T, DIFF>(c0, c1, c2, c3, c4, c5, c6, c7, c8, u0, u1, u2, ui0, ui1, ui2, y0, y1, y2);
                 }
             }
@@ -278,9 +339,8 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
                 } else {
                     j = ldm<int, NT>(cl + e);
                 }
-                const T u0 = fma(betaT, pold[3 * j], z[3 * j]);
-                const T u1 = fma(betaT, pold[3 * j + 1], z[3 * j + 1]);
-                const T u2 = fma(betaT, pold[3 * j + 2], z[3 * j + 2]);
+                T u0, u1, u2;
+                gat(j, u0, u1, u2);
                 const T *__restrict__ ae = a + e;
                 block_apply<T, DIFF>(ldm<T, NT>(ae), ldm<T, NT>(ae + nbs), ldm<T, NT>(ae + 2 * nbs),
                                      ldm<T, NT>(ae + 3 * nbs), ldm<T, NT>(ae + 4 * nbs), ldm<T, NT>(ae + 5 * nbs),
@@ -615,6 +675,8 @@ __global__ void k_scal_restart(SolverScalars *s, int k)
     s->reason = 0;
 }
 
+__global__ void k_scal_delta(SolverScalars *s, double delta2) { s->delta2 = delta2; }
+
 inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
 // Streaming copy (16-byte non-temporal loads and stores, four in flight per
@@ -664,34 +726,51 @@ __global__ void k_fill(int n, double v, double *__restrict__ out)
 }
 
 // SpMV launch variants (see k_pcg_spmv); the order is the one eolc_autotune reports.
-// The last column is the block layout; the symmetric (1) and record (2) layouts
-// apply to the FP32 difference form, and the FP64 product runs its SoA kernel for
-// them; the pipelined SoA loop (3) serves both precisions.
-#define EOLC_SPMV_VARIANT_LIST(X)   \
-    X(0, 1, false, false, 8, 0)     \
-    X(1, 1, true, false, 8, 0)      \
-    X(2, 1, true, true, 8, 0)       \
-    X(3, 1, false, true, 8, 0)      \
-    X(4, 1, true, true, 6, 0)       \
-    X(5, 2, true, true, 6, 0)       \
-    X(6, 2, true, false, 6, 0)      \
-    X(7, 2, true, true, 4, 0)       \
-    X(8, 1, true, true, 4, 0)       \
-    X(9, 1, false, false, 8, 1)     \
-    X(10, 1, false, false, 6, 1)    \
-    X(11, 1, false, true, 8, 1)     \
-    X(12, 1, false, false, 8, 2)    \
-    X(13, 1, false, true, 8, 2)     \
-    X(14, 1, false, false, 6, 2)    \
-    X(15, 1, false, false, 8, 3)    \
-    X(16, 1, false, true, 8, 3)     \
-    X(17, 1, false, true, 6, 3)
+// Columns: id, entries per trip, column prefetch, non-temporal loads, waves per SIMD,
+// block layout, LDS-staged gathers.  The symmetric (1) and record (2) layouts apply
+// to the FP32 difference form, and the FP64 product runs its SoA kernel for them;
+// the pipelined SoA loop (3) serves both precisions; layout 4 is the symmetric
+// records in either precision.  The staged variants (19-24) need every workgroup's
+// vertex window within kStageVerts; their LDS (65 KB FP32, 131 KB FP64) allows two
+// workgroups per CU in FP32 and one in FP64.
+#define EOLC_SPMV_VARIANT_LIST(X)      \
+    X(0, 1, false, false, 8, 0, false) \
+    X(1, 1, true, false, 8, 0, false) \
+    X(2, 1, true, true, 8, 0, false) \
+    X(3, 1, false, true, 8, 0, false) \
+    X(4, 1, true, true, 6, 0, false) \
+    X(5, 2, true, true, 6, 0, false) \
+    X(6, 2, true, false, 6, 0, false) \
+    X(7, 2, true, true, 4, 0, false) \
+    X(8, 1, true, true, 4, 0, false) \
+    X(9, 1, false, false, 8, 1, false) \
+    X(10, 1, false, false, 6, 1, false) \
+    X(11, 1, false, true, 8, 1, false) \
+    X(12, 1, false, false, 8, 2, false) \
+    X(13, 1, false, true, 8, 2, false) \
+    X(14, 1, false, false, 6, 2, false) \
+    X(15, 1, false, false, 8, 3, false) \
+    X(16, 1, false, true, 8, 3, false) \
+    X(17, 1, false, true, 6, 3, false) \
+    X(18, 1, false, false, 8, 4, false) \
+    X(19, 1, false, true, 8, 0, true) \
+    X(20, 1, false, true, 4, 0, true) \
+    X(21, 1, false, false, 8, 1, true) \
+    X(22, 1, false, false, 4, 4, true) \
+    X(23, 1, false, true, 8, 3, true) \
+    X(24, 1, false, true, 4, 3, true)
 
 // layout column of a variant -> the FP32 array it reads
-constexpr int lay_bit(int lay) { return lay == 1 ? eolc::kLaySym : lay == 2 ? eolc::kLayRecord : eolc::kLaySoA; }
-#define EOLC_SPMV_LAYOUT_CASE(id, U, PF, NT, W, LAY) \
+constexpr int lay_bit(int lay)
+{
+    return (lay == 1 || lay == 4) ? eolc::kLaySym : lay == 2 ? eolc::kLayRecord : eolc::kLaySoA;
+}
+#define EOLC_SPMV_LAYOUT_CASE(id, U, PF, NT, W, LAY, STG) \
     case id:                                         \
         return lay_bit(LAY);
+#define EOLC_SPMV_LAYOUT64_CASE(id, U, PF, NT, W, LAY, STG) \
+    case id:                                           \
+        return LAY == 4 ? eolc::kLaySym64 : 0;
 
 template <class T>
 void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm, const T *z, const T *pold, T *pnew,
@@ -699,19 +778,21 @@ void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm
 {
     const eolc::Topology &t = c->topo;
     constexpr bool diff = std::is_same<T, float>::value;
-    const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u, c->S32a};
-#define EOLC_SPMV_CASE(id, U, PF, NT, W, LAY)                                                                      \
+    const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u, c->S32a, c->S64u};
+    // a staged variant chosen for an earlier mesh whose windows were narrow enough
+    if (eolc::spmv_staged(variant) && t.wmax > eolc::kStageVerts) variant = eolc::spmv_unstaged(variant);
+#define EOLC_SPMV_CASE(id, U, PF, NT, W, LAY, STG)                                                                      \
     case id:                                                                                                       \
-        hipLaunchKernelGGL((k_pcg_spmv<T, diff, U, PF, NT, W, (diff || LAY == 3) ? LAY : 0>), dim3(G), dim3(WG), 0, c->stream, k, \
-                           t.N, t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z,  \
-                           pold, pnew, q, c->scal, c->partials, sym);                                             \
+        hipLaunchKernelGGL((k_pcg_spmv<T, diff || LAY == 4, U, PF, NT, W, (diff || LAY >= 3) ? LAY : 0, STG>), dim3(G), \
+                           dim3(WG), 0, c->stream, k, t.N, t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, \
+                           t.nbs, c->fixmask, z, pold, pnew, q, c->scal, c->partials, sym, c->wwin);                \
         break;
     switch (variant) {
         EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_CASE)
     default:
         hipLaunchKernelGGL((k_pcg_spmv<T, diff, 1, false, false, 8, 0>), dim3(G), dim3(WG), 0, c->stream, k, t.N,
                            t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, pold, pnew,
-                           q, c->scal, c->partials, sym);
+                           q, c->scal, c->partials, sym, c->wwin);
     }
 #undef EOLC_SPMV_CASE
 }
@@ -771,12 +852,49 @@ void launch_resid64(eolc_ctx *c, int G, const double *xin, const double *b, doub
 
 namespace eolc {
 
+#define EOLC_SPMV_STAGED_CASE(id, U, PF, NT, W, LAY, STG) \
+    case id:                                             \
+        return STG;
+bool spmv_staged(int variant)
+{
+    switch (variant) {
+        EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_STAGED_CASE)
+    default:
+        return false;
+    }
+}
+
+// 19, 20 read the SoA arrays like 3; 21 the symmetric FP32 records like 9; 22 the
+// symmetric records of both precisions like 18; 23, 24 the SoA arrays like 16
+int spmv_unstaged(int variant)
+{
+    switch (variant) {
+    case 19:
+    case 20: return 3;
+    case 21: return 9;
+    case 22: return 18;
+    case 23:
+    case 24: return 16;
+    default: return variant;
+    }
+}
+
 int fp32_layout(int variant)
 {
     switch (variant) {
         EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_LAYOUT_CASE)
     default:
         return kLaySoA;
+    }
+}
+
+// FP64 variants read A64 (always written) and, for layout 4, the FP64 records
+int fp64_layout(int variant)
+{
+    switch (variant) {
+        EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_LAYOUT64_CASE)
+    default:
+        return 0;
     }
 }
 
@@ -844,8 +962,7 @@ hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters)
         if (mixed)
             launch_iteration<float>(c, k, G, c->S32, c->M32, c->D32, c->z32, c->p32, c->q32, c->r32, c->xl32);
         else
-            launch_iteration<double>(c, k, G, c->A64, (const double *)nullptr, c->D64, c->z64, c->p64, c->q64, c->r64,
-                                     c->xs64);
+            launch_iteration<double>(c, k, G, c->A64, c->M64, c->D64, c->z64, c->p64, c->q64, c->r64, c->xs64);
     }
     return hipGetLastError();
 }
@@ -894,6 +1011,12 @@ hipError_t pcg_finish(eolc_ctx *c, double *out)
 hipError_t scal_restart(eolc_ctx *c, int k)
 {
     hipLaunchKernelGGL(k_scal_restart, dim3(1), dim3(1), 0, c->stream, c->scal, k);
+    return hipGetLastError();
+}
+
+hipError_t scal_delta(eolc_ctx *c, double delta2)
+{
+    hipLaunchKernelGGL(k_scal_delta, dim3(1), dim3(1), 0, c->stream, c->scal, delta2);
     return hipGetLastError();
 }
 
@@ -958,8 +1081,8 @@ hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms)
     const int N = c->topo.N;
     const int G = pcg_grid(N);
     if (variant < 0) variant = c->spmv_variant[mixed ? 0 : 1];
-    if (mixed) {
-        const hipError_t el = ensure_fp32_layout(c, variant);
+    {
+        const hipError_t el = mixed ? ensure_fp32_layout(c, variant) : ensure_fp64_layout(c, variant);
         if (el != hipSuccess) return el;
     }
     hipEvent_t a, b;
@@ -982,14 +1105,13 @@ hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms)
     if (mixed)
         launch_spmv<float>(c, variant, 1, G, c->S32, c->M32, c->z32, c->p32[1], c->p32[0], c->q32);
     else
-        launch_spmv<double>(c, variant, 1, G, c->A64, (const double *)nullptr, c->z64, c->p64[1], c->p64[0], c->q64);
+        launch_spmv<double>(c, variant, 1, G, c->A64, c->M64, c->z64, c->p64[1], c->p64[0], c->q64);
     e = hipEventRecord(a, c->stream);
     for (int r = 0; r < reps && e == hipSuccess; ++r) {
         if (mixed)
             launch_spmv<float>(c, variant, 1, G, c->S32, c->M32, c->z32, c->p32[1], c->p32[0], c->q32);
         else
-            launch_spmv<double>(c, variant, 1, G, c->A64, (const double *)nullptr, c->z64, c->p64[1], c->p64[0],
-                                c->q64);
+            launch_spmv<double>(c, variant, 1, G, c->A64, c->M64, c->z64, c->p64[1], c->p64[0], c->q64);
     }
     if (e == hipSuccess) e = hipEventRecord(b, c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(b);

@@ -37,13 +37,13 @@ EXPORTED_SYMBOLS = (
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
     "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
     "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol", "eolc_velocity_transfer_eol",
-    "eolc_collide_noalloc", "eolc_learned_set_multilevel",
+    "eolc_collide_noalloc", "eolc_learned_set_multilevel", "eolc_set_mixed_stall",
 )
 
 BATCH_EAGER = 0x1
 BATCH_VARIANTS = 3
 
-SPMV_VARIANTS = 18
+SPMV_VARIANTS = 25
 
 LEARNED_NPARAMS = 26246
 
@@ -142,6 +142,7 @@ def lib():
                                         C.c_void_p]
         L.eolc_learned_set_weights.argtypes = [vp, C.c_void_p, C.c_int]
         L.eolc_learned_set_multilevel.argtypes = [vp, C.c_int]
+        L.eolc_set_mixed_stall.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.eolc_learned_get_weights.argtypes = [vp, C.c_void_p, C.c_int]
         L.eolc_learned_infer.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.eolc_learned_loss_grad.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p,
@@ -411,6 +412,10 @@ class Context:
                 continue
             self._check(rc)
             return [buf[i] for i in range(cnt.value)]
+
+    def set_mixed_stall(self, first_cycle=10000, min_cycle=1000, sticky=8):
+        """Stall limits of the mixed modes (FP64 continuation after an FP32 stall)."""
+        return self._check(lib().eolc_set_mixed_stall(self._h, int(first_cycle), int(min_cycle), int(sticky)))
 
     def set_mixed_delta(self, delta):
         return self._check(lib().eolc_set_mixed_delta(self._h, float(delta)))

@@ -47,7 +47,11 @@ typedef enum {
 /* solve modes (eolc_velocity_solve / eolc_step) */
 #define EOLC_SOLVE_FP64_BJ       0  /* FP64 3x3 block-Jacobi PCG */
 #define EOLC_SOLVE_MIXED_BJ      1  /* FP32 matrix/vectors, FP64 dots and true residual,
-                                       reliable updates; block-Jacobi */
+                                       reliable updates; block-Jacobi.  When the FP32
+                                       iteration stalls (the FP32 operator no longer
+                                       resolves the system) the solve continues in FP64
+                                       from its iterate, and the context's next mixed
+                                       solves start in FP64 (eolc_set_mixed_stall) */
 #define EOLC_SOLVE_MIXED_LEARNED 2  /* as MIXED_BJ with the learned blocks P_i (weights
                                        from eolc_learned_set_weights); falls back to
                                        block-Jacobi after twice the iterations of the last
@@ -68,6 +72,15 @@ typedef enum {
  * whenever the recursive residual has dropped by this factor since the last update.
  * A value set here applies to every mode. */
 int eolc_set_mixed_delta(eolc_ctx *ctx, double delta);
+
+/* Stall limits of the mixed modes (no counterpart in the reference).  The FP32
+ * iteration is taken to have stalled when a reliable-update cycle runs longer than
+ * max(min_cycle, 10x the mean completed cycle) -- first_cycle for the first cycle --
+ * or when three reliable updates in a row each reduce the true residual by less than
+ * half the cycle's target.  The solve then continues in FP64 from its iterate, and
+ * the next `sticky` mixed solves on the context start in FP64 (a state upload ends
+ * that).  Defaults 10000, 1000, 8. */
+int eolc_set_mixed_stall(eolc_ctx *ctx, int first_cycle, int min_cycle, int sticky);
 
 /* Velocity transfer after a remesh, non-EOL vertices: Cloth::velocityTransfer
  * (Cloth.cpp:195-230, 265-281).  Each new vertex takes the velocity of the first old
@@ -125,7 +138,10 @@ typedef struct {
     int iters;              /* inner PCG iterations */
     int refinements;        /* FP64 true-residual replacements (mixed modes) */
     int status;             /* eolc_status of the solve */
-    int used_fallback;      /* 1 when the learned mode fell back to block-Jacobi */
+    int used_fallback;      /* bit 0: the learned mode fell back to block-Jacobi; bit 1:
+                               the mixed solve ran (part of) its iterations in FP64
+                               after its FP32 iteration stalled, or started in FP64
+                               because a recent mixed solve on the context had stalled */
     double relres;          /* recursive residual estimate at exit */
     double true_relres;     /* ||b - A v||_2 / ||b||_2, FP64, reduced system */
     double ms_assembly;     /* element evaluation + assembly (device time) */
@@ -389,13 +405,20 @@ int eolc_stream_read(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
 
 /* ---- launch autotuner (no counterpart in the reference) -----------------------
  * Times each SpMV launch variant (entries per loop trip, column-index prefetch,
- * non-temporal matrix loads, waves per SIMD, and for FP32 the symmetric layout that
- * stores each off-diagonal block once) for `reps` launches on the current
- * matrix and keeps the fastest for later solves of that precision (32 or 64) on
- * this context.  All variants give bitwise-identical results.  ms (may be NULL)
- * receives EOLC_SPMV_VARIANTS average times; *best (may be NULL) the choice.
- * eolc_set_spmv_variant sets the choice directly. */
-#define EOLC_SPMV_VARIANTS 18
+ * non-temporal matrix loads, waves per SIMD, and the symmetric layout that stores
+ * each off-diagonal block once) for `reps` launches on the current matrix and
+ * keeps the fastest for later solves of that precision (32 or 64) on this context.
+ * Variants 19-24 stage each workgroup's neighbour values in LDS; they apply when
+ * every workgroup's rows and their columns span at most 5,440 consecutive vertices
+ * (a locally numbered mesh, e.g. a row-major grid up to about 1,100 vertices wide);
+ * otherwise the autotuner reports them as infinite and eolc_set_spmv_variant
+ * refuses them.  All variants give bitwise-identical results within a precision,
+ * except that variants 18 and 22 read the symmetric records in FP64 too and there
+ * apply the operator in the difference form of the FP32 product (the same matrix,
+ * rounded differently; the true residual is always formed from the assembled FP64
+ * matrix).  ms (may be NULL) receives EOLC_SPMV_VARIANTS average times; *best (may
+ * be NULL) the choice.  eolc_set_spmv_variant sets the choice directly. */
+#define EOLC_SPMV_VARIANTS 25
 int eolc_autotune(eolc_ctx *ctx, int precision, int reps, int *best, double *ms);
 int eolc_set_spmv_variant(eolc_ctx *ctx, int precision, int variant);
 

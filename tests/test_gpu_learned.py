@@ -211,3 +211,31 @@ def test_trained_network_gain(ctx):
         assert all(r[6] <= 1e-8 for r in rows)
         assert le <= 1.25 * bj
 
+
+
+def test_committed_trained_weights_solve_a_1024_step():
+    """The committed network (eolc/weights/mpn_cloth_trained.npy, trained by
+    `python -m eolc.train_mpn --batched` on systems of the batched path) loads without
+    pickle and solves one step of the 1024x1024 benchmark sheet to 1e-8 inside the
+    multilevel preconditioner; its iteration count is reported next to the
+    block-Jacobi blocks' on the same system."""
+    import os
+    path = os.path.join(os.path.dirname(eolc.__file__), "weights", "mpn_cloth_trained.npy")
+    theta = np.load(path, allow_pickle=False)
+    assert theta.dtype == np.float32 and theta.size == eolc.LEARNED_NPARAMS and np.all(np.isfinite(theta))
+    sc = scene.Scene(1024, sigma_x=1e-3, sigma_v=0.05)
+    print("committed weights: one 1024x1024 step", flush=True)
+    with eolc.Context(0) as c:
+        c.set_topology(sc.X, sc.tri, sc.hinge)
+        c.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+        c.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+        out = {}
+        for name, mode, th in (("block-Jacobi blocks", eolc.SOLVE_MIXED_BJ, None),
+                               ("trained network", eolc.SOLVE_MIXED_LEARNED, theta)):
+            if th is not None:
+                c.learned_set_weights(th)
+            c.upload_state(sc.x0, sc.v0)
+            rc, st = c.step(GRAV, H, sc.fixed, tol=1e-8, mode=mode | eolc.SOLVE_MULTILEVEL)
+            assert rc == eolc.OK and st.true_relres <= 1e-8, st.as_dict()
+            out[name] = st.iters
+        print(f"  iterations: {out}", flush=True)

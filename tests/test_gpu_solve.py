@@ -12,7 +12,7 @@ import pytest
 import eolc
 import oracle as O
 from conftest import GRAV, H, rel_err
-from eolc import scene
+from eolc import learned, scene
 
 pytestmark = pytest.mark.gpu
 
@@ -143,9 +143,17 @@ def test_iteration_cap_reports_maxit(ctx):
     assert st.launched == 5 and st.host_syncs >= 1
 
 
+FP64_DIFF_VARIANTS = (18, 22)   # FP64 difference form on symmetric records
+
+
 def test_spmv_variants_bitwise_identical_and_autotune(ctx):
-    """Every SpMV launch variant gives the same bits (same arithmetic, same order);
-    the autotuner times them all and keeps one."""
+    """Every SpMV launch variant of one arithmetic family gives the same bits (same
+    arithmetic, same order; the LDS-staged variants 19-24 form the same p_new values
+    in LDS that the others form per gather): all FP32 variants, the FP64 variants of
+    the assembled form, and the two FP64 variants of the difference form on
+    symmetric records (18, 22: other rounding), whose solve meets the tolerance and
+    agrees with the assembled form to the solver tolerance.  The autotuner times
+    them all and keeps one."""
     sc = scene.Scene(40, sigma_x=1e-3, sigma_v=0.05)
     for prec, mode in ((32, eolc.SOLVE_MIXED_BJ), (64, eolc.SOLVE_FP64_BJ)):
         out = []
@@ -154,16 +162,26 @@ def test_spmv_variants_bitwise_identical_and_autotune(ctx):
             setup(ctx, sc)
             ctx.set_spmv_variant(prec, var)
             rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-9, maxit=20000, mode=mode)
-            assert rc == 0
+            assert rc == 0 and st.true_relres <= 1e-9
             out.append((ctx.download_state(), st.iters))
-        for (x, v), it in out[1:]:
-            assert it == out[0][1]
-            assert np.array_equal(x, out[0][0][0]) and np.array_equal(v, out[0][0][1])
+        families = [out] if prec == 32 else [
+            [o for k, o in enumerate(out) if k not in FP64_DIFF_VARIANTS], [out[k] for k in FP64_DIFF_VARIANTS]]
+        for family in families:
+            for (x, v), it in family[1:]:
+                assert it == family[0][1]
+                assert np.array_equal(x, family[0][0][0]) and np.array_equal(v, family[0][0][1])
+        if prec == 64:
+            (x18, v18), it18 = out[18]
+            print(f"  FP64 difference form on symmetric records: {it18} iterations "
+                  f"(assembled form {out[0][1]}), v rel diff {rel_err(v18, out[0][0][1]):.2e}", flush=True)
+            assert rel_err(v18, out[0][0][1]) <= 1e-5
+        ctx.set_spmv_variant(prec, 0)
         setup(ctx, sc)
         ctx.forces_fill(GRAV, H)
         best, ms = ctx.autotune(prec, reps=5)
         print(f"autotune fp{prec}: best {best}, ms {np.round(ms * 1e3, 1)} us", flush=True)
         assert 0 <= best < eolc.SPMV_VARIANTS and np.all(ms > 0) and ms[best] == ms.min()
+        ctx.set_spmv_variant(prec, 0)
 
 
 def test_headline_path_50_steps_vs_oracle(ctx):
@@ -193,3 +211,35 @@ def test_headline_path_50_steps_vs_oracle(ctx):
     ef = rel_err(fg, fo)
     print(f"  forces at the final state {ef:.2e}", flush=True)
     assert ef <= 1e-10
+
+def test_mixed_stall_continues_in_fp64():
+    """A mixed solve whose FP32 iteration stalls finishes in FP64 from its iterate and
+    meets the tolerance (stats.used_fallback bit 1); the context's next mixed solves
+    start in FP64 for the sticky count.  The stall is forced from the host: a
+    reliable-update trigger of 1e-4 (eolc_set_mixed_delta) keeps the first FP32 cycle
+    going past its first 16-iteration chunk, and a first-cycle limit of 1 iteration
+    (eolc_set_mixed_stall) then declares it stalled; the FP64 answer is compared with
+    a plain FP64 solve of the same system.  (Own contexts: both are context settings.)"""
+    sc = scene.Scene(48, sigma_x=1e-3, sigma_v=0.05)
+    for mode in (eolc.SOLVE_MIXED_BJ, eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL,
+                 eolc.SOLVE_MIXED_LEARNED | eolc.SOLVE_MULTILEVEL):
+        with eolc.Context(0) as c:
+            setup(c, sc, eolc.OPT_SPD_PROJECT_BENDING)
+            c.learned_set_weights(learned.init_weights(7))
+            c.set_mixed_delta(1e-4)
+            c.set_mixed_stall(first_cycle=1, min_cycle=1, sticky=2)
+            rc, st = c.step(GRAV, H, sc.fixed, tol=1e-10, maxit=100000, mode=mode)
+            print(f"  mode {mode}: forced stall -> {st.as_dict()}", flush=True)
+            assert rc == eolc.OK and st.true_relres <= 1e-10 and (st.used_fallback & 2)
+            x1, v1 = c.download_state()
+            # the next two solves start in FP64 directly, the third in FP32 again
+            c.set_mixed_delta(0.5)
+            for k, fp64 in ((0, True), (1, True), (2, False)):
+                c.set_mixed_stall(first_cycle=10000, min_cycle=1000, sticky=2 if fp64 else 0)
+                rc, stk = c.step(GRAV, H, sc.fixed, tol=1e-10, maxit=100000, mode=mode)
+                assert rc == eolc.OK and bool(stk.used_fallback & 2) == fp64, (k, stk.as_dict())
+            # against a plain FP64 solve of the first step's system
+            setup(c, sc, eolc.OPT_SPD_PROJECT_BENDING)
+            rc, st64 = c.step(GRAV, H, sc.fixed, tol=1e-13, maxit=100000, mode=eolc.SOLVE_FP64_BJ)
+            x64, v64 = c.download_state()
+            assert rel_err(v1, v64) <= 1e-7
