@@ -666,7 +666,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"cloth {args.n}x{args.n} quads, noise 1e-3 m, one step = "
                                f"assemble + solve to 1e-8 + update; trajectory steps 0..{args.window - 1} "
-                               f"from the initial state, repeated", "path": head_name,
+                               f"from the initial state, repeated",
+                   "path": head_name if wsrc.endswith(".npy") else
+                   f"{head_name} with zero-output weights (exactly the block-Jacobi blocks)",
                    "vertices": sc.N, "triangles": sc.F, "hinges": sc.H, "blocks": nb,
                    "bending_hessian": "spd-projected", "weights": wsrc,
                    "multilevel_aggregates": None if args.no_ml else ml_counts},
