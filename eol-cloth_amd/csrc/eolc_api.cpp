@@ -78,6 +78,8 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->supper); dfree(c->uoff); dfree(c->loff); dfree(c->lrec); dfree(c->e2u); dfree(c->S32u); dfree(c->S32a);
     dfree(c->wwin);
     dfree(c->S64u); dfree(c->M64);
+    dfree(c->pt_row); dfree(c->pt_cnt); dfree(c->pt_roff); dfree(c->pt_soff); dfree(c->pt_loff); dfree(c->pt_lo);
+    dfree(c->pt_e2r); dfree(c->pt_e2l); dfree(c->pt_woff); dfree(c->pt_win); dfree(c->P32); dfree(c->P64);
     dfree(c->fixmask); dfree(c->vfix); dfree(c->fix_dofs); dfree(c->fix_vals);
     dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
     dfree(c->p64[0]); dfree(c->p64[1]);
@@ -500,6 +502,217 @@ std::string build_topology(Topology &t, int N, int F, const int *tri, int H, con
     return "";
 }
 
+// Patch layout of SpMV layout 5 (Topology::pt_*).  The vertices are cut into
+// G = pcg_grid(N) patches by recursive coordinate bisection of the material
+// coordinates X: a range of `parts` patches is split at the proportional rank along
+// its wider axis (ties by vertex index), so every patch holds floor or ceil of N / G
+// vertices (at most kPcgWG) and is compact in the material plane, which keeps most
+// stencil pairs inside one patch.  Any mesh gets a valid layout: locality only
+// decides how many pairs are cross blocks.  Inside a patch the rows are ordered by
+// decreasing upper count (vertex index within equal counts).  Returns "" or why the
+// layout does not apply (the patch variants then run their symmetric-record twin).
+std::string build_patch_layout(Topology &t, const double *X)
+{
+    const int N = t.N, C = eolc::kSlice, S = eolc::kPcgWG / C;
+    const int G = eolc::pcg_grid(N);
+    if ((int64_t)G * eolc::kPcgWG < N) return "more vertices than PCG workgroup rows";
+    std::vector<int> ord(N);
+    for (int i = 0; i < N; ++i) ord[i] = i;
+    std::vector<int> poff(G + 1, 0);
+    // explicit stack; left halves are finished first, so patches come out in order
+    struct Rng { int lo, hi, parts, first; };
+    std::vector<Rng> st{{0, N, G, 0}};
+    while (!st.empty()) {
+        const Rng r = st.back();
+        st.pop_back();
+        if (r.parts == 1) {
+            poff[r.first] = r.lo;
+            poff[r.first + 1] = r.hi;
+            continue;
+        }
+        const int a = r.parts / 2;
+        const int cut = r.lo + (int)((int64_t)(r.hi - r.lo) * a / r.parts);
+        double lo0 = INFINITY, hi0 = -INFINITY, lo1 = INFINITY, hi1 = -INFINITY;
+        for (int k = r.lo; k < r.hi; ++k) {
+            const double u = X[2 * (size_t)ord[k]], w = X[2 * (size_t)ord[k] + 1];
+            lo0 = std::min(lo0, u); hi0 = std::max(hi0, u);
+            lo1 = std::min(lo1, w); hi1 = std::max(hi1, w);
+        }
+        const int ax = (hi1 - lo1 > hi0 - lo0) ? 1 : 0;
+        if (cut > r.lo && cut < r.hi)
+            std::nth_element(ord.begin() + r.lo, ord.begin() + cut, ord.begin() + r.hi, [&](int i, int j) {
+                const double ui = X[2 * (size_t)i + ax], uj = X[2 * (size_t)j + ax];
+                return ui < uj || (ui == uj && i < j);
+            });
+        st.push_back({cut, r.hi, r.parts - a, r.first + a});
+        st.push_back({r.lo, cut, a, r.first});
+    }
+    std::vector<int> patch(N, -1);
+    for (int p = 0; p < G; ++p) {
+        if (poff[p + 1] - poff[p] > eolc::kPcgWG) return "a patch holds more than one workgroup's rows";
+        for (int k = poff[p]; k < poff[p + 1]; ++k) patch[ord[k]] = p;
+    }
+    // per-row counts: upper blocks, cross lower blocks, lower blocks
+    std::vector<int> nup(N, 0), ncl(N, 0), nlo(N, 0);
+    host_parallel(N, 4096, [&](int64_t r0, int64_t r1) {
+        for (int i = (int)r0; i < (int)r1; ++i)
+            for (int b = t.rowptr[i]; b < t.rowptr[i + 1]; ++b) {
+                const int j = t.col[b];
+                if (j > i) ++nup[i];
+                else if (j < i) {
+                    ++nlo[i];
+                    if (patch[j] != patch[i]) ++ncl[i];
+                }
+            }
+    });
+    for (int i = 0; i < N; ++i)
+        if (nup[i] + ncl[i] > 0xffff || nlo[i] > 0xffff) return "a row has too many blocks";
+    // storage positions: within a patch, decreasing upper count, then vertex index
+    t.pt_row.assign((size_t)G * eolc::kPcgWG, -1);
+    t.pt_cnt.assign((size_t)G * eolc::kPcgWG, 0);
+    std::vector<int> pos(N, -1);
+    host_parallel(G, 8, [&](int64_t p0, int64_t p1) {
+        std::vector<int> v;
+        for (int p = (int)p0; p < (int)p1; ++p) {
+            v.assign(ord.begin() + poff[p], ord.begin() + poff[p + 1]);
+            std::sort(v.begin(), v.end(), [&](int i, int j) { return nup[i] > nup[j] || (nup[i] == nup[j] && i < j); });
+            for (int k = 0; k < (int)v.size(); ++k) {
+                const int g = p * eolc::kPcgWG + k;
+                t.pt_row[g] = v[k];
+                t.pt_cnt[g] = nup[v[k]] | (nlo[v[k]] << 16);
+                pos[v[k]] = g;
+            }
+        }
+    });
+    // slice widths -> record, lower-entry and LDS-slot offsets
+    const int ns = G * S;
+    t.pt_roff.assign(ns + 1, 0);
+    t.pt_loff.assign(ns + 1, 0);
+    t.pt_soff.assign(ns, 0);
+    int slots = 0, slots_max = 0;
+    for (int sl = 0; sl < ns; ++sl) {
+        int wr = 0, wu = 0, wl = 0;
+        for (int l = 0; l < C; ++l) {
+            const int i = t.pt_row[(size_t)sl * C + l];
+            if (i < 0) continue;
+            wr = std::max(wr, nup[i] + ncl[i]);
+            wu = std::max(wu, nup[i]);
+            wl = std::max(wl, nlo[i]);
+        }
+        if (sl % S == 0) slots = 0;
+        t.pt_soff[sl] = slots;
+        slots += wu * C;
+        slots_max = std::max(slots_max, slots);
+        if ((int64_t)t.pt_roff[sl] + (int64_t)wr * C > INT32_MAX || (int64_t)t.pt_loff[sl] + (int64_t)wl * C > INT32_MAX)
+            return "too many patch records for 32-bit offsets";
+        t.pt_roff[sl + 1] = t.pt_roff[sl] + wr * C;
+        t.pt_loff[sl + 1] = t.pt_loff[sl] + wl * C;
+    }
+    // windows: the patch's rows in storage order (window index = position in the
+    // patch), then the vertices of other patches its rows reference, in increasing
+    // index
+    std::vector<std::vector<int>> halo(G);
+    host_parallel(G, 8, [&](int64_t p0, int64_t p1) {
+        for (int p = (int)p0; p < (int)p1; ++p) {
+            std::vector<int> &h = halo[p];
+            for (int k = 0; k < poff[p + 1] - poff[p]; ++k) {
+                const int i = t.pt_row[(size_t)p * eolc::kPcgWG + k];
+                for (int b = t.rowptr[i]; b < t.rowptr[i + 1]; ++b)
+                    if (patch[t.col[b]] != p) h.push_back(t.col[b]);
+            }
+            std::sort(h.begin(), h.end());
+            h.erase(std::unique(h.begin(), h.end()), h.end());
+        }
+    });
+    t.pt_woff.assign(G + 1, 0);
+    int wmax = 0;
+    for (int p = 0; p < G; ++p) {
+        const int w = poff[p + 1] - poff[p] + (int)halo[p].size();
+        wmax = std::max(wmax, w);
+        t.pt_woff[p + 1] = t.pt_woff[p] + w;
+    }
+    if (wmax > eolc::kPatchWin)
+        return "a patch window holds " + std::to_string(wmax) + " vertices (limit " +
+               std::to_string(eolc::kPatchWin) + ")";
+    if (slots_max + wmax > eolc::kPatchPool)
+        return "a patch needs " + std::to_string(slots_max) + " LDS slots and " + std::to_string(wmax) +
+               " window vertices (limit " + std::to_string(eolc::kPatchPool) + " together)";
+    t.pt_win.assign(std::max(t.pt_woff[G], 1), 0);
+    for (int p = 0; p < G; ++p) {
+        const int n = poff[p + 1] - poff[p];
+        for (int k = 0; k < n; ++k) t.pt_win[t.pt_woff[p] + k] = t.pt_row[(size_t)p * eolc::kPcgWG + k];
+        std::copy(halo[p].begin(), halo[p].end(), t.pt_win.begin() + t.pt_woff[p] + n);
+    }
+    t.pt_wmax = wmax;
+    auto local = [&](int p, int j) {   // window index of vertex j in patch p's window
+        if (patch[j] == p) return pos[j] - p * eolc::kPcgWG;
+        return poff[p + 1] - poff[p] + (int)(std::lower_bound(halo[p].begin(), halo[p].end(), j) - halo[p].begin());
+    };
+    t.pt_slots = slots_max;
+    t.pt_nrec = t.pt_roff[ns];
+    t.pt_nlo = t.pt_loff[ns];
+    // records, and the LDS slot each in-patch upper block leaves for its lower row
+    // (indexed by the CSR block of that lower entry)
+    t.pt_e2r.assign(t.nbs, -1);
+    t.pt_e2l.assign(t.nbs, 0);
+    t.pt_lo.assign(std::max<int64_t>(t.pt_nlo, 1), eolc::kPatchCross);
+    std::vector<int> tslot(t.nb, -1);
+    std::atomic<bool> bad{false};
+    host_parallel(N, 4096, [&](int64_t r0, int64_t r1) {
+        for (int i = (int)r0; i < (int)r1; ++i) {
+            const int g = pos[i], sl = g / C, l = g % C;
+            int k = 0;
+            for (int b = t.rowptr[i]; b < t.rowptr[i + 1]; ++b) {
+                const int j = t.col[b];
+                if (j <= i) continue;
+                t.pt_e2r[t.sellpos[b]] = t.pt_roff[sl] + k * C + l;
+                t.pt_e2l[t.sellpos[b]] = local(patch[i], j);
+                if (patch[j] == patch[i]) {
+                    const int bt = find_block(t, j, i);
+                    if (bt < 0) {
+                        bad = true;
+                        return;
+                    }
+                    tslot[bt] = t.pt_soff[sl] + k * C + l;
+                }
+                ++k;
+            }
+            int c = 0;
+            for (int b = t.rowptr[i]; b < t.rowptr[i + 1]; ++b) {
+                const int j = t.col[b];
+                if (j < i && patch[j] != patch[i]) {
+                    t.pt_e2r[t.sellpos[b]] = t.pt_roff[sl] + (nup[i] + c++) * C + l;
+                    t.pt_e2l[t.sellpos[b]] = local(patch[i], j);
+                }
+            }
+        }
+    });
+    if (bad) return "block pattern is not symmetric";
+    host_parallel(N, 4096, [&](int64_t r0, int64_t r1) {
+        for (int i = (int)r0; i < (int)r1; ++i) {
+            const int g = pos[i], sl = g / C, l = g % C;
+            int k = 0;
+            for (int b = t.rowptr[i]; b < t.rowptr[i + 1]; ++b) {
+                const int j = t.col[b];
+                if (j >= i) continue;
+                const size_t at = (size_t)t.pt_loff[sl] + (size_t)k * C + l;
+                if (patch[j] == patch[i]) {
+                    if (tslot[b] < 0 || tslot[b] >= (int)eolc::kPatchCross) {
+                        bad = true;
+                        return;
+                    }
+                    t.pt_lo[at] = (unsigned short)tslot[b];
+                } else {
+                    t.pt_lo[at] = eolc::kPatchCross;
+                }
+                ++k;
+            }
+        }
+    });
+    if (bad) return "patch lower entry without its upper block";
+    return "";
+}
+
 template <class T> hipError_t upload(eolc_ctx *c, T **dst, const std::vector<T> &src)
 {
     hipError_t e = dalloc(c, dst, src.size());
@@ -765,13 +978,20 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     Topology t;
     std::string msg = build_topology(t, N, F, tri, H, hinge, seg);
     if (!msg.empty()) return fail(c, EOLC_ERR_INVALID, "set_topology: " + msg);
+    t.pt_why = build_patch_layout(t, X);
+    t.pt_ok = t.pt_why.empty();
+    if (!t.pt_ok) {   // the patch variants fall back to their symmetric-record twin
+        t.pt_row.clear(); t.pt_cnt.clear(); t.pt_roff.clear(); t.pt_soff.clear(); t.pt_loff.clear();
+        t.pt_lo.clear(); t.pt_e2r.clear(); t.pt_e2l.clear(); t.pt_woff.clear(); t.pt_win.clear();
+        t.pt_nrec = t.pt_nlo = 0;
+    }
     free_topology_buffers(c);
     c->have_topo = c->have_state = c->have_forces = false;
     c->topo = std::move(t);
     // an LDS-staged SpMV variant chosen for an earlier mesh: its unstaged twin when
     // this mesh's windows are too wide
     for (int &v : c->spmv_variant)
-        if (eolc::spmv_staged(v) && c->topo.wmax > eolc::kStageVerts) v = eolc::spmv_unstaged(v);
+        if (!eolc::spmv_usable(c->topo, v)) v = eolc::spmv_fallback(v);
     const Topology &T = c->topo;
     std::vector<double> Xv(X, X + 2 * (size_t)N);
     std::vector<int> triv(tri, tri + 3 * (size_t)F), hv(hinge, hinge + 4 * (size_t)H);
@@ -789,6 +1009,20 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(upload(c, &c->lrec, T.lrec));
     EOLC_HIP(upload(c, &c->e2u, T.e2u));
     EOLC_HIP(upload(c, &c->wwin, T.wwin));
+    if (T.pt_ok) {
+        EOLC_HIP(upload(c, &c->pt_row, T.pt_row));
+        EOLC_HIP(upload(c, &c->pt_cnt, T.pt_cnt));
+        EOLC_HIP(upload(c, &c->pt_roff, T.pt_roff));
+        EOLC_HIP(upload(c, &c->pt_soff, T.pt_soff));
+        EOLC_HIP(upload(c, &c->pt_loff, T.pt_loff));
+        EOLC_HIP(upload(c, &c->pt_lo, T.pt_lo));
+        EOLC_HIP(upload(c, &c->pt_e2r, T.pt_e2r));
+        EOLC_HIP(upload(c, &c->pt_e2l, T.pt_e2l));
+        EOLC_HIP(upload(c, &c->pt_woff, T.pt_woff));
+        EOLC_HIP(upload(c, &c->pt_win, T.pt_win));
+        EOLC_HIP(dalloc(c, &c->P32, 10 * (size_t)std::max<int64_t>(T.pt_nrec, 1)));
+        EOLC_HIP(dalloc(c, &c->P64, 10 * (size_t)std::max<int64_t>(T.pt_nrec, 1)));
+    }
     EOLC_HIP(upload(c, &c->cptr, T.cptr));
     EOLC_HIP(upload(c, &c->clist, T.clist));
     EOLC_HIP(upload(c, &c->fptr, T.fptr));
@@ -2108,6 +2342,36 @@ try {
     return eolc::report_exception(c, "time_spmv");
 }
 
+int eolc_spmv_layout_info(eolc_ctx *c, int64_t *info, int n)
+try {
+    if (!c) return EOLC_ERR_INVALID;
+    if (!c->have_topo || !info || n <= 0) return fail(c, EOLC_ERR_INVALID, "spmv_layout_info: no topology or no output");
+    const Topology &t = c->topo;
+    int64_t cross = 0, up = 0;
+    if (t.pt_ok)
+        for (size_t g = 0; g < t.pt_row.size(); ++g) {
+            if (t.pt_row[g] < 0) continue;
+            const unsigned cnt = (unsigned)t.pt_cnt[g];
+            up += cnt & 0xffffu;
+        }
+    // (pt_lo padding holds kPatchCross too: only the entries rows read count)
+    if (t.pt_ok) {
+        const int C = eolc::kSlice;
+        for (size_t g = 0; g < t.pt_row.size(); ++g) {
+            if (t.pt_row[g] < 0) continue;
+            const int sl = (int)(g / C), l = (int)(g % C), nlo = (int)((unsigned)t.pt_cnt[g] >> 16);
+            for (int k = 0; k < nlo; ++k) cross += t.pt_lo[(size_t)t.pt_loff[sl] + (size_t)k * C + l] == eolc::kPatchCross;
+        }
+    }
+    const int64_t v[8] = {t.pt_ok ? 1 : 0, up + cross, t.pt_ok ? (t.nb - t.N) - up : 0,
+                          t.pt_ok ? (int64_t)t.pt_row.size() : 0, cross, t.pt_slots,
+                          t.pt_ok ? (int64_t)t.pt_woff.back() : 0, t.pt_wmax};
+    for (int k = 0; k < std::min(n, 8); ++k) info[k] = v[k];
+    return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "spmv_layout_info");
+}
+
 int eolc_set_mixed_stall(eolc_ctx *c, int first_cycle, int min_cycle, int sticky)
 try {
     if (!c) return EOLC_ERR_INVALID;
@@ -2170,8 +2434,8 @@ try {
     int arg = 0;
     double tbest = 0.0;
     for (int v = 0; v < eolc::kSpmvVariants; ++v) {
-        if (eolc::spmv_staged(v) && c->topo.wmax > eolc::kStageVerts) {
-            if (ms) ms[v] = INFINITY;   // not applicable to this mesh's vertex order
+        if (!eolc::spmv_usable(c->topo, v)) {
+            if (ms) ms[v] = INFINITY;   // not applicable to this mesh
             continue;
         }
         float t = 0.f;
@@ -2198,6 +2462,9 @@ try {
     if (eolc::spmv_staged(variant) && (!c->have_topo || c->topo.wmax > eolc::kStageVerts))
         return fail(c, EOLC_ERR_INVALID, "set_spmv_variant: the LDS-staged variants need every workgroup's vertex "
                                          "window within " + std::to_string(eolc::kStageVerts) + " vertices");
+    if (eolc::spmv_patch(variant) && (!c->have_topo || !c->topo.pt_ok))
+        return fail(c, EOLC_ERR_INVALID, "set_spmv_variant: the patch layout does not apply to this mesh" +
+                                             (c->have_topo ? ": " + c->topo.pt_why : std::string()));
     c->spmv_variant[precision == 32 ? 0 : 1] = variant;
     return EOLC_OK;
 } catch (...) {

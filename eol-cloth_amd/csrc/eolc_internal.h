@@ -79,7 +79,41 @@ struct Topology {
     // wmax = the widest range (the LDS-staged SpMV variants need wmax <= kStageVerts)
     std::vector<int> wwin;
     int wmax = 0;
+    // Patch layout (SpMV layout 5): the vertices are cut into pcg_grid(N) compact
+    // patches of at most kPcgWG vertices (recursive coordinate bisection of the
+    // material coordinates), one per PCG workgroup, each stored as 16 slices of 64
+    // rows at positions 1024 lw + 64 slice + lane.  A row's records are its upper
+    // blocks (column > row) in increasing column order, then its lower blocks whose
+    // column lies in another patch (cross blocks); a lower block inside the patch is
+    // not stored again: its owner (the upper row) leaves the transposed product in
+    // LDS for it, and the row's lower list names that LDS slot (or kPatchCross: take
+    // the row's next cross record).
+    bool pt_ok = false;
+    std::string pt_why;                  // why the layout does not apply to this mesh
+    std::vector<int> pt_row;             // per storage position: row or -1
+    std::vector<int> pt_cnt;             // per storage position: upper records | lower entries << 16
+    std::vector<int> pt_roff;            // per slice (+1): first record
+    std::vector<int> pt_soff;            // per slice: patch-local LDS slot of its first upper position
+    std::vector<int> pt_loff;            // per slice (+1): first lower entry
+    std::vector<unsigned short> pt_lo;   // lower entries: LDS slot or kPatchCross
+    std::vector<int> pt_e2r;             // per SELL entry: its record or -1
+    std::vector<int> pt_e2l;             // per SELL entry with a record: its column's window index
+    std::vector<int> pt_woff;            // per patch (+1): first window vertex
+    std::vector<int> pt_win;             // windows: the patch's rows in storage order, then the
+                                         // other vertices they reference (increasing index)
+    int64_t pt_nrec = 0, pt_nlo = 0;
+    int pt_slots = 0;                    // LDS slots of the largest patch
+    int pt_wmax = 0;                     // largest window
 };
+
+// Patch layout limits.  A workgroup's LDS pool holds 3 values per transposed-product
+// slot and per window vertex: kPatchPool entries (81,000 B in FP32, two workgroups
+// per CU; 162,000 B in FP64, one).  A window holds at most kPatchWin vertices (staged
+// through registers before beta is known).  kPatchCross marks a cross lower entry.
+constexpr int kPatchPool = 6750;
+constexpr int kPatchWin = 2048;
+constexpr int kPatchSlots = kPatchPool;   // slot indices stay below this
+constexpr unsigned short kPatchCross = 0xffff;
 
 // Preconditioner blocks are symmetric by construction (block-Jacobi inverse by the
 // adjugate, learned G G^T) and stored as their 6 upper entries, SoA over vertices:
@@ -285,6 +319,12 @@ struct eolc_ctx {
     float *S32a = nullptr;      // 10 x nbs: {column bits, S'_ij} per SELL entry (record layout)
     int s32_valid = 0;          // layouts (kLay* bits) holding the last assembly's values
     double *S64u = nullptr;     // 10 x nup: the symmetric records in FP64 (FP64 SpMV layout 4)
+    // patch layout (Topology::pt_*) and its records {column bits, S'_ij (9)}
+    int *pt_row = nullptr, *pt_cnt = nullptr, *pt_roff = nullptr, *pt_soff = nullptr, *pt_loff = nullptr;
+    unsigned short *pt_lo = nullptr;
+    int *pt_e2r = nullptr, *pt_e2l = nullptr, *pt_woff = nullptr, *pt_win = nullptr;
+    float *P32 = nullptr;       // 10 x pt_nrec
+    double *P64 = nullptr;      // 10 x pt_nrec
     double *M64 = nullptr;      // N row masses sum_j m_ij in FP64 (written by k_rhs)
     int *entry_is_diag = nullptr;  // nbs
     double *Mval = nullptr;     // nbs mass scalars (M block = m I3)
@@ -370,6 +410,7 @@ hipError_t launch_hinge_eval(eolc_ctx *c, double h);
 // S32a (layout 2)
 constexpr int kLaySoA = 1, kLaySym = 2, kLayRecord = 4;
 constexpr int kLaySym64 = 8;                            // symmetric records in FP64 (S64u, FP64 layout 4)
+constexpr int kLayPatch = 16, kLayPatch64 = 32;          // patch records in FP32 (P32) and FP64 (P64), layout 5
 int fp32_layout(int variant);                          // the layout bit a FP32 SpMV variant reads
 int fp64_layout(int variant);                          // the extra layout bit a FP64 variant reads (or 0)
 hipError_t launch_assemble(eolc_ctx *c, int lay);      // FP64 values + the layouts in lay
@@ -395,12 +436,17 @@ hipError_t set_fixed(eolc_ctx *c, int n, const int *d_dofs, const double *d_vals
 hipError_t fixed_from_vold(eolc_ctx *c, int n, const int *d_dofs, double *d_vals);
 // EOL rows of the last forces_fill (k_eol.hip); after launch_rhs.
 hipError_t eol_fill(eolc_ctx *c);
-constexpr int kSpmvVariants = 25;
+constexpr int kSpmvVariants = 29;
 // LDS-staged SpMV variants: a workgroup's vertex window holds at most this many
 // vertices (3 x 5440 values of p: 65,280 B in FP32, 130,560 B in FP64)
 constexpr int kStageVerts = 5440;
 bool spmv_staged(int variant);   // the variant gathers p from LDS
 int spmv_unstaged(int variant);  // a variant without staging that reads the same layout
+bool spmv_patch(int variant);    // the variant reads the patch layout
+// the variant can run on this topology (staged: windows within kStageVerts; patch:
+// the patch layout was built), else spmv_fallback(variant) gives the same bits
+bool spmv_usable(const Topology &t, int variant);
+int spmv_fallback(int variant);
 hipError_t time_spmv(eolc_ctx *c, bool mixed, int variant, int reps, float *ms);
 hipError_t stream_copy(eolc_ctx *c, size_t bytes, int reps, float *ms, int read_only);
 // learned preconditioner (k_learned.hip); buffers allocated by the API layer

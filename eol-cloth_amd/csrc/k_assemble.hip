@@ -15,8 +15,9 @@
 //                which (A p)_i = Mbar_i p_i + sum_{j != i} S'_ij (p_j - p_i),
 //                Mbar_i = sum_j m_ij (since sum_j m_ij p_j = Mbar_i p_i +
 //                sum_{j != i} m_ij (p_j - p_i)).  The FP32 values go only to the
-//                layouts in `lay` (SoA, symmetric upper records, per-entry records):
-//                the one the chosen SpMV variant reads, the others on demand.
+//                layouts in `lay` (SoA, symmetric upper records, per-entry records,
+//                patch records in FP32 or FP64): the one the chosen SpMV variant
+//                reads, the others on demand.
 //   k_rhs      : one thread per vertex (SELL storage order); f_i = sum of its element force slots
 //                (faces, then hinges when the bending force is included),
 //                rhs_i = sum_j m_ij v_j + h f_i, the right-hand side M v + h f of
@@ -30,7 +31,9 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
            const int *__restrict__ entry_is_diag, const double *__restrict__ stageK, const double *__restrict__ stageM,
            double *__restrict__ A64, int lay, float *__restrict__ S32, const int *__restrict__ e2u,
            const int *__restrict__ scol, float *__restrict__ S32u, float *__restrict__ S32a,
-           double *__restrict__ S64u, double *__restrict__ Mval)
+           double *__restrict__ S64u, const int *__restrict__ e2r, const int *__restrict__ e2l,
+           float *__restrict__ P32, double *__restrict__ P64,
+           double *__restrict__ Mval)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbs) return;
@@ -86,6 +89,21 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
 #pragma unroll
             for (int c = 0; c < 9; ++c) rec[1 + c] = diag ? 0.0 : stf[c] + (c % 4 == 0 ? m : 0.0);
         }
+        // patch layout: upper blocks and cross lower blocks, in either precision, the
+        // column as its index in the patch's window
+        const int pr = (lay & (eolc::kLayPatch | eolc::kLayPatch64)) ? e2r[b] : -1;
+        if (pr >= 0 && (lay & eolc::kLayPatch)) {
+            float *rec = P32 + (size_t)10 * pr;
+            rec[0] = __int_as_float(e2l[b]);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) rec[1 + c] = sv[c];
+        }
+        if (pr >= 0 && (lay & eolc::kLayPatch64)) {
+            double *rec = P64 + (size_t)10 * pr;
+            rec[0] = __longlong_as_double((long long)e2l[b]);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) rec[1 + c] = diag ? 0.0 : stf[c] + (c % 4 == 0 ? m : 0.0);
+        }
     }
     Mval[b] = m;
 }
@@ -137,9 +155,10 @@ hipError_t launch_assemble(eolc_ctx *c, int lay)
     const int64_t nb = c->topo.nbs;
     c->s32_valid = 0;
     if (nb == 0) return hipSuccess;
+    if (!c->topo.pt_ok) lay &= ~(eolc::kLayPatch | eolc::kLayPatch64);
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
                        c->entry_is_diag, c->stageK, c->stageM, c->A64, lay, c->S32, c->e2u, c->scol, c->S32u,
-                       c->S32a, c->S64u, c->Mval);
+                       c->S32a, c->S64u, c->pt_e2r, c->pt_e2l, c->P32, c->P64, c->Mval);
     const hipError_t e = hipGetLastError();
     if (e == hipSuccess) c->s32_valid = lay;
     return e;
@@ -157,8 +176,15 @@ static hipError_t ensure_layout(eolc_ctx *c, int need)
     return e;
 }
 
-hipError_t ensure_fp32_layout(eolc_ctx *c, int variant) { return ensure_layout(c, fp32_layout(variant)); }
-hipError_t ensure_fp64_layout(eolc_ctx *c, int variant) { return ensure_layout(c, fp64_layout(variant)); }
+// (a variant this mesh cannot run is replaced by its twin, as the SpMV launch does)
+hipError_t ensure_fp32_layout(eolc_ctx *c, int variant)
+{
+    return ensure_layout(c, fp32_layout(spmv_usable(c->topo, variant) ? variant : spmv_fallback(variant)));
+}
+hipError_t ensure_fp64_layout(eolc_ctx *c, int variant)
+{
+    return ensure_layout(c, fp64_layout(spmv_usable(c->topo, variant) ? variant : spmv_fallback(variant)));
+}
 
 hipError_t launch_rhs(eolc_ctx *c, double h)
 {

@@ -398,6 +398,195 @@ k_pcg_spmv(int k, int N, int nslices, const int *__restrict__ slice_off, const i
 }
 
 // ---------------------------------------------------------------------------
+// Patch layout (LAY 5; difference form in either precision).  Workgroup lw owns the
+// rows of patch lw, a compact set of at most 1024 vertices (Topology::pt_*), and
+// stages p_new = z + beta p_old of the patch's window (its rows, then the vertices of
+// other patches they reference) in LDS; records name their column by window index,
+// so no product gathers from global memory.
+//   Phase 1: every row applies its upper records (column > row, increasing
+//   column); for an upper block whose column lies in the patch it also leaves the
+//   transposed product for that column's row in an LDS slot.
+//   Phase 2, after a workgroup barrier: every row adds its lower blocks in
+//   increasing column order, from the LDS slot when the column is in the patch, else
+//   from its next cross record (read from HBM like an upper record).
+// An off-diagonal block inside a patch is therefore read from HBM once (layout 1
+// reads it a second time, transposed and scattered through the caches).  Every
+// row's sum takes the same terms in the same order as every other difference-form
+// variant: upper entries, then lower entries, and the lower-row product
+// S'_ji (p_j - p_i) = S'_ij^T (p_j - p_i) is -(S'_ij^T (p_i - p_j)) exactly (IEEE
+// negation is exact and rounding is symmetric), so q is bitwise the q of layout 4.
+// The workgroup partial of p.q sums the patch's rows instead of 1,024 consecutive
+// rows, so p.q, and with it alpha, can differ in the last bits.
+//
+// LDS pool (kPatchPool entries of 3 values): transposed products, component c of
+// slot s at c S + s (S = slots of the largest patch), then the window, vertex w at
+// 3 S + 3 w.
+struct PatchLayout {
+    const int *row, *cnt, *roff, *soff, *loff, *woff, *win;
+    const unsigned short *lo;
+    const float *R32;
+    const double *R64;
+    int S;
+};
+template <class T> __device__ __forceinline__ const T *patch_records(const PatchLayout &pl);
+template <> __device__ __forceinline__ const float *patch_records<float>(const PatchLayout &pl) { return pl.R32; }
+template <> __device__ __forceinline__ const double *patch_records<double>(const PatchLayout &pl) { return pl.R64; }
+
+constexpr int kPatchLo = 8;   // lower entries per row loaded before phase 1 (the rest in phase 2)
+
+template <class T, bool NT, int W>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(W, W)))
+k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned char *__restrict__ fixmask,
+                 const T *__restrict__ z, const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q,
+                 SolverScalars *scal, double *partials)
+{
+    using R = SymRec<T>;
+    typedef typename R::v2 v2;
+    __shared__ double lds[3 * (WG / 64) + 3];
+    __shared__ T pool[3 * eolc::kPatchPool];
+    if (scal->done) return;
+    const int S = pl.S;
+    T *const su = pool + 3 * S;   // window values
+    // Everything that does not depend on beta is loaded before the partial sums are
+    // re-reduced: the row's counts, mass and fixed mask, its first lower entries, and
+    // p_old and z of the window.
+    const int lw = logical_wg();
+    const int lane = threadIdx.x & 63;
+    const int sl = lw * (WG / 64) + (threadIdx.x >> 6);
+    const int g = sl * eolc::kSlice + lane;
+    const int row = pl.row[g];
+    const bool active = row >= 0;
+    int nup = 0, nlo = 0;
+    T mb = 0;
+    unsigned char fm = 0;
+    unsigned short le[kPatchLo];
+    const unsigned short *__restrict__ lo = pl.lo + pl.loff[sl] + lane;
+    if (active) {
+        const unsigned cnt = (unsigned)pl.cnt[g];
+        nup = (int)(cnt & 0xffffu);
+        nlo = (int)(cnt >> 16);
+        mb = Mm[row];
+        fm = fixmask[row];
+#pragma unroll
+        for (int t = 0; t < kPatchLo; ++t) le[t] = t < nlo ? lo[t * eolc::kSlice] : eolc::kPatchCross;
+    }
+    const T *__restrict__ rb = patch_records<T>(pl) + (size_t)10 * (pl.roff[sl] + lane);
+    const int s0 = pl.soff[sl] + lane;
+    const int w0 = pl.woff[lw], nf = 3 * (pl.woff[lw + 1] - w0);
+    constexpr int kPer = (3 * eolc::kPatchWin + WG - 1) / WG;
+    T spo[kPer], sz[kPer];
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+        const int f = t * WG + (int)threadIdx.x;
+        spo[t] = 0;
+        sz[t] = 0;
+        if (f < nf) {
+            const int v = pl.win[w0 + f / 3], c = f % 3;
+            spo[t] = pold[3 * v + c];
+            sz[t] = z[3 * v + c];
+        }
+    }
+    double cur[2];
+    const bool restart_beta0 = (k == scal->first_iter);
+    double prev[1] = {1.0};
+    if (restart_beta0) reduce_partials<2>(PB(partials, k & 1), cur, lds);
+    else reduce_partials2<2, 1>(PB(partials, k & 1), PB(partials, (k + 1) & 1), cur, prev, lds);
+    const double rzk = cur[0], rrk = cur[1];
+    const double thr = fmax(scal->tol2 * scal->bnorm2, scal->delta2 * scal->rmax2);
+    if (!isfinite(rrk) || !isfinite(rzk) || rrk <= thr) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal->done = 1;
+            scal->reason = (isfinite(rrk) && isfinite(rzk)) ? 1 : 4;
+            scal->rr = rrk;
+            scal->rz = rzk;
+        }
+        return;
+    }
+    const double beta = restart_beta0 ? 0.0 : rzk / prev[0];
+    const T betaT = (T)beta;   // p_new = z + beta p is formed in T, identically everywhere
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+        const int f = t * WG + (int)threadIdx.x;
+        if (f < nf) su[f] = fma(betaT, spo[t], sz[t]);
+    }
+    __syncthreads();
+    // the row's own p_new: window index = position in the patch = thread index
+    T ui0 = 0, ui1 = 0, ui2 = 0;
+    if (active) {
+        ui0 = su[3 * threadIdx.x];
+        ui1 = su[3 * threadIdx.x + 1];
+        ui2 = su[3 * threadIdx.x + 2];
+    }
+    double y0 = (double)(mb * ui0), y1 = (double)(mb * ui1), y2 = (double)(mb * ui2);
+    if (active && nup > 0) {
+        // records streamed one ahead: the next record's loads are issued before the
+        // current record's LDS reads and products
+        const v2 *rec = reinterpret_cast<const v2 *>(rb);
+        v2 n0 = ldm<v2, NT>(rec), n1 = ldm<v2, NT>(rec + 1), n2 = ldm<v2, NT>(rec + 2), n3 = ldm<v2, NT>(rec + 3),
+           n4 = ldm<v2, NT>(rec + 4);
+#pragma unroll 1
+        for (int kk = 0; kk < nup; ++kk) {
+            const v2 r0 = n0, r1 = n1, r2 = n2, r3 = n3, r4 = n4;
+            if (kk + 1 < nup) {
+                const v2 *rn = reinterpret_cast<const v2 *>(rb + (size_t)640 * (kk + 1));
+                n0 = ldm<v2, NT>(rn); n1 = ldm<v2, NT>(rn + 1); n2 = ldm<v2, NT>(rn + 2);
+                n3 = ldm<v2, NT>(rn + 3); n4 = ldm<v2, NT>(rn + 4);
+            }
+            const T *sj = su + 3 * R::col(r0);
+            const T u0 = sj[0], u1 = sj[1], u2 = sj[2];
+            block_apply<T, true>(r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x, r4.y, u0, u1, u2, ui0, ui1, ui2, y0,
+                                 y1, y2);
+            // S'^T (p_j - p_i) for row j; stored negated: -(S'^T (p_j - p_i)) = S'^T (p_i - p_j).
+            // Written for cross upper blocks too (their slots are never read).
+            const T d0 = u0 - ui0, d1 = u1 - ui1, d2 = u2 - ui2;
+            const int slot = s0 + kk * eolc::kSlice;
+            pool[slot] = -fma(r3.y, d2, fma(r2.x, d1, r0.y * d0));
+            pool[S + slot] = -fma(r4.x, d2, fma(r2.y, d1, r1.x * d0));
+            pool[2 * S + slot] = -fma(r4.y, d2, fma(r3.x, d1, r1.y * d0));
+        }
+    }
+    __syncthreads();
+    double dot = 0.0;
+    if (active) {
+        int cr = nup;   // next cross record
+#pragma unroll 1
+        for (int kk = 0; kk < nlo; ++kk) {
+            unsigned e = eolc::kPatchCross;
+#pragma unroll
+            for (int t = 0; t < kPatchLo; ++t)
+                if (t == kk) e = le[t];
+            if (kk >= kPatchLo) e = lo[kk * eolc::kSlice];
+            if (e != eolc::kPatchCross) {
+                y0 += (double)pool[e];
+                y1 += (double)pool[S + e];
+                y2 += (double)pool[2 * S + e];
+            } else {
+                const v2 *rec = reinterpret_cast<const v2 *>(rb + (size_t)640 * cr);
+                ++cr;
+                const v2 r0 = ldm<v2, NT>(rec), r1 = ldm<v2, NT>(rec + 1), r2 = ldm<v2, NT>(rec + 2),
+                         r3 = ldm<v2, NT>(rec + 3), r4 = ldm<v2, NT>(rec + 4);
+                const T *sj = su + 3 * R::col(r0);
+                block_apply<T, true>(r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x, r4.y, sj[0], sj[1], sj[2], ui0,
+                                     ui1, ui2, y0, y1, y2);
+            }
+        }
+        if (fm & 1) y0 = 0;
+        if (fm & 2) y1 = 0;
+        if (fm & 4) y2 = 0;
+        pnew[3 * row] = ui0; pnew[3 * row + 1] = ui1; pnew[3 * row + 2] = ui2;
+        const T q0 = (T)y0, q1 = (T)y1, q2 = (T)y2;
+        q[3 * row] = q0; q[3 * row + 1] = q1; q[3 * row + 2] = q2;
+        dot = ((double)ui0 * (double)q0 + (double)ui1 * (double)q1) + (double)ui2 * (double)q2;
+    }
+    double v[1] = {dot};
+    write_partials<1>(v, PA(partials), lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal->rr = rrk;
+        scal->rz = rzk;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // TX is the solution accumulator (double in FP64 mode, float in mixed mode).  ML:
 // the multilevel kernels form z and r.z afterwards (k_ml.hip); only r.r here.
 template <class T, class TX, bool ML>
@@ -732,7 +921,8 @@ __global__ void k_fill(int n, double v, double *__restrict__ out)
 // the pipelined SoA loop (3) serves both precisions; layout 4 is the symmetric
 // records in either precision.  The staged variants (19-24) need every workgroup's
 // vertex window within kStageVerts; their LDS (65 KB FP32, 131 KB FP64) allows two
-// workgroups per CU in FP32 and one in FP64.
+// workgroups per CU in FP32 and one in FP64.  Layout 5 (25-28) is the patch layout
+// in either precision (k_pcg_spmv_patch; 81 KB of LDS in FP32, 162 KB in FP64).
 #define EOLC_SPMV_VARIANT_LIST(X)      \
     X(0, 1, false, false, 8, 0, false) \
     X(1, 1, true, false, 8, 0, false) \
@@ -758,41 +948,58 @@ __global__ void k_fill(int n, double v, double *__restrict__ out)
     X(21, 1, false, false, 8, 1, true) \
     X(22, 1, false, false, 4, 4, true) \
     X(23, 1, false, true, 8, 3, true) \
-    X(24, 1, false, true, 4, 3, true)
+    X(24, 1, false, true, 4, 3, true) \
+    X(25, 1, false, false, 8, 5, false) \
+    X(26, 1, false, true, 8, 5, false) \
+    X(27, 1, false, false, 4, 5, false) \
+    X(28, 1, false, true, 4, 5, false)
 
 // layout column of a variant -> the FP32 array it reads
 constexpr int lay_bit(int lay)
 {
-    return (lay == 1 || lay == 4) ? eolc::kLaySym : lay == 2 ? eolc::kLayRecord : eolc::kLaySoA;
+    return (lay == 1 || lay == 4) ? eolc::kLaySym : lay == 2 ? eolc::kLayRecord : lay == 5 ? eolc::kLayPatch
+                                                                                           : eolc::kLaySoA;
 }
 #define EOLC_SPMV_LAYOUT_CASE(id, U, PF, NT, W, LAY, STG) \
     case id:                                         \
         return lay_bit(LAY);
 #define EOLC_SPMV_LAYOUT64_CASE(id, U, PF, NT, W, LAY, STG) \
     case id:                                           \
-        return LAY == 4 ? eolc::kLaySym64 : 0;
+        return LAY == 4 ? eolc::kLaySym64 : LAY == 5 ? eolc::kLayPatch64 : 0;
+
+template <class T, int U, bool PF, bool NT, int W, int LAY, bool STG>
+void launch_spmv_one(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const T *z, const T *pold, T *pnew, T *q)
+{
+    const eolc::Topology &t = c->topo;
+    constexpr bool diff = std::is_same<T, float>::value;
+    if constexpr (LAY == 5) {
+        const PatchLayout pl{c->pt_row,  c->pt_cnt, c->pt_roff, c->pt_soff, c->pt_loff, c->pt_woff,
+                             c->pt_win,  c->pt_lo,  c->P32,     c->P64,     t.pt_slots};
+        hipLaunchKernelGGL((k_pcg_spmv_patch<T, NT, W>), dim3(G), dim3(WG), 0, c->stream, k, pl, Mm, c->fixmask, z,
+                           pold, pnew, q, c->scal, c->partials);
+    } else {
+        const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u, c->S32a, c->S64u};
+        hipLaunchKernelGGL((k_pcg_spmv<T, diff || LAY == 4, U, PF, NT, W, (diff || LAY >= 3) ? LAY : 0, STG>),
+                           dim3(G), dim3(WG), 0, c->stream, k, t.N, t.nslices, c->slice_off, c->srow, c->slen, c->scol,
+                           A, Mm, t.nbs, c->fixmask, z, pold, pnew, q, c->scal, c->partials, sym, c->wwin);
+    }
+}
 
 template <class T>
 void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm, const T *z, const T *pold, T *pnew,
                  T *q)
 {
-    const eolc::Topology &t = c->topo;
-    constexpr bool diff = std::is_same<T, float>::value;
-    const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u, c->S32a, c->S64u};
-    // a staged variant chosen for an earlier mesh whose windows were narrow enough
-    if (eolc::spmv_staged(variant) && t.wmax > eolc::kStageVerts) variant = eolc::spmv_unstaged(variant);
-#define EOLC_SPMV_CASE(id, U, PF, NT, W, LAY, STG)                                                                      \
-    case id:                                                                                                       \
-        hipLaunchKernelGGL((k_pcg_spmv<T, diff || LAY == 4, U, PF, NT, W, (diff || LAY >= 3) ? LAY : 0, STG>), dim3(G), \
-                           dim3(WG), 0, c->stream, k, t.N, t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, \
-                           t.nbs, c->fixmask, z, pold, pnew, q, c->scal, c->partials, sym, c->wwin);                \
+    // a staged or patch variant chosen for an earlier mesh that this mesh cannot run:
+    // its twin with the same bits
+    if (!eolc::spmv_usable(c->topo, variant)) variant = eolc::spmv_fallback(variant);
+#define EOLC_SPMV_CASE(id, U, PF, NT, W, LAY, STG)                                      \
+    case id:                                                                       \
+        launch_spmv_one<T, U, PF, NT, W, LAY, STG>(c, k, G, A, Mm, z, pold, pnew, q); \
         break;
     switch (variant) {
         EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_CASE)
     default:
-        hipLaunchKernelGGL((k_pcg_spmv<T, diff, 1, false, false, 8, 0>), dim3(G), dim3(WG), 0, c->stream, k, t.N,
-                           t.nslices, c->slice_off, c->srow, c->slen, c->scol, A, Mm, t.nbs, c->fixmask, z, pold, pnew,
-                           q, c->scal, c->partials, sym, c->wwin);
+        launch_spmv_one<T, 1, false, false, 8, 0, false>(c, k, G, A, Mm, z, pold, pnew, q);
     }
 #undef EOLC_SPMV_CASE
 }
@@ -878,6 +1085,29 @@ int spmv_unstaged(int variant)
     default: return variant;
     }
 }
+
+#define EOLC_SPMV_PATCH_CASE(id, U, PF, NT, W, LAY, STG) \
+    case id:                                            \
+        return LAY == 5;
+bool spmv_patch(int variant)
+{
+    switch (variant) {
+        EOLC_SPMV_VARIANT_LIST(EOLC_SPMV_PATCH_CASE)
+    default:
+        return false;
+    }
+}
+
+bool spmv_usable(const Topology &t, int variant)
+{
+    if (spmv_staged(variant) && t.wmax > kStageVerts) return false;
+    if (spmv_patch(variant) && !t.pt_ok) return false;
+    return true;
+}
+
+// the staged variants' unstaged twins; the patch variants' symmetric-record twin (18:
+// the same sums in the same order, in both precisions)
+int spmv_fallback(int variant) { return spmv_patch(variant) ? 18 : spmv_unstaged(variant); }
 
 int fp32_layout(int variant)
 {

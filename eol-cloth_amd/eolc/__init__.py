@@ -37,13 +37,13 @@ EXPORTED_SYMBOLS = (
     "eolc_ml_info", "eolc_ml_aggregates", "eolc_precond_apply", "eolc_ml_partition", "eolc_set_batch_multilevel",
     "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
     "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol", "eolc_velocity_transfer_eol",
-    "eolc_collide_noalloc", "eolc_learned_set_multilevel", "eolc_set_mixed_stall",
+    "eolc_collide_noalloc", "eolc_learned_set_multilevel", "eolc_set_mixed_stall", "eolc_spmv_layout_info",
 )
 
 BATCH_EAGER = 0x1
 BATCH_VARIANTS = 3
 
-SPMV_VARIANTS = 25
+SPMV_VARIANTS = 29
 
 LEARNED_NPARAMS = 26246
 
@@ -109,6 +109,7 @@ def lib():
                                    C.POINTER(CollisionRec), C.c_int, C.POINTER(C.c_int)]
         L.eolc_collide_noalloc.argtypes = L.eolc_collide.argtypes
         L.eolc_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.eolc_spmv_layout_info.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
         L.eolc_set_batch_topology.argtypes = [vp, C.c_int] + [C.c_void_p] * 9
         L.eolc_set_batch_material.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.eolc_batch_step.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p]
@@ -388,6 +389,16 @@ class Context:
         ms = C.c_double(0)
         self._check(lib().eolc_time_spmv(self._h, int(precision), int(reps), C.byref(ms)))
         return ms.value
+
+    def spmv_layout_info(self):
+        """Patch layout sizes: applies, records read per product, lower entries,
+        storage positions, cross blocks, LDS slots of the largest patch, window
+        vertices over all patches, largest window."""
+        info = (C.c_int64 * 8)()
+        self._check(lib().eolc_spmv_layout_info(self._h, info, 8))
+        keys = ("applies", "records", "lower_entries", "positions", "cross_blocks", "lds_slots", "window_vertices",
+                "window_max")
+        return {k: int(v) for k, v in zip(keys, info)}
 
     def collide(self, threshold, pts=None, nrm=None, whd=None, E=None, eol_points=False,
                 box_npts=8, box_nedges=12, cap=1 << 16, noalloc=False):

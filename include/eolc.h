@@ -394,6 +394,15 @@ int eolc_set_batch_variant(eolc_ctx *ctx, int variant);
  * PCG SpMV kernel (precision 64 or 32) over `reps` back-to-back launches on the
  * current matrix.  For roofline reporting; leaves the solver state undefined. */
 int eolc_time_spmv(eolc_ctx *ctx, int precision, int reps, double *avg_ms);
+/* Sizes of the patch layout of the current topology, for the algorithmic bytes of
+ * the patch SpMV variants (no counterpart in the reference): info[0] = 1 when the
+ * layout applies, [1] records read per product (upper blocks plus lower blocks whose
+ * columns lie in another patch), [2] lower entries (2 bytes each), [3] storage
+ * positions (8 bytes of row index and counts each), [4] the cross blocks among [1],
+ * [5] LDS slots of the largest patch, [6] window vertices over all patches (each
+ * read once per product with its 4-byte index), [7] the largest window.  n: entries
+ * of info (at most 8 are written). */
+int eolc_spmv_layout_info(eolc_ctx *ctx, int64_t *info, int n);
 
 /* Average device time of one launch of a streaming copy kernel (16-byte loads and
  * stores) reading and writing `bytes` each, over `reps` launches: the known-size
@@ -412,13 +421,20 @@ int eolc_stream_read(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
  * every workgroup's rows and their columns span at most 5,440 consecutive vertices
  * (a locally numbered mesh, e.g. a row-major grid up to about 1,100 vertices wide);
  * otherwise the autotuner reports them as infinite and eolc_set_spmv_variant
- * refuses them.  All variants give bitwise-identical results within a precision,
- * except that variants 18 and 22 read the symmetric records in FP64 too and there
- * apply the operator in the difference form of the FP32 product (the same matrix,
- * rounded differently; the true residual is always formed from the assembled FP64
- * matrix).  ms (may be NULL) receives EOLC_SPMV_VARIANTS average times; *best (may
- * be NULL) the choice.  eolc_set_spmv_variant sets the choice directly. */
-#define EOLC_SPMV_VARIANTS 25
+ * refuses them.  Variants 25-28 read the patch layout: the vertices are cut into
+ * one compact patch per workgroup (coordinate bisection of X), and a block whose
+ * two vertices share a patch is read once, its transposed product passed to the
+ * other row through LDS; they apply when every patch needs at most 6,400 LDS slots
+ * (the same reporting and refusal otherwise).  All variants give bitwise-identical
+ * results within a precision, except that variants 18, 22 and 25-28 read the
+ * symmetric or patch records in FP64 too and there apply the operator in the
+ * difference form of the FP32 product (the same matrix, rounded differently; the
+ * true residual is always formed from the assembled FP64 matrix), and that the
+ * patch variants sum the dot product p.q per patch (other rounding of alpha; their
+ * products q are the same bits).  ms (may be NULL)
+ * receives EOLC_SPMV_VARIANTS average times; *best (may be NULL) the choice.
+ * eolc_set_spmv_variant sets the choice directly. */
+#define EOLC_SPMV_VARIANTS 29
 int eolc_autotune(eolc_ctx *ctx, int precision, int reps, int *best, double *ms);
 int eolc_set_spmv_variant(eolc_ctx *ctx, int precision, int variant);
 

@@ -144,16 +144,21 @@ def test_iteration_cap_reports_maxit(ctx):
 
 
 FP64_DIFF_VARIANTS = (18, 22)   # FP64 difference form on symmetric records
+PATCH_VARIANTS = (25, 26, 27, 28)   # patch layout (either precision; its dot products group rows by patch)
 
 
 def test_spmv_variants_bitwise_identical_and_autotune(ctx):
     """Every SpMV launch variant of one arithmetic family gives the same bits (same
     arithmetic, same order; the LDS-staged variants 19-24 form the same p_new values
-    in LDS that the others form per gather): all FP32 variants, the FP64 variants of
-    the assembled form, and the two FP64 variants of the difference form on
-    symmetric records (18, 22: other rounding), whose solve meets the tolerance and
-    agrees with the assembled form to the solver tolerance.  The autotuner times
-    them all and keeps one."""
+    in LDS that the others form per gather): all FP32 variants but the patch ones,
+    the FP64 variants of the assembled form, and the FP64 variants of the difference
+    form on symmetric records (18, 22: other rounding), whose solve meets the
+    tolerance and agrees with the assembled form to the solver tolerance.  The patch
+    variants 25-28 form every row's product from the same terms in the same order,
+    but each workgroup's partial sum of p.q covers its patch instead of 1,024
+    consecutive rows, so the dot products round differently: they agree with each
+    other bit for bit and with the others to the solver tolerance.  The autotuner
+    times them all and keeps one."""
     sc = scene.Scene(40, sigma_x=1e-3, sigma_v=0.05)
     for prec, mode in ((32, eolc.SOLVE_MIXED_BJ), (64, eolc.SOLVE_FP64_BJ)):
         out = []
@@ -164,17 +169,19 @@ def test_spmv_variants_bitwise_identical_and_autotune(ctx):
             rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-9, maxit=20000, mode=mode)
             assert rc == 0 and st.true_relres <= 1e-9
             out.append((ctx.download_state(), st.iters))
-        families = [out] if prec == 32 else [
-            [o for k, o in enumerate(out) if k not in FP64_DIFF_VARIANTS], [out[k] for k in FP64_DIFF_VARIANTS]]
+        special = (FP64_DIFF_VARIANTS + PATCH_VARIANTS) if prec == 64 else PATCH_VARIANTS
+        families = [[o for k, o in enumerate(out) if k not in special], [out[k] for k in PATCH_VARIANTS]]
+        if prec == 64:
+            families.append([out[k] for k in FP64_DIFF_VARIANTS])
         for family in families:
             for (x, v), it in family[1:]:
                 assert it == family[0][1]
                 assert np.array_equal(x, family[0][0][0]) and np.array_equal(v, family[0][0][1])
-        if prec == 64:
-            (x18, v18), it18 = out[18]
-            print(f"  FP64 difference form on symmetric records: {it18} iterations "
-                  f"(assembled form {out[0][1]}), v rel diff {rel_err(v18, out[0][0][1]):.2e}", flush=True)
-            assert rel_err(v18, out[0][0][1]) <= 1e-5
+        for k in (FP64_DIFF_VARIANTS + PATCH_VARIANTS) if prec == 64 else PATCH_VARIANTS:
+            (xk, vk), itk = out[k]
+            print(f"  variant {k}: {itk} iterations (variant 0: {out[0][1]}), v rel diff "
+                  f"{rel_err(vk, out[0][0][1]):.2e}", flush=True)
+            assert rel_err(vk, out[0][0][1]) <= 1e-5
         ctx.set_spmv_variant(prec, 0)
         setup(ctx, sc)
         ctx.forces_fill(GRAV, H)
