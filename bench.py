@@ -313,7 +313,8 @@ def spmv_kernel_name(fp32, variant):
     u, pf, nt, w, lay, stg = SPMV_VARIANTS[variant]
     b = lambda x: "true" if x else "false"
     if lay == 5:
-        return f"k_pcg_spmv_patch<{'float' if fp32 else 'double'}, {b(nt)}, {w}>"
+        t = "float" if fp32 else "double"
+        return f"k_pcg_spmv_patch<{t}, {t}, {b(nt)}, {w}>"
     t = "float, true" if fp32 else ("double, true" if lay == 4 else "double, false")
     return f"k_pcg_spmv<{t}, {u}, {b(pf)}, {b(nt)}, {w}, {lay if (fp32 or lay >= 3) else 0}, {b(stg)}>"
 
@@ -613,7 +614,7 @@ def main():
         full = spmv_bytes(sc.N, nb, s, True)
         if SPMV_VARIANTS[variant][4] == 5:
             alg = (10 * s * layout["records"] + 2 * layout["lower_entries"] + 8 * layout["positions"] +
-                   (4 * s * 3 + s + 1) * sc.N)
+                   4 * layout["window_vertices"] + (4 * s * 3 + s + 1) * sc.N)
         elif SPMV_VARIANTS[variant][4] in (1, 4):   # symmetric records: upper records + lower records
             alg = (10 * s) * (nb - sc.N) // 2 + 8 * (nb - sc.N) // 2 + 8 * sc.N + (4 * s * 3 + s + 1) * sc.N
         else:

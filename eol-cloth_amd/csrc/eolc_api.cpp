@@ -80,10 +80,11 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->S64u); dfree(c->M64);
     dfree(c->pt_row); dfree(c->pt_cnt); dfree(c->pt_roff); dfree(c->pt_soff); dfree(c->pt_loff); dfree(c->pt_lo);
     dfree(c->pt_e2r); dfree(c->pt_e2l); dfree(c->pt_woff); dfree(c->pt_win); dfree(c->P32); dfree(c->P64);
+    dfree(c->stiff); dfree(c->smask); dfree(c->sellbase);
     dfree(c->fixmask); dfree(c->vfix); dfree(c->fix_dofs); dfree(c->fix_vals);
     dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
     dfree(c->p64[0]); dfree(c->p64[1]);
-    dfree(c->r32); dfree(c->z32); dfree(c->q32); dfree(c->xl32);
+    dfree(c->r32); dfree(c->z32); dfree(c->q32); dfree(c->xl32); dfree(c->xl64);
     dfree(c->p32[0]); dfree(c->p32[1]);
     dfree(c->cedges); dfree(c->cd_unit);
     eolc::LearnedState &L = c->lp;
@@ -565,8 +566,9 @@ std::string build_patch_layout(Topology &t, const double *X)
                 }
             }
     });
+    // (a row's stiff blocks are flagged in a 64-bit mask)
     for (int i = 0; i < N; ++i)
-        if (nup[i] + ncl[i] > 0xffff || nlo[i] > 0xffff) return "a row has too many blocks";
+        if (nup[i] + nlo[i] > 64) return "a row has more than 64 off-diagonal blocks";
     // storage positions: within a patch, decreasing upper count, then vertex index
     t.pt_row.assign((size_t)G * eolc::kPcgWG, -1);
     t.pt_cnt.assign((size_t)G * eolc::kPcgWG, 0);
@@ -1021,6 +1023,11 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
         EOLC_HIP(upload(c, &c->pt_woff, T.pt_woff));
         EOLC_HIP(upload(c, &c->pt_win, T.pt_win));
         EOLC_HIP(dalloc(c, &c->P32, 10 * (size_t)std::max<int64_t>(T.pt_nrec, 1)));
+        EOLC_HIP(dalloc(c, &c->stiff, (size_t)T.nbs));
+        EOLC_HIP(dalloc(c, &c->smask, (size_t)N));
+        std::vector<int> sb(N);
+        for (int g = 0; g < N; ++g) sb[T.srow[g]] = T.slice_off[g / eolc::kSlice] + g % eolc::kSlice;
+        EOLC_HIP(upload(c, &c->sellbase, sb));
         EOLC_HIP(dalloc(c, &c->P64, 10 * (size_t)std::max<int64_t>(T.pt_nrec, 1)));
     }
     EOLC_HIP(upload(c, &c->cptr, T.cptr));
@@ -1057,7 +1064,7 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
     EOLC_HIP(dalloc(c, &c->fix_vals, n3));
     EOLC_HIP(hipMemset(c->fixmask, 0, ((size_t)N + 3) & ~(size_t)3));
     EOLC_HIP(hipMemset(c->vfix, 0, n3 * sizeof(double)));
-    for (double **p : {&c->b64, &c->xs64, &c->r64, &c->z64, &c->q64, &c->p64[0], &c->p64[1]})
+    for (double **p : {&c->b64, &c->xs64, &c->r64, &c->z64, &c->q64, &c->p64[0], &c->p64[1], &c->xl64})
         EOLC_HIP(dalloc(c, p, n3));
     for (float **p : {&c->r32, &c->z32, &c->q32, &c->xl32, &c->p32[0], &c->p32[1]}) EOLC_HIP(dalloc(c, p, n3));
     // proximity-query tables (createEdges order, jitter stream)
@@ -1141,9 +1148,9 @@ static int forces_fill_async(eolc_ctx *c, const double grav[3], double h, bool w
     EOLC_HIP(hipMemsetAsync(c->counter + 1, 0, 2 * sizeof(unsigned), c->stream));
     EOLC_HIP(eolc::launch_face_eval(c, h, grav));
     EOLC_HIP(eolc::launch_hinge_eval(c, h));
+    c->h_last = h;   // (the assembly's stiff threshold scales with h^2)
     EOLC_HIP(eolc::launch_assemble(c, (want32 ? eolc::fp32_layout(c->spmv_variant[0]) : 0) |
                                           eolc::fp64_layout(c->spmv_variant[1])));
-    c->h_last = h;
     c->have_forces = true;
     return EOLC_OK;
 }
@@ -2093,6 +2100,8 @@ try {
         return lo;
     };
     for (int f = 0; f < t.F; ++f) ef[f] = E[inst_of(c->topo_tri[3 * f])];
+    c->e_batch_max = 0.0;
+    for (int b = 0; b < S.B; ++b) c->e_batch_max = std::max(c->e_batch_max, E[b]);
     std::vector<int> hh(4 * (size_t)t.H);
     if (t.H) EOLC_HIP(hipMemcpy(hh.data(), c->hinge, sizeof(int) * 4 * t.H, hipMemcpyDeviceToHost));
     for (int h = 0; h < t.H; ++h) bh[h] = beta[inst_of(hh[4 * h])];
@@ -2384,6 +2393,18 @@ try {
     return EOLC_OK;
 } catch (...) {
     return eolc::report_exception(c, "set_mixed_stall");
+}
+
+int eolc_set_stiff_threshold(eolc_ctx *c, double rel)
+try {
+    if (!c) return EOLC_ERR_INVALID;
+    if (!(rel >= 0.0) || !std::isfinite(rel)) return fail(c, EOLC_ERR_INVALID, "set_stiff_threshold: need rel >= 0");
+    c->stiff_rel = rel;
+    c->s32_valid = 0;   // the FP32 patch records depend on it
+    drop_batch_graph(c);
+    return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_stiff_threshold");
 }
 
 int eolc_set_mixed_delta(eolc_ctx *c, double delta)

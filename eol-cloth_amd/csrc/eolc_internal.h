@@ -323,6 +323,9 @@ struct eolc_ctx {
     int *pt_row = nullptr, *pt_cnt = nullptr, *pt_roff = nullptr, *pt_soff = nullptr, *pt_loff = nullptr;
     unsigned short *pt_lo = nullptr;
     int *pt_e2r = nullptr, *pt_e2l = nullptr, *pt_woff = nullptr, *pt_win = nullptr;
+    unsigned char *stiff = nullptr;   // nbs: the SELL entry's block is applied in FP64
+    unsigned long long *smask = nullptr;   // N: bit k = the row's SELL entry k is stiff
+    int *sellbase = nullptr;          // N: SELL entry 0 of the row (entry k at + 64 k)
     float *P32 = nullptr;       // 10 x pt_nrec
     double *P64 = nullptr;      // 10 x pt_nrec
     double *M64 = nullptr;      // N row masses sum_j m_ij in FP64 (written by k_rhs)
@@ -337,7 +340,8 @@ struct eolc_ctx {
     // PCG work vectors
     double *b64 = nullptr, *xs64 = nullptr, *r64 = nullptr, *z64 = nullptr, *q64 = nullptr;
     double *p64[2] = {nullptr, nullptr};
-    float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *xl32 = nullptr;
+    float *r32 = nullptr, *z32 = nullptr, *q32 = nullptr, *xl32 = nullptr;   // xl32: the batched solve's
+    double *xl64 = nullptr;     // the mixed solve's FP64 correction since the last reliable update
     float *p32[2] = {nullptr, nullptr};
     double *partials = nullptr;   // 6 x kMaxPartials
     unsigned int *counter = nullptr;
@@ -357,6 +361,10 @@ struct eolc_ctx {
     // learned preconditioner
     eolc::LearnedState lp;
     int spmv_variant[2] = {0, 0};   // SpMV launch variant for FP32 [0] and FP64 [1] solves
+    // FP32 patch records: a block whose largest entry exceeds stiff_rel x dampingB h^2 E
+    // is applied from its FP64 values by both its rows (Topology::sellbase, stiff masks)
+    double stiff_rel = 1e3;
+    double e_batch_max = 0.0;       // largest per-instance Young's modulus of a batch
     double mixed_delta = 0.5;       // reliable-update trigger of the mixed solves
     double mixed_delta_ml = 0.2;    // the same with the multilevel levels (shorter cycles)
     double batch_delta = 0.1;       // the batched solve (its FP64 residuals cost more per iteration)

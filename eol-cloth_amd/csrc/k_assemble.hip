@@ -22,6 +22,8 @@
 //                (faces, then hinges when the bending force is included),
 //                rhs_i = sum_j m_ij v_j + h f_i, the right-hand side M v + h f of
 //                Cloth::solve (Cloth.cpp:345), and the FP32 row mass Mbar_i.
+#include <algorithm>
+
 #include "eolc_internal.h"
 
 namespace {
@@ -32,7 +34,7 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
            double *__restrict__ A64, int lay, float *__restrict__ S32, const int *__restrict__ e2u,
            const int *__restrict__ scol, float *__restrict__ S32u, float *__restrict__ S32a,
            double *__restrict__ S64u, const int *__restrict__ e2r, const int *__restrict__ e2l,
-           float *__restrict__ P32, double *__restrict__ P64,
+           float *__restrict__ P32, double *__restrict__ P64, double tau, unsigned char *__restrict__ stiff,
            double *__restrict__ Mval)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,6 +100,14 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
 #pragma unroll
             for (int c = 0; c < 9; ++c) rec[1 + c] = sv[c];
         }
+        if (lay & eolc::kLayPatch) {
+            // a block too stiff for FP32 (k_stiff_mask moves it to FP64 in both rows:
+            // (i, j) and (j, i) have the same largest entry, so both are flagged)
+            double mx = 0.0;
+#pragma unroll
+            for (int c = 0; c < 9; ++c) mx = fmax(mx, fabs(stf[c] + (c % 4 == 0 ? m : 0.0)));
+            stiff[b] = (!diag && tau > 0.0 && mx > tau) ? 1 : 0;
+        }
         if (pr >= 0 && (lay & eolc::kLayPatch64)) {
             double *rec = P64 + (size_t)10 * pr;
             rec[0] = __longlong_as_double((long long)e2l[b]);
@@ -106,6 +116,35 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
         }
     }
     Mval[b] = m;
+}
+
+// One thread per SELL storage position: the row's stiff-entry mask (a row holds at
+// most 64 off-diagonal blocks where the patch layout applies) and the FP32 patch records of its stiff entries
+// zeroed, so that the FP32 product skips them and the patch SpMV applies them from
+// the FP64 values (A64) in both rows.
+__global__ void __launch_bounds__(256)
+k_stiff_mask(int N, const int *__restrict__ slice_off, const int *__restrict__ srow, const int *__restrict__ slen,
+             const unsigned char *__restrict__ stiff, const int *__restrict__ e2r, float *__restrict__ P32,
+             unsigned long long *__restrict__ smask)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    const int base = slice_off[g / eolc::kSlice] + (g % eolc::kSlice);
+    const int n = min(slen[g] - 1, 64);   // the diagonal block is the row's last entry
+    unsigned long long m = 0;
+    for (int k = 0; k < n; ++k) {
+        const int e = base + k * eolc::kSlice;
+        if (!stiff[e]) continue;
+        m |= 1ull << k;
+        // the entry's record, if it has one (a lower block inside a patch has none: its
+        // upper twin's record is zeroed by that row)
+        const int r = e2r[e];
+        if (r < 0) continue;
+        float *rec = P32 + (size_t)10 * r;
+#pragma unroll
+        for (int c = 1; c < 10; ++c) rec[c] = 0.0f;
+    }
+    smask[srow[g]] = m;
 }
 
 // one thread per SELL storage position (row srow[g])
@@ -156,9 +195,16 @@ hipError_t launch_assemble(eolc_ctx *c, int lay)
     c->s32_valid = 0;
     if (nb == 0) return hipSuccess;
     if (!c->topo.pt_ok) lay &= ~(eolc::kLayPatch | eolc::kLayPatch64);
+    // stiff threshold of the FP32 patch records: stiff_rel x dampingB h^2 E (E the
+    // largest Young's modulus on the context)
+    const double emax = std::max(c->E, c->batch.B > 0 ? c->e_batch_max : 0.0);
+    const double tau = c->stiff_rel > 0.0 ? c->stiff_rel * c->dampB * c->h_last * c->h_last * emax : 0.0;
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
                        c->entry_is_diag, c->stageK, c->stageM, c->A64, lay, c->S32, c->e2u, c->scol, c->S32u,
-                       c->S32a, c->S64u, c->pt_e2r, c->pt_e2l, c->P32, c->P64, c->Mval);
+                       c->S32a, c->S64u, c->pt_e2r, c->pt_e2l, c->P32, c->P64, tau, c->stiff, c->Mval);
+    if (lay & eolc::kLayPatch)
+        hipLaunchKernelGGL(k_stiff_mask, dim3(blocks_for(c->topo.N)), dim3(256), 0, c->stream, c->topo.N,
+                           c->slice_off, c->srow, c->slen, c->stiff, c->pt_e2r, c->P32, c->smask);
     const hipError_t e = hipGetLastError();
     if (e == hipSuccess) c->s32_valid = lay;
     return e;

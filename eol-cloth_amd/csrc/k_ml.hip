@@ -413,11 +413,11 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
 // arrive accumulated in C_3.
 // FINE = false: the coarse corrections alone (z_i = C(a_i) + C_3), for the learned
 // blocks' training loss, whose fine term is the network's.
-template <class T, bool FINE = true>
+template <class T, bool FINE = true, class TD = T>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ anc3,
              const double *__restrict__ Dinv, const double *__restrict__ R, const double *__restrict__ C,
-             const T *__restrict__ D, const T *__restrict__ r, T *__restrict__ z,
+             const TD *__restrict__ D, const T *__restrict__ r, T *__restrict__ z,
              const unsigned char *__restrict__ fixmask, const SolverScalars *scal, double *partials, int check_done)
 {
     __shared__ double lds[2 * (WG / 64) + 2];
@@ -450,12 +450,16 @@ k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const
             apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
         }
         const T r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
-        T f0 = (T)0, f1 = (T)0, f2 = (T)0;
-        if constexpr (FINE) eolc::apply_sym3<T>(D + i, N, r0, r1, r2, f0, f1, f2);
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        if constexpr (FINE) {
+            TD f0, f1, f2;
+            eolc::apply_sym3<TD>(D + i, N, (TD)r0, (TD)r1, (TD)r2, f0, f1, f2);
+            g0 = (double)f0; g1 = (double)f1; g2 = (double)f2;
+        }
         const unsigned fm = fixmask[i];
-        const T z0 = (fm & 1u) ? (T)0 : (T)((double)f0 + c0);
-        const T z1 = (fm & 2u) ? (T)0 : (T)((double)f1 + c1);
-        const T z2 = (fm & 4u) ? (T)0 : (T)((double)f2 + c2);
+        const T z0 = (fm & 1u) ? (T)0 : (T)(g0 + c0);
+        const T z1 = (fm & 2u) ? (T)0 : (T)(g1 + c1);
+        const T z2 = (fm & 4u) ? (T)0 : (T)(g2 + c2);
         z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
         srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
     }

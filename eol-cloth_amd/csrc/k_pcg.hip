@@ -427,6 +427,12 @@ struct PatchLayout {
     const float *R32;
     const double *R64;
     int S;
+    // blocks too stiff for FP32 (their FP32 records are zero): the row's mask of
+    // stiff SELL entries, its first SELL entry, the SELL columns and the FP64 blocks
+    const unsigned long long *smask;
+    const int *sbase, *scol;
+    const double *A64;
+    int64_t nbs;
 };
 template <class T> __device__ __forceinline__ const T *patch_records(const PatchLayout &pl);
 template <> __device__ __forceinline__ const float *patch_records<float>(const PatchLayout &pl) { return pl.R32; }
@@ -434,13 +440,21 @@ template <> __device__ __forceinline__ const double *patch_records<double>(const
 
 constexpr int kPatchLo = 8;   // lower entries per row loaded before phase 1 (the rest in phase 2)
 
-template <class T, bool NT, int W>
+// T: vectors and products; TR: the stored records (the same type in every launch
+// variant; an FP32 record with FP64 vectors would need reliable updates like the
+// mixed solve, and measured no faster per FP64 solve).  With FP32 records, a block whose
+// entries exceed the context's stiff threshold is applied from its FP64 values by
+// both of its rows (its FP32 record is zero): rounded to FP32, such a block (a
+// collapsed element, up to 1e10 times the others) adds errors of 6e-8 of its size
+// in every direction, also in its own null directions where the rest of the
+// matrix is soft, and no FP32 iteration then converges.
+template <class T, class TR, bool NT, int W>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned char *__restrict__ fixmask,
                  const T *__restrict__ z, const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q,
                  SolverScalars *scal, double *partials)
 {
-    using R = SymRec<T>;
+    using R = SymRec<TR>;
     typedef typename R::v2 v2;
     __shared__ double lds[3 * (WG / 64) + 3];
     __shared__ T pool[3 * eolc::kPatchPool];
@@ -470,7 +484,7 @@ k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned
 #pragma unroll
         for (int t = 0; t < kPatchLo; ++t) le[t] = t < nlo ? lo[t * eolc::kSlice] : eolc::kPatchCross;
     }
-    const T *__restrict__ rb = patch_records<T>(pl) + (size_t)10 * (pl.roff[sl] + lane);
+    const TR *__restrict__ rb = patch_records<TR>(pl) + (size_t)10 * (pl.roff[sl] + lane);
     const int s0 = pl.soff[sl] + lane;
     const int w0 = pl.woff[lw], nf = 3 * (pl.woff[lw + 1] - w0);
     constexpr int kPer = (3 * eolc::kPatchWin + WG - 1) / WG;
@@ -534,15 +548,16 @@ k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned
             }
             const T *sj = su + 3 * R::col(r0);
             const T u0 = sj[0], u1 = sj[1], u2 = sj[2];
-            block_apply<T, true>(r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x, r4.y, u0, u1, u2, ui0, ui1, ui2, y0,
-                                 y1, y2);
+            const T a0 = (T)r0.y, a1 = (T)r1.x, a2 = (T)r1.y, a3 = (T)r2.x, a4 = (T)r2.y, a5 = (T)r3.x,
+                    a6 = (T)r3.y, a7 = (T)r4.x, a8 = (T)r4.y;
+            block_apply<T, true>(a0, a1, a2, a3, a4, a5, a6, a7, a8, u0, u1, u2, ui0, ui1, ui2, y0, y1, y2);
             // S'^T (p_j - p_i) for row j; stored negated: -(S'^T (p_j - p_i)) = S'^T (p_i - p_j).
             // Written for cross upper blocks too (their slots are never read).
             const T d0 = u0 - ui0, d1 = u1 - ui1, d2 = u2 - ui2;
             const int slot = s0 + kk * eolc::kSlice;
-            pool[slot] = -fma(r3.y, d2, fma(r2.x, d1, r0.y * d0));
-            pool[S + slot] = -fma(r4.x, d2, fma(r2.y, d1, r1.x * d0));
-            pool[2 * S + slot] = -fma(r4.y, d2, fma(r3.x, d1, r1.y * d0));
+            pool[slot] = -fma(a6, d2, fma(a3, d1, a0 * d0));
+            pool[S + slot] = -fma(a7, d2, fma(a4, d1, a1 * d0));
+            pool[2 * S + slot] = -fma(a8, d2, fma(a5, d1, a2 * d0));
         }
     }
     __syncthreads();
@@ -566,8 +581,28 @@ k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned
                 const v2 r0 = ldm<v2, NT>(rec), r1 = ldm<v2, NT>(rec + 1), r2 = ldm<v2, NT>(rec + 2),
                          r3 = ldm<v2, NT>(rec + 3), r4 = ldm<v2, NT>(rec + 4);
                 const T *sj = su + 3 * R::col(r0);
-                block_apply<T, true>(r0.y, r1.x, r1.y, r2.x, r2.y, r3.x, r3.y, r4.x, r4.y, sj[0], sj[1], sj[2], ui0,
-                                     ui1, ui2, y0, y1, y2);
+                block_apply<T, true>((T)r0.y, (T)r1.x, (T)r1.y, (T)r2.x, (T)r2.y, (T)r3.x, (T)r3.y, (T)r4.x,
+                                     (T)r4.y, sj[0], sj[1], sj[2], ui0, ui1, ui2, y0, y1, y2);
+            }
+        }
+        if constexpr (std::is_same<TR, float>::value) {
+            unsigned long long m = pl.smask[row];
+            if (m) {
+                const int base = pl.sbase[row];
+                const double di0 = (double)ui0, di1 = (double)ui1, di2 = (double)ui2;
+                while (m) {
+                    const int e = base + __builtin_ctzll(m) * eolc::kSlice;
+                    m &= m - 1;
+                    const int j = pl.scol[e];
+                    const double d0 = (double)fma(betaT, pold[3 * j], z[3 * j]) - di0;
+                    const double d1 = (double)fma(betaT, pold[3 * j + 1], z[3 * j + 1]) - di1;
+                    const double d2 = (double)fma(betaT, pold[3 * j + 2], z[3 * j + 2]) - di2;
+                    const double *a = pl.A64 + e;
+                    const int64_t n = pl.nbs;
+                    y0 += fma(a[2 * n], d2, fma(a[n], d1, a[0] * d0));
+                    y1 += fma(a[5 * n], d2, fma(a[4 * n], d1, a[3 * n] * d0));
+                    y2 += fma(a[8 * n], d2, fma(a[7 * n], d1, a[6 * n] * d0));
+                }
             }
         }
         if (fm & 1) y0 = 0;
@@ -629,8 +664,14 @@ k_pcg_update(int k, int N, const T *__restrict__ D, const T *__restrict__ pnew, 
                 pv[c] = pnew[3 * i + c]; qv[c] = q[3 * i + c]; rv[c] = r[3 * i + c]; xv[c] = x[3 * i + c];
             }
         }
+        // the correction accumulates in TX: FP64 also in the mixed modes, whose FP32
+        // rounding of x would break the constraints of very stiff blocks (errors of
+        // 6e-8 |x| times the block) at every reliable update
 #pragma unroll
-        for (int c = 0; c < 3; ++c) x[3 * i + c] = (TX)fma(alphaT, pv[c], (T)xv[c]);
+        for (int c = 0; c < 3; ++c) {
+            if constexpr (std::is_same<TX, T>::value) x[3 * i + c] = fma(alphaT, pv[c], xv[c]);
+            else x[3 * i + c] = fma((TX)alpha, (TX)pv[c], xv[c]);
+        }
         const T r0 = fma(-alphaT, qv[0], rv[0]);
         const T r1 = fma(-alphaT, qv[1], rv[1]);
         const T r2 = fma(-alphaT, qv[2], rv[2]);
@@ -825,12 +866,12 @@ __global__ void k_masked_copy(int N, const double *__restrict__ vin, const unsig
     xout[t] = ((fixmask[vtx] >> c) & 1) ? 0.0 : vin[t];
 }
 
-__global__ void k_accumulate(int n, double *__restrict__ xs, float *__restrict__ xl)
+__global__ void k_accumulate(int n, double *__restrict__ xs, double *__restrict__ xl)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    xs[t] += (double)xl[t];
-    xl[t] = 0.0f;
+    xs[t] += xl[t];
+    xl[t] = 0.0;
 }
 
 __global__ void k_write_solution(int N, const double *__restrict__ xs, const unsigned char *__restrict__ fixmask,
@@ -973,10 +1014,12 @@ void launch_spmv_one(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const T
     const eolc::Topology &t = c->topo;
     constexpr bool diff = std::is_same<T, float>::value;
     if constexpr (LAY == 5) {
-        const PatchLayout pl{c->pt_row,  c->pt_cnt, c->pt_roff, c->pt_soff, c->pt_loff, c->pt_woff,
-                             c->pt_win,  c->pt_lo,  c->P32,     c->P64,     t.pt_slots};
-        hipLaunchKernelGGL((k_pcg_spmv_patch<T, NT, W>), dim3(G), dim3(WG), 0, c->stream, k, pl, Mm, c->fixmask, z,
-                           pold, pnew, q, c->scal, c->partials);
+        using TR = T;
+        const PatchLayout pl{c->pt_row, c->pt_cnt, c->pt_roff, c->pt_soff, c->pt_loff, c->pt_woff, c->pt_win,
+                             c->pt_lo,  c->P32,    c->P64,     t.pt_slots, c->smask,   c->sellbase, c->scol,
+                             c->A64,    t.nbs};
+        hipLaunchKernelGGL((k_pcg_spmv_patch<T, TR, NT, W>), dim3(G), dim3(WG), 0, c->stream, k, pl, Mm, c->fixmask,
+                           z, pold, pnew, q, c->scal, c->partials);
     } else {
         const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u, c->S32a, c->S64u};
         hipLaunchKernelGGL((k_pcg_spmv<T, diff || LAY == 4, U, PF, NT, W, (diff || LAY >= 3) ? LAY : 0, STG>),
@@ -1004,21 +1047,21 @@ void launch_spmv(eolc_ctx *c, int variant, int k, int G, const T *A, const T *Mm
 #undef EOLC_SPMV_CASE
 }
 
-template <class T>
+template <class T, class TX>
 void launch_iteration(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const T *D, T *z, T *const p[2], T *q, T *r,
-                      T *x)
+                      TX *x)
 {
     const eolc::Topology &t = c->topo;
     const int pin = k & 1, pout = pin ^ 1;
     constexpr bool diff = std::is_same<T, float>::value;
     launch_spmv<T>(c, c->spmv_variant[diff ? 0 : 1], k, G, A, Mm, z, p[pin], p[pout], q);
     if (c->ml_on) {
-        hipLaunchKernelGGL((k_pcg_update<T, T, true>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r,
+        hipLaunchKernelGGL((k_pcg_update<T, TX, true>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r,
                            z, c->scal, c->partials);
         (void)eolc::ml_apply(c, diff, (k + 1) & 1, 1);   // launch errors surface at the chunk's hipGetLastError
     } else {
-        hipLaunchKernelGGL((k_pcg_update<T, T, false>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r,
-                           z, c->scal, c->partials);
+        hipLaunchKernelGGL((k_pcg_update<T, TX, false>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x,
+                           r, z, c->scal, c->partials);
     }
 }
 
@@ -1177,7 +1220,7 @@ hipError_t pcg_init(eolc_ctx *c, bool mixed)
                                   (size_t)3 * N * (mixed ? sizeof(float) : sizeof(double)), c->stream);
     if (e != hipSuccess) return e;
     if (mixed) {
-        e = hipMemsetAsync(c->xl32, 0, (size_t)3 * N * sizeof(float), c->stream);
+        e = hipMemsetAsync(c->xl64, 0, (size_t)3 * N * sizeof(double), c->stream);
         if (e != hipSuccess) return e;
     }
     e = hipGetLastError();
@@ -1190,9 +1233,9 @@ hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters)
     const int G = pcg_grid(c->topo.N);
     for (int k = k0; k < k0 + iters; ++k) {
         if (mixed)
-            launch_iteration<float>(c, k, G, c->S32, c->M32, c->D32, c->z32, c->p32, c->q32, c->r32, c->xl32);
+            launch_iteration<float, double>(c, k, G, c->S32, c->M32, c->D32, c->z32, c->p32, c->q32, c->r32, c->xl64);
         else
-            launch_iteration<double>(c, k, G, c->A64, c->M64, c->D64, c->z64, c->p64, c->q64, c->r64, c->xs64);
+            launch_iteration<double, double>(c, k, G, c->A64, c->M64, c->D64, c->z64, c->p64, c->q64, c->r64, c->xs64);
     }
     return hipGetLastError();
 }
@@ -1203,7 +1246,7 @@ hipError_t pcg_reliable_update(eolc_ctx *c)
 {
     const int N = c->topo.N;
     const int G = pcg_grid(N);
-    hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, 3 * N, c->xs64, c->xl32);
+    hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, 3 * N, c->xs64, c->xl64);
     launch_resid64(c, G, c->xs64, c->b64, c->r64);
     hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, c->partials + 5 * kMaxPartials,
                        &c->scal->rmax2);

@@ -38,6 +38,7 @@ EXPORTED_SYMBOLS = (
     "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
     "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol", "eolc_velocity_transfer_eol",
     "eolc_collide_noalloc", "eolc_learned_set_multilevel", "eolc_set_mixed_stall", "eolc_spmv_layout_info",
+    "eolc_set_stiff_threshold",
 )
 
 BATCH_EAGER = 0x1
@@ -110,6 +111,7 @@ def lib():
         L.eolc_collide_noalloc.argtypes = L.eolc_collide.argtypes
         L.eolc_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.eolc_spmv_layout_info.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
+        L.eolc_set_stiff_threshold.argtypes = [vp, C.c_double]
         L.eolc_set_batch_topology.argtypes = [vp, C.c_int] + [C.c_void_p] * 9
         L.eolc_set_batch_material.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.eolc_batch_step.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p]
@@ -427,6 +429,9 @@ class Context:
     def set_mixed_stall(self, first_cycle=10000, min_cycle=1000, sticky=8):
         """Stall limits of the mixed modes (FP64 continuation after an FP32 stall)."""
         return self._check(lib().eolc_set_mixed_stall(self._h, int(first_cycle), int(min_cycle), int(sticky)))
+
+    def set_stiff_threshold(self, rel):
+        self._check(lib().eolc_set_stiff_threshold(self._h, float(rel)))
 
     def set_mixed_delta(self, delta):
         return self._check(lib().eolc_set_mixed_delta(self._h, float(delta)))

@@ -72,6 +72,13 @@ typedef enum {
  * whenever the recursive residual has dropped by this factor since the last update.
  * A value set here applies to every mode. */
 int eolc_set_mixed_delta(eolc_ctx *ctx, double delta);
+/* Stiff-block threshold of the FP32 patch records (SpMV variants 25-30), relative to
+ * dampingB h^2 E (default 1e3; 0 turns it off).  A block whose largest entry exceeds
+ * it is applied from its FP64 values by both of its rows, because rounded to FP32 a
+ * block of a collapsed element (up to 1e10 times the others) perturbs the soft
+ * directions of the matrix by more than their own stiffness.  No counterpart in the
+ * reference (its solve is FP64 throughout). */
+int eolc_set_stiff_threshold(eolc_ctx *ctx, double rel);
 
 /* Stall limits of the mixed modes (no counterpart in the reference).  The FP32
  * iteration is taken to have stalled when a reliable-update cycle runs longer than
@@ -429,9 +436,10 @@ int eolc_stream_read(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
  * results within a precision, except that variants 18, 22 and 25-28 read the
  * symmetric or patch records in FP64 too and there apply the operator in the
  * difference form of the FP32 product (the same matrix, rounded differently; the
- * true residual is always formed from the assembled FP64 matrix), and that the
- * patch variants sum the dot product p.q per patch (other rounding of alpha; their
- * products q are the same bits).  ms (may be NULL)
+ * true residual is always formed from the assembled FP64 matrix), that the patch
+ * variants sum the dot product p.q per patch (other rounding of alpha; their
+ * products q are the same bits, except for blocks above the stiff threshold, see
+ * eolc_set_stiff_threshold).  ms (may be NULL)
  * receives EOLC_SPMV_VARIANTS average times; *best (may be NULL) the choice.
  * eolc_set_spmv_variant sets the choice directly. */
 #define EOLC_SPMV_VARIANTS 29
