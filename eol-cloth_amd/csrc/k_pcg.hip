@@ -622,6 +622,73 @@ k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned
 }
 
 // ---------------------------------------------------------------------------
+// The multilevel modes' update (x += alpha p, r -= alpha q, r.r) touches no 3x3
+// block, so it runs over the flat DOF arrays: workgroup lw owns DOFs
+// [lw c, lw c + c) with c = ceil(3N / G) rounded up to 4, and every thread handles 4
+// consecutive DOFs per trip with 16-byte loads (FP32) or two 16-byte loads (FP64).
+// The same arithmetic per DOF as k_pcg_update; r.r is summed per DOF range.
+template <class T, class TX>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_pcg_update_flat(int k, int n3, const T *__restrict__ pnew, const T *__restrict__ q, TX *__restrict__ x,
+                  T *__restrict__ r, SolverScalars *scal, double *partials)
+{
+    __shared__ double lds[2 * (WG / 64) + 2];
+    if (scal->done) return;
+    const int G = gridDim.x;
+    const int chunk = (((n3 + G - 1) / G) + 3) & ~3;
+    const int b = min(logical_wg() * chunk, n3), e = min(b + chunk, n3);
+    double pq[1], rz[1];
+    reduce_partials2<1, 1>(PA(partials), PB(partials, k & 1), pq, rz, lds);
+    if (!(pq[0] > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal->done = 1;
+            scal->reason = isfinite(pq[0]) ? 2 : 4;
+            scal->pq = pq[0];
+        }
+        return;
+    }
+    const double alpha = rz[0] / pq[0];
+    const T alphaT = (T)alpha;
+    double srr = 0.0;
+    for (int i = b + 4 * (int)threadIdx.x; i < e; i += 4 * WG) {
+        if (i + 3 < e) {
+            // i is a multiple of 4: aligned 4-element vector loads and stores
+            typedef T v4 __attribute__((ext_vector_type(4)));
+            typedef TX x4 __attribute__((ext_vector_type(4)));
+            const v4 pv = *reinterpret_cast<const v4 *>(pnew + i), qv = *reinterpret_cast<const v4 *>(q + i);
+            const v4 rv = *reinterpret_cast<const v4 *>(r + i);
+            const x4 xv = *reinterpret_cast<const x4 *>(x + i);
+            x4 xo;
+            v4 ro;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if constexpr (std::is_same<TX, T>::value) xo[u] = fma(alphaT, pv[u], xv[u]);
+                else xo[u] = fma((TX)alpha, (TX)pv[u], xv[u]);
+                ro[u] = fma(-alphaT, qv[u], rv[u]);
+                srr += (double)ro[u] * (double)ro[u];
+            }
+            *reinterpret_cast<x4 *>(x + i) = xo;
+            *reinterpret_cast<v4 *>(r + i) = ro;
+        } else {
+            for (int u = i; u < e; ++u) {
+                if constexpr (std::is_same<TX, T>::value) x[u] = fma(alphaT, pnew[u], x[u]);
+                else x[u] = fma((TX)alpha, (TX)pnew[u], x[u]);
+                const T rn = fma(-alphaT, q[u], r[u]);
+                r[u] = rn;
+                srr += (double)rn * (double)rn;
+            }
+        }
+    }
+    double v[1] = {srr};
+    write_partials<1>(v, PB(partials, (k + 1) & 1) + eolc::kMaxPartials, lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal->iters = k + 1;
+        scal->pq = pq[0];
+        scal->alpha = alpha;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // TX is the solution accumulator (double in FP64 mode, float in mixed mode).  ML:
 // the multilevel kernels form z and r.z afterwards (k_ml.hip); only r.r here.
 template <class T, class TX, bool ML>
@@ -1056,8 +1123,8 @@ void launch_iteration(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const 
     constexpr bool diff = std::is_same<T, float>::value;
     launch_spmv<T>(c, c->spmv_variant[diff ? 0 : 1], k, G, A, Mm, z, p[pin], p[pout], q);
     if (c->ml_on) {
-        hipLaunchKernelGGL((k_pcg_update<T, TX, true>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x, r,
-                           z, c->scal, c->partials);
+        hipLaunchKernelGGL((k_pcg_update_flat<T, TX>), dim3(G), dim3(WG), 0, c->stream, k, 3 * t.N, p[pout], q, x, r,
+                           c->scal, c->partials);
         (void)eolc::ml_apply(c, diff, (k + 1) & 1, 1);   // launch errors surface at the chunk's hipGetLastError
     } else {
         hipLaunchKernelGGL((k_pcg_update<T, TX, false>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x,

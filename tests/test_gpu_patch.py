@@ -1,0 +1,102 @@
+"""The patch-layout SpMV (variants 25-28) on meshes other than the benchmark grid.
+
+The patch layout cuts the vertices into compact patches by coordinate bisection
+of X, whatever their numbering, and passes in-patch transposed products through
+LDS.  These tests solve on an irregular mesh (diagonals flipped at random, faces
+removed, vertex numbering shuffled) with the patch variants and with the SELL
+variants, in both precisions, and require the same answer to the solver
+tolerance; and they drive the stiff-block path (blocks applied from their FP64
+values in both rows) from the host by lowering the threshold until most blocks
+qualify."""
+import numpy as np
+import pytest
+
+import eolc
+from conftest import GRAV, H, rel_err
+from eolc import scene
+
+pytestmark = pytest.mark.gpu
+
+
+def irregular_sheet(n=48, seed=3, drop=0.04):
+    """A grid with random diagonals and a few faces removed, the vertices renumbered
+    at random (unused vertices dropped), noise 1e-3 on x and 0.05 on v."""
+    rng = np.random.default_rng(seed)
+    X, _ = scene.grid(n)
+    np1 = n + 1
+    tris = []
+    for j in range(n):
+        for i in range(n):
+            v00 = j * np1 + i
+            v10, v01, v11 = v00 + 1, v00 + np1, v00 + np1 + 1
+            if rng.random() < 0.5:
+                q = [(v00, v10, v11), (v00, v11, v01)]
+            else:
+                q = [(v00, v10, v01), (v10, v11, v01)]
+            for t in q:
+                if rng.random() >= drop or (j == n - 1):   # keep the top row (fixed corners)
+                    tris.append(t)
+    tri = np.array(tris, dtype=np.int64)
+    used = np.unique(tri)
+    perm = rng.permutation(len(used))
+    new_of = np.full(X.shape[0], -1, dtype=np.int64)
+    new_of[used] = perm
+    Xn = np.zeros((len(used), 2))
+    Xn[perm] = X[used]
+    tri = new_of[tri].astype(np.int32)
+    hinge = scene.hinges(len(used), tri)
+    N = len(used)
+    x0 = np.zeros((N, 3))
+    x0[:, :2] = Xn
+    x0 = x0.reshape(-1) + scene.gaussian(11, 3 * N, 1e-3)
+    v0 = scene.gaussian(12, 3 * N, 0.05)
+    corners = np.array([new_of[n * np1], new_of[np1 * np1 - 1]], dtype=np.int32)
+    return Xn, tri, hinge, x0, v0, corners
+
+
+def solve(ctx, mesh, prec, variant, rel=None):
+    Xn, tri, hinge, x0, v0, fixed = mesh
+    ctx.set_topology(Xn, tri, hinge)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    if rel is not None:
+        ctx.set_stiff_threshold(rel)
+    ctx.set_spmv_variant(prec, variant)
+    ctx.upload_state(x0, v0)
+    mode = eolc.SOLVE_MIXED_BJ if prec == 32 else eolc.SOLVE_FP64_BJ
+    rc, st = ctx.step(GRAV, H, fixed, tol=1e-9, maxit=50000, mode=mode | eolc.SOLVE_MULTILEVEL)
+    assert rc == 0 and st.true_relres <= 1e-9, st.as_dict()
+    return ctx.download_state()[1], st
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_patch_variants_on_an_irregular_renumbered_mesh(ctx, prec):
+    mesh = irregular_sheet()
+    ctx.set_topology(mesh[0], mesh[1], mesh[2])
+    info = ctx.spmv_layout_info()
+    print(f"irregular mesh: {len(mesh[0])} vertices, {len(mesh[1])} faces, layout {info}", flush=True)
+    assert info["applies"] == 1 and 0 < info["cross_blocks"] < info["records"]
+    ref, st0 = solve(ctx, mesh, prec, 0)
+    out = {}
+    for var in (25, 26, 27, 28):
+        out[var], st = solve(ctx, mesh, prec, var)
+        print(f"  fp{prec} variant {var}: {st.iters} iterations (variant 0: {st0.iters}), "
+              f"v rel diff {rel_err(out[var], ref):.2e}", flush=True)
+        assert rel_err(out[var], ref) <= 1e-6
+    for var in (26, 27, 28):   # launch shapes of one layout: the same bits
+        assert np.array_equal(out[var], out[25])
+    ctx.set_spmv_variant(prec, 0)
+
+
+def test_stiff_blocks_applied_in_fp64(ctx):
+    """Thresholds that send none, some and nearly all blocks to the FP64 path give
+    the same mixed solve to the tolerance (the path runs in both rows of a block)."""
+    mesh = irregular_sheet(n=40, seed=5)
+    ref, _ = solve(ctx, mesh, 64, 0)
+    for rel in (0.0, 1e-1, 1e-4, 1e-8):
+        v, st = solve(ctx, mesh, 32, 25, rel=rel)
+        print(f"  stiff threshold {rel:g}: {st.iters} iterations, v rel diff vs FP64 {rel_err(v, ref):.2e}",
+              flush=True)
+        assert rel_err(v, ref) <= 1e-6
+    ctx.set_stiff_threshold(1e3)
+    ctx.set_spmv_variant(32, 0)
