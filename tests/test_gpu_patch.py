@@ -100,3 +100,44 @@ def test_stiff_blocks_applied_in_fp64(ctx):
         assert rel_err(v, ref) <= 1e-6
     ctx.set_stiff_threshold(1e3)
     ctx.set_spmv_variant(32, 0)
+
+
+def test_patch_layout_refused_falls_back(ctx):
+    """A fan of 72 triangles around one vertex gives that row more than 64
+    off-diagonal blocks: the patch layout does not apply, the patch variants are
+    refused by name and reported infinite by the autotuner, and a context that had
+    chosen one solves with its symmetric-record twin (variant 18, the same
+    products)."""
+    K = 72
+    ang = 2 * np.pi * np.arange(K) / K
+    X = np.concatenate([[[0.5, 0.5]], 0.5 + 0.4 * np.stack([np.cos(ang), np.sin(ang)], 1)])
+    tri = np.array([[0, 1 + k, 1 + (k + 1) % K] for k in range(K)], dtype=np.int32)
+    hinge = scene.hinges(len(X), tri)
+    x0 = np.zeros((len(X), 3))
+    x0[:, :2] = X
+    x0 = x0.reshape(-1) + scene.gaussian(21, 3 * len(X), 1e-3)
+    v0 = scene.gaussian(22, 3 * len(X), 0.05)
+    fixed = np.array([1, 1 + K // 2], dtype=np.int32)
+    # a patch variant chosen on a grid first
+    sc = scene.Scene(16)
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.set_spmv_variant(32, 25)
+    ctx.set_spmv_variant(64, 25)
+    ctx.set_topology(X, tri, hinge)
+    info = ctx.spmv_layout_info()
+    print(f"fan of {K} triangles: layout {info}", flush=True)
+    assert info["applies"] == 0
+    with pytest.raises(eolc.EolcError):
+        ctx.set_spmv_variant(32, 25)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    out = {}
+    for prec, mode in ((32, eolc.SOLVE_MIXED_BJ), (64, eolc.SOLVE_FP64_BJ)):
+        ctx.upload_state(x0, v0)
+        rc, st = ctx.step(GRAV, H, fixed, tol=1e-10, maxit=20000, mode=mode)
+        assert rc == 0 and st.true_relres <= 1e-10
+        out[prec] = ctx.download_state()[1]
+        ctx.upload_state(x0, v0)
+        ctx.forces_fill(GRAV, H)
+        best, ms = ctx.autotune(prec, reps=2)
+        assert all(np.isinf(ms[v]) for v in (25, 26, 27, 28)) and best not in (25, 26, 27, 28)
+    assert rel_err(out[32], out[64]) <= 1e-6
