@@ -89,7 +89,7 @@ def test_forward_matches_oracle(ctx, n):
     oq, _, _ = MO.forward(th.astype(np.float64), X, W, sc.N, quant=True)
     err = np.max(np.abs(o - oq)) / np.max(np.abs(oq))
     print(f"  max|o| {np.max(np.abs(oq)):.3e} rel err vs quantised oracle {err:.3e}", flush=True)
-    assert err < 1e-2
+    assert err < 6e-3   # bfloat16 activations on both sides; measured 2e-3 (20^2), 4e-3 (33^2)
     # the blocks built from the GPU outputs
     Po, _ = MO.precond_from_outputs(o.astype(np.float64), B, sc.N, fixmask)
     sP = np.max(np.abs(Po))
@@ -122,7 +122,7 @@ def test_loss_and_gradient_match_oracle(ctx):
         k += m
         rel = np.linalg.norm(gg - gr) / max(np.linalg.norm(gr), 1e-30)
         print(f"  grad {name:6s} |g| {np.linalg.norm(gr):.3e} rel err {rel:.3e}", flush=True)
-        assert rel < 3e-2, name
+        assert rel < 2e-3, name   # measured 2e-5 .. 2e-4
 
 
 def test_training_lowers_loss_and_is_deterministic(ctx):
