@@ -428,6 +428,7 @@ hipError_t launch_rhs(eolc_ctx *c, double h);
 hipError_t launch_update_x(eolc_ctx *c, double h);
 
 int pcg_grid(int N);
+int pcg_grid_phys(int G);   // physical workgroups of the row kernels with a second logical one
 hipError_t pcg_setup(eolc_ctx *c, bool mixed, int precond_kind);
 hipError_t pcg_init(eolc_ctx *c, bool mixed);
 hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters);

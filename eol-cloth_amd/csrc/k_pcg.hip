@@ -629,16 +629,14 @@ k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned
 // The same arithmetic per DOF as k_pcg_update; r.r is summed per DOF range.
 template <class T, class TX>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
-k_pcg_update_flat(int k, int n3, const T *__restrict__ pnew, const T *__restrict__ q, TX *__restrict__ x,
+k_pcg_update_flat(int k, int G, int n3, const T *__restrict__ pnew, const T *__restrict__ q, TX *__restrict__ x,
                   T *__restrict__ r, SolverScalars *scal, double *partials)
 {
     __shared__ double lds[2 * (WG / 64) + 2];
     if (scal->done) return;
-    const int G = gridDim.x;
     const int chunk = (((n3 + G - 1) / G) + 3) & ~3;
-    const int b = min(logical_wg() * chunk, n3), e = min(b + chunk, n3);
     double pq[1], rz[1];
-    reduce_partials2<1, 1>(PA(partials), PB(partials, k & 1), pq, rz, lds);
+    reduce_partials2<1, 1>(PA(partials), PB(partials, k & 1), pq, rz, lds, G);
     if (!(pq[0] > 0.0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             scal->done = 1;
@@ -649,38 +647,47 @@ k_pcg_update_flat(int k, int n3, const T *__restrict__ pnew, const T *__restrict
     }
     const double alpha = rz[0] / pq[0];
     const T alphaT = (T)alpha;
-    double srr = 0.0;
-    for (int i = b + 4 * (int)threadIdx.x; i < e; i += 4 * WG) {
-        if (i + 3 < e) {
-            // i is a multiple of 4: aligned 4-element vector loads and stores
-            typedef T v4 __attribute__((ext_vector_type(4)));
-            typedef TX x4 __attribute__((ext_vector_type(4)));
-            const v4 pv = *reinterpret_cast<const v4 *>(pnew + i), qv = *reinterpret_cast<const v4 *>(q + i);
-            const v4 rv = *reinterpret_cast<const v4 *>(r + i);
-            const x4 xv = *reinterpret_cast<const x4 *>(x + i);
-            x4 xo;
-            v4 ro;
+    // the G logical ranges may run on fewer workgroups: the first ones then take a
+    // second range
+    const int lw0 = logical_wg(), lwx = extra_logical_wg(G);
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        const int lw = pass == 0 ? lw0 : lwx;
+        if (lw < 0) break;   // uniform in the workgroup
+        const int b = min(lw * chunk, n3), e = min(b + chunk, n3);
+        double srr = 0.0;
+        for (int i = b + 4 * (int)threadIdx.x; i < e; i += 4 * WG) {
+            if (i + 3 < e) {
+                // i is a multiple of 4: aligned 4-element vector loads and stores
+                typedef T v4 __attribute__((ext_vector_type(4)));
+                typedef TX x4 __attribute__((ext_vector_type(4)));
+                const v4 pv = *reinterpret_cast<const v4 *>(pnew + i), qv = *reinterpret_cast<const v4 *>(q + i);
+                const v4 rv = *reinterpret_cast<const v4 *>(r + i);
+                const x4 xv = *reinterpret_cast<const x4 *>(x + i);
+                x4 xo;
+                v4 ro;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if constexpr (std::is_same<TX, T>::value) xo[u] = fma(alphaT, pv[u], xv[u]);
-                else xo[u] = fma((TX)alpha, (TX)pv[u], xv[u]);
-                ro[u] = fma(-alphaT, qv[u], rv[u]);
-                srr += (double)ro[u] * (double)ro[u];
-            }
-            *reinterpret_cast<x4 *>(x + i) = xo;
-            *reinterpret_cast<v4 *>(r + i) = ro;
-        } else {
-            for (int u = i; u < e; ++u) {
-                if constexpr (std::is_same<TX, T>::value) x[u] = fma(alphaT, pnew[u], x[u]);
-                else x[u] = fma((TX)alpha, (TX)pnew[u], x[u]);
-                const T rn = fma(-alphaT, q[u], r[u]);
-                r[u] = rn;
-                srr += (double)rn * (double)rn;
+                for (int u = 0; u < 4; ++u) {
+                    if constexpr (std::is_same<TX, T>::value) xo[u] = fma(alphaT, pv[u], xv[u]);
+                    else xo[u] = fma((TX)alpha, (TX)pv[u], xv[u]);
+                    ro[u] = fma(-alphaT, qv[u], rv[u]);
+                    srr += (double)ro[u] * (double)ro[u];
+                }
+                *reinterpret_cast<x4 *>(x + i) = xo;
+                *reinterpret_cast<v4 *>(r + i) = ro;
+            } else {
+                for (int u = i; u < e; ++u) {
+                    if constexpr (std::is_same<TX, T>::value) x[u] = fma(alphaT, pnew[u], x[u]);
+                    else x[u] = fma((TX)alpha, (TX)pnew[u], x[u]);
+                    const T rn = fma(-alphaT, q[u], r[u]);
+                    r[u] = rn;
+                    srr += (double)rn * (double)rn;
+                }
             }
         }
+        double v[1] = {srr};
+        write_partials<1>(v, PB(partials, (k + 1) & 1) + eolc::kMaxPartials, lds, lw);
     }
-    double v[1] = {srr};
-    write_partials<1>(v, PB(partials, (k + 1) & 1) + eolc::kMaxPartials, lds);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal->iters = k + 1;
         scal->pq = pq[0];
@@ -1123,8 +1130,8 @@ void launch_iteration(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const 
     constexpr bool diff = std::is_same<T, float>::value;
     launch_spmv<T>(c, c->spmv_variant[diff ? 0 : 1], k, G, A, Mm, z, p[pin], p[pout], q);
     if (c->ml_on) {
-        hipLaunchKernelGGL((k_pcg_update_flat<T, TX>), dim3(G), dim3(WG), 0, c->stream, k, 3 * t.N, p[pout], q, x, r,
-                           c->scal, c->partials);
+        hipLaunchKernelGGL((k_pcg_update_flat<T, TX>), dim3(eolc::pcg_grid_phys(G)), dim3(WG), 0, c->stream, k, G,
+                           3 * t.N, p[pout], q, x, r, c->scal, c->partials);
         (void)eolc::ml_apply(c, diff, (k + 1) & 1, 1);   // launch errors surface at the chunk's hipGetLastError
     } else {
         hipLaunchKernelGGL((k_pcg_update<T, TX, false>), dim3(G), dim3(WG), 0, c->stream, k, t.N, D, p[pout], q, x,
@@ -1236,6 +1243,17 @@ int fp64_layout(int variant)
     default:
         return 0;
     }
+}
+
+// Physical workgroups of the row kernels that can take a second logical workgroup
+// (the patch SpMV, the multilevel update and prolongation): when G exceeds a
+// multiple of 512 (two resident 1,024-thread workgroups on each of the 256 CUs; four
+// rounds of one for the FP64 patch SpMV) by at most 64, the excess rides on the first
+// workgroups, which start first, instead of running as a nearly empty last round.
+int pcg_grid_phys(int G)
+{
+    const int E = G % 512;
+    return (G > 512 && E > 0 && E <= 64) ? G - E : G;
 }
 
 int pcg_grid(int N)

@@ -28,6 +28,24 @@ __device__ __forceinline__ void wg_rows(int N, int *begin, int *end)
     *end = e < N ? e : N;
 }
 
+// A row kernel may run on fewer workgroups than the G logical ones (pcg_grid): the
+// first G - gridDim.x workgroups (the first dispatched, one per XCD in turn) then
+// also do logical workgroup gridDim.x + blockIdx.x.  Returns that index or -1.
+__device__ __forceinline__ int extra_logical_wg(int G)
+{
+    const int x = (int)gridDim.x + (int)blockIdx.x;
+    return x < G ? x : -1;
+}
+
+// rows of logical workgroup lw of G
+__device__ __forceinline__ void wg_rows_at(int N, int G, int lw, int *begin, int *end)
+{
+    int per = (N + G - 1) / G;
+    int b = lw * per, e = b + per;
+    *begin = b < N ? b : N;
+    *end = e < N ? e : N;
+}
+
 // slice owned by this wavefront (16 per workgroup)
 __device__ __forceinline__ int wave_slice() { return logical_wg() * (WG / 64) + (threadIdx.x >> 6); }
 
@@ -68,9 +86,10 @@ __device__ __forceinline__ void block_allsum(double (&v)[K], double *lds /* K * 
 // Fixed-order sum of K partial arrays (stride kMaxPartials) of length G, done by
 // every workgroup; the result is identical everywhere.
 template <int K>
-__device__ __forceinline__ void reduce_partials(const double *__restrict__ part, double (&out)[K], double *lds)
+__device__ __forceinline__ void reduce_partials(const double *__restrict__ part, double (&out)[K], double *lds,
+                                                int G = -1)
 {
-    const int G = gridDim.x;
+    if (G < 0) G = gridDim.x;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double s = 0.0;
@@ -85,9 +104,9 @@ __device__ __forceinline__ void reduce_partials(const double *__restrict__ part,
 // workgroup sum instead of two.  lds: (K1 + K2) * WG/64 + K1 + K2 doubles.
 template <int K1, int K2>
 __device__ __forceinline__ void reduce_partials2(const double *__restrict__ p1, const double *__restrict__ p2,
-                                                 double (&o1)[K1], double (&o2)[K2], double *lds)
+                                                 double (&o1)[K1], double (&o2)[K2], double *lds, int G = -1)
 {
-    const int G = gridDim.x;
+    if (G < 0) G = gridDim.x;
     double v[K1 + K2];
 #pragma unroll
     for (int k = 0; k < K1 + K2; ++k) v[k] = 0.0;
@@ -105,11 +124,11 @@ __device__ __forceinline__ void reduce_partials2(const double *__restrict__ p1, 
 }
 
 template <int K>
-__device__ __forceinline__ void write_partials(double (&v)[K], double *__restrict__ part, double *lds)
+__device__ __forceinline__ void write_partials(double (&v)[K], double *__restrict__ part, double *lds, int lw = -1)
 {
     block_allsum<K>(v, lds);
     if (threadIdx.x == 0) {
-        const int lw = logical_wg();
+        if (lw < 0) lw = logical_wg();
 #pragma unroll
         for (int k = 0; k < K; ++k) part[k * eolc::kMaxPartials + lw] = v[k];
     }

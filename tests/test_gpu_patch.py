@@ -141,3 +141,31 @@ def test_patch_layout_refused_falls_back(ctx):
         best, ms = ctx.autotune(prec, reps=2)
         assert all(np.isinf(ms[v]) for v in (25, 26, 27, 28)) and best not in (25, 26, 27, 28)
     assert rel_err(out[32], out[64]) <= 1e-6
+
+
+def test_patch_kernels_with_a_second_logical_workgroup(ctx):
+    """At 730x730 the row kernels have 528 logical workgroups: 512 run, and the first
+    16 also take logical workgroups 512-527 (a second patch of the SpMV, a second
+    row range of the update and the prolongation).  The mixed multilevel step with
+    the patch SpMV must meet the FP64 true residual (formed by the SELL kernel, so a
+    wrong row in the second pass would show) and agree with the SELL variant."""
+    n = 730
+    sc = scene.Scene(n, sigma_x=1e-3, sigma_v=0.05)
+    ctx.set_topology(sc.X, sc.tri, sc.hinge)
+    ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
+    ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    out = {}
+    for var in (0, 25):
+        ctx.set_spmv_variant(32, var)
+        ctx.set_spmv_variant(64, var)
+        for name, mode in (("mixed", eolc.SOLVE_MIXED_BJ), ("fp64", eolc.SOLVE_FP64_BJ)):
+            ctx.upload_state(sc.x0, sc.v0)
+            rc, st = ctx.step(GRAV, H, sc.fixed, tol=1e-8, maxit=60000, mode=mode | eolc.SOLVE_MULTILEVEL)
+            print(f"  {n}^2 {name} variant {var}: rc {rc} {st.iters} iterations, true residual {st.true_relres:.2e}",
+                  flush=True)
+            assert rc == 0 and st.true_relres <= 1e-8
+            out[(name, var)] = ctx.download_state()[1]
+    for name in ("mixed", "fp64"):
+        assert rel_err(out[(name, 25)], out[(name, 0)]) <= 1e-4
+    ctx.set_spmv_variant(32, 0)
+    ctx.set_spmv_variant(64, 0)
