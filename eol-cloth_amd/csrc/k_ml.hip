@@ -415,64 +415,56 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
 // blocks' training loss, whose fine term is the network's.
 template <class T, bool FINE = true, class TD = T>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
-k_ml_prolong(int parity, int N, int G, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ anc3,
+k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ anc3,
              const double *__restrict__ Dinv, const double *__restrict__ R, const double *__restrict__ C,
              const TD *__restrict__ D, const T *__restrict__ r, T *__restrict__ z,
              const unsigned char *__restrict__ fixmask, const SolverScalars *scal, double *partials, int check_done)
 {
     __shared__ double lds[2 * (WG / 64) + 2];
-    // the G logical row ranges may run on fewer workgroups: the first ones then take
-    // a second range
-    const int lw0 = logical_wg(), lwx = extra_logical_wg(G);
-#pragma unroll 1
-    for (int pass = 0; pass < 2; ++pass) {
-        const int lw = pass == 0 ? lw0 : lwx;
-        if (lw < 0) break;   // uniform in the workgroup
-        int rb, re;
-        wg_rows_at(N, G, lw, &rb, &re);
-        const int i0 = rb + (int)threadIdx.x;
-        // the first vertex's aggregate indices are read together with the stop flag, so
-        // both corrections below load in one round
-        int a = 0, u = 0;
-        if (i0 < re) {
-            a = agg1[i0];
-            if (lv.L >= 3) u = anc3[i0];
-        }
-        if (pass == 0 && check_done && scal->done) return;
-        double srz = 0.0;
-        for (int i = i0; i < re; i += WG) {
-            if (i != i0) {
-                a = agg1[i];
-                if (lv.L >= 3) u = anc3[i];
-            }
-            double c0, c1, c2;
-            if (lv.L >= 2) {
-                // levels 1 and 2, combined per level-1 aggregate by the restriction
-                c0 = C[3 * a]; c1 = C[3 * a + 1]; c2 = C[3 * a + 2];
-                if (lv.L >= 3) {
-                    const double *up = C + 3 * (int64_t)u;
-                    c0 += up[0]; c1 += up[1]; c2 += up[2];
-                }
-            } else {
-                apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
-            }
-            const T r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
-            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-            if constexpr (FINE) {
-                TD f0, f1, f2;
-                eolc::apply_sym3<TD>(D + i, N, (TD)r0, (TD)r1, (TD)r2, f0, f1, f2);
-                g0 = (double)f0; g1 = (double)f1; g2 = (double)f2;
-            }
-            const unsigned fm = fixmask[i];
-            const T z0 = (fm & 1u) ? (T)0 : (T)(g0 + c0);
-            const T z1 = (fm & 2u) ? (T)0 : (T)(g1 + c1);
-            const T z2 = (fm & 4u) ? (T)0 : (T)(g2 + c2);
-            z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
-            srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
-        }
-        double v[1] = {srz};
-        write_partials<1>(v, PB(partials, parity), lds, lw);
+    int rb, re;
+    wg_rows(N, &rb, &re);
+    const int i0 = rb + (int)threadIdx.x;
+    // the first vertex's aggregate indices are read together with the stop flag, so
+    // both corrections below load in one round
+    int a = 0, u = 0;
+    if (i0 < re) {
+        a = agg1[i0];
+        if (lv.L >= 3) u = anc3[i0];
     }
+    if (check_done && scal->done) return;
+    double srz = 0.0;
+    for (int i = i0; i < re; i += WG) {
+        if (i != i0) {
+            a = agg1[i];
+            if (lv.L >= 3) u = anc3[i];
+        }
+        double c0, c1, c2;
+        if (lv.L >= 2) {
+            // levels 1 and 2, combined per level-1 aggregate by the restriction
+            c0 = C[3 * a]; c1 = C[3 * a + 1]; c2 = C[3 * a + 2];
+            if (lv.L >= 3) {
+                const double *up = C + 3 * (int64_t)u;
+                c0 += up[0]; c1 += up[1]; c2 += up[2];
+            }
+        } else {
+            apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
+        }
+        const T r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        if constexpr (FINE) {
+            TD f0, f1, f2;
+            eolc::apply_sym3<TD>(D + i, N, (TD)r0, (TD)r1, (TD)r2, f0, f1, f2);
+            g0 = (double)f0; g1 = (double)f1; g2 = (double)f2;
+        }
+        const unsigned fm = fixmask[i];
+        const T z0 = (fm & 1u) ? (T)0 : (T)(g0 + c0);
+        const T z1 = (fm & 2u) ? (T)0 : (T)(g1 + c1);
+        const T z2 = (fm & 4u) ? (T)0 : (T)(g2 + c2);
+        z[3 * i] = z0; z[3 * i + 1] = z1; z[3 * i + 2] = z2;
+        srz += ((double)r0 * (double)z0 + (double)r1 * (double)z1) + (double)r2 * (double)z2;
+    }
+    double v[1] = {srz};
+    write_partials<1>(v, PB(partials, parity), lds);
 }
 
 inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
@@ -501,9 +493,6 @@ hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
     const MLLevels &lv = m.lv;
     const int N = c->topo.N;
     const int G = pcg_grid(N);
-    // one logical row range per workgroup: taking two on fewer workgroups
-    // (pcg_grid_phys) measured 24.6 against 23.4 us per FP64 prolongation at 1024^2
-    const int Gp = G;
     const int waves = lv.L >= 2 ? lv.n[2] : (lv.n[1] + 63) / 64;
     const int rblocks = blocks_for((int64_t)64 * waves, kRestrictWG);
     if (mixed) {
@@ -517,10 +506,10 @@ hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
         hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
                            c->scal, check_done);
     if (mixed) {
-        hipLaunchKernelGGL(k_ml_prolong<float>, dim3(Gp), dim3(WG), 0, c->stream, parity, N, G, lv, m.agg1, m.anc3,
-                           m.Dinv, m.R, m.C, c->D32, c->r32, c->z32, c->fixmask, c->scal, c->partials, check_done);
+        hipLaunchKernelGGL(k_ml_prolong<float>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.anc3, m.Dinv,
+                           m.R, m.C, c->D32, c->r32, c->z32, c->fixmask, c->scal, c->partials, check_done);
     } else {
-        hipLaunchKernelGGL(k_ml_prolong<double>, dim3(Gp), dim3(WG), 0, c->stream, parity, N, G, lv, m.agg1, m.anc3,
+        hipLaunchKernelGGL(k_ml_prolong<double>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.anc3,
                            m.Dinv, m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, check_done);
     }
     return hipGetLastError();
@@ -541,8 +530,8 @@ hipError_t ml_coarse64(eolc_ctx *c)
     if (lv.L >= 3)
         hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
                            c->scal, 0);
-    hipLaunchKernelGGL((k_ml_prolong<double, false>), dim3(G), dim3(WG), 0, c->stream, 0, N, G, lv, m.agg1, m.anc3,
-                       m.Dinv, m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, 0);
+    hipLaunchKernelGGL((k_ml_prolong<double, false>), dim3(G), dim3(WG), 0, c->stream, 0, N, lv, m.agg1, m.anc3, m.Dinv,
+                       m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, 0);
     return hipGetLastError();
 }
 
