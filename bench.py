@@ -305,8 +305,8 @@ SPMV_VARIANTS = [(1, False, False, 8, 0, False), (1, True, False, 8, 0, False), 
                  (1, False, False, 8, 3, False), (1, False, True, 8, 3, False), (1, False, True, 6, 3, False),
                  (1, False, False, 8, 4, False), (1, False, True, 8, 0, True), (1, False, True, 4, 0, True),
                  (1, False, False, 8, 1, True), (1, False, False, 4, 4, True), (1, False, True, 8, 3, True),
-                 (1, False, True, 4, 3, True), (1, False, False, 8, 5, False), (1, False, True, 8, 5, False),
-                 (1, False, False, 4, 5, False), (1, False, True, 4, 5, False)]
+                 (1, False, True, 4, 3, True), (1, False, False, 8, 5, False), (2, False, False, 8, 5, False),
+                 (1, False, False, 4, 5, False), (2, False, False, 4, 5, False)]
 
 
 def spmv_kernel_name(fp32, variant):
@@ -314,7 +314,7 @@ def spmv_kernel_name(fp32, variant):
     b = lambda x: "true" if x else "false"
     if lay == 5:
         t = "float" if fp32 else "double"
-        return f"k_pcg_spmv_patch<{t}, {t}, {b(nt)}, {w}>"
+        return f"k_pcg_spmv_patch<{t}, {t}, {b(nt)}, {w}, {u}>"
     t = "float, true" if fp32 else ("double, true" if lay == 4 else "double, false")
     return f"k_pcg_spmv<{t}, {u}, {b(pf)}, {b(nt)}, {w}, {lay if (fp32 or lay >= 3) else 0}, {b(stg)}>"
 

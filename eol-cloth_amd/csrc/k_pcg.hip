@@ -440,6 +440,8 @@ template <> __device__ __forceinline__ const double *patch_records<double>(const
 
 constexpr int kPatchLo = 8;   // lower entries per row loaded before phase 1 (the rest in phase 2)
 
+// PD: how many records of a row are in flight ahead of the one in use (2 fits the
+// FP64 kernel's 128 VGPRs at 4 waves per SIMD; in FP32 at 8 waves it spills).
 // T: vectors and products; TR: the stored records (the same type in every launch
 // variant; an FP32 record with FP64 vectors would need reliable updates like the
 // mixed solve, and measured no faster per FP64 solve).  With FP32 records, a block whose
@@ -448,7 +450,7 @@ constexpr int kPatchLo = 8;   // lower entries per row loaded before phase 1 (th
 // collapsed element, up to 1e10 times the others) adds errors of 6e-8 of its size
 // in every direction, also in its own null directions where the rest of the
 // matrix is soft, and no FP32 iteration then converges.
-template <class T, class TR, bool NT, int W>
+template <class T, class TR, bool NT, int W, int PD>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned char *__restrict__ fixmask,
                  const T *__restrict__ z, const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q,
@@ -533,18 +535,28 @@ k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned
     }
     double y0 = (double)(mb * ui0), y1 = (double)(mb * ui1), y2 = (double)(mb * ui2);
     if (active && nup > 0) {
-        // records streamed one ahead: the next record's loads are issued before the
-        // current record's LDS reads and products
-        const v2 *rec = reinterpret_cast<const v2 *>(rb);
-        v2 n0 = ldm<v2, NT>(rec), n1 = ldm<v2, NT>(rec + 1), n2 = ldm<v2, NT>(rec + 2), n3 = ldm<v2, NT>(rec + 3),
-           n4 = ldm<v2, NT>(rec + 4);
+        // records streamed PD ahead: record kk + PD's loads are issued before record
+        // kk's LDS reads and products
+        v2 nb[PD][5];
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            if (d == 0 || d < nup) {
+                const v2 *rn = reinterpret_cast<const v2 *>(rb + (size_t)640 * d);
+#pragma unroll
+                for (int h = 0; h < 5; ++h) nb[d][h] = ldm<v2, NT>(rn + h);
+            }
+        }
 #pragma unroll 1
         for (int kk = 0; kk < nup; ++kk) {
-            const v2 r0 = n0, r1 = n1, r2 = n2, r3 = n3, r4 = n4;
-            if (kk + 1 < nup) {
-                const v2 *rn = reinterpret_cast<const v2 *>(rb + (size_t)640 * (kk + 1));
-                n0 = ldm<v2, NT>(rn); n1 = ldm<v2, NT>(rn + 1); n2 = ldm<v2, NT>(rn + 2);
-                n3 = ldm<v2, NT>(rn + 3); n4 = ldm<v2, NT>(rn + 4);
+            const v2 r0 = nb[0][0], r1 = nb[0][1], r2 = nb[0][2], r3 = nb[0][3], r4 = nb[0][4];
+#pragma unroll
+            for (int d = 0; d + 1 < PD; ++d)
+#pragma unroll
+                for (int h = 0; h < 5; ++h) nb[d][h] = nb[d + 1][h];
+            if (kk + PD < nup) {
+                const v2 *rn = reinterpret_cast<const v2 *>(rb + (size_t)640 * (kk + PD));
+#pragma unroll
+                for (int h = 0; h < 5; ++h) nb[PD - 1][h] = ldm<v2, NT>(rn + h);
             }
             const T *sj = su + 3 * R::col(r0);
             const T u0 = sj[0], u1 = sj[1], u2 = sj[2];
@@ -1065,9 +1077,9 @@ __global__ void k_fill(int n, double v, double *__restrict__ out)
     X(23, 1, false, true, 8, 3, true) \
     X(24, 1, false, true, 4, 3, true) \
     X(25, 1, false, false, 8, 5, false) \
-    X(26, 1, false, true, 8, 5, false) \
+    X(26, 2, false, false, 8, 5, false) \
     X(27, 1, false, false, 4, 5, false) \
-    X(28, 1, false, true, 4, 5, false)
+    X(28, 2, false, false, 4, 5, false)
 
 // layout column of a variant -> the FP32 array it reads
 constexpr int lay_bit(int lay)
@@ -1092,7 +1104,7 @@ void launch_spmv_one(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const T
         const PatchLayout pl{c->pt_row, c->pt_cnt, c->pt_roff, c->pt_soff, c->pt_loff, c->pt_woff, c->pt_win,
                              c->pt_lo,  c->P32,    c->P64,     t.pt_slots, c->smask,   c->sellbase, c->scol,
                              c->A64,    t.nbs};
-        hipLaunchKernelGGL((k_pcg_spmv_patch<T, TR, NT, W>), dim3(G), dim3(WG), 0, c->stream, k, pl, Mm, c->fixmask,
+        hipLaunchKernelGGL((k_pcg_spmv_patch<T, TR, NT, W, U>), dim3(G), dim3(WG), 0, c->stream, k, pl, Mm, c->fixmask,
                            z, pold, pnew, q, c->scal, c->partials);
     } else {
         const SymLayout sym{c->supper, c->uoff, c->loff, c->lrec, c->S32u, c->S32a, c->S64u};

@@ -72,7 +72,7 @@ typedef enum {
  * whenever the recursive residual has dropped by this factor since the last update.
  * A value set here applies to every mode. */
 int eolc_set_mixed_delta(eolc_ctx *ctx, double delta);
-/* Stiff-block threshold of the FP32 patch records (SpMV variants 25-30), relative to
+/* Stiff-block threshold of the FP32 patch records (SpMV variants 25-28), relative to
  * dampingB h^2 E (default 1e3; 0 turns it off).  A block whose largest entry exceeds
  * it is applied from its FP64 values by both of its rows, because rounded to FP32 a
  * block of a collapsed element (up to 1e10 times the others) perturbs the soft
@@ -431,8 +431,9 @@ int eolc_stream_read(eolc_ctx *ctx, int64_t bytes, int reps, double *avg_ms);
  * refuses them.  Variants 25-28 read the patch layout: the vertices are cut into
  * one compact patch per workgroup (coordinate bisection of X), and a block whose
  * two vertices share a patch is read once, its transposed product passed to the
- * other row through LDS; they apply when every patch needs at most 6,400 LDS slots
- * (the same reporting and refusal otherwise).  All variants give bitwise-identical
+ * other row through LDS; they apply when every patch needs at most 6,750 LDS entries
+ * (the same reporting and refusal otherwise).  26 and 28 load each row's records
+ * two ahead instead of one (8 and 4 waves per SIMD).  All variants give bitwise-identical
  * results within a precision, except that variants 18, 22 and 25-28 read the
  * symmetric or patch records in FP64 too and there apply the operator in the
  * difference form of the FP32 product (the same matrix, rounded differently; the
