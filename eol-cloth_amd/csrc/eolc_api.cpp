@@ -111,6 +111,11 @@ void free_topology_buffers(eolc_ctx *c)
     eolc::MLState &ml = c->ml;
     dfree(ml.hord); dfree(ml.ptr1); dfree(ml.agg1); dfree(ml.gpos); dfree(ml.cptr); dfree(ml.par); dfree(ml.lam);
     dfree(ml.V); dfree(ml.Dinv); dfree(ml.R); dfree(ml.C); dfree(ml.anc3); dfree(ml.kptr2);
+    dfree(ml.geo); dfree(ml.gchunk); dfree(ml.gcp); dfree(ml.gpart); dfree(ml.D9); dfree(ml.feat2); dfree(ml.feat3);
+    dfree(ml.R9); dfree(ml.C9);
+    ml.lin_ok = ml.lin_on = ml.lin_built = false;
+    ml.hord_h.clear();
+    ml.nchunks = 0;
     ml.built = false;
     ml.lv = eolc::MLLevels{};
     ml.counts.clear();
@@ -771,6 +776,106 @@ int learned_buffers(eolc_ctx *c)
     return EOLC_OK;
 }
 
+// Geometry and set-up chunks of the linear coarse fields (levels >= kMLLinFrom): the
+// centroid, RMS radius and inverse radius of every aggregate in the material
+// coordinates, and per aggregate of those levels its hord range cut into chunks of
+// at most kGal9Rows positions.  A hierarchy whose aggregates are not contiguous in
+// hord keeps the translations alone (lin_ok false).
+hipError_t ml_prepare_linear(eolc_ctx *c, const std::vector<int> &hord)
+{
+    eolc::MLState &m = c->ml;
+    const eolc::MLLevels &lv = m.lv;
+    const int N = c->topo.N, L = lv.L;
+    std::vector<double> X((size_t)2 * N);
+    hipError_t e = hipMemcpy(X.data(), c->X, sizeof(double) * 2 * N, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    const size_t na = (size_t)lv.off[L + 1];
+    std::vector<double> geo(4 * na, 0.0), cnt(na, 0.0);
+    for (int l = 1; l <= L; ++l)
+        for (int i = 0; i < N; ++i) {
+            const size_t g = (size_t)lv.off[l] + m.maps[l][i];
+            geo[4 * g] += X[2 * (size_t)i];
+            geo[4 * g + 1] += X[2 * (size_t)i + 1];
+            cnt[g] += 1.0;
+        }
+    for (size_t g = 0; g < na; ++g) {
+        geo[4 * g] /= cnt[g];
+        geo[4 * g + 1] /= cnt[g];
+    }
+    for (int l = 1; l <= L; ++l)
+        for (int i = 0; i < N; ++i) {
+            const size_t g = (size_t)lv.off[l] + m.maps[l][i];
+            const double du = X[2 * (size_t)i] - geo[4 * g], dv = X[2 * (size_t)i + 1] - geo[4 * g + 1];
+            geo[4 * g + 2] += du * du + dv * dv;
+        }
+    for (size_t g = 0; g < na; ++g) {
+        double s = std::sqrt(geo[4 * g + 2] / cnt[g]);
+        if (!(s > 0.0)) s = 1.0;
+        geo[4 * g + 2] = s;
+        geo[4 * g + 3] = 1.0 / s;
+    }
+    // hord ranges of the aggregates of levels >= kMLLinFrom, and their chunks
+    const int l0 = eolc::kMLLinFrom;
+    const int n_hi = lv.off[L + 1] - lv.off[l0];
+    std::vector<int> k0(n_hi, INT_MAX), k1(n_hi, -1), seen(n_hi, 0);
+    bool contiguous = true;
+    for (int l = l0; l <= L && contiguous; ++l) {
+        int prev = -1;
+        for (int k = 0; k < N; ++k) {
+            const int A = m.maps[l][hord[k]];
+            const int h = lv.off[l] - lv.off[l0] + A;
+            if (A != prev) {
+                if (seen[h]) { contiguous = false; break; }
+                seen[h] = 1;
+                k0[h] = k;
+                prev = A;
+            }
+            k1[h] = k + 1;
+        }
+    }
+    std::vector<int> chunk, gcp(n_hi + 1, 0);
+    if (contiguous) {
+        for (int l = l0; l <= L; ++l)
+            for (int A = 0; A < lv.n[l]; ++A) {
+                const int h = lv.off[l] - lv.off[l0] + A;
+                gcp[h] = (int)chunk.size() / 4;
+                for (int k = k0[h]; k < k1[h]; k += eolc::kGal9Rows) {
+                    chunk.push_back(l);
+                    chunk.push_back(lv.off[l] + A);
+                    chunk.push_back(k);
+                    chunk.push_back(std::min(k + eolc::kGal9Rows, k1[h]));
+                }
+            }
+        gcp[n_hi] = (int)chunk.size() / 4;
+    }
+    if ((e = upload(c, &m.geo, geo)) != hipSuccess) return e;
+    m.lin_ok = contiguous && n_hi > 0;
+    if (!m.lin_ok) return hipSuccess;
+    m.nchunks = (int)chunk.size() / 4;
+    if ((e = upload(c, &m.gchunk, chunk)) != hipSuccess) return e;
+    if ((e = upload(c, &m.gcp, gcp)) != hipSuccess) return e;
+    if ((e = dalloc(c, &m.gpart, (size_t)45 * m.nchunks)) != hipSuccess) return e;
+    if ((e = dalloc(c, &m.D9, (size_t)54 * n_hi)) != hipSuccess) return e;
+    // per-vertex features of levels 2 (in hord order, for the restriction) and 3 (for
+    // the prolongation), formed as the kernels would form them from X and geo
+    std::vector<double> f2((size_t)2 * N), f3((size_t)2 * N);
+    for (int k = 0; k < N; ++k) {
+        const int i = hord[k];
+        const size_t g = (size_t)lv.off[2] + m.maps[2][i];
+        f2[2 * (size_t)k] = (X[2 * (size_t)i] - geo[4 * g]) * geo[4 * g + 3];
+        f2[2 * (size_t)k + 1] = (X[2 * (size_t)i + 1] - geo[4 * g + 1]) * geo[4 * g + 3];
+    }
+    for (int i = 0; i < N; ++i) {
+        const size_t g = (size_t)lv.off[3] + m.maps[3][i];
+        f3[2 * (size_t)i] = (X[2 * (size_t)i] - geo[4 * g]) * geo[4 * g + 3];
+        f3[2 * (size_t)i + 1] = (X[2 * (size_t)i + 1] - geo[4 * g + 1]) * geo[4 * g + 3];
+    }
+    if ((e = upload(c, &m.feat2, f2)) != hipSuccess) return e;
+    if ((e = upload(c, &m.feat3, f3)) != hipSuccess) return e;
+    if ((e = dalloc(c, &m.R9, (size_t)9 * n_hi)) != hipSuccess) return e;
+    return dalloc(c, &m.C9, (size_t)9 * n_hi);
+}
+
 // Aggregation hierarchy of the multilevel preconditioner for the current topology
 // (built once, on first use).
 int ml_prepare(eolc_ctx *c)
@@ -809,9 +914,25 @@ int ml_prepare(eolc_ctx *c)
             std::vector<int> anc3((size_t)t.N);
             for (int i = 0; i < t.N; ++i) anc3[i] = lv.off[3] + par[(size_t)lv.off[2] + par[agg1[i]]];
             EOLC_HIP(upload(c, &m.anc3, anc3));
+            m.hord_h = hord;   // for the linear coarse fields, built on first use
         }
     }
     m.built = true;
+    return EOLC_OK;
+}
+
+// Whether the multilevel levels >= 3 carry linear fields for iterations in this
+// precision (eolc_set_ml_linear).
+static bool ml_lin_for(const eolc_ctx *c, bool mixed) { return c->ml_linear == 2 || (c->ml_linear == 1 && mixed); }
+
+// The linear coarse fields' geometry, features and set-up chunks, built on the first
+// call that uses them for this hierarchy.
+static int ml_lin_prepare(eolc_ctx *c, bool want)
+{
+    eolc::MLState &m = c->ml;
+    if (!want || m.lin_built || !m.built || m.lv.L < eolc::kMLLinFrom) return EOLC_OK;
+    EOLC_HIP(ml_prepare_linear(c, m.hord_h));
+    m.lin_built = true;
     return EOLC_OK;
 }
 
@@ -1451,7 +1572,11 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     const int precond = (mode == EOLC_SOLVE_FP64_JACOBI) ? 1 : 0;
     EOLC_HIP(eolc::ensure_fp64_layout(c, c->spmv_variant[1]));   // FP64 iterations, or a stall's continuation
     if (mixed) EOLC_HIP(eolc::ensure_fp32_layout(c, c->spmv_variant[0]));
-    EOLC_HIP(eolc::pcg_setup(c, mixed, precond));
+    if (multilevel) {
+        const int rc = ml_lin_prepare(c, ml_lin_for(c, mixed));
+        if (rc) return rc;
+    }
+    EOLC_HIP(eolc::pcg_setup(c, mixed, precond, ml_lin_for(c, mixed)));
     // Learned mode: P replaces the FP32 block-Jacobi blocks; after twice the
     // iterations of the last converged block-Jacobi solve on this context (maxit
     // when there was none) the solve falls back to block-Jacobi from its current
@@ -1460,7 +1585,8 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     int cap = maxit;
     if (learned) {
         EOLC_HIP(eolc::learned_precond(c));
-        if (bj_hint > 0) cap = std::min(maxit, 2 * bj_hint);
+        // a zero output layer gives exactly the block-Jacobi blocks: no cap, no fallback
+        if (bj_hint > 0 && !c->lp.zero_output) cap = std::min(maxit, 2 * bj_hint);
     }
     const double delta = mixed ? (c->ml_on ? c->mixed_delta_ml : c->mixed_delta) : 0.0;
     EOLC_HIP(eolc::scal_set(c, tol * tol, delta * delta, 1));
@@ -1479,11 +1605,19 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
     // FP32 stall detection: an inner cycle far longer than the completed ones, or the
     // true residual hardly falling over consecutive reliable updates, means the FP32
     // operator no longer resolves the system (extreme stiffness ratios after the sheet
-    // deforms); the solve then continues in FP64 from its current iterate.
+    // deforms); the solve then continues in FP64 from its current iterate (from its
+    // initial guess when the FP32 iterations used linear coarse fields), with the
+    // translations alone on the multilevel levels (eolc_set_ml_linear).
     double last_true2 = -1.0;
     int slow_updates = 0;
     auto fp64_continue = [&]() -> int {
-        EOLC_HIP(eolc::pcg_reliable_update(c));   // x64 += the FP32 correction; r = b - A x64
+        // with the translations alone the FP64 iteration continues from the stalled
+        // iterate (x64 += the FP32 correction; r = b - A x64); after FP32 iterations
+        // with linear coarse fields it restarts from the initial guess
+        if (c->ml_on && c->ml.lin_on)
+            EOLC_HIP(eolc::pcg_restart_from_guess(c));
+        else
+            EOLC_HIP(eolc::pcg_reliable_update(c));
         mixed = false;
         learned = false;
         used_fallback |= 2;
@@ -1491,7 +1625,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
         const size_t n3 = 3 * (size_t)c->topo.N;
         EOLC_HIP(hipMemsetAsync(c->p64[0], 0, n3 * sizeof(double), c->stream));
         EOLC_HIP(hipMemsetAsync(c->p64[1], 0, n3 * sizeof(double), c->stream));
-        EOLC_HIP(eolc::pcg_setup(c, false, 0));
+        EOLC_HIP(eolc::pcg_setup(c, false, 0, ml_lin_for(c, false)));
         EOLC_HIP(eolc::scal_delta(c, 0.0));
         EOLC_HIP(eolc::pcg_restart_inner(c, false, k));
         EOLC_HIP(eolc::scal_restart(c, k));
@@ -1505,7 +1639,7 @@ static int solve_impl(eolc_ctx *c, int n_fixed, const int *fixed_dof, const doub
             // fallback: block-Jacobi blocks, true residual of the current iterate, beta = 0
             learned = false;
             used_fallback = 1;
-            EOLC_HIP(eolc::pcg_setup(c, mixed, 0));
+            EOLC_HIP(eolc::pcg_setup(c, mixed, 0, ml_lin_for(c, mixed)));
             EOLC_HIP(eolc::pcg_reliable_update(c));
             EOLC_HIP(eolc::pcg_restart_inner(c, mixed, k));
             EOLC_HIP(eolc::scal_restart(c, k));
@@ -1744,7 +1878,8 @@ try {
     MlScope mls(c, multilevel);
     std::vector<double> rm(r, r + n3);
     for (int i = 0; i < n_fixed; ++i) rm[(size_t)fixed_dof[i]] = 0.0;
-    EOLC_HIP(eolc::pcg_setup(c, mixed, base == EOLC_SOLVE_FP64_JACOBI ? 1 : 0));
+    if (multilevel && (rc = ml_lin_prepare(c, ml_lin_for(c, mixed)))) return rc;
+    EOLC_HIP(eolc::pcg_setup(c, mixed, base == EOLC_SOLVE_FP64_JACOBI ? 1 : 0, ml_lin_for(c, mixed)));
     if (base == EOLC_SOLVE_MIXED_LEARNED) EOLC_HIP(eolc::learned_precond(c));
     EOLC_HIP(hipMemcpyAsync(c->r64, rm.data(), n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     EOLC_HIP(eolc::pcg_precondition(c, mixed));
@@ -1856,6 +1991,8 @@ try {
     EOLC_HIP(hipStreamSynchronize(c->stream));
     L.adam_t = 0;
     L.have_weights = true;
+    L.zero_output = true;
+    for (int k = eolc::kLpOffOut; k < n; ++k) L.zero_output = L.zero_output && theta[k] == 0.0f;
     return EOLC_OK;
 } catch (...) {
     return eolc::report_exception(c, "learned_set_weights");
@@ -1929,7 +2066,9 @@ try {
 // current fixed set) and the Galerkin blocks of every level for the current MDK.
 static int learned_ml_setup(eolc_ctx *c)
 {
-    EOLC_HIP(eolc::pcg_setup(c, false, 0));
+    const int rc = ml_lin_prepare(c, ml_lin_for(c, true));
+    if (rc) return rc;
+    EOLC_HIP(eolc::pcg_setup(c, false, 0, ml_lin_for(c, true)));   // the coarse levels of the learned mixed solve
     return EOLC_OK;
 }
 
@@ -1974,6 +2113,7 @@ try {
         EOLC_HIP(eolc::learned_loss_grad(c));
         if (losses) EOLC_HIP(hipMemcpyAsync(&losses[s], c->lp.scal + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         EOLC_HIP(eolc::learned_adam(c, (float)lr));
+        c->lp.zero_output = false;
     }
     EOLC_HIP(hipStreamSynchronize(c->stream));
     return EOLC_OK;
@@ -2379,6 +2519,16 @@ try {
     return EOLC_OK;
 } catch (...) {
     return eolc::report_exception(c, "spmv_layout_info");
+}
+
+int eolc_set_ml_linear(eolc_ctx *c, int where)
+try {
+    if (!c) return EOLC_ERR_INVALID;
+    if (where < 0 || where > 2) return fail(c, EOLC_ERR_INVALID, "set_ml_linear: 0, 1 or 2");
+    c->ml_linear = where;
+    return EOLC_OK;
+} catch (...) {
+    return eolc::report_exception(c, "set_ml_linear");
 }
 
 int eolc_set_mixed_stall(eolc_ctx *c, int first_cycle, int min_cycle, int sticky)

@@ -219,6 +219,8 @@ struct BatchState {
 // of level-(l-1) ids.  Per-aggregate arrays of all levels are concatenated: level l
 // starts at aggregate `off[l]`.
 constexpr int kMLMaxLevels = 8;
+constexpr int kMLLinFrom = 3;     // first level with linear coarse fields
+constexpr int kGal9Rows = 2048;   // hord positions per set-up chunk of a 9 x 9 Galerkin block
 struct MLLevels {
     int L;                        // coarse levels built (0: none)
     int n[kMLMaxLevels + 1];      // aggregates per level (n[0] = vertices)
@@ -244,6 +246,26 @@ struct MLState {
                                   // level-1 aggregate a (L >= 2): D_1^-1 R_1(a) + D_2^-1 R_2(parent)
     int *anc3 = nullptr;          // N (L >= 3): global index of each vertex's level-3 aggregate
     int *kptr2 = nullptr;         // n_2 + 1 (L >= 2): member range in hord of each level-2 aggregate
+    // Linear coarse fields on the levels >= 3 (single-cloth solves; the batched solve
+    // keeps the translations): per aggregate of every level its centroid in the
+    // material coordinates, RMS radius s and 1/s; the Galerkin blocks on the basis
+    // phi = (1, (X - c) / s) of each component (9 x 9, upper triangle packed, 45
+    // values) are summed per chunk of at most kGal9Rows hord positions of one
+    // aggregate, then per aggregate, and factored (ml_setup).
+    bool lin_built = false;       // the data below built for this hierarchy (on first use)
+    std::vector<int> hord_h;      // host copy of hord
+    bool lin_ok = false;          // hierarchy has levels >= 3 with contiguous hord ranges
+    bool lin_on = false;          // the last ml_setup formed the 9 x 9 blocks
+    double *geo = nullptr;        // 4 per aggregate (all levels): c_u, c_v, s, 1/s
+    int *gchunk = nullptr;        // 4 per chunk: level, global aggregate, first and end hord position
+    int *gcp = nullptr;           // n_3 + ... + n_L + 1: chunk range of each aggregate of levels >= 3
+    int nchunks = 0;
+    double *gpart = nullptr;      // 45 per chunk
+    double *D9 = nullptr;         // 54 per aggregate of levels >= 3: factors of the Galerkin block
+    double *feat2 = nullptr;      // 2 per hord position: (phi_1, phi_2) of the vertex on its level-2 aggregate
+    double *feat3 = nullptr;      // 2 per vertex: (phi_1, phi_2) on its level-3 aggregate
+    double *R9 = nullptr;         // 9 per aggregate of levels >= 3: restricted sums (level 3 from the restriction)
+    double *C9 = nullptr;         // 9 per aggregate of levels >= 3: accumulated coarse field
     std::vector<int> counts;      // host copy of n[0..L]
     std::vector<std::vector<int>> maps;   // host: level-l aggregate of every vertex (l = 1..L)
 };
@@ -267,6 +289,7 @@ constexpr int kLpGradWG = 256;   // maximum workgroups of the weight-gradient ke
 
 struct LearnedState {
     bool have_weights = false;
+    bool zero_output = false;   // output layer all zero: the blocks are the block-Jacobi ones
     bool with_ml = false;        // the training loss adds the multilevel coarse corrections to z
     int N = 0;                   // vertex count the buffers were sized for
     int64_t nbs = 0;
@@ -367,6 +390,7 @@ struct eolc_ctx {
     double e_batch_max = 0.0;       // largest per-instance Young's modulus of a batch
     double mixed_delta = 0.5;       // reliable-update trigger of the mixed solves
     double mixed_delta_ml = 0.2;    // the same with the multilevel levels (shorter cycles)
+    int ml_linear = 0;              // linear coarse fields: 0 never, 1 in FP32 iterations, 2 always
     double batch_delta = 0.1;       // the batched solve (its FP64 residuals cost more per iteration)
     // per-element material (batched cloths): Young's modulus per face, bending
     // stiffness per hinge; null = the context's scalar material
@@ -429,10 +453,11 @@ hipError_t launch_update_x(eolc_ctx *c, double h);
 
 int pcg_grid(int N);
 int pcg_grid_phys(int G);   // physical workgroups of the row kernels with a second logical one
-hipError_t pcg_setup(eolc_ctx *c, bool mixed, int precond_kind);
+hipError_t pcg_setup(eolc_ctx *c, bool mixed, int precond_kind, bool ml_linear = true);
 hipError_t pcg_init(eolc_ctx *c, bool mixed);
 hipError_t pcg_chunk(eolc_ctx *c, bool mixed, int k0, int iters);
 hipError_t pcg_reliable_update(eolc_ctx *c);
+hipError_t pcg_restart_from_guess(eolc_ctx *c);   // a stalled mixed solve: FP64 from the initial guess
 hipError_t pcg_restart_inner(eolc_ctx *c, bool mixed, int k);
 hipError_t pcg_precondition(eolc_ctx *c, bool mixed);   // z = P r for r in r64 (parity 0)
 hipError_t pcg_replace_residual64(eolc_ctx *c);         // FP64 mode: r = b - A x, ||r||^2 -> rmax2
@@ -471,7 +496,7 @@ hipError_t scal_restart(eolc_ctx *c, int k);           // iteration k starts wit
 std::string ml_build(const Topology &t, std::vector<std::vector<int>> &agg_levels, std::vector<int> &hord,
                      std::vector<int> &ptr1, std::vector<int> &agg1, std::vector<int> &cptr, std::vector<int> &par,
                      std::vector<unsigned char> &lam, MLLevels &lv);
-hipError_t ml_setup(eolc_ctx *c);                             // Galerkin diagonal blocks of every level
+hipError_t ml_setup(eolc_ctx *c, bool linear = true);         // Galerkin diagonal blocks of every level
 hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done);   // z = D0 r + coarse, r.z partials
 hipError_t ml_coarse64(eolc_ctx *c);                          // z64 = coarse corrections of r64 (no fine term)
 // velocity transfer after a remesh (k_transfer.hip); host arrays in and out

@@ -521,8 +521,8 @@ hipError_t batch_solve_async(eolc_ctx *c, double tol, int maxit, int variant)
         hipLaunchKernelGGL(k_batch_vfix, dim3(blocks_for(3 * nu)), dim3(256), 0, c->stream, nu, S.d_user_fixed, S.d_scripted,
                            c->v, c->vfix);
     const bool ml = S.ml && S.d_ml_lo && c->ml.lv.L > 0;
-    c->ml_on = ml;   // pcg_setup then also forms the coarse levels' blocks
-    hipError_t e = pcg_setup(c, true, 0);
+    c->ml_on = ml;   // pcg_setup then also forms the coarse levels' blocks (translations only)
+    hipError_t e = pcg_setup(c, true, 0, false);
     c->ml_on = false;
     if (e != hipSuccess) return e;
     BatchArgs a;

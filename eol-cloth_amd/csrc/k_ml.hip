@@ -27,6 +27,20 @@
 //   k_ml_prolong  : z_i = D0 r_i + C(a_i) + C_3(level-3 aggregate of i); r.z partials
 // All coarse arithmetic is FP64 in a fixed order, so the result does not depend on
 // the launch shape.
+//
+// Linear coarse fields (single-cloth solves, levels >= kMLLinFrom = 3): there Z_l
+// carries, per aggregate A and component, the basis phi = (1, (X - c_A) / s_A) of
+// the material coordinates X (c_A the centroid, s_A the RMS radius of A's members),
+// so a coarse correction is a field linear on each aggregate instead of a constant,
+// and D_l is the 9 x 9 diagonal block of Z_l^T A Z_l, applied through the LDL^T
+// factors of its unit-diagonal scaling, which leave out a basis function whose
+// scaled pivot falls to 1e-10 (fact9_psd).  The smooth bending modes of a sheet are linear to
+// first order over an aggregate; at 512^2 this cut the PCG iterations by a quarter
+// (826 -> 617 at step 0, 890 -> 696 at step 6 of the benchmark trajectory).  The
+// restriction forms the first moments sum (phi_1, phi_2) r of each level-2 aggregate,
+// from which the coarse kernel gets every level >= 3 by the exact change of
+// centroid and radius; the prolongation evaluates the level-3 field at each vertex.
+// The batched solve keeps the translations on every level.
 #include "eolc_internal.h"
 #include "pcg_device.h"
 
@@ -150,6 +164,156 @@ __global__ void __launch_bounds__(256) k_ml_diagup(int l, MLLevels lv, const int
     inv_spd3(d, Dinv + 6 * (int64_t)(lv.off[l] + A));
 }
 
+// Upper triangle of a symmetric 9 x 9 block, row by row: 45 entries.
+__device__ __forceinline__ constexpr int pk9(int p, int q) { return p * 9 - p * (p - 1) / 2 + (q - p); }
+// Strictly lower triangle of a 9 x 9 factor, row by row: 36 entries.
+__device__ __forceinline__ constexpr int lo9(int i, int j) { return i * (i - 1) / 2 + j; }
+
+constexpr double kPivotTol = 1e-10;
+constexpr int kFact9 = 54;   // per aggregate: 36 factor entries, 9 inverse pivots, 9 scales
+
+// Factorisation of a symmetric positive semi-definite 9 x 9 block G (packed) for
+// applying its inverse: with the scales s_p = G_pp^-1/2 the scaled block s G s has
+// a unit diagonal; LDL^T of it in index order leaves out a basis function whose
+// pivot is at or below kPivotTol (its share of the block not spanned by the earlier
+// ones), or whose diagonal is not positive, and inverts the rest.  Stored: the unit
+// lower factor (its columns of left-out functions zero), 1/d (0 when left out) and
+// s.  The scaling keeps a block whose stretch modes are 1e10 times stiffer than its
+// translations (a collapsed element inside the aggregate) from losing the
+// translations, and applying the factors keeps the operator positive where an
+// explicit inverse of such a block would not be.
+__device__ void fact9_psd(const double *__restrict__ G, double *__restrict__ F)
+{
+    double Lm[9][9], d[9], sc[9];
+    bool kept[9];
+    for (int p = 0; p < 9; ++p) {
+        const double g = G[pk9(p, p)];
+        sc[p] = (g > 0.0 && isfinite(g)) ? 1.0 / sqrt(g) : 0.0;
+    }
+    for (int k = 0; k < 9; ++k) {
+        double dk = sc[k] * sc[k] * G[pk9(k, k)];
+        for (int j = 0; j < k; ++j)
+            if (kept[j]) dk -= Lm[k][j] * Lm[k][j] * d[j];
+        kept[k] = sc[k] > 0.0 && dk > kPivotTol && isfinite(dk);
+        for (int i = 0; i < 9; ++i) Lm[i][k] = 0.0;
+        d[k] = 0.0;
+        if (!kept[k]) continue;
+        d[k] = dk;
+        for (int i = k + 1; i < 9; ++i) {
+            double v = sc[k] * sc[i] * G[pk9(k, i)];
+            for (int j = 0; j < k; ++j)
+                if (kept[j]) v -= Lm[i][j] * Lm[k][j] * d[j];
+            Lm[i][k] = v / dk;
+        }
+    }
+    for (int i = 1; i < 9; ++i)
+        for (int j = 0; j < i; ++j) F[lo9(i, j)] = Lm[i][j];
+    for (int p = 0; p < 9; ++p) {
+        F[36 + p] = kept[p] ? 1.0 / d[p] : 0.0;
+        F[45 + p] = sc[p];
+    }
+}
+
+// out = G^+ r from fact9_psd's factors: s L^-T D^-1 L^-1 s r.
+__device__ __forceinline__ void apply9(const double *__restrict__ F, const double *r, double *out)
+{
+    double y[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double v = F[45 + i] * r[i];
+#pragma unroll
+        for (int j = 0; j < i; ++j) v = fma(-F[lo9(i, j)], y[j], v);
+        y[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) y[i] *= F[36 + i];
+#pragma unroll
+    for (int i = 8; i >= 0; --i) {
+        double v = y[i];
+#pragma unroll
+        for (int j = i + 1; j < 9; ++j) v = fma(-F[lo9(j, i)], y[j], v);
+        y[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) out[i] = F[45 + i] * y[i];
+}
+
+// One workgroup per set-up chunk (level l, aggregate g, hord positions [k0, k1)):
+// sum over the chunk's rows i and their entries (i, j) inside g (first common level
+// <= l) of phi(i) phi(j)^T (x) A_ij, fixed DOFs masked, in the aggregate's basis;
+// the upper triangle (45 sums) in a fixed order per thread, then over the workgroup.
+constexpr int kGalWG = 256;
+__global__ void __launch_bounds__(kGalWG) k_ml_gal9_part(const int *__restrict__ gchunk, const int *__restrict__ hord,
+                                                         const int *__restrict__ gpos,
+                                                         const int *__restrict__ slice_off,
+                                                         const int *__restrict__ slen, const int *__restrict__ scol,
+                                                         const unsigned char *__restrict__ lam,
+                                                         const double *__restrict__ A, int64_t nbs,
+                                                         const unsigned char *__restrict__ fixmask,
+                                                         const double *__restrict__ X, const double *__restrict__ geo,
+                                                         double *__restrict__ gpart)
+{
+    __shared__ double red[kGalWG / 64][45];
+    const int ch = blockIdx.x;
+    const int l = gchunk[4 * ch], g = gchunk[4 * ch + 1], k0 = gchunk[4 * ch + 2], k1 = gchunk[4 * ch + 3];
+    const double cu = geo[4 * (int64_t)g], cv = geo[4 * (int64_t)g + 1], is = geo[4 * (int64_t)g + 3];
+    double acc[45];
+#pragma unroll
+    for (int p = 0; p < 45; ++p) acc[p] = 0.0;
+    for (int k = k0 + (int)threadIdx.x; k < k1; k += kGalWG) {
+        const int i = hord[k];
+        const int gp = gpos[i];
+        const int len = slen[gp];
+        const int base = slice_off[gp / eolc::kSlice] + (gp % eolc::kSlice);
+        const unsigned fi = fixmask[i];
+        const double pi[3] = {1.0, (X[2 * (int64_t)i] - cu) * is, (X[2 * (int64_t)i + 1] - cv) * is};
+        for (int kk = 0; kk < len; ++kk) {
+            const int e = base + kk * eolc::kSlice;
+            if ((int)lam[e] > l) continue;
+            const int j = scol[e];
+            const unsigned fj = fixmask[j];
+            const double pj[3] = {1.0, (X[2 * (int64_t)j] - cu) * is, (X[2 * (int64_t)j + 1] - cv) * is};
+            double b[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                const int r = c / 3, t = c % 3;
+                b[c] = (((fi >> r) | (fj >> t)) & 1u) ? 0.0 : A[(int64_t)c * nbs + e];
+            }
+#pragma unroll
+            for (int p = 0; p < 9; ++p)
+#pragma unroll
+                for (int q = p; q < 9; ++q) acc[pk9(p, q)] = fma(pi[p / 3] * pj[q / 3], b[(p % 3) * 3 + q % 3], acc[pk9(p, q)]);
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int p = 0; p < 45; ++p) {
+        const double v = wave_sum(acc[p]);
+        if (lane == 0) red[wid][p] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 45) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < kGalWG / 64; ++w) v += red[w][threadIdx.x];
+        gpart[45 * (int64_t)ch + threadIdx.x] = v;
+    }
+}
+
+// One thread per aggregate of levels >= 3: its chunks' sums in order, then the
+// block's factors.
+__global__ void __launch_bounds__(64) k_ml_gal9_fin(int n_hi, const int *__restrict__ gcp,
+                                                    const double *__restrict__ gpart, double *__restrict__ D9)
+{
+    const int h = blockIdx.x * 64 + threadIdx.x;
+    if (h >= n_hi) return;
+    double G[45];
+    for (int p = 0; p < 45; ++p) G[p] = 0.0;
+    for (int ch = gcp[h]; ch < gcp[h + 1]; ++ch)
+        for (int p = 0; p < 45; ++p) G[p] += gpart[45 * (int64_t)ch + p];
+    fact9_psd(G, D9 + kFact9 * (int64_t)h);
+}
+
 // Ordering of LDS traffic between the lanes of one wavefront (no workgroup barrier:
 // the wavefronts of a workgroup run different trip counts).
 __device__ __forceinline__ void wave_sync()
@@ -168,33 +332,54 @@ __device__ __forceinline__ void wave_sync()
 // two wavefronts per workgroup: 13.9 us per restriction at 1024^2, against 14.8 (1),
 // 14.2 (4) and 14.4 (8) in the same kernel trace
 constexpr int kRestrictWG = 128;
-template <class T>
-__global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const int *__restrict__ hord,
-                                                             const int *__restrict__ ptr1, const int *__restrict__ cptr,
-                                                             const int *__restrict__ kptr2,
-                                                             const T *__restrict__ r, double *__restrict__ R,
-                                                             const double *__restrict__ Dinv, double *__restrict__ C,
-                                                             const SolverScalars *scal, int check_done)
+
+__device__ __forceinline__ void lift9(const double *__restrict__ q, double al, double bu, double bv, double *sm)
 {
-    __shared__ double stage[kRestrictWG / 64][3 * 64];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sm[a] += q[a];
+        sm[3 + a] += fma(al, q[3 + a], bu * q[a]);
+        sm[6 + a] += fma(al, q[6 + a], bv * q[a]);
+    }
+}
+__device__ __forceinline__ void lower9(const double *__restrict__ P, double al, double bu, double bv, double *cf)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        cf[a] += fma(bv, P[6 + a], fma(bu, P[3 + a], P[a]));
+        cf[3 + a] = fma(al, P[3 + a], cf[3 + a]);
+        cf[6 + a] = fma(al, P[6 + a], cf[6 + a]);
+    }
+}
+
+// The restriction of one level-2 aggregate w by one wavefront (two levels or more),
+// or of the 64 level-1 aggregates from w * 64 (one level).  Writes R_1 of its
+// children, R_2(w) and C of its children; returns R_2(w) in every lane and, with
+// feat2 (the members' level-2 coordinate features in hord order), the first moments
+// sum phi_1 r and sum phi_2 r in lane 0.
+template <class T>
+__device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const int *__restrict__ hord,
+                                              const int *__restrict__ ptr1, const int *__restrict__ cptr,
+                                              const int *__restrict__ kptr2, const T *__restrict__ r,
+                                              double *__restrict__ R, const double *__restrict__ Dinv,
+                                              double *__restrict__ C, const double *__restrict__ feat2,
+                                              double *__restrict__ buf, double &s0, double &s1, double &s2,
+                                              double (&mom)[6], const SolverScalars *scal, int check_done)
+{
     const int lane = threadIdx.x % 64;
-    double *__restrict__ buf = stage[threadIdx.x / 64];
-    const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;   // uniform in the wavefront
     const bool two = lv.L >= 2;
     int a0, a1, ka = 0, kz = 0;
-    // the aggregate's child and member ranges are read together with the stop flag
-    // (one latency round; a stopped launch only reads these words)
     if (two) {
-        if (w >= lv.n[2]) return;
         a0 = cptr[lv.coff[2] + w];
         a1 = cptr[lv.coff[2] + w + 1];
         ka = kptr2[w];
         kz = kptr2[w + 1];
     } else {
         a0 = w * 64;
-        if (a0 >= lv.n[1]) return;
         a1 = min(a0 + 64, lv.n[1]);
     }
+    // the stop flag is read together with the ranges (one latency round; a stopped
+    // launch only reads these words)
     if (check_done && scal->done) return;
     // inverse blocks of the level-2 aggregate and of this lane's first child: they do
     // not depend on r, so they load with the member indices
@@ -207,8 +392,9 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
 #pragma unroll
             for (int k = 0; k < 6; ++k) da[k] = Dinv[6 * (int64_t)(a0 + lane) + k];
     }
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    s0 = s1 = s2 = 0.0;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;   // R_1 of this lane's first child
+    double mu0 = 0.0, mu1 = 0.0, mu2 = 0.0, mv0 = 0.0, mv1 = 0.0, mv2 = 0.0;
     int k0 = two ? ka : ptr1[a0];
     for (int g0 = a0; g0 < a1; g0 += 64) {   // groups of up to 64 children
         const int g1 = min(g0 + 64, a1);
@@ -221,9 +407,15 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
             const int k = kb + lane;
             if (k < k1) {
                 const int i = hord[k];
-                buf[3 * lane] = (double)r[3 * i];
-                buf[3 * lane + 1] = (double)r[3 * i + 1];
-                buf[3 * lane + 2] = (double)r[3 * i + 2];
+                const double q0 = (double)r[3 * i], q1 = (double)r[3 * i + 1], q2 = (double)r[3 * i + 2];
+                buf[3 * lane] = q0;
+                buf[3 * lane + 1] = q1;
+                buf[3 * lane + 2] = q2;
+                if (feat2) {
+                    const double u = feat2[2 * (int64_t)k], v = feat2[2 * (int64_t)k + 1];
+                    mu0 = fma(u, q0, mu0); mu1 = fma(u, q1, mu1); mu2 = fma(u, q2, mu2);
+                    mv0 = fma(v, q0, mv0); mv1 = fma(v, q1, mv1); mv2 = fma(v, q2, mv2);
+                }
             }
             wave_sync();
             const int e0 = max(m0, kb), e1 = min(m1, kb + 64);
@@ -241,32 +433,104 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
         }
         k0 = k1;   // the next group starts where this one ends
     }
-    if (two) {
-        // wave_sum leaves the total in lane 0; every lane needs it below
-        s0 = __shfl(wave_sum(s0), 0, 64);
-        s1 = __shfl(wave_sum(s1), 0, 64);
-        s2 = __shfl(wave_sum(s2), 0, 64);
-        if (lane == 0) {
-            double *o = R + 3 * (lv.off[2] + (int64_t)w);
-            o[0] = s0; o[1] = s1; o[2] = s2;
-        }
-        // levels 1 and 2 of each child, for the prolongation: C(a) = D_1^-1 R_1(a) +
-        // D_2^-1 R_2(w)
-        double e0, e1, e2, c0, c1, c2;
-        apply_inv(dw, s0, s1, s2, e0, e1, e2);
-        if (a0 + lane < a1) {
-            const int64_t a = a0 + lane;
-            apply_inv(da, f0, f1, f2, c0, c1, c2);
+    if (!two) return;
+    // wave_sum leaves the total in lane 0; every lane needs it below
+    s0 = __shfl(wave_sum(s0), 0, 64);
+    s1 = __shfl(wave_sum(s1), 0, 64);
+    s2 = __shfl(wave_sum(s2), 0, 64);
+    if (lane == 0) {
+        double *o = R + 3 * (lv.off[2] + (int64_t)w);
+        o[0] = s0; o[1] = s1; o[2] = s2;
+    }
+    if (feat2) {
+        mom[0] = wave_sum(mu0); mom[1] = wave_sum(mu1); mom[2] = wave_sum(mu2);
+        mom[3] = wave_sum(mv0); mom[4] = wave_sum(mv1); mom[5] = wave_sum(mv2);
+    }
+    // levels 1 and 2 of each child, for the prolongation: C(a) = D_1^-1 R_1(a) +
+    // D_2^-1 R_2(w)
+    double e0, e1, e2, c0, c1, c2;
+    apply_inv(dw, s0, s1, s2, e0, e1, e2);
+    if (a0 + lane < a1) {
+        const int64_t a = a0 + lane;
+        apply_inv(da, f0, f1, f2, c0, c1, c2);
+        C[3 * a] = c0 + e0; C[3 * a + 1] = c1 + e1; C[3 * a + 2] = c2 + e2;
+    }
+    // more than 64 children: each lane re-reads the R_1 it wrote above
+    for (int g0 = a0 + 64; g0 < a1; g0 += 64) {
+        const int a = g0 + lane;
+        if (a < a1) {
+            apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
             C[3 * a] = c0 + e0; C[3 * a + 1] = c1 + e1; C[3 * a + 2] = c2 + e2;
         }
-        // more than 64 children: each lane re-reads the R_1 it wrote above
-        for (int g0 = a0 + 64; g0 < a1; g0 += 64) {
-            const int a = g0 + lane;
-            if (a < a1) {
-                apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
-                C[3 * a] = c0 + e0; C[3 * a + 1] = c1 + e1; C[3 * a + 2] = c2 + e2;
-            }
+    }
+}
+
+// Translations only: one wavefront per level-2 aggregate (or per 64 level-1 ones).
+template <class T>
+__global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const int *__restrict__ hord,
+                                                             const int *__restrict__ ptr1, const int *__restrict__ cptr,
+                                                             const int *__restrict__ kptr2,
+                                                             const T *__restrict__ r, double *__restrict__ R,
+                                                             const double *__restrict__ Dinv, double *__restrict__ C,
+                                                             const SolverScalars *scal, int check_done)
+{
+    __shared__ double stage[kRestrictWG / 64][3 * 64];
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;   // uniform in the wavefront
+    if (lv.L >= 2 ? w >= lv.n[2] : w * 64 >= lv.n[1]) return;
+    double s0, s1, s2, mom[6];
+    restrict_wave<T>(lv, w, hord, ptr1, cptr, kptr2, r, R, Dinv, C, nullptr, stage[threadIdx.x / 64], s0, s1, s2,
+                     mom, scal, check_done);
+}
+
+// Linear coarse fields: one workgroup per level-3 aggregate U, its wavefronts taking
+// U's level-2 children in turn.  Each child's R_2 and first moments are re-expressed
+// on U's basis, (R_2, alpha M_1 + beta_u R_2, alpha M_2 + beta_v R_2) with alpha =
+// s_w / s_U and beta = (c_w - c_U) / s_U, summed per wavefront in child order and
+// over the wavefronts in order: R9_3(U).
+constexpr int kRestrict3WG = 512;   // all level-3 groups resident at 1024^2 (379 workgroups, 2 per CU)
+template <class T>
+__global__ void __launch_bounds__(kRestrict3WG) k_ml_restrict3(MLLevels lv, const int *__restrict__ hord,
+                                                               const int *__restrict__ ptr1,
+                                                               const int *__restrict__ cptr,
+                                                               const int *__restrict__ kptr2, const T *__restrict__ r,
+                                                               double *__restrict__ R, const double *__restrict__ Dinv,
+                                                               double *__restrict__ C,
+                                                               const double *__restrict__ feat2,
+                                                               const double *__restrict__ geo, double *__restrict__ R9,
+                                                               const SolverScalars *scal, int check_done)
+{
+    constexpr int NW = kRestrict3WG / 64;
+    __shared__ double stage[NW][3 * 64];
+    __shared__ double red[NW][9];
+    const int U = blockIdx.x;
+    const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+    const int c0 = cptr[lv.coff[3] + U], c1 = cptr[lv.coff[3] + U + 1];
+    const int64_t gU = lv.off[3] + U;
+    const double cu = geo[4 * gU], cv = geo[4 * gU + 1], is = geo[4 * gU + 3];
+    if (check_done && scal->done) return;
+    // lane 0 keeps the wavefront's running sum in LDS
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) red[wid][q] = 0.0;
+    for (int w = c0 + wid; w < c1; w += NW) {
+        double s0, s1, s2, mom[6];
+        restrict_wave<T>(lv, w, hord, ptr1, cptr, kptr2, r, R, Dinv, C, feat2, stage[wid], s0, s1, s2, mom, scal, 0);
+        if (lane == 0) {
+            const int64_t gw = lv.off[2] + w;
+            const double al = geo[4 * gw + 2] * is, bu = (geo[4 * gw] - cu) * is, bv = (geo[4 * gw + 1] - cv) * is;
+            double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            const double q[9] = {s0, s1, s2, mom[0], mom[1], mom[2], mom[3], mom[4], mom[5]};
+            lift9(q, al, bu, bv, t);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) red[wid][k] += t[k];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) v += red[k][threadIdx.x];
+        R9[9 * (int64_t)U + threadIdx.x] = v;
     }
 }
 
@@ -408,6 +672,132 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
         for (int t = threadIdx.x; t < 3 * lv.n[3]; t += kCoarseWG) C[3 * (int64_t)base + t] = Cb[t];
 }
 
+// Levels >= 3 with linear fields, in one workgroup, from the level-3 sums R9_3 of
+// the restriction: R9_l (l > 3) = sum over the level-(l-1) children c of (R9_c[0],
+// alpha R9_c[1] + beta_u R9_c[0], alpha R9_c[2] + beta_v R9_c[0]) with alpha = s_c /
+// s_A, beta = (c_c - c_A) / s_A (the child's basis on the parent's; R9_c[m] the
+// three components of feature m).  Then from the top down C9_l = D9_l^+ R9_l plus the
+// parent's field re-expressed on the aggregate's basis (the same alpha, beta); the
+// level-3 fields go to C9.
+//   k_ml_coarse9     : at most kCoarse9WG aggregates on these levels, one per thread:
+//                      every load (the level-3 sums, geometry, parents, child ranges
+//                      and the thread's own factors) is issued in one round, then the
+//                      levels run in LDS and registers.
+//   k_ml_coarse9_any : any count, sums and fields in R9 / C9 in global memory.
+constexpr int kCoarse9WG = 512;
+__global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const int *__restrict__ cptr,
+                                                           const int *__restrict__ par, const double *__restrict__ geo,
+                                                           const double *__restrict__ D9, const double *__restrict__ R9,
+                                                           double *__restrict__ C9, const SolverScalars *scal,
+                                                           int check_done)
+{
+    __shared__ double sR[9 * kCoarse9WG], sC[9 * kCoarse9WG], sg[4 * kCoarse9WG];
+    __shared__ int sp[kCoarse9WG];
+    const int L = lv.L, base = lv.off[3];
+    const int M = lv.off[L + 1] - base, n3 = lv.n[3];
+    const int h = threadIdx.x;   // this thread's aggregate (levels >= 3 concatenated)
+    int l = 0, A = 0, c0 = 0, c1 = 0;
+    double F[kFact9];
+    if (h < M) {
+        l = 3;
+        while (l < L && h >= lv.off[l + 1] - base) ++l;
+        A = h - (lv.off[l] - base);
+        if (l > 3) {
+            c0 = cptr[lv.coff[l] + A];
+            c1 = cptr[lv.coff[l] + A + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kFact9; ++k) F[k] = D9[kFact9 * (int64_t)h + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sg[4 * h + k] = geo[4 * (int64_t)(base + h) + k];
+        sp[h] = l < L ? par[base + h] : 0;
+    }
+    for (int t = threadIdx.x; t < 9 * n3; t += kCoarse9WG) sR[t] = R9[t];
+    if (check_done && scal->done) return;
+    __syncthreads();
+    for (int lev = 4; lev <= L; ++lev) {
+        if (h < M && l == lev) {
+            const double cu = sg[4 * h], cv = sg[4 * h + 1], is = sg[4 * h + 3];
+            double sm[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) sm[q] = 0.0;
+            const int cb = lv.off[lev - 1] - base;
+            for (int ch = c0; ch < c1; ++ch) {
+                const int hc = cb + ch;
+                lift9(sR + 9 * hc, sg[4 * hc + 2] * is, (sg[4 * hc] - cu) * is, (sg[4 * hc + 1] - cv) * is, sm);
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) sR[9 * h + q] = sm[q];
+        }
+        __syncthreads();
+    }
+    for (int lev = L; lev >= 3; --lev) {
+        if (h < M && l == lev) {
+            double rv[9], cf[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) rv[q] = sR[9 * h + q];
+            apply9(F, rv, cf);
+            if (lev < L) {
+                const int hp = lv.off[lev + 1] - base + sp[h];
+                const double ip = sg[4 * hp + 3];
+                lower9(sC + 9 * hp, sg[4 * h + 2] * ip, (sg[4 * h] - sg[4 * hp]) * ip, (sg[4 * h + 1] - sg[4 * hp + 1]) * ip,
+                       cf);
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) sC[9 * h + q] = cf[q];
+        }
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < 9 * n3; t += kCoarse9WG) C9[t] = sC[t];
+}
+
+__global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9_any(MLLevels lv, const int *__restrict__ cptr,
+                                                               const int *__restrict__ par,
+                                                               const double *__restrict__ geo,
+                                                               const double *__restrict__ D9, double *__restrict__ R9,
+                                                               double *__restrict__ C9, const SolverScalars *scal,
+                                                               int check_done)
+{
+    if (check_done && scal->done) return;
+    const int L = lv.L, base = lv.off[3];
+    const double *G = geo + 4 * (int64_t)base;
+    for (int l = 4; l <= L; ++l) {
+        for (int A = threadIdx.x; A < lv.n[l]; A += kCoarse9WG) {
+            const int h = lv.off[l] - base + A;
+            const double cu = G[4 * h], cv = G[4 * h + 1], is = G[4 * h + 3];
+            const int c0 = cptr[lv.coff[l] + A], c1 = cptr[lv.coff[l] + A + 1];
+            double sm[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) sm[q] = 0.0;
+            for (int ch = c0; ch < c1; ++ch) {
+                const int hc = lv.off[l - 1] - base + ch;
+                lift9(R9 + 9 * (int64_t)hc, G[4 * hc + 2] * is, (G[4 * hc] - cu) * is, (G[4 * hc + 1] - cv) * is, sm);
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) R9[9 * (int64_t)h + q] = sm[q];
+        }
+        __syncthreads();
+    }
+    for (int l = L; l >= 3; --l) {
+        for (int A = threadIdx.x; A < lv.n[l]; A += kCoarse9WG) {
+            const int h = lv.off[l] - base + A;
+            double rv[9], cf[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) rv[q] = R9[9 * (int64_t)h + q];
+            apply9(D9 + kFact9 * (int64_t)h, rv, cf);
+            if (l < L) {
+                const int hp = lv.off[l + 1] - base + par[base + h];
+                const double ip = G[4 * hp + 3];
+                lower9(C9 + 9 * (int64_t)hp, G[4 * h + 2] * ip, (G[4 * h] - G[4 * hp]) * ip,
+                       (G[4 * h + 1] - G[4 * hp + 1]) * ip, cf);
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) C9[9 * (int64_t)h + q] = cf[q];
+        }
+        __syncthreads();
+    }
+}
+
 // z = D0 r + coarse correction (zero on fixed DOFs); r.z partials -> PB[parity] slot 0.
 // Levels 1 and 2 are applied here, per vertex, from the restricted sums; levels >= 3
 // arrive accumulated in C_3.
@@ -418,7 +808,8 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const int *__restrict__ anc3,
              const double *__restrict__ Dinv, const double *__restrict__ R, const double *__restrict__ C,
              const TD *__restrict__ D, const T *__restrict__ r, T *__restrict__ z,
-             const unsigned char *__restrict__ fixmask, const SolverScalars *scal, double *partials, int check_done)
+             const unsigned char *__restrict__ fixmask, const SolverScalars *scal, double *partials, int check_done,
+             const double *__restrict__ feat3, const double *__restrict__ C9)
 {
     __shared__ double lds[2 * (WG / 64) + 2];
     int rb, re;
@@ -443,8 +834,16 @@ k_ml_prolong(int parity, int N, MLLevels lv, const int *__restrict__ agg1, const
             // levels 1 and 2, combined per level-1 aggregate by the restriction
             c0 = C[3 * a]; c1 = C[3 * a + 1]; c2 = C[3 * a + 2];
             if (lv.L >= 3) {
-                const double *up = C + 3 * (int64_t)u;
-                c0 += up[0]; c1 += up[1]; c2 += up[2];
+                if (C9) {   // the level-3 aggregate's linear field at this vertex
+                    const double *P = C9 + 9 * (int64_t)(u - lv.off[3]);
+                    const double gu = feat3[2 * (int64_t)i], gv = feat3[2 * (int64_t)i + 1];
+                    c0 += fma(gv, P[6], fma(gu, P[3], P[0]));
+                    c1 += fma(gv, P[7], fma(gu, P[4], P[1]));
+                    c2 += fma(gv, P[8], fma(gu, P[5], P[2]));
+                } else {
+                    const double *up = C + 3 * (int64_t)u;
+                    c0 += up[0]; c1 += up[1]; c2 += up[2];
+                }
             }
         } else {
             apply_inv(Dinv + 6 * (int64_t)a, R[3 * a], R[3 * a + 1], R[3 * a + 2], c0, c1, c2);
@@ -473,10 +872,11 @@ inline int blocks_for(int64_t n, int t = 256) { return (int)((n + t - 1) / t); }
 
 namespace eolc {
 
-hipError_t ml_setup(eolc_ctx *c)
+hipError_t ml_setup(eolc_ctx *c, bool linear)
 {
-    const MLState &m = c->ml;
+    MLState &m = c->ml;
     const MLLevels &lv = m.lv;
+    m.lin_on = false;
     if (lv.L <= 0) return hipSuccess;
     const Topology &t = c->topo;
     hipLaunchKernelGGL(k_ml_diag1, dim3(blocks_for(lv.n[1])), dim3(256), 0, c->stream, lv, m.hord, m.ptr1, m.gpos,
@@ -484,7 +884,52 @@ hipError_t ml_setup(eolc_ctx *c)
     for (int l = 2; l <= lv.L; ++l)
         hipLaunchKernelGGL(k_ml_diagup, dim3(blocks_for(lv.n[l])), dim3(256), 0, c->stream, l, lv, m.cptr, m.V,
                            m.Dinv);
+    if (linear && m.lin_ok) {
+        const int n_hi = lv.off[lv.L + 1] - lv.off[kMLLinFrom];
+        hipLaunchKernelGGL(k_ml_gal9_part, dim3(m.nchunks), dim3(kGalWG), 0, c->stream, m.gchunk, m.hord, m.gpos,
+                           c->slice_off, c->slen, c->scol, m.lam, c->A64, t.nbs, c->fixmask, c->X, m.geo, m.gpart);
+        hipLaunchKernelGGL(k_ml_gal9_fin, dim3(blocks_for(n_hi, 64)), dim3(64), 0, c->stream, n_hi, m.gcp, m.gpart,
+                           m.D9);
+        m.lin_on = true;
+    }
     return hipGetLastError();
+}
+
+// The restriction, the coarse levels and the prolongation's coarse term, with the
+// linear fields when the last ml_setup formed them.
+static void ml_coarse_chain(eolc_ctx *c, bool mixed, int check_done)
+{
+    const MLState &m = c->ml;
+    const MLLevels &lv = m.lv;
+    const bool lin = m.lin_on && lv.L >= kMLLinFrom;
+    if (lin) {
+        if (mixed)
+            hipLaunchKernelGGL(k_ml_restrict3<float>, dim3(lv.n[3]), dim3(kRestrict3WG), 0, c->stream, lv, m.hord,
+                               m.ptr1, m.cptr, m.kptr2, c->r32, m.R, m.Dinv, m.C, m.feat2, m.geo, m.R9, c->scal,
+                               check_done);
+        else
+            hipLaunchKernelGGL(k_ml_restrict3<double>, dim3(lv.n[3]), dim3(kRestrict3WG), 0, c->stream, lv, m.hord,
+                               m.ptr1, m.cptr, m.kptr2, c->r64, m.R, m.Dinv, m.C, m.feat2, m.geo, m.R9, c->scal,
+                               check_done);
+        if (lv.off[lv.L + 1] - lv.off[3] <= kCoarse9WG)
+            hipLaunchKernelGGL(k_ml_coarse9, dim3(1), dim3(kCoarse9WG), 0, c->stream, lv, m.cptr, m.par, m.geo, m.D9,
+                               m.R9, m.C9, c->scal, check_done);
+        else
+            hipLaunchKernelGGL(k_ml_coarse9_any, dim3(1), dim3(kCoarse9WG), 0, c->stream, lv, m.cptr, m.par, m.geo,
+                               m.D9, m.R9, m.C9, c->scal, check_done);
+        return;
+    }
+    const int waves = lv.L >= 2 ? lv.n[2] : (lv.n[1] + 63) / 64;
+    const int rblocks = blocks_for((int64_t)64 * waves, kRestrictWG);
+    if (mixed)
+        hipLaunchKernelGGL(k_ml_restrict<float>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1,
+                           m.cptr, m.kptr2, c->r32, m.R, m.Dinv, m.C, c->scal, check_done);
+    else
+        hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1,
+                           m.cptr, m.kptr2, c->r64, m.R, m.Dinv, m.C, c->scal, check_done);
+    if (lv.L >= 3)
+        hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
+                           c->scal, check_done);
 }
 
 hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
@@ -493,24 +938,16 @@ hipError_t ml_apply(eolc_ctx *c, bool mixed, int parity, int check_done)
     const MLLevels &lv = m.lv;
     const int N = c->topo.N;
     const int G = pcg_grid(N);
-    const int waves = lv.L >= 2 ? lv.n[2] : (lv.n[1] + 63) / 64;
-    const int rblocks = blocks_for((int64_t)64 * waves, kRestrictWG);
-    if (mixed) {
-        hipLaunchKernelGGL(k_ml_restrict<float>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
-                           m.kptr2, c->r32, m.R, m.Dinv, m.C, c->scal, check_done);
-    } else {
-        hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
-                           m.kptr2, c->r64, m.R, m.Dinv, m.C, c->scal, check_done);
-    }
-    if (lv.L >= 3)
-        hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
-                           c->scal, check_done);
+    ml_coarse_chain(c, mixed, check_done);
+    const double *c9 = (m.lin_on && lv.L >= kMLLinFrom) ? m.C9 : nullptr;
     if (mixed) {
         hipLaunchKernelGGL(k_ml_prolong<float>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.anc3, m.Dinv,
-                           m.R, m.C, c->D32, c->r32, c->z32, c->fixmask, c->scal, c->partials, check_done);
+                           m.R, m.C, c->D32, c->r32, c->z32, c->fixmask, c->scal, c->partials, check_done, m.feat3,
+                           c9);
     } else {
         hipLaunchKernelGGL(k_ml_prolong<double>, dim3(G), dim3(WG), 0, c->stream, parity, N, lv, m.agg1, m.anc3,
-                           m.Dinv, m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, check_done);
+                           m.Dinv, m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, check_done,
+                           m.feat3, c9);
     }
     return hipGetLastError();
 }
@@ -523,15 +960,10 @@ hipError_t ml_coarse64(eolc_ctx *c)
     const int N = c->topo.N;
     const int G = pcg_grid(N);
     if (lv.L <= 0) return hipMemsetAsync(c->z64, 0, sizeof(double) * 3 * (size_t)N, c->stream);
-    const int waves = lv.L >= 2 ? lv.n[2] : (lv.n[1] + 63) / 64;
-    const int rblocks = blocks_for((int64_t)64 * waves, kRestrictWG);
-    hipLaunchKernelGGL(k_ml_restrict<double>, dim3(rblocks), dim3(kRestrictWG), 0, c->stream, lv, m.hord, m.ptr1, m.cptr,
-                       m.kptr2, c->r64, m.R, m.Dinv, m.C, c->scal, 0);
-    if (lv.L >= 3)
-        hipLaunchKernelGGL(k_ml_coarse, dim3(1), dim3(kCoarseWG), 0, c->stream, lv, m.cptr, m.par, m.Dinv, m.R, m.C,
-                           c->scal, 0);
+    ml_coarse_chain(c, false, 0);
+    const double *c9 = (m.lin_on && lv.L >= kMLLinFrom) ? m.C9 : nullptr;
     hipLaunchKernelGGL((k_ml_prolong<double, false>), dim3(G), dim3(WG), 0, c->stream, 0, N, lv, m.agg1, m.anc3, m.Dinv,
-                       m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, 0);
+                       m.R, m.C, c->D64, c->r64, c->z64, c->fixmask, c->scal, c->partials, 0, m.feat3, c9);
     return hipGetLastError();
 }
 

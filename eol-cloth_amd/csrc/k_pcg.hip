@@ -1276,13 +1276,13 @@ int pcg_grid(int N)
     return g;
 }
 
-hipError_t pcg_setup(eolc_ctx *c, bool /*mixed*/, int precond_kind)
+hipError_t pcg_setup(eolc_ctx *c, bool /*mixed*/, int precond_kind, bool ml_linear)
 {
     int N = c->topo.N;
     hipLaunchKernelGGL(k_block_jacobi, dim3(blocks_for(N)), dim3(256), 0, c->stream, N, c->diagpos, c->A64,
                        c->topo.nbs, c->fixmask, precond_kind, c->D64, c->D32);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess && c->ml_on) e = ml_setup(c);
+    if (e == hipSuccess && c->ml_on) e = ml_setup(c, ml_linear);
     return e;
 }
 
@@ -1344,6 +1344,25 @@ hipError_t pcg_reliable_update(eolc_ctx *c)
     const int N = c->topo.N;
     const int G = pcg_grid(N);
     hipLaunchKernelGGL(k_accumulate, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, 3 * N, c->xs64, c->xl64);
+    launch_resid64(c, G, c->xs64, c->b64, c->r64);
+    hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, c->partials + 5 * kMaxPartials,
+                       &c->scal->rmax2);
+    return hipGetLastError();
+}
+
+// A mixed solve that stalled with linear coarse fields restarts in FP64 from its
+// initial guess: x = the resident v on the free DOFs, the FP32 correction dropped,
+// r = b - A x; ||r||^2 -> scal->rmax2.  (Continued from such a stalled iterate, the
+// FP64 iteration on a 1024^2 state with collapsed elements stagnated at a true
+// residual of 1e-7, where the same preconditioner from the initial guess converged in
+// 2,242 iterations.)
+hipError_t pcg_restart_from_guess(eolc_ctx *c)
+{
+    const int N = c->topo.N;
+    const int G = pcg_grid(N);
+    hipLaunchKernelGGL(k_masked_copy, dim3(blocks_for(3 * N)), dim3(256), 0, c->stream, N, c->v, c->fixmask, c->xs64);
+    hipError_t e = hipMemsetAsync(c->xl64, 0, (size_t)3 * N * sizeof(double), c->stream);
+    if (e != hipSuccess) return e;
     launch_resid64(c, G, c->xs64, c->b64, c->r64);
     hipLaunchKernelGGL(k_reduce_to, dim3(1), dim3(WG), 0, c->stream, G, c->partials + 5 * kMaxPartials,
                        &c->scal->rmax2);

@@ -38,7 +38,7 @@ EXPORTED_SYMBOLS = (
     "eolc_velocity_transfer", "eolc_stream_read", "eolc_step_dofs", "eolc_set_batch_scripted",
     "eolc_rigid_integrate", "eolc_box_step", "eolc_set_eol", "eolc_velocity_transfer_eol",
     "eolc_collide_noalloc", "eolc_learned_set_multilevel", "eolc_set_mixed_stall", "eolc_spmv_layout_info",
-    "eolc_set_stiff_threshold",
+    "eolc_set_stiff_threshold", "eolc_set_ml_linear",
 )
 
 BATCH_EAGER = 0x1
@@ -112,6 +112,7 @@ def lib():
         L.eolc_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.eolc_spmv_layout_info.argtypes = [vp, C.POINTER(C.c_int64), C.c_int]
         L.eolc_set_stiff_threshold.argtypes = [vp, C.c_double]
+        L.eolc_set_ml_linear.argtypes = [vp, C.c_int]
         L.eolc_set_batch_topology.argtypes = [vp, C.c_int] + [C.c_void_p] * 9
         L.eolc_set_batch_material.argtypes = [vp, C.c_void_p, C.c_void_p]
         L.eolc_batch_step.argtypes = [vp, _dp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p]
@@ -429,6 +430,11 @@ class Context:
     def set_mixed_stall(self, first_cycle=10000, min_cycle=1000, sticky=8):
         """Stall limits of the mixed modes (FP64 continuation after an FP32 stall)."""
         return self._check(lib().eolc_set_mixed_stall(self._h, int(first_cycle), int(min_cycle), int(sticky)))
+
+    def set_ml_linear(self, where):
+        """Linear coarse fields on the multilevel levels >= 3: 0 never (default), 1 in
+        FP32 iterations only, 2 in every mode (eolc_set_ml_linear)."""
+        self._check(lib().eolc_set_ml_linear(self._h, int(where)))
 
     def set_stiff_threshold(self, rel):
         self._check(lib().eolc_set_stiff_threshold(self._h, float(rel)))

@@ -50,12 +50,16 @@ typedef enum {
                                        reliable updates; block-Jacobi.  When the FP32
                                        iteration stalls (the FP32 operator no longer
                                        resolves the system) the solve continues in FP64
-                                       from its iterate, and the context's next mixed
-                                       solves start in FP64 (eolc_set_mixed_stall) */
+                                       from its iterate (restarts from its initial guess
+                                       after linear coarse fields, eolc_set_ml_linear),
+                                       and the context's next mixed solves start in FP64
+                                       (eolc_set_mixed_stall) */
 #define EOLC_SOLVE_MIXED_LEARNED 2  /* as MIXED_BJ with the learned blocks P_i (weights
                                        from eolc_learned_set_weights); falls back to
                                        block-Jacobi after twice the iterations of the last
-                                       converged block-Jacobi solve on the context */
+                                       converged block-Jacobi solve on the context, unless
+                                       the network's output layer is zero (its blocks are
+                                       then the block-Jacobi ones) */
 #define EOLC_SOLVE_FP64_JACOBI   3  /* FP64 scalar-Jacobi PCG (Eigen's default) */
 #define EOLC_SOLVE_MULTILEVEL 0x10  /* flag, OR-ed into any mode: the mode's fine blocks plus
                                        the coarse levels of an additive multilevel
@@ -84,10 +88,24 @@ int eolc_set_stiff_threshold(eolc_ctx *ctx, double rel);
  * iteration is taken to have stalled when a reliable-update cycle runs longer than
  * max(min_cycle, 10x the mean completed cycle) -- first_cycle for the first cycle --
  * or when three reliable updates in a row each reduce the true residual by less than
- * half the cycle's target.  The solve then continues in FP64 from its iterate, and
- * the next `sticky` mixed solves on the context start in FP64 (a state upload ends
- * that).  Defaults 10000, 1000, 8. */
+ * half the cycle's target.  The solve then continues in FP64 (see
+ * EOLC_SOLVE_MIXED_BJ), and the next `sticky` mixed solves on the context start in
+ * FP64 (a state upload ends that).  Defaults 10000, 1000, 8. */
 int eolc_set_mixed_stall(eolc_ctx *ctx, int first_cycle, int min_cycle, int sticky);
+
+/* Linear coarse fields of EOLC_SOLVE_MULTILEVEL (no counterpart in the reference):
+ * on the levels >= 3 each aggregate carries, per component, the basis (1, (X - c)/s)
+ * of the material coordinates instead of the constant (9 x 9 Galerkin blocks), so a
+ * coarse correction is linear on the aggregate and resolves the sheet's smooth
+ * bending modes better: 20-25% fewer iterations at 512^2 and 1024^2 on most states.
+ * `where`: 0 never (default), 1 in FP32 iterations only (FP64 solves, the FP64
+ * restart of a stalled mixed solve and its sticky successors keep the
+ * translations), 2 in every mode.  Off by default: on states with collapsed
+ * elements an FP64 solve with them took up to 3x the iterations of the translations
+ * alone, and their restriction costs 35 against 15 us per iteration at 1024^2, which
+ * over the benchmark trajectory cancels the iteration gain.  The batched solve
+ * always keeps the translations. */
+int eolc_set_ml_linear(eolc_ctx *ctx, int where);
 
 /* Velocity transfer after a remesh, non-EOL vertices: Cloth::velocityTransfer
  * (Cloth.cpp:195-230, 265-281).  Each new vertex takes the velocity of the first old

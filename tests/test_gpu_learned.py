@@ -173,6 +173,15 @@ def test_learned_solve_reaches_tolerance_and_falls_back(ctx):
     rc, st2 = ctx.velocity_solve(fd, vals, tol=1e-9, mode=eolc.SOLVE_MIXED_LEARNED)
     print(f"  learned solve after a tight block-Jacobi hint {st2.as_dict()}", flush=True)
     assert rc == eolc.OK and st2.used_fallback == 0 and st2.true_relres <= 1e-9
+    # a zero output layer gives the block-Jacobi blocks themselves: no cap, so no
+    # fallback even after the loose hint
+    ctx.learned_set_weights(learned.init_weights(3))
+    for mode, tol in ((eolc.SOLVE_MIXED_BJ, 1e-2), (eolc.SOLVE_MIXED_LEARNED, 1e-9)):
+        ctx.upload_state(sc.x0, sc.v0)
+        ctx.forces_fill(GRAV, H)
+        rc, st3 = ctx.velocity_solve(fd, vals, tol=tol, mode=mode)
+    print(f"  zero-output learned solve after a loose hint {st3.as_dict()}", flush=True)
+    assert rc == eolc.OK and st3.used_fallback == 0 and st3.true_relres <= 1e-9 and st3.iters > 2 * st_bj.iters
 
 
 def test_learned_mode_requires_weights():

@@ -1,11 +1,13 @@
 """Multilevel preconditioner on the GPU against its FP64 oracle (ml_oracle).
 
 Bars: the hierarchy is the host rule's partition on every level; P r in FP64 within
-1e-12 of the oracle's (relative to max |z|), in the mixed modes within 1e-5 (FP32
-fine blocks and vectors, FP64 coarse levels); solves with the coarse levels reach
-the FP64 true-residual tolerance with fewer iterations than block-Jacobi, agree with
-the block-Jacobi solution to 1e-6 relative, and the FP64 iteration count is within
-3% of the oracle's own PCG with the same preconditioner."""
+1e-12 of the oracle's (relative to max |z|), with the translations alone and with
+the linear fields on the levels >= 3, in the mixed modes within 1e-5 (FP32 fine
+blocks and vectors, FP64 coarse levels); solves with the coarse levels reach the
+FP64 true-residual tolerance with fewer iterations than block-Jacobi (and with the
+linear fields fewer than with the translations), agree with the block-Jacobi
+solution to 1e-6 relative, and the FP64 iteration counts are within 3% of the
+oracle's own PCG with the same preconditioner."""
 import numpy as np
 import pytest
 
@@ -52,18 +54,30 @@ def test_hierarchy_and_apply_match_oracle(ctx, n):
     fd = learned.fixed_dofs(sc.fixed)
     rng = np.random.default_rng(n)
     r = rng.normal(size=3 * sc.N)
-    zref = ML.apply(A, sc.N, maps, r, fd)
+    assert len(counts) >= 4   # levels >= 3 exist: the linear fields are exercised
+    zlin = ML.apply(A, sc.N, maps, r, fd, X=sc.X)       # linear fields on the levels >= 3
+    ztr = ML.apply(A, sc.N, maps, r, fd)                 # translations only
+    # default: translations alone; eolc_set_ml_linear 1: linear fields in FP32
+    # iterations only; 2: in every mode
     z64 = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
-    e64 = rel_err(z64, zref)
+    z32t = ctx.precond_apply(r, fd, mode=eolc.SOLVE_MIXED_BJ | MLF)
+    ctx.set_ml_linear(1)
     z32 = ctx.precond_apply(r, fd, mode=eolc.SOLVE_MIXED_BJ | MLF)
-    e32 = rel_err(z32, zref)
+    z64m = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
+    ctx.set_ml_linear(2)
+    z64l = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
+    ctx.set_ml_linear(0)
     zbj = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ)
     ebj = rel_err(zbj, ML.apply(A, sc.N, [], r, fd))
-    print(f"  rel err FP64 {e64:.2e}, mixed {e32:.2e}, block-Jacobi only {ebj:.2e}", flush=True)
-    assert e64 <= 1e-12 and ebj <= 1e-12 and e32 <= 1e-5
-    assert np.all(z64[fd] == 0.0) and np.all(z32[fd] == 0.0)
-    # the coarse levels change z (they are not a no-op)
-    assert rel_err(zbj, zref) > 1e-3
+    e64, e64l, e32, e32t = rel_err(z64, ztr), rel_err(z64l, zlin), rel_err(z32, zlin), rel_err(z32t, ztr)
+    print(f"  rel err FP64 translations {e64:.2e}, FP64 linear {e64l:.2e}, mixed linear {e32:.2e}, "
+          f"mixed translations {e32t:.2e}, block-Jacobi only {ebj:.2e}", flush=True)
+    assert e64 <= 1e-12 and e64l <= 1e-12 and ebj <= 1e-12 and e32 <= 1e-5 and e32t <= 1e-5
+    assert np.array_equal(z64m, z64)   # setting 1 keeps the translations in FP64
+    for z in (z64, z64l, z32, z32t):
+        assert np.all(z[fd] == 0.0)
+    # the coarse levels change z (they are not a no-op), and so do the linear fields
+    assert rel_err(zbj, ztr) > 1e-3 and rel_err(zlin, ztr) > 1e-3
 
 
 def test_multilevel_solves(ctx):
@@ -75,7 +89,9 @@ def test_multilevel_solves(ctx):
     vals = 0.99 * sc.v0[fd]
     out = {}
     for name, mode, tol in (("fp64-bj", eolc.SOLVE_FP64_BJ, 1e-10), ("fp64-ml", eolc.SOLVE_FP64_BJ | MLF, 1e-10),
-                            ("mixed-ml", eolc.SOLVE_MIXED_BJ | MLF, 1e-10)):
+                            ("mixed-ml", eolc.SOLVE_MIXED_BJ | MLF, 1e-10), ("fp64-ml-lin", eolc.SOLVE_FP64_BJ | MLF, 1e-10),
+                            ("mixed-ml-lin", eolc.SOLVE_MIXED_BJ | MLF, 1e-10)):
+        ctx.set_ml_linear({"fp64-ml-lin": 2, "mixed-ml-lin": 1}.get(name, 0))
         ctx.upload_state(sc.x0, sc.v0)
         ctx.forces_fill(GRAV, H)
         rc, st = ctx.velocity_solve(fd, vals, tol=tol, mode=mode)
@@ -83,9 +99,12 @@ def test_multilevel_solves(ctx):
         out[name] = (st, v)
         print(f"  {name}: {st.iters} iterations, true relres {st.true_relres:.2e}", flush=True)
         assert rc == eolc.OK and st.true_relres <= tol
+    ctx.set_ml_linear(0)
     assert out["fp64-ml"][0].iters < 0.7 * out["fp64-bj"][0].iters
+    assert out["fp64-ml-lin"][0].iters < out["fp64-ml"][0].iters
+    assert out["mixed-ml-lin"][0].iters < out["mixed-ml"][0].iters
     vbj = out["fp64-bj"][1]
-    for k in ("fp64-ml", "mixed-ml"):
+    for k in ("fp64-ml", "mixed-ml", "fp64-ml-lin", "mixed-ml-lin"):
         assert rel_err(out[k][1], vbj) <= 1e-6
     # the oracle's PCG with the same preconditioner
     A = ctx.download_csr(eolc.MAT_MDK).tocsr()
@@ -94,12 +113,13 @@ def test_multilevel_solves(ctx):
     b = M @ sc.v0 + H * f
     maps = [ctx.ml_aggregates(l) for l in range(1, len(ctx.ml_info()))]
     D0 = ML.block_jacobi(A, sc.N, fd)
-    levels = ML.coarse_levels(A, sc.N, maps, fd)
     x0 = sc.v0.copy()
     x0[fd] = vals
-    k_ref, _ = ML.pcg(A, b, x0, lambda r: ML.apply(A, sc.N, maps, r, fd, D0=D0, levels=levels), fd, tol=1e-10)
-    print(f"  oracle PCG with the same preconditioner: {k_ref} iterations", flush=True)
-    assert abs(out["fp64-ml"][0].iters - k_ref) <= max(2, 0.03 * k_ref)
+    for name, X in (("fp64-ml", None), ("fp64-ml-lin", sc.X)):
+        levels = ML.coarse_levels(A, sc.N, maps, fd, X)
+        k_ref, _ = ML.pcg(A, b, x0, lambda r: ML.apply(A, sc.N, maps, r, fd, D0=D0, levels=levels), fd, tol=1e-10)
+        print(f"  oracle PCG with the preconditioner of {name}: {k_ref} iterations", flush=True)
+        assert abs(out[name][0].iters - k_ref) <= max(2, 0.03 * k_ref)
 
 
 def test_learned_with_multilevel(ctx):
@@ -186,8 +206,10 @@ def test_multilevel_edge_cases(ctx, case):
         assert len(counts) == 2
     rng = np.random.default_rng(3)
     r = rng.normal(size=3 * N)
+    ctx.set_ml_linear(2)
     z = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
-    zref = ML.apply(A, N, maps, r, fd)
+    ctx.set_ml_linear(0)
+    zref = ML.apply(A, N, maps, r, fd, X=X)
     print(f"  P r rel err {rel_err(z, zref):.2e}", flush=True)
     assert rel_err(z, zref) <= 1e-12 and np.all(z[fd] == 0.0)
     vals = 0.99 * v0[fd]

@@ -222,23 +222,26 @@ def test_headline_path_50_steps_vs_oracle(ctx):
     assert ef <= 1e-10
 
 def test_mixed_stall_continues_in_fp64():
-    """A mixed solve whose FP32 iteration stalls finishes in FP64 from its iterate and
-    meets the tolerance (stats.used_fallback bit 1); the context's next mixed solves
+    """A mixed solve whose FP32 iteration stalls finishes in FP64 (from its iterate, or
+    from its initial guess after linear coarse fields) and meets the tolerance
+    (stats.used_fallback bit 1); the context's next mixed solves
     start in FP64 for the sticky count.  The stall is forced from the host: a
     reliable-update trigger of 1e-4 (eolc_set_mixed_delta) keeps the first FP32 cycle
     going past its first 16-iteration chunk, and a first-cycle limit of 1 iteration
     (eolc_set_mixed_stall) then declares it stalled; the FP64 answer is compared with
     a plain FP64 solve of the same system.  (Own contexts: both are context settings.)"""
     sc = scene.Scene(48, sigma_x=1e-3, sigma_v=0.05)
-    for mode in (eolc.SOLVE_MIXED_BJ, eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL,
-                 eolc.SOLVE_MIXED_LEARNED | eolc.SOLVE_MULTILEVEL):
+    for mode, lin in ((eolc.SOLVE_MIXED_BJ, 0), (eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL, 0),
+                      (eolc.SOLVE_MIXED_LEARNED | eolc.SOLVE_MULTILEVEL, 0),
+                      (eolc.SOLVE_MIXED_BJ | eolc.SOLVE_MULTILEVEL, 1)):
         with eolc.Context(0) as c:
             setup(c, sc, eolc.OPT_SPD_PROJECT_BENDING)
+            c.set_ml_linear(lin)   # 1: linear coarse fields in the FP32 iterations
             c.learned_set_weights(learned.init_weights(7))
             c.set_mixed_delta(1e-4)
             c.set_mixed_stall(first_cycle=1, min_cycle=1, sticky=2)
             rc, st = c.step(GRAV, H, sc.fixed, tol=1e-10, maxit=100000, mode=mode)
-            print(f"  mode {mode}: forced stall -> {st.as_dict()}", flush=True)
+            print(f"  mode {mode} (linear fields {lin}): forced stall -> {st.as_dict()}", flush=True)
             assert rc == eolc.OK and st.true_relres <= 1e-10 and (st.used_fallback & 2)
             x1, v1 = c.download_state()
             # the next two solves start in FP64 directly, the third in FP32 again
