@@ -456,6 +456,9 @@ def main():
                     help="headline without the multilevel coarse levels (and skip the multilevel lines)")
     ap.add_argument("--profile-solve", action="store_true",
                     help="kernel-trace runs: one step of each solve mode capped at 400 PCG iterations, then exit")
+    ap.add_argument("--ml-linear", type=int, default=0, choices=(0, 1, 2),
+                    help="with --profile-solve: linear coarse fields on the multilevel levels >= 3 "
+                         "(eolc_set_ml_linear; 0 off, the library's default)")
     ap.add_argument("--profile-fill", action="store_true",
                     help="counter runs: 10 mixed-mode element evaluations + assemblies (the step's fill) and "
                          "a 1 GiB device copy for calibration, then exit")
@@ -505,6 +508,7 @@ def main():
     if args.profile_solve:
         # one step of each mode, PCG capped at 400 iterations
         ML = eolc.SOLVE_MULTILEVEL
+        ctx.set_ml_linear(args.ml_linear)
         for mode, name in ((eolc.SOLVE_MIXED_BJ, "mixed-bj"), (eolc.SOLVE_FP64_BJ, "fp64-bj"),
                            (eolc.SOLVE_MIXED_BJ | ML, "mixed-bj-ml"), (eolc.SOLVE_FP64_BJ | ML, "fp64-bj-ml")):
             ctx.upload_state(sc.x0, sc.v0)
