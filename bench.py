@@ -457,7 +457,7 @@ def main():
     ap.add_argument("--profile-solve", action="store_true",
                     help="kernel-trace runs: one step of each solve mode capped at 400 PCG iterations, then exit")
     ap.add_argument("--ml-linear", type=int, default=0, choices=(0, 1, 2),
-                    help="with --profile-solve: linear coarse fields on the multilevel levels >= 3 "
+                    help="linear coarse fields on the multilevel levels >= 3 for every leg "
                          "(eolc_set_ml_linear; 0 off, the library's default)")
     ap.add_argument("--profile-fill", action="store_true",
                     help="counter runs: 10 mixed-mode element evaluations + assemblies (the step's fill) and "
@@ -479,6 +479,7 @@ def main():
     ctx.set_topology(sc.X, sc.tri, sc.hinge)
     ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
     ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+    ctx.set_ml_linear(args.ml_linear)
     log(f"topology set-up {time.perf_counter() - t:.1f} s")
     # what a remesh costs on a live context: the pattern, SELL layout and gather maps
     # rebuilt and every per-topology device buffer replaced
@@ -508,7 +509,6 @@ def main():
     if args.profile_solve:
         # one step of each mode, PCG capped at 400 iterations
         ML = eolc.SOLVE_MULTILEVEL
-        ctx.set_ml_linear(args.ml_linear)
         for mode, name in ((eolc.SOLVE_MIXED_BJ, "mixed-bj"), (eolc.SOLVE_FP64_BJ, "fp64-bj"),
                            (eolc.SOLVE_MIXED_BJ | ML, "mixed-bj-ml"), (eolc.SOLVE_FP64_BJ | ML, "fp64-bj-ml")):
             ctx.upload_state(sc.x0, sc.v0)
@@ -661,6 +661,7 @@ def main():
         ctx.set_topology(sc.X, sc.tri, sc.hinge)
         ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
         ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
+        ctx.set_ml_linear(args.ml_linear)
         ctx.upload_state(sc.x0, sc.v0)
         ctx.forces_fill(GRAV, H)
         ctx.set_spmv_variant(32, tuned[32]["variant"])
@@ -697,7 +698,8 @@ def main():
                    f"{head_name} with zero-output weights (exactly the block-Jacobi blocks)",
                    "vertices": sc.N, "triangles": sc.F, "hinges": sc.H, "blocks": nb,
                    "bending_hessian": "spd-projected", "weights": wsrc,
-                   "multilevel_aggregates": None if args.no_ml else ml_counts},
+                   "multilevel_aggregates": None if args.no_ml else ml_counts,
+                   "multilevel_linear_fields": args.ml_linear},
         "speedup_vs_fp64_bj": legs.get("fp64_bj", {}).get("speedup_of_headline"),
         "roofline": roof,
         "roofline_step": head.get("roofline_step"),

@@ -221,6 +221,7 @@ struct BatchState {
 constexpr int kMLMaxLevels = 8;
 constexpr int kMLLinFrom = 3;     // first level with linear coarse fields
 constexpr int kGal9Rows = 2048;   // hord positions per set-up chunk of a 9 x 9 Galerkin block
+constexpr int kRestrict3Parts = 4;   // restriction workgroups per level-3 aggregate (linear fields)
 struct MLLevels {
     int L;                        // coarse levels built (0: none)
     int n[kMLMaxLevels + 1];      // aggregates per level (n[0] = vertices)
@@ -264,7 +265,8 @@ struct MLState {
     double *D9 = nullptr;         // 54 per aggregate of levels >= 3: factors of the Galerkin block
     double *feat2 = nullptr;      // 2 per hord position: (phi_1, phi_2) of the vertex on its level-2 aggregate
     double *feat3 = nullptr;      // 2 per vertex: (phi_1, phi_2) on its level-3 aggregate
-    double *R9 = nullptr;         // 9 per aggregate of levels >= 3: restricted sums (level 3 from the restriction)
+    double *R9p = nullptr;        // 9 per restriction workgroup (4 per level-3 aggregate): partial level-3 sums
+    double *R9 = nullptr;         // 9 per aggregate of levels >= 3: restricted sums
     double *C9 = nullptr;         // 9 per aggregate of levels >= 3: accumulated coarse field
     std::vector<int> counts;      // host copy of n[0..L]
     std::vector<std::vector<int>> maps;   // host: level-l aggregate of every vertex (l = 1..L)

@@ -482,12 +482,18 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
                      mom, scal, check_done);
 }
 
-// Linear coarse fields: one workgroup per level-3 aggregate U, its wavefronts taking
-// U's level-2 children in turn.  Each child's R_2 and first moments are re-expressed
-// on U's basis, (R_2, alpha M_1 + beta_u R_2, alpha M_2 + beta_v R_2) with alpha =
-// s_w / s_U and beta = (c_w - c_U) / s_U, summed per wavefront in child order and
-// over the wavefronts in order: R9_3(U).
-constexpr int kRestrict3WG = 512;   // all level-3 groups resident at 1024^2 (379 workgroups, 2 per CU)
+// Linear coarse fields: kRestrict3Parts workgroups per level-3 aggregate U, whose
+// kRestrict3Parts x (kRestrict3WG / 64) wavefronts take U's level-2 children in turn
+// (wavefront g of U takes children c0 + g, c0 + g + 16, ...).  Each child's R_2 and
+// first moments are re-expressed on U's basis, (R_2, alpha M_1 + beta_u R_2, alpha
+// M_2 + beta_v R_2) with alpha = s_w / s_U and beta = (c_w - c_U) / s_U, summed per
+// wavefront in child order and over the workgroup's wavefronts in order: one partial
+// R9 per workgroup, which the coarse kernel adds in order.  Small workgroups keep all
+// of them resident (1,516 at 1024^2): 35 us per restriction with one 512-thread
+// workgroup per level-3 aggregate, which ran in 1.5 rounds with two children per
+// wavefront.
+constexpr int kRestrict3WG = 256;
+using eolc::kRestrict3Parts;
 template <class T>
 __global__ void __launch_bounds__(kRestrict3WG) k_ml_restrict3(MLLevels lv, const int *__restrict__ hord,
                                                                const int *__restrict__ ptr1,
@@ -502,7 +508,7 @@ __global__ void __launch_bounds__(kRestrict3WG) k_ml_restrict3(MLLevels lv, cons
     constexpr int NW = kRestrict3WG / 64;
     __shared__ double stage[NW][3 * 64];
     __shared__ double red[NW][9];
-    const int U = blockIdx.x;
+    const int U = blockIdx.x / kRestrict3Parts, part = blockIdx.x % kRestrict3Parts;
     const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
     const int c0 = cptr[lv.coff[3] + U], c1 = cptr[lv.coff[3] + U + 1];
     const int64_t gU = lv.off[3] + U;
@@ -512,7 +518,7 @@ __global__ void __launch_bounds__(kRestrict3WG) k_ml_restrict3(MLLevels lv, cons
     if (lane == 0)
 #pragma unroll
         for (int q = 0; q < 9; ++q) red[wid][q] = 0.0;
-    for (int w = c0 + wid; w < c1; w += NW) {
+    for (int w = c0 + part * NW + wid; w < c1; w += NW * kRestrict3Parts) {
         double s0, s1, s2, mom[6];
         restrict_wave<T>(lv, w, hord, ptr1, cptr, kptr2, r, R, Dinv, C, feat2, stage[wid], s0, s1, s2, mom, scal, 0);
         if (lane == 0) {
@@ -530,7 +536,7 @@ __global__ void __launch_bounds__(kRestrict3WG) k_ml_restrict3(MLLevels lv, cons
         double v = 0.0;
 #pragma unroll
         for (int k = 0; k < NW; ++k) v += red[k][threadIdx.x];
-        R9[9 * (int64_t)U + threadIdx.x] = v;
+        R9[9 * (int64_t)blockIdx.x + threadIdx.x] = v;
     }
 }
 
@@ -687,7 +693,7 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
 constexpr int kCoarse9WG = 512;
 __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const int *__restrict__ cptr,
                                                            const int *__restrict__ par, const double *__restrict__ geo,
-                                                           const double *__restrict__ D9, const double *__restrict__ R9,
+                                                           const double *__restrict__ D9, const double *__restrict__ R9p,
                                                            double *__restrict__ C9, const SolverScalars *scal,
                                                            int check_done)
 {
@@ -712,8 +718,22 @@ __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const in
         for (int k = 0; k < 4; ++k) sg[4 * h + k] = geo[4 * (int64_t)(base + h) + k];
         sp[h] = l < L ? par[base + h] : 0;
     }
-    for (int t = threadIdx.x; t < 9 * n3; t += kCoarse9WG) sR[t] = R9[t];
+    // the level-3 sums: the restriction's partial sums of each aggregate, in order
+    double r3[kRestrict3Parts][9];
+    if (h < n3)
+#pragma unroll
+        for (int q = 0; q < kRestrict3Parts; ++q)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) r3[q][k] = R9p[9 * ((int64_t)h * kRestrict3Parts + q) + k];
     if (check_done && scal->done) return;
+    if (h < n3)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            double v = r3[0][k];
+#pragma unroll
+            for (int q = 1; q < kRestrict3Parts; ++q) v += r3[q][k];
+            sR[9 * h + k] = v;
+        }
     __syncthreads();
     for (int lev = 4; lev <= L; ++lev) {
         if (h < M && l == lev) {
@@ -754,13 +774,22 @@ __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const in
 __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9_any(MLLevels lv, const int *__restrict__ cptr,
                                                                const int *__restrict__ par,
                                                                const double *__restrict__ geo,
-                                                               const double *__restrict__ D9, double *__restrict__ R9,
+                                                               const double *__restrict__ D9,
+                                                               const double *__restrict__ R9p, double *__restrict__ R9,
                                                                double *__restrict__ C9, const SolverScalars *scal,
                                                                int check_done)
 {
     if (check_done && scal->done) return;
     const int L = lv.L, base = lv.off[3];
     const double *G = geo + 4 * (int64_t)base;
+    // the level-3 sums: the restriction's partial sums of each aggregate, in order
+    for (int A = threadIdx.x; A < lv.n[3]; A += kCoarse9WG)
+        for (int k = 0; k < 9; ++k) {
+            double v = R9p[9 * ((int64_t)A * kRestrict3Parts) + k];
+            for (int q = 1; q < kRestrict3Parts; ++q) v += R9p[9 * ((int64_t)A * kRestrict3Parts + q) + k];
+            R9[9 * (int64_t)A + k] = v;
+        }
+    __syncthreads();
     for (int l = 4; l <= L; ++l) {
         for (int A = threadIdx.x; A < lv.n[l]; A += kCoarse9WG) {
             const int h = lv.off[l] - base + A;
@@ -904,19 +933,19 @@ static void ml_coarse_chain(eolc_ctx *c, bool mixed, int check_done)
     const bool lin = m.lin_on && lv.L >= kMLLinFrom;
     if (lin) {
         if (mixed)
-            hipLaunchKernelGGL(k_ml_restrict3<float>, dim3(lv.n[3]), dim3(kRestrict3WG), 0, c->stream, lv, m.hord,
-                               m.ptr1, m.cptr, m.kptr2, c->r32, m.R, m.Dinv, m.C, m.feat2, m.geo, m.R9, c->scal,
-                               check_done);
+            hipLaunchKernelGGL(k_ml_restrict3<float>, dim3(lv.n[3] * kRestrict3Parts), dim3(kRestrict3WG), 0,
+                               c->stream, lv, m.hord, m.ptr1, m.cptr, m.kptr2, c->r32, m.R, m.Dinv, m.C, m.feat2,
+                               m.geo, m.R9p, c->scal, check_done);
         else
-            hipLaunchKernelGGL(k_ml_restrict3<double>, dim3(lv.n[3]), dim3(kRestrict3WG), 0, c->stream, lv, m.hord,
-                               m.ptr1, m.cptr, m.kptr2, c->r64, m.R, m.Dinv, m.C, m.feat2, m.geo, m.R9, c->scal,
-                               check_done);
+            hipLaunchKernelGGL(k_ml_restrict3<double>, dim3(lv.n[3] * kRestrict3Parts), dim3(kRestrict3WG), 0,
+                               c->stream, lv, m.hord, m.ptr1, m.cptr, m.kptr2, c->r64, m.R, m.Dinv, m.C, m.feat2,
+                               m.geo, m.R9p, c->scal, check_done);
         if (lv.off[lv.L + 1] - lv.off[3] <= kCoarse9WG)
             hipLaunchKernelGGL(k_ml_coarse9, dim3(1), dim3(kCoarse9WG), 0, c->stream, lv, m.cptr, m.par, m.geo, m.D9,
-                               m.R9, m.C9, c->scal, check_done);
+                               m.R9p, m.C9, c->scal, check_done);
         else
             hipLaunchKernelGGL(k_ml_coarse9_any, dim3(1), dim3(kCoarse9WG), 0, c->stream, lv, m.cptr, m.par, m.geo,
-                               m.D9, m.R9, m.C9, c->scal, check_done);
+                               m.D9, m.R9p, m.R9, m.C9, c->scal, check_done);
         return;
     }
     const int waves = lv.L >= 2 ? lv.n[2] : (lv.n[1] + 63) / 64;
