@@ -406,13 +406,17 @@ __device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const i
         for (int kb = k0; kb < k1; kb += 64) {
             const int k = kb + lane;
             if (k < k1) {
+                double u = 0.0, v = 0.0;   // features first: they do not wait for hord
+                if (feat2) {
+                    u = feat2[2 * (int64_t)k];
+                    v = feat2[2 * (int64_t)k + 1];
+                }
                 const int i = hord[k];
                 const double q0 = (double)r[3 * i], q1 = (double)r[3 * i + 1], q2 = (double)r[3 * i + 2];
                 buf[3 * lane] = q0;
                 buf[3 * lane + 1] = q1;
                 buf[3 * lane + 2] = q2;
                 if (feat2) {
-                    const double u = feat2[2 * (int64_t)k], v = feat2[2 * (int64_t)k + 1];
                     mu0 = fma(u, q0, mu0); mu1 = fma(u, q1, mu1); mu2 = fma(u, q2, mu2);
                     mv0 = fma(v, q0, mv0); mv1 = fma(v, q1, mv1); mv2 = fma(v, q2, mv2);
                 }
@@ -488,10 +492,10 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
 // first moments are re-expressed on U's basis, (R_2, alpha M_1 + beta_u R_2, alpha
 // M_2 + beta_v R_2) with alpha = s_w / s_U and beta = (c_w - c_U) / s_U, summed per
 // wavefront in child order and over the workgroup's wavefronts in order: one partial
-// R9 per workgroup, which the coarse kernel adds in order.  Small workgroups keep all
-// of them resident (1,516 at 1024^2): 35 us per restriction with one 512-thread
-// workgroup per level-3 aggregate, which ran in 1.5 rounds with two children per
-// wavefront.
+// R9 per workgroup, which the coarse kernel adds in order.  At 1024^2 the kernel takes
+// 31 us against 14 us for the translations' restriction, the same with one
+// 512-thread workgroup per level-3 aggregate: timing variants put about 10 us on the
+// first moments (the feature stream and its sums) and 1.5 us on the re-expression.
 constexpr int kRestrict3WG = 256;
 using eolc::kRestrict3Parts;
 template <class T>
