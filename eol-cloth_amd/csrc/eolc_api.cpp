@@ -112,7 +112,7 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(ml.hord); dfree(ml.ptr1); dfree(ml.agg1); dfree(ml.gpos); dfree(ml.cptr); dfree(ml.par); dfree(ml.lam);
     dfree(ml.V); dfree(ml.Dinv); dfree(ml.R); dfree(ml.C); dfree(ml.anc3); dfree(ml.kptr2);
     dfree(ml.geo); dfree(ml.gchunk); dfree(ml.gcp); dfree(ml.gpart); dfree(ml.D9); dfree(ml.feat2); dfree(ml.feat3);
-    dfree(ml.R9); dfree(ml.R9p); dfree(ml.C9);
+    dfree(ml.R9p); dfree(ml.C9);
     ml.lin_ok = ml.lin_on = ml.lin_built = false;
     ml.hord_h.clear();
     ml.nchunks = 0;
@@ -849,7 +849,9 @@ hipError_t ml_prepare_linear(eolc_ctx *c, const std::vector<int> &hord)
         gcp[n_hi] = (int)chunk.size() / 4;
     }
     if ((e = upload(c, &m.geo, geo)) != hipSuccess) return e;
-    m.lin_ok = contiguous && n_hi > 0;
+    // the coarse kernel keeps one aggregate of these levels per thread (about 1.4 M
+    // vertices on a grid); larger hierarchies keep the translations
+    m.lin_ok = contiguous && n_hi > 0 && n_hi <= eolc::kCoarse9WG;
     if (!m.lin_ok) return hipSuccess;
     m.nchunks = (int)chunk.size() / 4;
     if ((e = upload(c, &m.gchunk, chunk)) != hipSuccess) return e;
@@ -872,7 +874,6 @@ hipError_t ml_prepare_linear(eolc_ctx *c, const std::vector<int> &hord)
     }
     if ((e = upload(c, &m.feat2, f2)) != hipSuccess) return e;
     if ((e = upload(c, &m.feat3, f3)) != hipSuccess) return e;
-    if ((e = dalloc(c, &m.R9, (size_t)9 * n_hi)) != hipSuccess) return e;
     if ((e = dalloc(c, &m.R9p, (size_t)9 * eolc::kRestrict3Parts * lv.n[3])) != hipSuccess) return e;
     return dalloc(c, &m.C9, (size_t)9 * n_hi);
 }

@@ -222,6 +222,7 @@ constexpr int kMLMaxLevels = 8;
 constexpr int kMLLinFrom = 3;     // first level with linear coarse fields
 constexpr int kGal9Rows = 2048;   // hord positions per set-up chunk of a 9 x 9 Galerkin block
 constexpr int kRestrict3Parts = 4;   // restriction workgroups per level-3 aggregate (linear fields)
+constexpr int kCoarse9WG = 512;      // the linear fields' coarse kernel: at most this many aggregates on levels >= 3
 struct MLLevels {
     int L;                        // coarse levels built (0: none)
     int n[kMLMaxLevels + 1];      // aggregates per level (n[0] = vertices)
@@ -255,7 +256,7 @@ struct MLState {
     // aggregate, then per aggregate, and factored (ml_setup).
     bool lin_built = false;       // the data below built for this hierarchy (on first use)
     std::vector<int> hord_h;      // host copy of hord
-    bool lin_ok = false;          // hierarchy has levels >= 3 with contiguous hord ranges
+    bool lin_ok = false;          // levels >= 3 exist, hold at most kCoarse9WG aggregates, contiguous in hord
     bool lin_on = false;          // the last ml_setup formed the 9 x 9 blocks
     double *geo = nullptr;        // 4 per aggregate (all levels): c_u, c_v, s, 1/s
     int *gchunk = nullptr;        // 4 per chunk: level, global aggregate, first and end hord position
@@ -266,7 +267,6 @@ struct MLState {
     double *feat2 = nullptr;      // 2 per hord position: (phi_1, phi_2) of the vertex on its level-2 aggregate
     double *feat3 = nullptr;      // 2 per vertex: (phi_1, phi_2) on its level-3 aggregate
     double *R9p = nullptr;        // 9 per restriction workgroup (4 per level-3 aggregate): partial level-3 sums
-    double *R9 = nullptr;         // 9 per aggregate of levels >= 3: restricted sums
     double *C9 = nullptr;         // 9 per aggregate of levels >= 3: accumulated coarse field
     std::vector<int> counts;      // host copy of n[0..L]
     std::vector<std::vector<int>> maps;   // host: level-l aggregate of every vertex (l = 1..L)

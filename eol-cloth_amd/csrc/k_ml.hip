@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(kRestrict3WG) k_ml_restrict3(MLLevels lv, cons
                                                                double *__restrict__ R, const double *__restrict__ Dinv,
                                                                double *__restrict__ C,
                                                                const double *__restrict__ feat2,
-                                                               const double *__restrict__ geo, double *__restrict__ R9,
+                                                               const double *__restrict__ geo, double *__restrict__ R9p,
                                                                const SolverScalars *scal, int check_done)
 {
     constexpr int NW = kRestrict3WG / 64;
@@ -540,7 +540,7 @@ __global__ void __launch_bounds__(kRestrict3WG) k_ml_restrict3(MLLevels lv, cons
         double v = 0.0;
 #pragma unroll
         for (int k = 0; k < NW; ++k) v += red[k][threadIdx.x];
-        R9[9 * (int64_t)blockIdx.x + threadIdx.x] = v;
+        R9p[9 * (int64_t)blockIdx.x + threadIdx.x] = v;
     }
 }
 
@@ -689,12 +689,11 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
 // three components of feature m).  Then from the top down C9_l = D9_l^+ R9_l plus the
 // parent's field re-expressed on the aggregate's basis (the same alpha, beta); the
 // level-3 fields go to C9.
-//   k_ml_coarse9     : at most kCoarse9WG aggregates on these levels, one per thread:
-//                      every load (the level-3 sums, geometry, parents, child ranges
-//                      and the thread's own factors) is issued in one round, then the
-//                      levels run in LDS and registers.
-//   k_ml_coarse9_any : any count, sums and fields in R9 / C9 in global memory.
-constexpr int kCoarse9WG = 512;
+// One aggregate per thread (the linear fields apply when these levels hold at most
+// kCoarse9WG aggregates, ml_prepare_linear): every load (the level-3 sums, geometry,
+// parents, child ranges and the thread's own factors) is issued in one round, then
+// the levels run in LDS and registers.
+using eolc::kCoarse9WG;
 __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const int *__restrict__ cptr,
                                                            const int *__restrict__ par, const double *__restrict__ geo,
                                                            const double *__restrict__ D9, const double *__restrict__ R9p,
@@ -773,62 +772,6 @@ __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const in
         __syncthreads();
     }
     for (int t = threadIdx.x; t < 9 * n3; t += kCoarse9WG) C9[t] = sC[t];
-}
-
-__global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9_any(MLLevels lv, const int *__restrict__ cptr,
-                                                               const int *__restrict__ par,
-                                                               const double *__restrict__ geo,
-                                                               const double *__restrict__ D9,
-                                                               const double *__restrict__ R9p, double *__restrict__ R9,
-                                                               double *__restrict__ C9, const SolverScalars *scal,
-                                                               int check_done)
-{
-    if (check_done && scal->done) return;
-    const int L = lv.L, base = lv.off[3];
-    const double *G = geo + 4 * (int64_t)base;
-    // the level-3 sums: the restriction's partial sums of each aggregate, in order
-    for (int A = threadIdx.x; A < lv.n[3]; A += kCoarse9WG)
-        for (int k = 0; k < 9; ++k) {
-            double v = R9p[9 * ((int64_t)A * kRestrict3Parts) + k];
-            for (int q = 1; q < kRestrict3Parts; ++q) v += R9p[9 * ((int64_t)A * kRestrict3Parts + q) + k];
-            R9[9 * (int64_t)A + k] = v;
-        }
-    __syncthreads();
-    for (int l = 4; l <= L; ++l) {
-        for (int A = threadIdx.x; A < lv.n[l]; A += kCoarse9WG) {
-            const int h = lv.off[l] - base + A;
-            const double cu = G[4 * h], cv = G[4 * h + 1], is = G[4 * h + 3];
-            const int c0 = cptr[lv.coff[l] + A], c1 = cptr[lv.coff[l] + A + 1];
-            double sm[9];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) sm[q] = 0.0;
-            for (int ch = c0; ch < c1; ++ch) {
-                const int hc = lv.off[l - 1] - base + ch;
-                lift9(R9 + 9 * (int64_t)hc, G[4 * hc + 2] * is, (G[4 * hc] - cu) * is, (G[4 * hc + 1] - cv) * is, sm);
-            }
-#pragma unroll
-            for (int q = 0; q < 9; ++q) R9[9 * (int64_t)h + q] = sm[q];
-        }
-        __syncthreads();
-    }
-    for (int l = L; l >= 3; --l) {
-        for (int A = threadIdx.x; A < lv.n[l]; A += kCoarse9WG) {
-            const int h = lv.off[l] - base + A;
-            double rv[9], cf[9];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) rv[q] = R9[9 * (int64_t)h + q];
-            apply9(D9 + kFact9 * (int64_t)h, rv, cf);
-            if (l < L) {
-                const int hp = lv.off[l + 1] - base + par[base + h];
-                const double ip = G[4 * hp + 3];
-                lower9(C9 + 9 * (int64_t)hp, G[4 * h + 2] * ip, (G[4 * h] - G[4 * hp]) * ip,
-                       (G[4 * h + 1] - G[4 * hp + 1]) * ip, cf);
-            }
-#pragma unroll
-            for (int q = 0; q < 9; ++q) C9[9 * (int64_t)h + q] = cf[q];
-        }
-        __syncthreads();
-    }
 }
 
 // z = D0 r + coarse correction (zero on fixed DOFs); r.z partials -> PB[parity] slot 0.
@@ -944,12 +887,8 @@ static void ml_coarse_chain(eolc_ctx *c, bool mixed, int check_done)
             hipLaunchKernelGGL(k_ml_restrict3<double>, dim3(lv.n[3] * kRestrict3Parts), dim3(kRestrict3WG), 0,
                                c->stream, lv, m.hord, m.ptr1, m.cptr, m.kptr2, c->r64, m.R, m.Dinv, m.C, m.feat2,
                                m.geo, m.R9p, c->scal, check_done);
-        if (lv.off[lv.L + 1] - lv.off[3] <= kCoarse9WG)
-            hipLaunchKernelGGL(k_ml_coarse9, dim3(1), dim3(kCoarse9WG), 0, c->stream, lv, m.cptr, m.par, m.geo, m.D9,
-                               m.R9p, m.C9, c->scal, check_done);
-        else
-            hipLaunchKernelGGL(k_ml_coarse9_any, dim3(1), dim3(kCoarse9WG), 0, c->stream, lv, m.cptr, m.par, m.geo,
-                               m.D9, m.R9p, m.R9, m.C9, c->scal, check_done);
+        hipLaunchKernelGGL(k_ml_coarse9, dim3(1), dim3(kCoarse9WG), 0, c->stream, lv, m.cptr, m.par, m.geo, m.D9,
+                           m.R9p, m.C9, c->scal, check_done);
         return;
     }
     const int waves = lv.L >= 2 ? lv.n[2] : (lv.n[1] + 63) / 64;
