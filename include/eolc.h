@@ -102,8 +102,8 @@ int eolc_set_mixed_stall(eolc_ctx *ctx, int first_cycle, int min_cycle, int stic
  * restart of a stalled mixed solve and its sticky successors keep the
  * translations), 2 in every mode.  Off by default: on states with collapsed
  * elements an FP64 solve with them took up to 3x the iterations of the translations
- * alone, and their restriction costs 31 against 14 us per iteration at 1024^2, which
- * over the benchmark trajectory cancels most of the iteration gain.  They apply when
+ * alone, and their restriction costs 31 against 14 us per iteration at 1024^2; over
+ * the benchmark trajectory setting 1 measured 3% more steps/s.  They apply when
  * the levels >= 3 hold at most 512 aggregates (a grid of about 1.4 M vertices);
  * larger hierarchies and the batched solve keep the translations. */
 int eolc_set_ml_linear(eolc_ctx *ctx, int where);

@@ -100,6 +100,8 @@ def test_multilevel_solves(ctx):
         print(f"  {name}: {st.iters} iterations, true relres {st.true_relres:.2e}", flush=True)
         assert rc == eolc.OK and st.true_relres <= tol
     ctx.set_ml_linear(0)
+    with pytest.raises(eolc.EolcError):
+        ctx.set_ml_linear(3)
     assert out["fp64-ml"][0].iters < 0.7 * out["fp64-bj"][0].iters
     assert out["fp64-ml-lin"][0].iters < out["fp64-ml"][0].iters
     assert out["mixed-ml-lin"][0].iters < out["mixed-ml"][0].iters
