@@ -456,9 +456,9 @@ def main():
                     help="headline without the multilevel coarse levels (and skip the multilevel lines)")
     ap.add_argument("--profile-solve", action="store_true",
                     help="kernel-trace runs: one step of each solve mode capped at 400 PCG iterations, then exit")
-    ap.add_argument("--ml-linear", type=int, default=0, choices=(0, 1, 2),
+    ap.add_argument("--ml-linear", type=int, default=None, choices=(0, 1, 2),
                     help="linear coarse fields on the multilevel levels >= 3 for every leg "
-                         "(eolc_set_ml_linear; 0 off, the library's default)")
+                         "(eolc_set_ml_linear; default: the library's, 1 = in FP32 iterations)")
     ap.add_argument("--profile-fill", action="store_true",
                     help="counter runs: 10 mixed-mode element evaluations + assemblies (the step's fill) and "
                          "a 1 GiB device copy for calibration, then exit")
@@ -479,7 +479,8 @@ def main():
     ctx.set_topology(sc.X, sc.tri, sc.hinge)
     ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
     ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
-    ctx.set_ml_linear(args.ml_linear)
+    if args.ml_linear is not None:
+        ctx.set_ml_linear(args.ml_linear)
     log(f"topology set-up {time.perf_counter() - t:.1f} s")
     # what a remesh costs on a live context: the pattern, SELL layout and gather maps
     # rebuilt and every per-topology device buffer replaced
@@ -661,7 +662,8 @@ def main():
         ctx.set_topology(sc.X, sc.tri, sc.hinge)
         ctx.set_material(0.1, 500.0, 0.3, 1e-5, 0.0, 1.0)
         ctx.set_options(eolc.OPT_SPD_PROJECT_BENDING)
-        ctx.set_ml_linear(args.ml_linear)
+        if args.ml_linear is not None:
+            ctx.set_ml_linear(args.ml_linear)
         ctx.upload_state(sc.x0, sc.v0)
         ctx.forces_fill(GRAV, H)
         ctx.set_spmv_variant(32, tuned[32]["variant"])
@@ -699,7 +701,7 @@ def main():
                    "vertices": sc.N, "triangles": sc.F, "hinges": sc.H, "blocks": nb,
                    "bending_hessian": "spd-projected", "weights": wsrc,
                    "multilevel_aggregates": None if args.no_ml else ml_counts,
-                   "multilevel_linear_fields": args.ml_linear},
+                   "multilevel_linear_fields": 1 if args.ml_linear is None else args.ml_linear},
         "speedup_vs_fp64_bj": legs.get("fp64_bj", {}).get("speedup_of_headline"),
         "roofline": roof,
         "roofline_step": head.get("roofline_step"),

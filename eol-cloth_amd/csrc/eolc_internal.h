@@ -388,11 +388,11 @@ struct eolc_ctx {
     int spmv_variant[2] = {0, 0};   // SpMV launch variant for FP32 [0] and FP64 [1] solves
     // FP32 patch records: a block whose largest entry exceeds stiff_rel x dampingB h^2 E
     // is applied from its FP64 values by both its rows (Topology::sellbase, stiff masks)
-    double stiff_rel = 1e3;
+    double stiff_rel = 10.0;
     double e_batch_max = 0.0;       // largest per-instance Young's modulus of a batch
     double mixed_delta = 0.5;       // reliable-update trigger of the mixed solves
     double mixed_delta_ml = 0.2;    // the same with the multilevel levels (shorter cycles)
-    int ml_linear = 0;              // linear coarse fields: 0 never, 1 in FP32 iterations, 2 always
+    int ml_linear = 1;              // linear coarse fields: 0 never, 1 in FP32 iterations, 2 always
     double batch_delta = 0.1;       // the batched solve (its FP64 residuals cost more per iteration)
     // per-element material (batched cloths): Young's modulus per face, bending
     // stiffness per hinge; null = the context's scalar material

@@ -432,8 +432,8 @@ class Context:
         return self._check(lib().eolc_set_mixed_stall(self._h, int(first_cycle), int(min_cycle), int(sticky)))
 
     def set_ml_linear(self, where):
-        """Linear coarse fields on the multilevel levels >= 3: 0 never (default), 1 in
-        FP32 iterations only, 2 in every mode (eolc_set_ml_linear)."""
+        """Linear coarse fields on the multilevel levels >= 3: 0 never, 1 in FP32
+        iterations only (default), 2 in every mode (eolc_set_ml_linear)."""
         self._check(lib().eolc_set_ml_linear(self._h, int(where)))
 
     def set_stiff_threshold(self, rel):

@@ -77,7 +77,7 @@ typedef enum {
  * A value set here applies to every mode. */
 int eolc_set_mixed_delta(eolc_ctx *ctx, double delta);
 /* Stiff-block threshold of the FP32 patch records (SpMV variants 25-28), relative to
- * dampingB h^2 E (default 1e3; 0 turns it off).  A block whose largest entry exceeds
+ * dampingB h^2 E (default 10; 0 turns it off).  A block whose largest entry exceeds
  * it is applied from its FP64 values by both of its rows, because rounded to FP32 a
  * block of a collapsed element (up to 1e10 times the others) perturbs the soft
  * directions of the matrix by more than their own stiffness.  No counterpart in the
@@ -98,14 +98,14 @@ int eolc_set_mixed_stall(eolc_ctx *ctx, int first_cycle, int min_cycle, int stic
  * of the material coordinates instead of the constant (9 x 9 Galerkin blocks), so a
  * coarse correction is linear on the aggregate and resolves the sheet's smooth
  * bending modes better: 20-25% fewer iterations at 512^2 and 1024^2 on most states.
- * `where`: 0 never (default), 1 in FP32 iterations only (FP64 solves, the FP64
- * restart of a stalled mixed solve and its sticky successors keep the
- * translations), 2 in every mode.  Off by default: on states with collapsed
- * elements an FP64 solve with them took up to 3x the iterations of the translations
- * alone, and their restriction costs 31 against 14 us per iteration at 1024^2; over
- * the benchmark trajectory setting 1 measured 3% more steps/s.  They apply when
- * the levels >= 3 hold at most 512 aggregates (a grid of about 1.4 M vertices);
- * larger hierarchies and the batched solve keep the translations. */
+ * `where`: 0 never, 1 (default) in FP32 iterations only -- FP64 solves, the FP64
+ * restart of a stalled mixed solve and its sticky successors keep the translations,
+ * because on states with collapsed elements an FP64 solve with the fields took up to
+ * 3x the iterations of the translations alone -- 2 in every mode.  With the stiff
+ * blocks in FP64 (eolc_set_stiff_threshold) they keep the mixed solves of the
+ * benchmark trajectory in FP32 where the translations stall.  They apply when the
+ * levels >= 3 hold at most 512 aggregates (a grid of about 1.4 M vertices); larger
+ * hierarchies and the batched solve keep the translations. */
 int eolc_set_ml_linear(eolc_ctx *ctx, int where);
 
 /* Velocity transfer after a remesh, non-EOL vertices: Cloth::velocityTransfer

@@ -57,23 +57,23 @@ def test_hierarchy_and_apply_match_oracle(ctx, n):
     assert len(counts) >= 4   # levels >= 3 exist: the linear fields are exercised
     zlin = ML.apply(A, sc.N, maps, r, fd, X=sc.X)       # linear fields on the levels >= 3
     ztr = ML.apply(A, sc.N, maps, r, fd)                 # translations only
-    # default: translations alone; eolc_set_ml_linear 1: linear fields in FP32
-    # iterations only; 2: in every mode
-    z64 = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
-    z32t = ctx.precond_apply(r, fd, mode=eolc.SOLVE_MIXED_BJ | MLF)
-    ctx.set_ml_linear(1)
+    # eolc_set_ml_linear 0: translations alone; 1 (the default): linear fields in
+    # FP32 iterations only; 2: in every mode
     z32 = ctx.precond_apply(r, fd, mode=eolc.SOLVE_MIXED_BJ | MLF)
     z64m = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
+    ctx.set_ml_linear(0)
+    z64 = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
+    z32t = ctx.precond_apply(r, fd, mode=eolc.SOLVE_MIXED_BJ | MLF)
     ctx.set_ml_linear(2)
     z64l = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
-    ctx.set_ml_linear(0)
+    ctx.set_ml_linear(1)
     zbj = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ)
     ebj = rel_err(zbj, ML.apply(A, sc.N, [], r, fd))
     e64, e64l, e32, e32t = rel_err(z64, ztr), rel_err(z64l, zlin), rel_err(z32, zlin), rel_err(z32t, ztr)
     print(f"  rel err FP64 translations {e64:.2e}, FP64 linear {e64l:.2e}, mixed linear {e32:.2e}, "
           f"mixed translations {e32t:.2e}, block-Jacobi only {ebj:.2e}", flush=True)
     assert e64 <= 1e-12 and e64l <= 1e-12 and ebj <= 1e-12 and e32 <= 1e-5 and e32t <= 1e-5
-    assert np.array_equal(z64m, z64)   # setting 1 keeps the translations in FP64
+    assert np.array_equal(z64m, z64)   # the default keeps the translations in FP64
     for z in (z64, z64l, z32, z32t):
         assert np.all(z[fd] == 0.0)
     # the coarse levels change z (they are not a no-op), and so do the linear fields
@@ -99,7 +99,7 @@ def test_multilevel_solves(ctx):
         out[name] = (st, v)
         print(f"  {name}: {st.iters} iterations, true relres {st.true_relres:.2e}", flush=True)
         assert rc == eolc.OK and st.true_relres <= tol
-    ctx.set_ml_linear(0)
+    ctx.set_ml_linear(1)
     with pytest.raises(eolc.EolcError):
         ctx.set_ml_linear(3)
     assert out["fp64-ml"][0].iters < 0.7 * out["fp64-bj"][0].iters
@@ -210,7 +210,7 @@ def test_multilevel_edge_cases(ctx, case):
     r = rng.normal(size=3 * N)
     ctx.set_ml_linear(2)
     z = ctx.precond_apply(r, fd, mode=eolc.SOLVE_FP64_BJ | MLF)
-    ctx.set_ml_linear(0)
+    ctx.set_ml_linear(1)
     zref = ML.apply(A, N, maps, r, fd, X=X)
     print(f"  P r rel err {rel_err(z, zref):.2e}", flush=True)
     assert rel_err(z, zref) <= 1e-12 and np.all(z[fd] == 0.0)

@@ -98,7 +98,7 @@ def test_stiff_blocks_applied_in_fp64(ctx):
         print(f"  stiff threshold {rel:g}: {st.iters} iterations, v rel diff vs FP64 {rel_err(v, ref):.2e}",
               flush=True)
         assert rel_err(v, ref) <= 1e-6
-    ctx.set_stiff_threshold(1e3)
+    ctx.set_stiff_threshold(10.0)
     ctx.set_spmv_variant(32, 0)
 
 
