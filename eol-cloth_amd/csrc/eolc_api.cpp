@@ -80,7 +80,7 @@ void free_topology_buffers(eolc_ctx *c)
     dfree(c->S64u); dfree(c->M64);
     dfree(c->pt_row); dfree(c->pt_cnt); dfree(c->pt_roff); dfree(c->pt_soff); dfree(c->pt_loff); dfree(c->pt_lo);
     dfree(c->pt_e2r); dfree(c->pt_e2l); dfree(c->pt_woff); dfree(c->pt_win); dfree(c->P32); dfree(c->P64);
-    dfree(c->stiff); dfree(c->smask); dfree(c->sellbase);
+    dfree(c->stiff); dfree(c->smask); dfree(c->sellbase); dfree(c->A64s);
     dfree(c->fixmask); dfree(c->vfix); dfree(c->fix_dofs); dfree(c->fix_vals);
     dfree(c->b64); dfree(c->xs64); dfree(c->r64); dfree(c->z64); dfree(c->q64);
     dfree(c->p64[0]); dfree(c->p64[1]);
@@ -1147,6 +1147,7 @@ static int set_topology_impl(eolc_ctx *c, int N, const double *X, int F, const i
         EOLC_HIP(upload(c, &c->pt_win, T.pt_win));
         EOLC_HIP(dalloc(c, &c->P32, 10 * (size_t)std::max<int64_t>(T.pt_nrec, 1)));
         EOLC_HIP(dalloc(c, &c->stiff, (size_t)T.nbs));
+        EOLC_HIP(dalloc(c, &c->A64s, (size_t)9 * T.nbs));   // written for the stiff entries only
         EOLC_HIP(dalloc(c, &c->smask, (size_t)N));
         std::vector<int> sb(N);
         for (int g = 0; g < N; ++g) sb[T.srow[g]] = T.slice_off[g / eolc::kSlice] + g % eolc::kSlice;

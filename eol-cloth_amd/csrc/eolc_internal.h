@@ -349,6 +349,7 @@ struct eolc_ctx {
     unsigned short *pt_lo = nullptr;
     int *pt_e2r = nullptr, *pt_e2l = nullptr, *pt_woff = nullptr, *pt_win = nullptr;
     unsigned char *stiff = nullptr;   // nbs: the SELL entry's block is applied in FP64
+    double *A64s = nullptr;           // 9 x nbs: such a block's FP64 values, contiguous per entry
     unsigned long long *smask = nullptr;   // N: bit k = the row's SELL entry k is stiff
     int *sellbase = nullptr;          // N: SELL entry 0 of the row (entry k at + 64 k)
     float *P32 = nullptr;       // 10 x pt_nrec

@@ -35,7 +35,7 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
            const int *__restrict__ scol, float *__restrict__ S32u, float *__restrict__ S32a,
            double *__restrict__ S64u, const int *__restrict__ e2r, const int *__restrict__ e2l,
            float *__restrict__ P32, double *__restrict__ P64, double tau, unsigned char *__restrict__ stiff,
-           double *__restrict__ Mval)
+           double *__restrict__ A64s, double *__restrict__ Mval)
 {
     int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbs) return;
@@ -106,7 +106,11 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
             double mx = 0.0;
 #pragma unroll
             for (int c = 0; c < 9; ++c) mx = fmax(mx, fabs(stf[c] + (c % 4 == 0 ? m : 0.0)));
-            stiff[b] = (!diag && tau > 0.0 && mx > tau) ? 1 : 0;
+            const bool st = !diag && tau > 0.0 && mx > tau;
+            stiff[b] = st ? 1 : 0;
+            if (st)   // the FP64 values once more, contiguous: the SpMV reads them as one 72-byte piece
+#pragma unroll
+                for (int c = 0; c < 9; ++c) A64s[9 * b + c] = acc[c];
         }
         if (pr >= 0 && (lay & eolc::kLayPatch64)) {
             double *rec = P64 + (size_t)10 * pr;
@@ -201,7 +205,7 @@ hipError_t launch_assemble(eolc_ctx *c, int lay)
     const double tau = c->stiff_rel > 0.0 ? c->stiff_rel * c->dampB * c->h_last * c->h_last * emax : 0.0;
     hipLaunchKernelGGL(k_assemble, dim3(blocks_for(nb)), dim3(256), 0, c->stream, nb, 6 * c->topo.F, c->cptr, c->clist,
                        c->entry_is_diag, c->stageK, c->stageM, c->A64, lay, c->S32, c->e2u, c->scol, c->S32u,
-                       c->S32a, c->S64u, c->pt_e2r, c->pt_e2l, c->P32, c->P64, tau, c->stiff, c->Mval);
+                       c->S32a, c->S64u, c->pt_e2r, c->pt_e2l, c->P32, c->P64, tau, c->stiff, c->A64s, c->Mval);
     if (lay & eolc::kLayPatch)
         hipLaunchKernelGGL(k_stiff_mask, dim3(blocks_for(c->topo.N)), dim3(256), 0, c->stream, c->topo.N,
                            c->slice_off, c->srow, c->slen, c->stiff, c->pt_e2r, c->P32, c->smask);

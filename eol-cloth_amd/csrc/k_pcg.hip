@@ -428,11 +428,10 @@ struct PatchLayout {
     const double *R64;
     int S;
     // blocks too stiff for FP32 (their FP32 records are zero): the row's mask of
-    // stiff SELL entries, its first SELL entry, the SELL columns and the FP64 blocks
+    // stiff SELL entries, its first SELL entry, the SELL columns and the FP64 values
     const unsigned long long *smask;
     const int *sbase, *scol;
-    const double *A64;
-    int64_t nbs;
+    const double *A64s;   // the stiff blocks' FP64 values, 9 contiguous per SELL entry
 };
 template <class T> __device__ __forceinline__ const T *patch_records(const PatchLayout &pl);
 template <> __device__ __forceinline__ const float *patch_records<float>(const PatchLayout &pl) { return pl.R32; }
@@ -609,11 +608,10 @@ k_pcg_spmv_patch(int k, PatchLayout pl, const T *__restrict__ Mm, const unsigned
                     const double d0 = (double)fma(betaT, pold[3 * j], z[3 * j]) - di0;
                     const double d1 = (double)fma(betaT, pold[3 * j + 1], z[3 * j + 1]) - di1;
                     const double d2 = (double)fma(betaT, pold[3 * j + 2], z[3 * j + 2]) - di2;
-                    const double *a = pl.A64 + e;
-                    const int64_t n = pl.nbs;
-                    y0 += fma(a[2 * n], d2, fma(a[n], d1, a[0] * d0));
-                    y1 += fma(a[5 * n], d2, fma(a[4 * n], d1, a[3 * n] * d0));
-                    y2 += fma(a[8 * n], d2, fma(a[7 * n], d1, a[6 * n] * d0));
+                    const double *a = pl.A64s + 9 * (int64_t)e;
+                    y0 += fma(a[2], d2, fma(a[1], d1, a[0] * d0));
+                    y1 += fma(a[5], d2, fma(a[4], d1, a[3] * d0));
+                    y2 += fma(a[8], d2, fma(a[7], d1, a[6] * d0));
                 }
             }
         }
@@ -1103,7 +1101,7 @@ void launch_spmv_one(eolc_ctx *c, int k, int G, const T *A, const T *Mm, const T
         using TR = T;
         const PatchLayout pl{c->pt_row, c->pt_cnt, c->pt_roff, c->pt_soff, c->pt_loff, c->pt_woff, c->pt_win,
                              c->pt_lo,  c->P32,    c->P64,     t.pt_slots, c->smask,   c->sellbase, c->scol,
-                             c->A64,    t.nbs};
+                             c->A64s};
         hipLaunchKernelGGL((k_pcg_spmv_patch<T, TR, NT, W, U>), dim3(G), dim3(WG), 0, c->stream, k, pl, Mm, c->fixmask,
                            z, pold, pnew, q, c->scal, c->partials);
     } else {
