@@ -356,8 +356,10 @@ __device__ __forceinline__ void lower9(const double *__restrict__ P, double al, 
 // or of the 64 level-1 aggregates from w * 64 (one level).  Writes R_1 of its
 // children, R_2(w) and C of its children; returns R_2(w) in every lane and, with
 // feat2 (the members' level-2 coordinate features in hord order), the first moments
-// sum phi_1 r and sum phi_2 r in lane 0.
-template <class T>
+// sum phi_1 r and sum phi_2 r in lane 0.  Without the moments (LIN false) the
+// inverse blocks load before the member loop, hiding their latency; with them they
+// load after it, which keeps the loop's registers at six waves per SIMD.
+template <class T, bool LIN>
 __device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const int *__restrict__ hord,
                                               const int *__restrict__ ptr1, const int *__restrict__ cptr,
                                               const int *__restrict__ kptr2, const T *__restrict__ r,
@@ -384,14 +386,15 @@ __device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const i
     // inverse blocks of the level-2 aggregate and of this lane's first child: they do
     // not depend on r, so they load with the member indices
     double dw[6] = {0, 0, 0, 0, 0, 0}, da[6] = {0, 0, 0, 0, 0, 0};
-    if (two) {
+    auto load_inv = [&]() {
         const int64_t gw = lv.off[2] + w;
 #pragma unroll
         for (int k = 0; k < 6; ++k) dw[k] = Dinv[6 * gw + k];
         if (a0 + lane < a1)
 #pragma unroll
             for (int k = 0; k < 6; ++k) da[k] = Dinv[6 * (int64_t)(a0 + lane) + k];
-    }
+    };
+    if (two && !LIN) load_inv();
     s0 = s1 = s2 = 0.0;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;   // R_1 of this lane's first child
     double mu0 = 0.0, mu1 = 0.0, mu2 = 0.0, mv0 = 0.0, mv1 = 0.0, mv2 = 0.0;
@@ -407,7 +410,7 @@ __device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const i
             const int k = kb + lane;
             if (k < k1) {
                 double u = 0.0, v = 0.0;   // features first: they do not wait for hord
-                if (feat2) {
+                if (LIN) {
                     u = feat2[2 * (int64_t)k];
                     v = feat2[2 * (int64_t)k + 1];
                 }
@@ -416,7 +419,7 @@ __device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const i
                 buf[3 * lane] = q0;
                 buf[3 * lane + 1] = q1;
                 buf[3 * lane + 2] = q2;
-                if (feat2) {
+                if (LIN) {
                     mu0 = fma(u, q0, mu0); mu1 = fma(u, q1, mu1); mu2 = fma(u, q2, mu2);
                     mv0 = fma(v, q0, mv0); mv1 = fma(v, q1, mv1); mv2 = fma(v, q2, mv2);
                 }
@@ -438,6 +441,7 @@ __device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const i
         k0 = k1;   // the next group starts where this one ends
     }
     if (!two) return;
+    if (LIN) load_inv();
     // wave_sum leaves the total in lane 0; every lane needs it below
     s0 = __shfl(wave_sum(s0), 0, 64);
     s1 = __shfl(wave_sum(s1), 0, 64);
@@ -446,7 +450,7 @@ __device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const i
         double *o = R + 3 * (lv.off[2] + (int64_t)w);
         o[0] = s0; o[1] = s1; o[2] = s2;
     }
-    if (feat2) {
+    if (LIN) {
         mom[0] = wave_sum(mu0); mom[1] = wave_sum(mu1); mom[2] = wave_sum(mu2);
         mom[3] = wave_sum(mv0); mom[4] = wave_sum(mv1); mom[5] = wave_sum(mv2);
     }
@@ -482,7 +486,7 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
     const int w = (blockIdx.x * blockDim.x + threadIdx.x) / 64;   // uniform in the wavefront
     if (lv.L >= 2 ? w >= lv.n[2] : w * 64 >= lv.n[1]) return;
     double s0, s1, s2, mom[6];
-    restrict_wave<T>(lv, w, hord, ptr1, cptr, kptr2, r, R, Dinv, C, nullptr, stage[threadIdx.x / 64], s0, s1, s2,
+    restrict_wave<T, false>(lv, w, hord, ptr1, cptr, kptr2, r, R, Dinv, C, nullptr, stage[threadIdx.x / 64], s0, s1, s2,
                      mom, scal, check_done);
 }
 
@@ -524,7 +528,7 @@ __global__ void __launch_bounds__(kRestrict3WG) k_ml_restrict3(MLLevels lv, cons
         for (int q = 0; q < 9; ++q) red[wid][q] = 0.0;
     for (int w = c0 + part * NW + wid; w < c1; w += NW * kRestrict3Parts) {
         double s0, s1, s2, mom[6];
-        restrict_wave<T>(lv, w, hord, ptr1, cptr, kptr2, r, R, Dinv, C, feat2, stage[wid], s0, s1, s2, mom, scal, 0);
+        restrict_wave<T, true>(lv, w, hord, ptr1, cptr, kptr2, r, R, Dinv, C, feat2, stage[wid], s0, s1, s2, mom, scal, 0);
         if (lane == 0) {
             const int64_t gw = lv.off[2] + w;
             const double al = geo[4 * gw + 2] * is, bu = (geo[4 * gw] - cu) * is, bv = (geo[4 * gw + 1] - cv) * is;
