@@ -415,6 +415,9 @@ __device__ __forceinline__ void restrict_wave(const MLLevels &lv, int w, const i
                     v = feat2[2 * (int64_t)k + 1];
                 }
                 const int i = hord[k];
+                // the compiler otherwise issues the feature load after the gather of r
+                // below, a third dependent memory round per 64 members
+                if (LIN) asm volatile("" ::: "memory");
                 const double q0 = (double)r[3 * i], q1 = (double)r[3 * i + 1], q2 = (double)r[3 * i + 2];
                 buf[3 * lane] = q0;
                 buf[3 * lane + 1] = q1;
