@@ -500,9 +500,11 @@ __global__ void __launch_bounds__(kRestrictWG) k_ml_restrict(MLLevels lv, const 
 // M_2 + beta_v R_2) with alpha = s_w / s_U and beta = (c_w - c_U) / s_U, summed per
 // wavefront in child order and over the workgroup's wavefronts in order: one partial
 // R9 per workgroup, which the coarse kernel adds in order.  At 1024^2 the kernel takes
-// 31 us against 14 us for the translations' restriction, the same with one
-// 512-thread workgroup per level-3 aggregate: timing variants put about 10 us on the
-// first moments (the feature stream and its sums) and 1.5 us on the re-expression.
+// 26 us against 14 us for the translations' restriction.  It ran at 31 us with the
+// inverse blocks loaded before the member loop (101 VGPRs, four waves per SIMD) and
+// at 28 us with the feature load issued after the gather of r; 512-thread workgroups
+// (32 wavefronts per level-3 aggregate) took 36 us, and loading the next 64 members'
+// indices and features during the gather gained nothing.
 constexpr int kRestrict3WG = 256;
 using eolc::kRestrict3Parts;
 template <class T>
