@@ -643,8 +643,19 @@ k_pcg_update_flat(int k, int G, int n3, const T *__restrict__ pnew, const T *__r
                   T *__restrict__ r, SolverScalars *scal, double *partials)
 {
     __shared__ double lds[2 * (WG / 64) + 2];
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    typedef TX x4 __attribute__((ext_vector_type(4)));
     if (scal->done) return;
     const int chunk = (((n3 + G - 1) / G) + 3) & ~3;
+    const int lw0 = logical_wg(), lwx = extra_logical_wg(G);
+    // mixed precision: the first trip's FP64 accumulator (half the bytes read) loads
+    // before the partial sums are re-reduced (it does not depend on alpha), so the two
+    // latencies overlap.  The other vectors too, or the FP64 mode's, would spill at
+    // eight waves per SIMD.
+    constexpr bool PRE = sizeof(T) == 4;
+    const int b0 = min(lw0 * chunk, n3), e0 = min(b0 + chunk, n3), i0 = b0 + 4 * (int)threadIdx.x;
+    x4 xv0 = {};
+    if (PRE && i0 + 3 < e0) xv0 = *reinterpret_cast<const x4 *>(x + i0);
     double pq[1], rz[1];
     reduce_partials2<1, 1>(PA(partials), PB(partials, k & 1), pq, rz, lds, G);
     if (!(pq[0] > 0.0)) {
@@ -659,7 +670,6 @@ k_pcg_update_flat(int k, int G, int n3, const T *__restrict__ pnew, const T *__r
     const T alphaT = (T)alpha;
     // the G logical ranges may run on fewer workgroups: the first ones then take a
     // second range
-    const int lw0 = logical_wg(), lwx = extra_logical_wg(G);
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
         const int lw = pass == 0 ? lw0 : lwx;
@@ -669,11 +679,9 @@ k_pcg_update_flat(int k, int G, int n3, const T *__restrict__ pnew, const T *__r
         for (int i = b + 4 * (int)threadIdx.x; i < e; i += 4 * WG) {
             if (i + 3 < e) {
                 // i is a multiple of 4: aligned 4-element vector loads and stores
-                typedef T v4 __attribute__((ext_vector_type(4)));
-                typedef TX x4 __attribute__((ext_vector_type(4)));
                 const v4 pv = *reinterpret_cast<const v4 *>(pnew + i), qv = *reinterpret_cast<const v4 *>(q + i);
                 const v4 rv = *reinterpret_cast<const v4 *>(r + i);
-                const x4 xv = *reinterpret_cast<const x4 *>(x + i);
+                const x4 xv = (PRE && pass == 0 && i == i0) ? xv0 : *reinterpret_cast<const x4 *>(x + i);
                 x4 xo;
                 v4 ro;
 #pragma unroll
