@@ -714,30 +714,52 @@ __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const in
     const int L = lv.L, base = lv.off[3];
     const int M = lv.off[L + 1] - base, n3 = lv.n[3];
     const int h = threadIdx.x;   // this thread's aggregate (levels >= 3 concatenated)
-    int l = 0, A = 0, c0 = 0, c1 = 0;
-    double F[kFact9];
-    if (h < M) {
-        l = 3;
-        while (l < L && h >= lv.off[l + 1] - base) ++l;
-        A = h - (lv.off[l] - base);
-        if (l > 3) {
-            c0 = cptr[lv.coff[l] + A];
-            c1 = cptr[lv.coff[l] + A + 1];
-        }
-#pragma unroll
-        for (int k = 0; k < kFact9; ++k) F[k] = D9[kFact9 * (int64_t)h + k];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sg[4 * h + k] = geo[4 * (int64_t)(base + h) + k];
-        sp[h] = l < L ? par[base + h] : 0;
-    }
-    // the level-3 sums: the restriction's partial sums of each aggregate, in order
+    // Every load is issued before the first wait: the level-3 partial sums first (they
+    // depend on h only), then the aggregate's factors, geometry, parent and child
+    // range.  The level and the offsets of its neighbour levels come from the uniform
+    // offsets with constant indices (scalar loads; a variable index reads the argument
+    // block with a vector load, one more memory round per level), and the LDS stores
+    // follow the loads.
     double r3[kRestrict3Parts][9];
     if (h < n3)
 #pragma unroll
         for (int q = 0; q < kRestrict3Parts; ++q)
 #pragma unroll
             for (int k = 0; k < 9; ++k) r3[q][k] = R9p[9 * ((int64_t)h * kRestrict3Parts + q) + k];
+    int l = 0, A = 0, c0 = 0, c1 = 0, pp = 0, offm1 = 0, offp1 = 0;
+    double F[kFact9], g4[4];
+    if (h < M) {
+        l = 3;
+        int offl = lv.off[3], coffl = lv.coff[3];
+        offm1 = lv.off[2];
+        offp1 = lv.off[4];
+#pragma unroll
+        for (int k = 4; k <= eolc::kMLMaxLevels; ++k)
+            if (k <= L && h >= lv.off[k] - base) {   // offsets increase with the level
+                l = k;
+                offl = lv.off[k];
+                coffl = lv.coff[k];
+                offm1 = lv.off[k - 1];
+                offp1 = lv.off[k + 1];
+            }
+        A = h - (offl - base);
+        if (l > 3) {
+            c0 = cptr[coffl + A];
+            c1 = cptr[coffl + A + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kFact9; ++k) F[k] = D9[kFact9 * (int64_t)h + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) g4[k] = geo[4 * (int64_t)(base + h) + k];
+        pp = l < L ? par[base + h] : 0;
+    }
     if (check_done && scal->done) return;
+    if (h < M) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sg[4 * h + k] = g4[k];
+        sp[h] = pp;
+    }
+    // the level-3 sums: the restriction's partial sums of each aggregate, in order
     if (h < n3)
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
@@ -753,7 +775,7 @@ __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const in
             double sm[9];
 #pragma unroll
             for (int q = 0; q < 9; ++q) sm[q] = 0.0;
-            const int cb = lv.off[lev - 1] - base;
+            const int cb = offm1 - base;   // level lev - 1
             for (int ch = c0; ch < c1; ++ch) {
                 const int hc = cb + ch;
                 lift9(sR + 9 * hc, sg[4 * hc + 2] * is, (sg[4 * hc] - cu) * is, (sg[4 * hc + 1] - cv) * is, sm);
@@ -770,7 +792,7 @@ __global__ void __launch_bounds__(kCoarse9WG) k_ml_coarse9(MLLevels lv, const in
             for (int q = 0; q < 9; ++q) rv[q] = sR[9 * h + q];
             apply9(F, rv, cf);
             if (lev < L) {
-                const int hp = lv.off[lev + 1] - base + sp[h];
+                const int hp = offp1 - base + sp[h];   // the parent on level lev + 1
                 const double ip = sg[4 * hp + 3];
                 lower9(sC + 9 * hp, sg[4 * h + 2] * ip, (sg[4 * h] - sg[4 * hp]) * ip, (sg[4 * h + 1] - sg[4 * hp + 1]) * ip,
                        cf);
