@@ -573,18 +573,31 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
         // loaded up front (with level 3's children sums), then every phase runs in LDS
         double *Rl = lds, *Cl = lds + 3 * kCoarseLds;
         const int j = threadIdx.x;
-        int l = 0, pa = 0, c0 = 0, c1 = 0;
+        int l = 0, pa = 0, c0 = 0, c1 = 0, offm1 = 0, offp1 = 0;
         double d[6] = {0, 0, 0, 0, 0, 0};
         if (j < M) {
+            // the level and its neighbours' offsets from constant indices into the
+            // arguments (scalar loads; a variable index costs a vector load per use)
             l = 3;
-            while (l < L && j >= lv.off[l + 1] - base) ++l;
-            const int a = j - (lv.off[l] - base);
+            int offl = lv.off[3], coffl = lv.coff[3];
+            offm1 = lv.off[2];
+            offp1 = lv.off[4];
+#pragma unroll
+            for (int k = 4; k <= eolc::kMLMaxLevels; ++k)
+                if (k <= L && j >= lv.off[k] - base) {   // offsets increase with the level
+                    l = k;
+                    offl = lv.off[k];
+                    coffl = lv.coff[k];
+                    offm1 = lv.off[k - 1];
+                    offp1 = lv.off[k + 1];
+                }
+            const int a = j - (offl - base);
             const int64_t g = base + j;
 #pragma unroll
             for (int k = 0; k < 6; ++k) d[k] = Dinv[6 * g + k];
             if (l < L) pa = par[g];
-            c0 = cptr[lv.coff[l] + a];
-            c1 = cptr[lv.coff[l] + a + 1];
+            c0 = cptr[coffl + a];
+            c1 = cptr[coffl + a + 1];
         }
         // the stop flag is read with the first loads (uniform: every thread leaves
         // before the first barrier, or none does)
@@ -614,7 +627,7 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
         __syncthreads();
         for (int lev = 4; lev <= L; ++lev) {
             if (j < M && l == lev) {
-                const int cb = lv.off[lev - 1] - base;
+                const int cb = offm1 - base;   // level lev - 1
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;
                 for (int b = c0; b < c1; ++b) {
                     s0 += Rl[3 * (cb + b)]; s1 += Rl[3 * (cb + b) + 1]; s2 += Rl[3 * (cb + b) + 2];
@@ -628,7 +641,7 @@ __global__ void __launch_bounds__(kCoarseWG) k_ml_coarse(MLLevels lv, const int 
                 double e0, e1, e2;
                 apply_inv(d, Rl[3 * j], Rl[3 * j + 1], Rl[3 * j + 2], e0, e1, e2);
                 if (lev < L) {
-                    const int up = lv.off[lev + 1] - base + pa;
+                    const int up = offp1 - base + pa;   // the parent on level lev + 1
                     e0 += Cl[3 * up]; e1 += Cl[3 * up + 1]; e2 += Cl[3 * up + 2];
                 }
                 Cl[3 * j] = e0; Cl[3 * j + 1] = e1; Cl[3 * j + 2] = e2;
