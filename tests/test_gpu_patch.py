@@ -90,14 +90,26 @@ def test_patch_variants_on_an_irregular_renumbered_mesh(ctx, prec):
 
 def test_stiff_blocks_applied_in_fp64(ctx):
     """Thresholds that send none, some and nearly all blocks to the FP64 path give
-    the same mixed solve to the tolerance (the path runs in both rows of a block)."""
+    the same mixed solve to the tolerance (the path runs in both rows of a block).
+    The answer alone would not expose a wrong FP64 copy of a block: the mixed solve's
+    FP64 true residual corrects an inconsistent FP32 operator, at the cost of more
+    refinements or a fallback.  So the iteration and refinement counts must also
+    match the run without stiff blocks (the operators differ by FP32 rounding only),
+    and no run may fall back to FP64."""
     mesh = irregular_sheet(n=40, seed=5)
     ref, _ = solve(ctx, mesh, 64, 0)
+    base = None
     for rel in (0.0, 1e-1, 1e-4, 1e-8):
         v, st = solve(ctx, mesh, 32, 25, rel=rel)
-        print(f"  stiff threshold {rel:g}: {st.iters} iterations, v rel diff vs FP64 {rel_err(v, ref):.2e}",
-              flush=True)
+        print(f"  stiff threshold {rel:g}: {st.iters} iterations, {st.refinements} refinements, "
+              f"fallback {st.used_fallback}, v rel diff vs FP64 {rel_err(v, ref):.2e}", flush=True)
         assert rel_err(v, ref) <= 1e-6
+        assert st.used_fallback == 0
+        if base is None:   # threshold 0: no stiff blocks
+            base = (st.iters, st.refinements)
+        else:
+            assert abs(st.iters - base[0]) <= 0.1 * base[0] + 5, (st.iters, base)
+            assert st.refinements <= base[1] + 3, (st.refinements, base)
     ctx.set_stiff_threshold(10.0)
     ctx.set_spmv_variant(32, 0)
 
