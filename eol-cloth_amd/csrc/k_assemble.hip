@@ -125,7 +125,7 @@ k_assemble(int64_t nbs, int ntrislots, const int *__restrict__ cptr, const int *
 // One thread per SELL storage position: the row's stiff-entry mask (a row holds at
 // most 64 off-diagonal blocks where the patch layout applies) and the FP32 patch records of its stiff entries
 // zeroed, so that the FP32 product skips them and the patch SpMV applies them from
-// the FP64 values (A64) in both rows.
+// their contiguous FP64 copies (A64s, written by k_assemble) in both rows.
 __global__ void __launch_bounds__(256)
 k_stiff_mask(int N, const int *__restrict__ slice_off, const int *__restrict__ srow, const int *__restrict__ slen,
              const unsigned char *__restrict__ stiff, const int *__restrict__ e2r, float *__restrict__ P32,
